@@ -1,0 +1,287 @@
+"""Python host mirror of the MI355X EVP1D engine (ctypes on libevp1d.so).
+
+The product boundary is the C ABI in include/evp1d.h; the production host is
+the Fortran layer in cice-evp1d_amd/fortran.  This module binds the same .so
+for the verification harness and bench.py:
+
+    from evp1d import Engine, set_evp_parameters, build_index
+    p = set_evp_parameters(ndte=120)
+    eng = Engine(na, navel, p)               # evp1d_create
+    eng.set_index("ee", ee) ...              # 1-based, as findXinY produces
+    eng.set_field("uvel", uvel) ...
+    eng.run(120)                             # evp1d_run
+    eng.get_field("stressp_1")
+
+Names and meaning follow the reference (src/bench_v2b.F90, src/vars.F90,
+integration/ice_dyn_evp1d.F90).  Every entry point raises Evp1dError with the
+library's message on a nonzero status; nothing here falls back to a CPU path:
+if the shared library is missing the import fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent.parent
+LIBDIR = PKG / "lib"
+
+FIELDS = (
+    "strength", "uvel", "vvel", "dxT", "dyT", "uarear", "cdn_ocn", "aiu", "uocn", "vocn",
+    "waterxU", "wateryU", "forcexU", "forceyU", "umassdti", "fmU", "strintxU", "strintyU", "Tbu",
+    "stressp_1", "stressp_2", "stressp_3", "stressp_4",
+    "stressm_1", "stressm_2", "stressm_3", "stressm_4",
+    "stress12_1", "stress12_2", "stress12_3", "stress12_4",
+    "HTE", "HTN", "HTEm1", "HTNm1", "uvel_init", "vvel_init", "Cb",
+    "str1", "str2", "str3", "str4", "str5", "str6", "str7", "str8",
+)
+FIELD_ID = {n: k for k, n in enumerate(FIELDS)}
+NAVEL_FIELDS = frozenset({"uvel", "vvel", "str1", "str2", "str3", "str4", "str5", "str6", "str7", "str8"})
+STRESS_FIELDS = FIELDS[19:31]
+NBR = ("ee", "ne", "se", "nw", "sw", "sse")
+MODES = {"launch": 0, "graph": 1, "persistent": 2, "auto": 3}
+
+# every symbol include/evp1d.h declares
+EXPORTS = (
+    "evp1d_set_evp_parameters", "evp1d_create", "evp1d_destroy", "evp1d_set_params",
+    "evp1d_get_params", "evp1d_get_sizes", "evp1d_set_field", "evp1d_get_field",
+    "evp1d_set_index", "evp1d_set_skip", "evp1d_set_halo", "evp1d_stress", "evp1d_stepu",
+    "evp1d_halo", "evp1d_run", "evp1d_set_mode", "evp1d_get_mode", "evp1d_run_individual",
+    "evp1d_synchronize", "evp1d_timer_start", "evp1d_timer_stop", "evp1d_index_count",
+    "evp1d_index_build", "evp1d_last_error", "evp1d_build_info",
+)
+
+
+class Evp1dError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[evp1d status {code}] {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    """evp1d_params (include/evp1d.h); mirrors ice_dyn_shared, src/ice.F90:80-107."""
+
+    _fields_ = [("ndte", ctypes.c_int32), ("reserved", ctypes.c_int32)] + [
+        (n, ctypes.c_double) for n in (
+            "capping", "e_factor", "epp2i", "deltaminEVP", "arlx", "arlx1i", "brlx", "denom1",
+            "revp", "Ktens", "u0", "cosw", "sinw", "rhow")]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int32)
+_H = ctypes.c_void_p
+
+_libs: dict = {}
+
+
+def lib_path(fma: bool = False) -> Path:
+    return LIBDIR / ("libevp1d_fma.so" if fma else "libevp1d.so")
+
+
+def load(fma: bool = False) -> ctypes.CDLL:
+    """Load libevp1d(.so|_fma.so) from the package's lib/ (raises if absent)."""
+    key = bool(fma)
+    if key in _libs:
+        return _libs[key]
+    path = lib_path(fma)
+    if not path.exists():
+        raise ImportError(f"{path} is not built: run __graft_entry__.build() or python -m evp1d.build")
+    L = ctypes.CDLL(str(path))
+    i32 = ctypes.c_int32
+    sig = {
+        "evp1d_set_evp_parameters": [i32, ctypes.c_double, ctypes.POINTER(Params)],
+        "evp1d_create": [i32, i32, ctypes.POINTER(Params), ctypes.POINTER(_H)],
+        "evp1d_destroy": [_H],
+        "evp1d_set_params": [_H, ctypes.POINTER(Params)],
+        "evp1d_get_params": [_H, ctypes.POINTER(Params)],
+        "evp1d_get_sizes": [_H, _IP, _IP],
+        "evp1d_set_field": [_H, i32, _DP, i32],
+        "evp1d_get_field": [_H, i32, _DP, i32],
+        "evp1d_set_index": [_H, i32, _IP, i32],
+        "evp1d_set_skip": [_H, i32, _IP, i32],
+        "evp1d_set_halo": [_H, i32, _IP, _IP],
+        "evp1d_stress": [_H], "evp1d_stepu": [_H], "evp1d_halo": [_H],
+        "evp1d_run": [_H, i32],
+        "evp1d_set_mode": [_H, i32],
+        "evp1d_get_mode": [_H, _IP],
+        "evp1d_run_individual": [_H, i32, i32],
+        "evp1d_synchronize": [_H],
+        "evp1d_timer_start": [_H],
+        "evp1d_timer_stop": [_H, ctypes.POINTER(ctypes.c_float)],
+        "evp1d_index_count": [i32, i32, i32, _IP, _IP, _IP],
+        "evp1d_index_build": [i32, i32, i32, _IP, i32, i32] + [_IP] * 9,
+        "evp1d_last_error": [ctypes.c_char_p, i32],
+        "evp1d_build_info": [_IP, _IP],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    _libs[key] = L
+    return L
+
+
+def _check(L, rc: int) -> None:
+    if rc != 0:
+        buf = ctypes.create_string_buffer(512)
+        L.evp1d_last_error(buf, 512)
+        raise Evp1dError(rc, buf.value.decode(errors="replace"))
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(_DP)
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(_IP)
+
+
+def set_evp_parameters(ndte: int, dt: float = 3600.0, fma: bool = False, **override) -> Params:
+    """set_evp_parameters (src/ice.F90:109-130) plus optional overrides such as
+    the four scalars readin_1d_v2 takes from the input file (vars.F90:2116)."""
+    L = load(fma)
+    p = Params()
+    _check(L, L.evp1d_set_evp_parameters(int(ndte), float(dt), ctypes.byref(p)))
+    for k, v in override.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def build_info(fma: bool = False) -> dict:
+    L = load(fma)
+    a, f = ctypes.c_int32(), ctypes.c_int32()
+    _check(L, L.evp1d_build_info(ctypes.byref(a), ctypes.byref(f)))
+    return {"abi": a.value, "fma": f.value}
+
+
+def build_index(tmask: np.ndarray, nghost: int = 1, fma: bool = False) -> dict:
+    """1D point list of a Fortran-ordered (nx, ny) T mask (rim included):
+    evp1d_index_count / evp1d_index_build."""
+    L = load(fma)
+    nx, ny = tmask.shape
+    m = np.asfortranarray(tmask != 0).astype(np.int32, order="F")
+    na, navel = ctypes.c_int32(), ctypes.c_int32()
+    _check(L, L.evp1d_index_count(nx, ny, nghost, _ip(m), ctypes.byref(na), ctypes.byref(navel)))
+    na, navel = na.value, navel.value
+    ti = np.zeros(max(na, 1), np.int32)
+    tj = np.zeros(max(na, 1), np.int32)
+    tij = np.zeros(max(navel, 1), np.int32)
+    nb = {n: np.zeros(max(na, 1), np.int32) for n in NBR}
+    _check(L, L.evp1d_index_build(nx, ny, nghost, _ip(m), na, navel, _ip(ti), _ip(tj), _ip(tij),
+                                  *[_ip(nb[n]) for n in NBR]))
+    out = dict(na=na, navel=navel, indxti=ti[:na], indxtj=tj[:na], indxtij=tij[:navel])
+    out.update({n: nb[n][:na] for n in NBR})
+    return out
+
+
+class Engine:
+    """One evp1d handle (device-resident 1D state of one domain or batch)."""
+
+    def __init__(self, na: int, navel: int, params: Params, fma: bool = False):
+        self.L = load(fma)
+        self.na, self.navel = int(na), int(navel)
+        self.fma = fma
+        h = _H()
+        _check(self.L, self.L.evp1d_create(self.na, self.navel, ctypes.byref(params), ctypes.byref(h)))
+        self.h = h
+
+    # lifetime ------------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.L.evp1d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # data movement --------------------------------------------------------
+    def length(self, name: str) -> int:
+        return self.navel if name in NAVEL_FIELDS else self.na
+
+    def set_field(self, name: str, a) -> None:
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        _check(self.L, self.L.evp1d_set_field(self.h, FIELD_ID[name], _dp(a), a.size))
+
+    def get_field(self, name: str) -> np.ndarray:
+        out = np.empty(self.length(name))
+        _check(self.L, self.L.evp1d_get_field(self.h, FIELD_ID[name], _dp(out), out.size))
+        return out
+
+    def set_index(self, name: str, idx) -> None:
+        a = np.ascontiguousarray(idx, dtype=np.int32)
+        _check(self.L, self.L.evp1d_set_index(self.h, NBR.index(name), _ip(a), a.size))
+
+    def set_skip(self, which: str, flags) -> None:
+        a = np.ascontiguousarray(flags, dtype=np.int32)
+        _check(self.L, self.L.evp1d_set_skip(self.h, {"skipT": 0, "skipU": 1}[which], _ip(a), a.size))
+
+    def set_halo(self, dst, src) -> None:
+        d = np.ascontiguousarray(dst, dtype=np.int32)
+        s = np.ascontiguousarray(src, dtype=np.int32)
+        if d.size != s.size:
+            raise ValueError("dst and src differ in length")
+        _check(self.L, self.L.evp1d_set_halo(self.h, d.size, _ip(d), _ip(s)))
+
+    def set_params(self, p: Params) -> None:
+        _check(self.L, self.L.evp1d_set_params(self.h, ctypes.byref(p)))
+
+    def get_params(self) -> Params:
+        p = Params()
+        _check(self.L, self.L.evp1d_get_params(self.h, ctypes.byref(p)))
+        return p
+
+    def load_state(self, fields: dict, idx: dict, skipT, skipU) -> None:
+        """Upload a whole 1D state (dicts keyed by the FIELDS / NBR names)."""
+        for n in NBR:
+            self.set_index(n, idx[n])
+        self.set_skip("skipT", skipT)
+        self.set_skip("skipU", skipU)
+        for n in FIELDS:
+            if n in fields:
+                self.set_field(n, fields[n])
+
+    # compute --------------------------------------------------------------
+    def set_mode(self, mode: str) -> None:
+        _check(self.L, self.L.evp1d_set_mode(self.h, MODES[mode]))
+
+    def stress(self) -> None:
+        _check(self.L, self.L.evp1d_stress(self.h))
+
+    def stepu(self) -> None:
+        _check(self.L, self.L.evp1d_stepu(self.h))
+
+    def halo(self) -> None:
+        _check(self.L, self.L.evp1d_halo(self.h))
+
+    def run(self, nsub: int) -> None:
+        _check(self.L, self.L.evp1d_run(self.h, int(nsub)))
+
+    def run_individual(self, which: str, nsub: int) -> None:
+        _check(self.L, self.L.evp1d_run_individual(self.h, {"stress": 0, "stepu": 1}[which], int(nsub)))
+
+    def synchronize(self) -> None:
+        _check(self.L, self.L.evp1d_synchronize(self.h))
+
+    def timer_start(self) -> None:
+        _check(self.L, self.L.evp1d_timer_start(self.h))
+
+    def timer_stop(self) -> float:
+        ms = ctypes.c_float()
+        _check(self.L, self.L.evp1d_timer_stop(self.h, ctypes.byref(ms)))
+        return ms.value

@@ -1,0 +1,184 @@
+/*
+ * evp1d.h - C ABI of the MI355X EVP1D subcycle engine (libevp1d.so).
+ *
+ * This is the boundary that replaces the reference's kernel module `bench`
+ * (src/bench_v2b.F90:62-312 `stress`, :444-549 `stepu`) and the subcycle loop
+ * of its drivers (src/evp_v2b.F90:91-110, integration/ice_dyn_evp1d.F90:188-207).
+ * Everything crosses as plain pointers, sizes and int status codes so that it
+ * binds from Fortran (ISO_C_BINDING, see cice-evp1d_amd/fortran/evp1d_capi.F90),
+ * C, C++ or Python ctypes.  No device pointer is ever handed to the caller: a
+ * handle owns its device buffers and copies happen only in set/get calls.
+ *
+ * Conventions
+ *   - neighbour indices are 1-based positions in 1..navel, as produced by
+ *     findXinY (integration/ice_dyn_evp1d.F90:1000-1045);
+ *   - skip flags are int32 (0 = active, nonzero = skip); Fortran logicals are
+ *     converted at the boundary because their bit pattern is compiler specific;
+ *   - all arithmetic is IEEE binary64; the default library is built with
+ *     floating-point contraction off and reproduces the reference's serial
+ *     operation order bit for bit.  libevp1d_fma.so exports the same ABI with
+ *     contraction on (fused multiply-add);
+ *   - every entry point returns EVP1D_OK (0) or a positive error code;
+ *     evp1d_last_error() returns the message of the most recent failure.
+ *   - one handle is driven from one host thread; work is issued on the
+ *     handle's own HIP stream and every call that returns data synchronises it.
+ */
+#ifndef EVP1D_H
+#define EVP1D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EVP1D_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+enum evp1d_status {
+  EVP1D_OK = 0,
+  EVP1D_EINVAL = 1,   /* bad argument (size, enum, null pointer)            */
+  EVP1D_EINDEX = 2,   /* neighbour index outside 1..navel / inconsistent list */
+  EVP1D_EHIP = 3,     /* HIP runtime error                                    */
+  EVP1D_ENOMEM = 4,   /* device or host allocation failed                     */
+  EVP1D_ESTATE = 5    /* call out of order (e.g. run before indices are set)  */
+};
+
+/* ---- scalar parameters --------------------------------------------------
+ * Mirrors module ice_dyn_shared (src/ice.F90:80-107) plus the scalars that
+ * readin_1d_v2 overrides from the input file (src/vars.F90:2116).  Field order
+ * is part of the ABI (Fortran binds it as a bind(C) derived type).          */
+typedef struct evp1d_params {
+  int32_t ndte;        /* subcycles per model step (used by derived values)  */
+  int32_t reserved;    /* keep 8-byte alignment of the doubles               */
+  double capping;      /* replacement-pressure capping (1 = max form)        */
+  double e_factor;     /* e_yieldcurve^2 / e_plasticpot^4                    */
+  double epp2i;        /* 1 / e_plasticpot^2                                 */
+  double deltaminEVP;  /* minimum Delta (times tarea)                        */
+  double arlx;         /* 2 * elasticDamp * ndte                             */
+  double arlx1i;       /* 1 / arlx                                           */
+  double brlx;         /* ndte                                               */
+  double denom1;       /* 1 / (1 + arlx1i)                                   */
+  double revp;         /* 0 = classic EVP, 1 = revised EVP                   */
+  double Ktens;        /* tensile strength factor                            */
+  double u0;           /* residual velocity for basal stress (m/s)           */
+  double cosw;         /* cos(ocean turning angle)                           */
+  double sinw;         /* sin(ocean turning angle)                           */
+  double rhow;         /* seawater density (kg/m^3)                          */
+} evp1d_params;
+
+/* set_evp_parameters (src/ice.F90:109-130): fills every member of *p from
+ * ndte with the reference defaults (elasticDamp 0.36, e_yieldcurve =
+ * e_plasticpot = 2, capping 1, deltaminEVP 1e-8, revp = Ktens = 0, u0 5e-5,
+ * cosw 1, sinw 0, rhow 1026).  dt is accepted and unused, as in the reference. */
+int evp1d_set_evp_parameters(int32_t ndte, double dt, evp1d_params *p);
+
+/* ---- 1D fields ----------------------------------------------------------
+ * Names follow src/vars.F90:185-199.  Fields marked [navel] have length navel
+ * (active points 1..na then ghost slots na+1..navel); all others length na. */
+enum evp1d_field {
+  EVP1D_STRENGTH = 0,
+  EVP1D_UVEL,          /* [navel] */
+  EVP1D_VVEL,          /* [navel] */
+  EVP1D_DXT, EVP1D_DYT, EVP1D_UAREAR,
+  EVP1D_CDN_OCN,       /* Cw in stepu */
+  EVP1D_AIU,           /* aiX in stepu */
+  EVP1D_UOCN, EVP1D_VOCN, EVP1D_WATERX, EVP1D_WATERY,
+  EVP1D_FORCEX, EVP1D_FORCEY, EVP1D_UMASSDTI, EVP1D_FM,
+  EVP1D_STRINTX, EVP1D_STRINTY, EVP1D_TBU,
+  EVP1D_STRESSP_1, EVP1D_STRESSP_2, EVP1D_STRESSP_3, EVP1D_STRESSP_4,
+  EVP1D_STRESSM_1, EVP1D_STRESSM_2, EVP1D_STRESSM_3, EVP1D_STRESSM_4,
+  EVP1D_STRESS12_1, EVP1D_STRESS12_2, EVP1D_STRESS12_3, EVP1D_STRESS12_4,
+  EVP1D_HTE, EVP1D_HTN, EVP1D_HTEM1, EVP1D_HTNM1,
+  EVP1D_UVEL_INIT, EVP1D_VVEL_INIT, EVP1D_CB,
+  EVP1D_STR1, EVP1D_STR2, EVP1D_STR3, EVP1D_STR4,   /* [navel] */
+  EVP1D_STR5, EVP1D_STR6, EVP1D_STR7, EVP1D_STR8,   /* [navel] */
+  EVP1D_NFIELDS
+};
+
+/* neighbour lists (integration/ice_dyn_evp1d.F90:766-772, 792-797) */
+enum evp1d_index { EVP1D_EE = 0, EVP1D_NE, EVP1D_SE, EVP1D_NW, EVP1D_SW, EVP1D_SSE };
+/* skip flags (integration/ice_dyn_evp1d.F90:521-536) */
+enum evp1d_skip { EVP1D_SKIPT = 0, EVP1D_SKIPU = 1 };
+/* how evp1d_run keeps the subcycle loop on the device */
+enum evp1d_mode {
+  EVP1D_MODE_LAUNCH = 0,      /* one stress and one stepu launch per subcycle */
+  EVP1D_MODE_GRAPH = 1,       /* the nsub x (stress, stepu, halo) sequence captured once as a hipGraph */
+  EVP1D_MODE_PERSISTENT = 2,  /* one cooperative kernel, grid barrier between phases */
+  EVP1D_MODE_AUTO = 3         /* library default (fastest measured form) */
+};
+
+typedef struct evp1d_handle evp1d_handle;
+
+/* Allocate a handle for na active points and navel-na ghost slots on the
+ * current HIP device.  All state starts at zero (in particular str1..8 at the
+ * ghost and skipped slots, which the stepu gathers read).  Replaces the
+ * allocations of alloc_1d_v2 (src/vars.F90:2054-2100) and
+ * evp1d_alloc_static_na/navel (integration/ice_dyn_evp1d.F90:378-461).     */
+int evp1d_create(int32_t na, int32_t navel, const evp1d_params *p, evp1d_handle **out);
+int evp1d_destroy(evp1d_handle *h);
+int evp1d_set_params(evp1d_handle *h, const evp1d_params *p);
+int evp1d_get_params(const evp1d_handle *h, evp1d_params *p);
+int evp1d_get_sizes(const evp1d_handle *h, int32_t *na, int32_t *navel);
+
+/* Host -> device copies.  n must equal the field's length.  Index lists are
+ * validated (1 <= idx <= navel) before upload.                              */
+int evp1d_set_field(evp1d_handle *h, int32_t field, const double *src, int32_t n);
+int evp1d_get_field(evp1d_handle *h, int32_t field, double *dst, int32_t n);
+int evp1d_set_index(evp1d_handle *h, int32_t which, const int32_t *idx, int32_t n);
+int evp1d_set_skip(evp1d_handle *h, int32_t which, const int32_t *flags, int32_t n);
+
+/* Ghost copy list: after every stepu, uvel/vvel of ghost slot dst[k] becomes
+ * uvel/vvel of slot src[k] (src[k] == 0: fill with 0).  1-based; dst in
+ * na+1..navel, src in 0..na.  This is the serial halo copy list of
+ * src/vars.F90:100-143 (ice_haloUpdate) in 1D form.  n = 0 (the default)
+ * reproduces the single-domain reference, whose ghost slots stay constant.  */
+int evp1d_set_halo(evp1d_handle *h, int32_t n, const int32_t *dst, const int32_t *src);
+
+/* One kernel each, over all na points (bench_v2b.F90 `stress` / `stepu` with
+ * lb=1, ub=na; the ghost update `evp1d_halo` is a no-op with an empty list). */
+int evp1d_stress(evp1d_handle *h);
+int evp1d_stepu(evp1d_handle *h);
+int evp1d_halo(evp1d_handle *h);
+
+/* nsub subcycles of (stress; stepu; halo) without leaving the device
+ * (src/evp_v2b.F90:91-110).  Asynchronous with respect to the host.          */
+int evp1d_run(evp1d_handle *h, int32_t nsub);
+int evp1d_set_mode(evp1d_handle *h, int32_t mode);
+int evp1d_get_mode(const evp1d_handle *h, int32_t *mode);
+/* nsub launches of stress only (which = 0) or stepu only (which = 1): the
+ * lindividual timing loops of src/evp_v2b.F90:128-155.                       */
+int evp1d_run_individual(evp1d_handle *h, int32_t which, int32_t nsub);
+int evp1d_synchronize(evp1d_handle *h);
+
+/* Device timing on the handle's stream (HIP events): start, then stop
+ * returns the elapsed milliseconds between the two after synchronising.     */
+int evp1d_timer_start(evp1d_handle *h);
+int evp1d_timer_stop(evp1d_handle *h, float *ms);
+
+/* Build the 1D point list and neighbour positions from a 2D T mask, as
+ * calc_nActiveTU / calc_2d_indices_init / calc_navel / convert_2d_1d_init
+ * do (integration/ice_dyn_evp1d.F90:494-519, 559-589, 743-816, 1047-1087).
+ * nx, ny are the full block sizes including the ghost rim (cnx + 2*nghost);
+ * tmask is int32 (nonzero = ocean), Fortran order (i fastest).  The rim cells
+ * with i or j <= nghost must be dry (the reference silently leaves zero
+ * indices there).  Outputs: indxti/indxtj (na, 1-based 2D indices),
+ * indxtij (navel, 1-based linear index i + (j-1)*nx of every 1D slot; ghost
+ * slots are the sorted set difference), and ee..sse (na each, 1-based).     */
+int evp1d_index_count(int32_t nx, int32_t ny, int32_t nghost, const int32_t *tmask,
+                      int32_t *na, int32_t *navel);
+int evp1d_index_build(int32_t nx, int32_t ny, int32_t nghost, const int32_t *tmask,
+                      int32_t na, int32_t navel, int32_t *indxti, int32_t *indxtj,
+                      int32_t *indxtij, int32_t *ee, int32_t *ne, int32_t *se,
+                      int32_t *nw, int32_t *sw, int32_t *sse);
+
+/* Copy the message of the last failure on this thread into buf (always NUL
+ * terminated); returns the code of that failure (0 if none).                */
+int evp1d_last_error(char *buf, int32_t n);
+/* ABI version and build flavour: fma = 1 for libevp1d_fma.so.               */
+int evp1d_build_info(int32_t *abi_version, int32_t *fma);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EVP1D_H */

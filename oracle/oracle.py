@@ -1,0 +1,386 @@
+"""Oracle for the EVP1D subcycle loop (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module; it is the checker, never the thing measured or shipped.
+
+Contents
+  OracleLib          ctypes wrapper of lib/liboracle.so (C restatement of
+                     src/bench_v2b.F90 stress/stepu, see evp1d_oracle.c)
+  index_reference()  restatement of the reference's 1D-list construction:
+                     calc_nActiveTU, calc_2d_indices_init, union, calc_navel,
+                     setdiff, findXinY, convert_2d_1d_init
+                     (integration/ice_dyn_evp1d.F90:494-519, 559-633, 743-816,
+                     951-1087), written as the reference's two-pointer loops
+  pack_dyn()/unpack_dyn()  convert_2d_1d_dyn / convert_1d_2d_dyn
+                     (integration/ice_dyn_evp1d.F90:818-947) and set_skipMe
+                     (521-536)
+  write_inputs()/read_outputs()/write_outputs()
+                     the reference binary formats of readin_1d_v2 and
+                     writeout_1d (src/vars.F90:2101-2135, 2233-2304) and the
+                     kernel_nml namelist (src/vars.F90:24-25)
+Pinning: see evp1d_oracle.c; the Python index restatement is pinned against
+the compiled reference's own input/output through the golden fixtures
+(tests/golden) and checked against hand-built tiny grids in tests/test_oracle.py.
+"""
+from __future__ import annotations
+
+import ctypes
+import hashlib
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIBDIR = HERE / "lib"
+REFDIR = HERE / "_ref"
+
+# order of include/evp1d.h enum evp1d_field
+FIELDS = (
+    "strength", "uvel", "vvel", "dxT", "dyT", "uarear", "cdn_ocn", "aiu", "uocn", "vocn",
+    "waterxU", "wateryU", "forcexU", "forceyU", "umassdti", "fmU", "strintxU", "strintyU", "Tbu",
+    "stressp_1", "stressp_2", "stressp_3", "stressp_4",
+    "stressm_1", "stressm_2", "stressm_3", "stressm_4",
+    "stress12_1", "stress12_2", "stress12_3", "stress12_4",
+    "HTE", "HTN", "HTEm1", "HTNm1", "uvel_init", "vvel_init", "Cb",
+    "str1", "str2", "str3", "str4", "str5", "str6", "str7", "str8",
+)
+NAVEL_FIELDS = {"uvel", "vvel", "str1", "str2", "str3", "str4", "str5", "str6", "str7", "str8"}
+STRESS_FIELDS = FIELDS[19:31]
+NBR = ("ee", "ne", "se", "nw", "sw", "sse")
+
+
+class Params(ctypes.Structure):
+    """Same layout as evp1d_params (include/evp1d.h)."""
+
+    _fields_ = [("ndte", ctypes.c_int32), ("reserved", ctypes.c_int32)] + [
+        (n, ctypes.c_double) for n in (
+            "capping", "e_factor", "epp2i", "deltaminEVP", "arlx", "arlx1i", "brlx", "denom1",
+            "revp", "Ktens", "u0", "cosw", "sinw", "rhow")]
+
+
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int32)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_IP)
+
+
+class OracleLib:
+    """The C restatement; `omp=True` loads the OpenMP build."""
+
+    def __init__(self, omp: bool = False):
+        path = LIBDIR / ("liboracle_omp.so" if omp else "liboracle.so")
+        if not path.exists():
+            subprocess.run(["make", "-C", str(HERE), "port"], check=True, capture_output=True)
+        self.lib = ctypes.CDLL(str(path))
+        self.lib.oracle_set_evp_parameters.argtypes = [ctypes.c_int32, ctypes.POINTER(Params)]
+        self.lib.oracle_run.argtypes = [ctypes.POINTER(Params), ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.POINTER(_DP), ctypes.POINTER(_IP), _IP, _IP,
+                                        ctypes.c_int32, _IP, _IP]
+
+    def stress(self, p: Params, st: "State1D") -> None:
+        """One oracle_stress call over 1..na (bench_v2b.F90 stress)."""
+        f = st.f
+        sig = (_DP * 12)(*[_dp(f[n]) for n in STRESS_FIELDS])
+        strp = (_DP * 8)(*[_dp(f[f"str{k}"]) for k in range(1, 9)])
+        self.lib.oracle_stress(ctypes.byref(p), 1, st.na, _ip(st.idx["ee"]), _ip(st.idx["ne"]),
+                               _ip(st.idx["se"]), _dp(f["uvel"]), _dp(f["vvel"]), _dp(f["dxT"]),
+                               _dp(f["dyT"]), _ip(st.skipT), _dp(f["strength"]), _dp(f["HTE"]),
+                               _dp(f["HTN"]), _dp(f["HTEm1"]), _dp(f["HTNm1"]), sig, strp)
+
+    def stepu(self, p: Params, st: "State1D") -> None:
+        """One oracle_stepu call over 1..na (bench_v2b.F90 stepu)."""
+        f = st.f
+        strp = (_DP * 8)(*[_dp(f[f"str{k}"]) for k in range(1, 9)])
+        self.lib.oracle_stepu(ctypes.byref(p), 1, st.na, _dp(f["cdn_ocn"]), _dp(f["aiu"]),
+                              _dp(f["uocn"]), _dp(f["vocn"]), _dp(f["waterxU"]), _dp(f["wateryU"]),
+                              _dp(f["forcexU"]), _dp(f["forceyU"]), _dp(f["umassdti"]), _dp(f["fmU"]),
+                              _dp(f["uarear"]), _dp(f["strintxU"]), _dp(f["strintyU"]),
+                              _dp(f["uvel_init"]), _dp(f["vvel_init"]), _dp(f["uvel"]), _dp(f["vvel"]),
+                              strp, _ip(st.idx["nw"]), _ip(st.idx["sw"]), _ip(st.idx["sse"]),
+                              _ip(st.skipU), _dp(f["Tbu"]), _dp(f["Cb"]))
+
+    def params(self, ndte: int, **override) -> Params:
+        p = Params()
+        self.lib.oracle_set_evp_parameters(ndte, ctypes.byref(p))
+        for k, v in override.items():
+            setattr(p, k, v)
+        return p
+
+    def run(self, p: Params, st: "State1D", nsub: int, halo=None) -> None:
+        """nsub subcycles in place on st (stress; stepu; halo)."""
+        for name in FIELDS:
+            a = st.f[name]
+            assert a.dtype == np.float64 and a.flags.c_contiguous, name
+        fptr = (_DP * len(FIELDS))(*[_dp(st.f[n]) for n in FIELDS])
+        iptr = (_IP * 6)(*[_ip(st.idx[n]) for n in NBR])
+        if halo is None:
+            nh, hd, hs = 0, np.zeros(1, np.int32), np.zeros(1, np.int32)
+        else:
+            hd = np.ascontiguousarray(halo[0], dtype=np.int32)
+            hs = np.ascontiguousarray(halo[1], dtype=np.int32)
+            nh = hd.size
+        self.lib.oracle_run(ctypes.byref(p), st.na, nsub, fptr, iptr, _ip(st.skipT), _ip(st.skipU),
+                            nh, _ip(hd), _ip(hs))
+
+
+class State1D:
+    """1D arrays of one domain (copy semantics: .copy() gives a deep copy)."""
+
+    def __init__(self, na: int, navel: int):
+        self.na, self.navel = na, navel
+        self.f = {n: np.zeros(navel if n in NAVEL_FIELDS else na) for n in FIELDS}
+        self.idx = {n: np.ones(na, dtype=np.int32) for n in NBR}
+        self.skipT = np.zeros(na, dtype=np.int32)
+        self.skipU = np.zeros(na, dtype=np.int32)
+        self.scalars = dict(capping=1.0, e_factor=0.25, epp2i=0.25, deltaminEVP=1e-8)
+
+    def copy(self) -> "State1D":
+        s = State1D(self.na, self.navel)
+        s.f = {k: v.copy() for k, v in self.f.items()}
+        s.idx = {k: v.copy() for k, v in self.idx.items()}
+        s.skipT, s.skipU = self.skipT.copy(), self.skipU.copy()
+        s.scalars = dict(self.scalars)
+        return s
+
+
+# ---------------------------------------------------------------------------
+# 1D list construction, restated from integration/ice_dyn_evp1d.F90
+# ---------------------------------------------------------------------------
+def _union(x, y):
+    """union (ice_dyn_evp1d.F90:591-633): merge of two sorted int vectors."""
+    i = j = 0
+    out = []
+    while i < len(x) and j < len(y):
+        if x[i] < y[j]:
+            out.append(x[i]); i += 1
+        elif x[i] > y[j]:
+            out.append(y[j]); j += 1
+        else:
+            out.append(x[i]); i += 1; j += 1
+    out.extend(x[i:])
+    out.extend(y[j:])
+    return out
+
+
+def _setdiff(x, y):
+    """setdiff (ice_dyn_evp1d.F90:951-998): symmetric difference of sorted vectors."""
+    i = j = 0
+    out = []
+    while i < len(x) and j < len(y):
+        if x[i] < y[j]:
+            out.append(x[i]); i += 1
+        elif x[i] > y[j]:
+            out.append(y[j]); j += 1
+        else:
+            i += 1; j += 1
+    out.extend(x[i:])
+    out.extend(y[j:])
+    return out
+
+
+def _find_x_in_y(x, y, nx_len):
+    """findXinY (ice_dyn_evp1d.F90:1000-1045), 1-based result."""
+    indx = [0] * len(x)
+    i, j1, j2 = 0, 0, nx_len
+    ny_len = len(y)
+    while i < len(x):
+        y1 = y[j1] if j1 < ny_len else None
+        y2 = y[j2] if j2 < ny_len else None
+        if x[i] == y1:
+            indx[i] = j1 + 1; i += 1; j1 += 1
+        elif x[i] == y2:
+            indx[i] = j2 + 1; i += 1; j2 += 1
+        elif y1 is not None and x[i] > y1:
+            j1 += 1
+        elif y2 is not None and x[i] > y2:
+            j2 += 1
+        else:
+            raise RuntimeError("findXinY: conditions not met")  # the reference stops here
+    return indx
+
+
+def index_reference(tmask: np.ndarray, nghost: int) -> dict:
+    """1D list of a (nx, ny) Fortran-ordered T mask (nx, ny include the rim)."""
+    nx, ny = tmask.shape
+    na = int(np.count_nonzero(tmask))                               # calc_nActiveTU
+    ti, tj = [], []
+    for j in range(1 + nghost, ny + 1):                             # calc_2d_indices_init
+        for i in range(1 + nghost, nx + 1):
+            if tmask[i - 1, j - 1]:
+                ti.append(i); tj.append(j)
+    if len(ti) != na:
+        raise ValueError("wet cells in the west/south rim")
+    Iin = [i + (j - 1) * nx for i, j in zip(ti, tj)]
+    Iee = [i - 1 + (j - 1) * nx for i, j in zip(ti, tj)]
+    Ine = [i - 1 + (j - 2) * nx for i, j in zip(ti, tj)]
+    Ise = [i + (j - 2) * nx for i, j in zip(ti, tj)]
+    Inw = [i + 1 + (j - 1) * nx for i, j in zip(ti, tj)]
+    Isw = [i + 1 + (j - 0) * nx for i, j in zip(ti, tj)]
+    Isse = [i + (j - 0) * nx for i, j in zip(ti, tj)]
+    u = _union(Iin, Iee)
+    u = _union(u, Ine)
+    u = _union(u, Ise)
+    u = _union(u, Inw)
+    u = _union(u, Isw)
+    u = _union(u, Isse)
+    navel = len(u)                                                  # calc_navel
+    ghosts = _setdiff(u, Iin)
+    indxtij = Iin + ghosts
+    out = dict(na=na, navel=navel, indxti=np.array(ti, np.int32), indxtj=np.array(tj, np.int32),
+               indxtij=np.array(indxtij, np.int64))
+    for name, lst in zip(NBR, (Iee, Ine, Ise, Inw, Isw, Isse)):
+        out[name] = np.array(_find_x_in_y(lst, indxtij, na), dtype=np.int32)
+    return out
+
+
+_DYN_2D = (
+    "stressp_1", "stressp_2", "stressp_3", "stressp_4", "stressm_1", "stressm_2", "stressm_3",
+    "stressm_4", "stress12_1", "stress12_2", "stress12_3", "stress12_4", "strength", "cdn_ocn",
+    "aiu", "uocn", "vocn", "waterxU", "wateryU", "forcexU", "forceyU", "umassdti", "fmU",
+    "strintxU", "strintyU", "Tbu", "Cb", "uvel", "vvel")
+
+
+def pack_dyn(grid, ix: dict) -> State1D:
+    """State1D from a synth.Grid2D: convert_2d_1d_init (803-815) for the static
+    fields, convert_2d_1d_dyn (845-881) for the dynamic ones, set_skipMe
+    (521-536) for the flags; uvel_init = uvel as readin_1d_v2 does."""
+    na, navel = ix["na"], ix["navel"]
+    st = State1D(na, navel)
+    i = ix["indxti"].astype(np.int64) - 1
+    j = ix["indxtj"].astype(np.int64) - 1
+    F = grid.fields
+    for n in _DYN_2D:
+        st.f[n][:na] = F[n][i, j]
+    st.f["uvel"][na:] = 0.0
+    st.f["vvel"][na:] = 0.0
+    st.f["uarear"][:] = F["uarear"][i, j]
+    st.f["dxT"][:] = F["dxT"][i, j]
+    st.f["dyT"][:] = F["dyT"][i, j]
+    st.f["HTE"][:] = F["HTE"][i, j]
+    st.f["HTN"][:] = F["HTN"][i, j]
+    st.f["HTEm1"][:] = F["HTE"][i - 1, j]
+    st.f["HTNm1"][:] = F["HTN"][i, j - 1]
+    st.f["uvel_init"][:] = st.f["uvel"][:na]
+    st.f["vvel_init"][:] = st.f["vvel"][:na]
+    st.skipT[:] = (~grid.icetmask[i, j]).astype(np.int32)
+    st.skipU[:] = (~grid.iceumask[i, j]).astype(np.int32)
+    for n in NBR:
+        st.idx[n][:] = ix[n]
+    return st
+
+
+def unpack_dyn(st: State1D, ix: dict, shape) -> dict:
+    """convert_1d_2d_dyn (ice_dyn_evp1d.F90:884-947): 1D -> 2D, zero elsewhere."""
+    i = ix["indxti"].astype(np.int64) - 1
+    j = ix["indxtj"].astype(np.int64) - 1
+    out = {}
+    for n in _DYN_2D:
+        a = np.zeros(shape, order="F")
+        a[i, j] = st.f[n][:st.na]
+        out[n] = a
+    return out
+
+
+# ---------------------------------------------------------------------------
+# reference binary formats (src/vars.F90)
+# ---------------------------------------------------------------------------
+_IN_ORDER = ("strength", "uvel", "vvel", "dxT", "dyT", "uarear", "cdn_ocn", "aiu", "uocn", "vocn",
+             "waterxU", "wateryU", "forcexU", "forceyU", "umassdti", "fmU", "strintxU", "strintyU",
+             "Tbu") + STRESS_FIELDS
+
+
+def write_inputs(d: Path, st: State1D, ndte: int, nx_block: int = -1, ny_block: int = -1) -> None:
+    """input_double_1d_v2.bin, input_integer_1d.bin, input_logical_1d.bin and
+    kernel.nml as readin_1d_v2 / read_nml consume them (vars.F90:2101-2135,
+    19-69).  Logicals are written as 4-byte 0/1."""
+    d = Path(d)
+    d.mkdir(parents=True, exist_ok=True)
+    na, navel = st.na, st.navel
+    with open(d / "input_double_1d_v2.bin", "wb") as fh:
+        for n in _IN_ORDER:
+            a = st.f[n]
+            fh.write(np.ascontiguousarray(a[:navel] if n in ("uvel", "vvel") else a[:na]).tobytes())
+        s = st.scalars
+        fh.write(np.array([s["capping"], s["e_factor"], s["epp2i"], s["deltaminEVP"]]).tobytes())
+        for n in ("HTE", "HTN", "HTEm1", "HTNm1"):
+            fh.write(st.f[n][:na].tobytes())
+    with open(d / "input_integer_1d.bin", "wb") as fh:
+        for n in NBR:
+            fh.write(st.idx[n].astype("<i4").tobytes())
+    with open(d / "input_logical_1d.bin", "wb") as fh:
+        fh.write(st.skipU.astype("<i4").tobytes())
+        fh.write(st.skipT.astype("<i4").tobytes())
+    nb = int(na - np.count_nonzero(st.skipU))
+    (d / "kernel.nml").write_text(
+        "&kernel_nml\n"
+        f" ndte = {ndte}\n testscale = 1\n binoutput = .true.\n lindividual = .false.\n"
+        f" nx_block = {nx_block}\n ny_block = {ny_block}\n na = {na}\n nb = {nb}\n navel = {navel}\n/\n")
+
+
+def read_inputs(d: Path, na: int, navel: int) -> State1D:
+    """Inverse of write_inputs (same layout as readin_1d_v2)."""
+    d = Path(d)
+    st = State1D(na, navel)
+    raw = np.fromfile(d / "input_double_1d_v2.bin", dtype="<f8")
+    need = 2 * navel + (len(_IN_ORDER) - 2) * na + 4 + 4 * na
+    if raw.size != need:
+        raise ValueError(f"input_double_1d_v2.bin has {raw.size} doubles, expected {need}")
+    p = 0
+    for n in _IN_ORDER:
+        m = navel if n in ("uvel", "vvel") else na
+        st.f[n][:m] = raw[p:p + m]
+        p += m
+    st.scalars = dict(zip(("capping", "e_factor", "epp2i", "deltaminEVP"), raw[p:p + 4].tolist()))
+    p += 4
+    for n in ("HTE", "HTN", "HTEm1", "HTNm1"):
+        st.f[n][:] = raw[p:p + na]
+        p += na
+    st.f["uvel_init"][:] = st.f["uvel"][:na]
+    st.f["vvel_init"][:] = st.f["vvel"][:na]
+    ints = np.fromfile(d / "input_integer_1d.bin", dtype="<i4").reshape(6, na)
+    for k, n in enumerate(NBR):
+        st.idx[n][:] = ints[k]
+    logi = np.fromfile(d / "input_logical_1d.bin", dtype="<i4").reshape(2, na)
+    st.skipU[:] = (logi[0] != 0)
+    st.skipT[:] = (logi[1] != 0)
+    return st
+
+
+def output_bytes(st: State1D) -> tuple[bytes, bytes]:
+    """Contents of output_stress_*_1d.bin and output_stepu_*_1d.bin
+    (writeout_1d, vars.F90:2233-2304): the 12 stresses over 1..na, then
+    strintx, strinty, uvel, vvel compacted over .not.skipU."""
+    na = st.na
+    stress = b"".join(np.ascontiguousarray(st.f[n][:na]).tobytes() for n in STRESS_FIELDS)
+    keep = st.skipU == 0
+    stepu = b"".join(np.ascontiguousarray(st.f[n][:na][keep]).tobytes()
+                     for n in ("strintxU", "strintyU", "uvel", "vvel"))
+    return stress, stepu
+
+
+def md5(b: bytes) -> str:
+    return hashlib.md5(b).hexdigest()
+
+
+def run_reference(d: Path, binary: str = "evp_v2b_f0", threads: int = 1, timeout: float = 3600,
+                  env_extra: dict | None = None) -> subprocess.CompletedProcess:
+    """Run the compiled reference driver in directory d (inputs must exist).
+
+    Writes output_stress_v2b_1d.bin / output_stepu_v2b_1d.bin there."""
+    exe = REFDIR / binary
+    if not exe.exists():
+        raise FileNotFoundError(f"{exe} not built (make -C oracle ref)")
+    env = dict(os.environ)
+    env["OMP_NUM_THREADS"] = str(threads)
+    if env_extra:
+        env.update(env_extra)
+    return subprocess.run([str(exe)], cwd=str(d), env=env, capture_output=True, text=True,
+                          timeout=timeout, check=True)

@@ -1,0 +1,70 @@
+"""Helpers shared by the GPU parity tests (engine <-> oracle state plumbing)."""
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+
+import evp1d
+from oracle import oracle as O
+
+GOLD = Path(__file__).resolve().parent / "golden"
+SCALAR_NAMES = ("capping", "e_factor", "epp2i", "deltaminEVP")
+
+
+def load_golden(name: str):
+    """(State1D, ndte, expected stress bytes, expected stepu bytes)."""
+    z = np.load(GOLD / f"{name}.npz", allow_pickle=False)
+    na, navel = int(z["na"]), int(z["navel"])
+    st = O.State1D(na, navel)
+    for k in st.f:
+        st.f[k][:] = z[f"f_{k}"]
+    for k in st.idx:
+        st.idx[k][:] = z[f"i_{k}"]
+    st.skipT[:] = z["skipT"]
+    st.skipU[:] = z["skipU"]
+    st.scalars = dict(zip(SCALAR_NAMES, z["scalars"].tolist()))
+    return st, int(z["ndte"]), z["out_stress"].tobytes(), z["out_stepu"].tobytes()
+
+
+def golden_cases():
+    return sorted(p.stem for p in GOLD.glob("*.npz"))
+
+
+def make_engine(st: O.State1D, ndte: int, mode: str = "auto", fma: bool = False) -> evp1d.Engine:
+    p = evp1d.set_evp_parameters(ndte, fma=fma, **st.scalars)
+    eng = evp1d.Engine(st.na, st.navel, p, fma=fma)
+    eng.set_mode(mode)
+    eng.load_state(st.f, st.idx, st.skipT, st.skipU)
+    return eng
+
+
+def fetch_state(eng: evp1d.Engine, like: O.State1D) -> O.State1D:
+    out = like.copy()
+    for k in out.f:
+        out.f[k][:] = eng.get_field(k)
+    return out
+
+
+def oracle_params(st: O.State1D, ndte: int):
+    lib = O.OracleLib()
+    return lib, lib.params(ndte, **st.scalars)
+
+
+def bits(a: np.ndarray) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def assert_state_bitexact(got: O.State1D, want: O.State1D, fields=None) -> None:
+    for k in fields or want.f:
+        g, w = bits(got.f[k]), bits(want.f[k])
+        bad = np.nonzero(g != w)[0]
+        assert bad.size == 0, (f"{k}: {bad.size} of {w.size} values differ; first at {bad[0]}: "
+                               f"got {got.f[k][bad[0]]!r} want {want.f[k][bad[0]]!r}")
+
+
+def synthetic_state(nx: int, ny: int, seed: int, **kw) -> O.State1D:
+    from evp1d import synth
+    g = synth.make_grid(nx, ny, seed, **kw)
+    ix = evp1d.build_index(g.tmask, g.nghost)
+    return O.pack_dyn(g, ix)

@@ -1,0 +1,166 @@
+"""GPU parity of the HIP EVP1D kernels (bit-exact against the reference).
+
+Golden fixtures come from the flang-built reference v2b at f0 flags
+(tests/golden/make_golden.py); random states are checked against the C
+oracle, which is itself pinned to the reference (tests/test_oracle.py).
+Exact build: every output bit must match.  FMA build: norm-wise relative
+error max|d|/max|ref| <= 1e-12 per field.
+"""
+import json
+import sys
+
+import numpy as np
+import pytest
+
+import evp1d
+from oracle import oracle as O
+from gpu_common import (GOLD, assert_state_bitexact, fetch_state, golden_cases, load_golden,
+                        make_engine, oracle_params, synthetic_state)
+
+pytestmark = pytest.mark.gpu
+MODES = ("launch", "graph", "persistent")
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("case", golden_cases())
+def test_golden_outputs_bitexact(gpu, case, mode):
+    """Both reference output files, byte for byte, for every golden case."""
+    _log(f"golden {case} mode={mode}")
+    st, ndte, want_stress, want_stepu = load_golden(case)
+    with make_engine(st, ndte, mode) as eng:
+        eng.run(ndte)
+        got = fetch_state(eng, st)
+    a, b = O.output_bytes(got)
+    assert O.md5(a) == O.md5(want_stress)
+    assert O.md5(b) == O.md5(want_stepu)
+
+
+@pytest.mark.parametrize("case", ["small_64x48", "scalars_32x24", "ragged_40x7"])
+def test_full_state_matches_oracle(gpu, case):
+    """Every field (incl. str1..8 at all navel slots, Cb, strint) vs the oracle."""
+    st, ndte, _, _ = load_golden(case)
+    lib, p = oracle_params(st, ndte)
+    want = st.copy()
+    lib.run(p, want, ndte)
+    with make_engine(st, ndte, "graph") as eng:
+        eng.run(ndte)
+        got = fetch_state(eng, st)
+    assert_state_bitexact(got, want)
+
+
+def test_single_kernels_match_oracle(gpu):
+    """evp1d_stress and evp1d_stepu, one call each (lb=1, ub=na), vs oracle."""
+    st = synthetic_state(96, 80, seed=21)
+    lib, p = oracle_params(st, 7)
+    lib.run(p, st, 1)                     # start from a non-zero stress state
+    want = st.copy()
+    with make_engine(st, 7) as eng:
+        eng.stress()
+        lib.stress(p, want)
+        assert_state_bitexact(fetch_state(eng, st), want)
+        eng.stepu()
+        lib.stepu(p, want)
+        assert_state_bitexact(fetch_state(eng, st), want)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_repeated_runs_and_restart(gpu, mode):
+    """run(n1); run(n2) equals one oracle run of n1+n2 (state stays on device)."""
+    st = synthetic_state(70, 50, seed=31)
+    lib, p = oracle_params(st, 30)
+    want = st.copy()
+    lib.run(p, want, 30)
+    with make_engine(st, 30, mode) as eng:
+        eng.run(12)
+        eng.run(18)
+        got = fetch_state(eng, st)
+    assert_state_bitexact(got, want)
+
+
+def test_halo_copy_list(gpu):
+    """Ghost copy list (vars.F90:100-143 in 1D form) vs the oracle's halo."""
+    st = synthetic_state(60, 44, seed=41)
+    rng = np.random.default_rng(5)
+    ghosts = np.arange(st.na + 1, st.navel + 1, dtype=np.int32)
+    dst = rng.choice(ghosts, size=ghosts.size // 2, replace=False).astype(np.int32)
+    src = rng.integers(0, st.na + 1, size=dst.size).astype(np.int32)   # 0 = fill with zero
+    lib, p = oracle_params(st, 9)
+    want = st.copy()
+    lib.run(p, want, 9, halo=(dst, src))
+    for mode in MODES:
+        with make_engine(st, 9, mode) as eng:
+            eng.set_halo(dst, src)
+            eng.run(9)
+            got = fetch_state(eng, st)
+        assert_state_bitexact(got, want)
+
+
+def test_evp_c0_md5(gpu):
+    """The 400x300, ndte=120 workload: both output files' md5 equal the
+    compiled reference's (tests/golden/golden.json)."""
+    from evp1d import synth
+    rec = json.loads((GOLD / "golden.json").read_text())["evp_c0"]
+    w = synth.WORKLOADS["evp_c0"]
+    st = synthetic_state(w["nx"], w["ny"], w["seed"])
+    assert (st.na, st.navel) == (rec["na"], rec["navel"])
+    # the regenerated inputs must be the ones the reference ran on
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        O.write_inputs(d, st, w["ndte"])
+        from pathlib import Path
+        for f, m in rec["inputs"].items():
+            assert O.md5((Path(d) / f).read_bytes()) == m, f
+    for mode in MODES:
+        _log(f"evp_c0 mode={mode}")
+        with make_engine(st, w["ndte"], mode) as eng:
+            eng.run(w["ndte"])
+            got = fetch_state(eng, st)
+        a, b = O.output_bytes(got)
+        assert O.md5(a) == rec["output_stress_md5"]
+        assert O.md5(b) == rec["output_stepu_md5"]
+
+
+def _normwise(a, b):
+    d = np.max(np.abs(a - b)) if a.size else 0.0
+    r = np.max(np.abs(b)) if b.size else 0.0
+    return d / r if r > 0 else d
+
+
+@pytest.mark.parametrize("case", ["small_64x48", "long_16x12"])
+def test_fma_build_within_1e12(gpu, case):
+    """libevp1d_fma.so (contraction on): max|d|/max|ref| <= 1e-12 per field."""
+    st, ndte, _, _ = load_golden(case)
+    lib, p = oracle_params(st, ndte)
+    want = st.copy()
+    lib.run(p, want, ndte)
+    assert evp1d.build_info(fma=True)["fma"] == 1
+    with make_engine(st, ndte, "graph", fma=True) as eng:
+        eng.run(ndte)
+        got = fetch_state(eng, st)
+    worst = max(_normwise(got.f[k], want.f[k]) for k in want.f)
+    assert worst <= 1e-12, worst
+
+
+def test_bad_inputs_rejected(gpu):
+    st = synthetic_state(20, 16, seed=3)
+    p = evp1d.set_evp_parameters(5)
+    with evp1d.Engine(st.na, st.navel, p) as eng:
+        bad = st.idx["ee"].copy()
+        bad[3] = st.navel + 1
+        with pytest.raises(evp1d.Evp1dError) as e:
+            eng.set_index("ee", bad)
+        assert e.value.code == 2
+        with pytest.raises(evp1d.Evp1dError) as e:
+            eng.run(3)                       # lists not set yet
+        assert e.value.code == 5
+        with pytest.raises(evp1d.Evp1dError):
+            eng.set_field("uvel", np.zeros(st.na))   # uvel has navel entries
+        with pytest.raises(evp1d.Evp1dError):
+            eng.set_halo([1], [1])                   # slot 1 is not a ghost
+    with pytest.raises(evp1d.Evp1dError):
+        evp1d.Engine(10, 5, p)
