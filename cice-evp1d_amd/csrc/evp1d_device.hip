@@ -1,22 +1,33 @@
 // evp1d_device.hip - gfx950 kernels and the C ABI of libevp1d.so.
 //
-// Device layout (one handle = one 1D point list, see include/evp1d.h):
-//   vel   double2[navel]      (uvel, vvel) interleaved: one 16-byte load per
-//                             neighbour gather in stress
+// Device layout (one handle = one 1D point list, see include/evp1d.h)
+//
+// Slot order.  The caller's 1D order (row-major from calc_2d_indices_init)
+// is permuted on the device into four classes, each kept in caller order:
+//     [ stress only | stress and stepu | stepu only | neither | ghosts ]
+//       skipT=F       skipT=F            skipT=T      both T     na..navel-1
+//       skipU=T       skipU=F            skipU=F                 (identity)
+// so that stress runs over the contiguous slot range [0, t_end) and stepu
+// over [u_begin, u_end) with no skip test and no partly used cache lines
+// (with skips scattered through the caller's order, HBM moves every line
+// that holds one active point).  Neighbour lists are stored in slot space;
+// set/get of fields go through the permutation, so callers only ever see
+// their own order.  The permutation is rebuilt when skip flags change.
+//
+//   vel   double2[navel]      (uvel, vvel): one 16-byte load per stress gather
 //   sig   double[12][ldsig]   stressp_1..4, stressm_1..4, stress12_1..4, SoA
 //   s15, s27, s36, s48 double2[navel]
 //                             str1/str5, str2/str7, str3/str6, str4/str8: the
 //                             pairs stepu gathers from one neighbour each
 //                             (own, nw, sse, sw), so every gather is 16 bytes
-//   skip  uint8[na]           bit0 = skipT, bit1 = skipU
-//   idx   int32[6][na]        ee, ne, se, nw, sw, sse, 0-based
+//   idx   int32[6][na]        ee, ne, se, nw, sw, sse, slot space, 0-based
 //   everything else           double[na] SoA, streamed once per kernel
 //
-// Kernels: one thread per 1D point; stress and stepu are separate phases
-// because stepu at a point gathers the str of three neighbours written by
-// stress in the same subcycle, and the next stress gathers the velocities of
-// three neighbours written by stepu.  The subcycle loop stays on the device as
-// a captured hipGraph or as one cooperative kernel with grid barriers.
+// Kernels: one thread per slot.  stress and stepu are separate phases because
+// stepu at a point gathers the str of three neighbours written by stress in
+// the same subcycle and the next stress gathers the velocities of three
+// neighbours written by stepu.  The subcycle loop stays on the device as a
+// captured hipGraph (default) or as one cooperative kernel with grid barriers.
 #include <hip/hip_runtime.h>
 #include <hip/hip_cooperative_groups.h>
 
@@ -32,6 +43,9 @@
 
 #ifndef EVP1D_FMA
 #define EVP1D_FMA 0
+#endif
+#ifndef EVP1D_BLOCK
+#define EVP1D_BLOCK 256
 #endif
 
 namespace cg = cooperative_groups;
@@ -60,14 +74,21 @@ static int fail(int code, const char *fmt, ...) {
                   hipGetErrorString(e_), __FILE__, __LINE__);                  \
   } while (0)
 
+#define RC_TRY(expr)              \
+  do {                            \
+    const int rc_ = (expr);       \
+    if (rc_) return rc_;          \
+  } while (0)
+
 // ---------------------------------------------------------------------------
 // device view of a handle (passed by value to every kernel)
 // ---------------------------------------------------------------------------
 struct DevView {
   int na, navel;
   long ldsig;
+  int t_end;              // stress slots [0, t_end)
+  int u_begin, u_end;     // stepu slots [u_begin, u_end)
   const int32_t *ee, *ne, *se, *nw, *sw, *sse;
-  const uint8_t *skip;
   double2 *vel;
   double *sig;
   double2 *s15, *s27, *s36, *s48;
@@ -81,7 +102,6 @@ struct DevView {
 };
 
 __device__ __forceinline__ void stress_at(const DevView &d, int iw) {
-  if (d.skip[iw] & 1) return;                       // bench_v2b.F90:127
   const double2 cc = d.vel[iw];
   const double2 ve = d.vel[d.ee[iw]];
   const double2 vn = d.vel[d.ne[iw]];
@@ -106,7 +126,6 @@ __device__ __forceinline__ void stress_at(const DevView &d, int iw) {
 }
 
 __device__ __forceinline__ void stepu_at(const DevView &d, int iw, bool write_diag) {
-  if (d.skip[iw] & 2) return;                       // bench_v2b.F90:507
   const double2 v = d.vel[iw];
   const double2 a = d.s15[iw];
   const double2 b = d.s27[d.nw[iw]];
@@ -134,16 +153,16 @@ __device__ __forceinline__ void halo_at(const DevView &d, int k) {
   d.vel[dst] = (src >= 0) ? d.vel[src] : make_double2(0.0, 0.0);
 }
 
-constexpr int kBlock = 256;
+constexpr int kBlock = EVP1D_BLOCK;
 
 __global__ __launch_bounds__(kBlock) void k_stress(DevView d) {
   const int iw = blockIdx.x * kBlock + threadIdx.x;
-  if (iw < d.na) stress_at(d, iw);
+  if (iw < d.t_end) stress_at(d, iw);
 }
 
 __global__ __launch_bounds__(kBlock) void k_stepu(DevView d, int write_diag) {
-  const int iw = blockIdx.x * kBlock + threadIdx.x;
-  if (iw < d.na) stepu_at(d, iw, write_diag != 0);
+  const int iw = d.u_begin + blockIdx.x * kBlock + threadIdx.x;
+  if (iw < d.u_end) stepu_at(d, iw, write_diag != 0);
 }
 
 __global__ __launch_bounds__(kBlock) void k_halo(DevView d) {
@@ -158,10 +177,10 @@ __global__ __launch_bounds__(kBlock) void k_persistent(DevView d, int nsub) {
   const int stride = gridDim.x * kBlock;
   const int t0 = blockIdx.x * kBlock + threadIdx.x;
   for (int k = 0; k < nsub; ++k) {
-    for (int iw = t0; iw < d.na; iw += stride) stress_at(d, iw);
+    for (int iw = t0; iw < d.t_end; iw += stride) stress_at(d, iw);
     grid.sync();
     const bool last = (k == nsub - 1);
-    for (int iw = t0; iw < d.na; iw += stride) stepu_at(d, iw, last);
+    for (int iw = d.u_begin + t0; iw < d.u_end; iw += stride) stepu_at(d, iw, last);
     grid.sync();
     if (d.nhalo > 0) {
       for (int g = t0; g < d.nhalo; g += stride) halo_at(d, g);
@@ -170,16 +189,38 @@ __global__ __launch_bounds__(kBlock) void k_persistent(DevView d, int nsub) {
   }
 }
 
-__global__ void k_put_comp(double2 *dst, const double *src, int n, int comp) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    if (comp == 0) dst[i].x = src[i]; else dst[i].y = src[i];
+// data movement between caller order (q) and slot order (s = inv[q]);
+// positions q >= na are ghost slots and map to themselves.
+__global__ void k_scatter_plain(double *dst, const double *src, const int32_t *inv, int na, int n) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) dst[q < na ? inv[q] : q] = src[q];
+}
+__global__ void k_gather_plain(double *dst, const double *src, const int32_t *inv, int na, int n) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) dst[q] = src[q < na ? inv[q] : q];
+}
+__global__ void k_scatter_comp(double2 *dst, const double *src, const int32_t *inv, int na, int n, int comp) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) {
+    const int s = q < na ? inv[q] : q;
+    if (comp == 0) dst[s].x = src[q]; else dst[s].y = src[q];
   }
 }
-
-__global__ void k_get_comp(double *dst, const double2 *src, int n, int comp) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = (comp == 0) ? src[i].x : src[i].y;
+__global__ void k_gather_comp(double *dst, const double2 *src, const int32_t *inv, int na, int n, int comp) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) {
+    const int s = q < na ? inv[q] : q;
+    dst[q] = (comp == 0) ? src[s].x : src[s].y;
+  }
+}
+// relayout: new slot s takes old slot map[s] (s < na)
+__global__ void k_permute_plain(double *dst, const double *src, const int32_t *map, int na) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < na) dst[s] = src[map[s]];
+}
+__global__ void k_permute_pair(double2 *dst, const double2 *src, const int32_t *map, int na) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < na) dst[s] = src[map[s]];
 }
 
 // ---------------------------------------------------------------------------
@@ -194,13 +235,21 @@ struct evp1d_handle {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<void *> allocs;
   DevView v{};
-  double *scratch = nullptr;      // navel doubles, staging for component copies
+  double *scratch = nullptr;       // navel doubles: staging for host copies
+  double2 *ptmp = nullptr;         // na double2: relayout staging
   int32_t *idx_dev[6] = {};
-  uint8_t *skip_dev = nullptr;
-  std::vector<uint8_t> skip_host;
-  bool idx_set[6] = {};
-  int32_t *halo_dev = nullptr;    // 2*cap: dst then src
+  int32_t *inv_dev = nullptr;      // caller position -> slot
+  int32_t *map_dev = nullptr;      // relayout map
+  int32_t *halo_dev = nullptr;     // 2*halo_cap: dst then src (slot space)
   int halo_cap = 0;
+  // host mirrors in caller order (0-based)
+  std::vector<int32_t> idx_host[6];
+  bool idx_set[6] = {};
+  std::vector<uint8_t> skip_host;  // bit0 skipT, bit1 skipU
+  std::vector<int32_t> perm, inv;  // slot -> caller position, and inverse
+  std::vector<int32_t> halo_dst_host, halo_src_host;   // caller order, -1 = fill
+  bool layout_dirty = false;       // skip flags changed since the last relayout
+  bool idx_dirty = false;          // slot-space lists need re-upload
   int mode = EVP1D_MODE_AUTO;
   int coop_blocks = 0;
   hipGraph_t graph = nullptr;
@@ -247,30 +296,31 @@ static void release(evp1d_handle *h) {
   delete h;
 }
 
+static inline int blocks_for(int n) { return (n + kBlock - 1) / kBlock; }
+
 // field table ---------------------------------------------------------------
-enum FieldKind { FK_PLAIN, FK_VEL, FK_STR, FK_SIG };
+enum FieldKind { FK_PLAIN, FK_PAIR };
 struct FieldRef {
   FieldKind kind;
-  double *plain;   // FK_PLAIN
-  double2 *pair;   // FK_VEL / FK_STR
+  double *plain;   // FK_PLAIN (na doubles, slot order)
+  double2 *pair;   // FK_PAIR  (navel double2, slot order)
   int comp;        // component of pair
-  int len;
+  int len;         // caller-visible length
 };
 
 static bool field_ref(evp1d_handle *h, int f, FieldRef &r) {
   DevView &v = h->v;
   r = FieldRef{FK_PLAIN, nullptr, nullptr, 0, h->na};
   auto plain = [&](const double *p) { r.kind = FK_PLAIN; r.plain = const_cast<double *>(p); };
-  auto pair = [&](double2 *p, int c, FieldKind k) { r.kind = k; r.pair = p; r.comp = c; r.len = h->navel; };
+  auto pair = [&](double2 *p, int c) { r.kind = FK_PAIR; r.pair = p; r.comp = c; r.len = h->navel; };
   if (f >= EVP1D_STRESSP_1 && f <= EVP1D_STRESS12_4) {
-    r.kind = FK_SIG;
-    r.plain = v.sig + (long)(f - EVP1D_STRESSP_1) * v.ldsig;
+    plain(v.sig + (long)(f - EVP1D_STRESSP_1) * v.ldsig);
     return true;
   }
   switch (f) {
     case EVP1D_STRENGTH: plain(v.strength); return true;
-    case EVP1D_UVEL: pair(v.vel, 0, FK_VEL); return true;
-    case EVP1D_VVEL: pair(v.vel, 1, FK_VEL); return true;
+    case EVP1D_UVEL: pair(v.vel, 0); return true;
+    case EVP1D_VVEL: pair(v.vel, 1); return true;
     case EVP1D_DXT: plain(v.dxT); return true;
     case EVP1D_DYT: plain(v.dyT); return true;
     case EVP1D_UAREAR: plain(v.uarear); return true;
@@ -294,14 +344,14 @@ static bool field_ref(evp1d_handle *h, int f, FieldRef &r) {
     case EVP1D_UVEL_INIT: plain(v.uvel_init); return true;
     case EVP1D_VVEL_INIT: plain(v.vvel_init); return true;
     case EVP1D_CB: plain(v.cb); return true;
-    case EVP1D_STR1: pair(v.s15, 0, FK_STR); return true;
-    case EVP1D_STR5: pair(v.s15, 1, FK_STR); return true;
-    case EVP1D_STR2: pair(v.s27, 0, FK_STR); return true;
-    case EVP1D_STR7: pair(v.s27, 1, FK_STR); return true;
-    case EVP1D_STR3: pair(v.s36, 0, FK_STR); return true;
-    case EVP1D_STR6: pair(v.s36, 1, FK_STR); return true;
-    case EVP1D_STR4: pair(v.s48, 0, FK_STR); return true;
-    case EVP1D_STR8: pair(v.s48, 1, FK_STR); return true;
+    case EVP1D_STR1: pair(v.s15, 0); return true;
+    case EVP1D_STR5: pair(v.s15, 1); return true;
+    case EVP1D_STR2: pair(v.s27, 0); return true;
+    case EVP1D_STR7: pair(v.s27, 1); return true;
+    case EVP1D_STR3: pair(v.s36, 0); return true;
+    case EVP1D_STR6: pair(v.s36, 1); return true;
+    case EVP1D_STR4: pair(v.s48, 0); return true;
+    case EVP1D_STR8: pair(v.s48, 1); return true;
     default: return false;
   }
 }
@@ -312,22 +362,114 @@ static bool ready(const evp1d_handle *h) {
   return true;
 }
 
+// every na-length device array (slot order), for relayout
+static void plain_arrays(evp1d_handle *h, std::vector<double *> &out) {
+  DevView &v = h->v;
+  out = {const_cast<double *>(v.dxT), const_cast<double *>(v.dyT), const_cast<double *>(v.strength),
+         const_cast<double *>(v.hte), const_cast<double *>(v.htn), const_cast<double *>(v.htem1),
+         const_cast<double *>(v.htnm1), const_cast<double *>(v.cw), const_cast<double *>(v.aiu),
+         const_cast<double *>(v.uocn), const_cast<double *>(v.vocn), const_cast<double *>(v.waterx),
+         const_cast<double *>(v.watery), const_cast<double *>(v.forcex), const_cast<double *>(v.forcey),
+         const_cast<double *>(v.umassdti), const_cast<double *>(v.fm), const_cast<double *>(v.uarear),
+         const_cast<double *>(v.uvel_init), const_cast<double *>(v.vvel_init), const_cast<double *>(v.tbu),
+         v.strintx, v.strinty, v.cb};
+  for (int c = 0; c < 12; ++c) out.push_back(v.sig + (long)c * v.ldsig);
+}
+
+// Rebuild the class permutation from the skip flags and move every resident
+// array into the new slot order.
+static int relayout(evp1d_handle *h) {
+  const int na = h->na;
+  std::vector<int32_t> nperm((size_t)na), ninv((size_t)na);
+  int cnt[4] = {0, 0, 0, 0};
+  auto cls = [](uint8_t b) {
+    const bool t = !(b & 1), u = !(b & 2);
+    return t && !u ? 0 : (t && u ? 1 : (u ? 2 : 3));
+  };
+  for (int q = 0; q < na; ++q) ++cnt[cls(h->skip_host[(size_t)q])];
+  int start[4] = {0, cnt[0], cnt[0] + cnt[1], cnt[0] + cnt[1] + cnt[2]};
+  for (int q = 0; q < na; ++q) {
+    const int s = start[cls(h->skip_host[(size_t)q])]++;
+    nperm[(size_t)s] = q;
+    ninv[(size_t)q] = s;
+  }
+  if (na > 0 && nperm != h->perm) {
+    std::vector<int32_t> map((size_t)na);
+    for (int s = 0; s < na; ++s) map[(size_t)s] = h->inv[(size_t)nperm[(size_t)s]];
+    HIP_TRY(hipMemcpyAsync(h->map_dev, map.data(), (size_t)na * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
+    std::vector<double *> arrs;
+    plain_arrays(h, arrs);
+    double *tmp = reinterpret_cast<double *>(h->ptmp);
+    for (double *a : arrs) {
+      hipLaunchKernelGGL(k_permute_plain, dim3(blocks_for(na)), dim3(kBlock), 0, h->stream, tmp, a, h->map_dev, na);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemcpyAsync(a, tmp, (size_t)na * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    }
+    double2 *pairs[5] = {h->v.vel, h->v.s15, h->v.s27, h->v.s36, h->v.s48};
+    for (double2 *a : pairs) {
+      hipLaunchKernelGGL(k_permute_pair, dim3(blocks_for(na)), dim3(kBlock), 0, h->stream, h->ptmp, a, h->map_dev, na);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemcpyAsync(a, h->ptmp, (size_t)na * sizeof(double2), hipMemcpyDeviceToDevice, h->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(h->inv_dev, ninv.data(), (size_t)na * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->perm.swap(nperm);
+    h->inv.swap(ninv);
+  }
+  h->v.t_end = cnt[0] + cnt[1];
+  h->v.u_begin = cnt[0];
+  h->v.u_end = cnt[0] + cnt[1] + cnt[2];
+  h->layout_dirty = false;
+  h->idx_dirty = true;
+  h->graph_dirty = true;
+  return EVP1D_OK;
+}
+
+static inline int32_t slot_of(const evp1d_handle *h, int32_t q) { return q < h->na ? h->inv[(size_t)q] : q; }
+
+// bring the device copy of layout, lists and halo up to date
+static int ensure(evp1d_handle *h) {
+  if (h->layout_dirty) RC_TRY(relayout(h));
+  if (!h->idx_dirty) return EVP1D_OK;
+  const int na = h->na;
+  std::vector<int32_t> z((size_t)na);
+  for (int k = 0; k < 6; ++k) {
+    if (!h->idx_set[k] || na == 0) continue;
+    const std::vector<int32_t> &src = h->idx_host[k];
+    for (int s = 0; s < na; ++s) z[(size_t)s] = slot_of(h, src[(size_t)h->perm[(size_t)s]]);
+    HIP_TRY(hipMemcpyAsync(h->idx_dev[k], z.data(), (size_t)na * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+  }
+  const int n = (int)h->halo_dst_host.size();
+  if (n > 0) {
+    std::vector<int32_t> buf((size_t)2 * n);
+    for (int k = 0; k < n; ++k) {
+      buf[(size_t)k] = h->halo_dst_host[(size_t)k];
+      const int32_t src = h->halo_src_host[(size_t)k];
+      buf[(size_t)n + k] = src < 0 ? -1 : slot_of(h, src);
+    }
+    HIP_TRY(hipMemcpyAsync(h->halo_dev, buf.data(), (size_t)2 * n * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+  }
+  h->idx_dirty = false;
+  h->graph_dirty = true;
+  return EVP1D_OK;
+}
+
 // ---------------------------------------------------------------------------
 // launches
 // ---------------------------------------------------------------------------
-static inline int blocks_for(int n) { return (n + kBlock - 1) / kBlock; }
-
 static int launch_stress(evp1d_handle *h) {
-  if (h->na == 0) return EVP1D_OK;
-  hipLaunchKernelGGL(k_stress, dim3(blocks_for(h->na)), dim3(kBlock), 0, h->stream, h->v);
+  if (h->v.t_end == 0) return EVP1D_OK;
+  hipLaunchKernelGGL(k_stress, dim3(blocks_for(h->v.t_end)), dim3(kBlock), 0, h->stream, h->v);
   HIP_TRY(hipGetLastError());
   return EVP1D_OK;
 }
 
 static int launch_stepu(evp1d_handle *h, bool write_diag) {
-  if (h->na == 0) return EVP1D_OK;
-  hipLaunchKernelGGL(k_stepu, dim3(blocks_for(h->na)), dim3(kBlock), 0, h->stream, h->v,
-                     write_diag ? 1 : 0);
+  const int n = h->v.u_end - h->v.u_begin;
+  if (n == 0) return EVP1D_OK;
+  hipLaunchKernelGGL(k_stepu, dim3(blocks_for(n)), dim3(kBlock), 0, h->stream, h->v, write_diag ? 1 : 0);
   HIP_TRY(hipGetLastError());
   return EVP1D_OK;
 }
@@ -341,12 +483,9 @@ static int launch_halo(evp1d_handle *h) {
 
 static int issue_subcycles(evp1d_handle *h, int nsub) {
   for (int k = 0; k < nsub; ++k) {
-    int rc = launch_stress(h);
-    if (rc) return rc;
-    rc = launch_stepu(h, k == nsub - 1);
-    if (rc) return rc;
-    rc = launch_halo(h);
-    if (rc) return rc;
+    RC_TRY(launch_stress(h));
+    RC_TRY(launch_stepu(h, k == nsub - 1));
+    RC_TRY(launch_halo(h));
   }
   return EVP1D_OK;
 }
@@ -355,15 +494,14 @@ static int run_graph(evp1d_handle *h, int nsub) {
   if (h->graph_dirty || h->graph_nsub != nsub || !h->graph_exec) {
     drop_graph(h);
     HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    int rc = issue_subcycles(h, nsub);
+    const int rc = issue_subcycles(h, nsub);
     hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    const hipError_t e = hipStreamEndCapture(h->stream, &g);
     if (rc) {
       if (g) (void)hipGraphDestroy(g);
       return rc;
     }
-    if (e != hipSuccess)
-      return fail(EVP1D_EHIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return fail(EVP1D_EHIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
     h->graph = g;
     HIP_TRY(hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
     h->graph_nsub = nsub;
@@ -381,12 +519,11 @@ static int run_persistent(evp1d_handle *h, int nsub) {
     if (per_cu <= 0 || cus <= 0) return fail(EVP1D_EHIP, "no co-resident blocks for k_persistent");
     h->coop_blocks = per_cu * cus;
   }
-  const int needed = std::max(blocks_for(h->na), 1);
-  int grid = std::min(h->coop_blocks, needed);
+  const int work = std::max(std::max(h->v.t_end, h->v.u_end - h->v.u_begin), h->v.nhalo);
+  const int grid = std::max(1, std::min(h->coop_blocks, blocks_for(work)));
   DevView v = h->v;
   void *args[] = {&v, &nsub};
-  HIP_TRY(hipLaunchCooperativeKernel((const void *)k_persistent, dim3(grid), dim3(kBlock), args, 0,
-                                     h->stream));
+  HIP_TRY(hipLaunchCooperativeKernel((const void *)k_persistent, dim3(grid), dim3(kBlock), args, 0, h->stream));
   return EVP1D_OK;
 }
 
@@ -462,15 +599,28 @@ int evp1d_create(int32_t na, int32_t navel, const evp1d_params *p, evp1d_handle 
   for (const double **pp : cplain) A((void **)pp, nA);
   A((void **)&v.strintx, nA); A((void **)&v.strinty, nA); A((void **)&v.cb, nA);
   for (int k = 0; k < 6; ++k) A((void **)&h->idx_dev[k], (size_t)na * sizeof(int32_t));
-  A((void **)&h->skip_dev, (size_t)na);
+  A((void **)&h->inv_dev, (size_t)na * sizeof(int32_t));
+  A((void **)&h->map_dev, (size_t)na * sizeof(int32_t));
   A((void **)&h->scratch, (size_t)navel * sizeof(double));
+  A((void **)&h->ptmp, (size_t)na * sizeof(double2));
   if (rc) { release(h); return rc; }
   v.ee = h->idx_dev[0]; v.ne = h->idx_dev[1]; v.se = h->idx_dev[2];
   v.nw = h->idx_dev[3]; v.sw = h->idx_dev[4]; v.sse = h->idx_dev[5];
-  v.skip = h->skip_dev;
   v.nhalo = 0;
   v.s = scalars_from(*p);
+  // no skips yet: every point is in the "stress and stepu" class, identity order
+  v.t_end = na;
+  v.u_begin = 0;
+  v.u_end = na;
   h->skip_host.assign((size_t)na, 0);
+  h->perm.resize((size_t)na);
+  h->inv.resize((size_t)na);
+  for (int q = 0; q < na; ++q) h->perm[(size_t)q] = h->inv[(size_t)q] = q;
+  if (na > 0 &&
+      hipMemcpyAsync(h->inv_dev, h->inv.data(), (size_t)na * sizeof(int32_t), hipMemcpyHostToDevice, h->stream) != hipSuccess) {
+    release(h);
+    return fail(EVP1D_EHIP, "evp1d_create: upload failed");
+  }
   if (hipStreamSynchronize(h->stream) != hipSuccess) {
     release(h);
     return fail(EVP1D_EHIP, "evp1d_create: initial clear failed");
@@ -512,14 +662,15 @@ int evp1d_set_field(evp1d_handle *h, int32_t field, const double *src, int32_t n
   if (n != r.len) return fail(EVP1D_EINVAL, "evp1d_set_field: field %d has length %d, got %d", field, r.len, n);
   if (n == 0) return EVP1D_OK;
   if (!src) return fail(EVP1D_EINVAL, "evp1d_set_field: src is NULL");
-  if (r.kind == FK_PLAIN || r.kind == FK_SIG) {
-    HIP_TRY(hipMemcpyAsync(r.plain, src, (size_t)n * sizeof(double), hipMemcpyHostToDevice, h->stream));
-  } else {
-    HIP_TRY(hipMemcpyAsync(h->scratch, src, (size_t)n * sizeof(double), hipMemcpyHostToDevice, h->stream));
-    hipLaunchKernelGGL(k_put_comp, dim3(blocks_for(n)), dim3(kBlock), 0, h->stream, r.pair,
-                       (const double *)h->scratch, n, r.comp);
-    HIP_TRY(hipGetLastError());
-  }
+  RC_TRY(ensure(h));
+  HIP_TRY(hipMemcpyAsync(h->scratch, src, (size_t)n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  if (r.kind == FK_PLAIN)
+    hipLaunchKernelGGL(k_scatter_plain, dim3(blocks_for(n)), dim3(kBlock), 0, h->stream, r.plain,
+                       (const double *)h->scratch, (const int32_t *)h->inv_dev, h->na, n);
+  else
+    hipLaunchKernelGGL(k_scatter_comp, dim3(blocks_for(n)), dim3(kBlock), 0, h->stream, r.pair,
+                       (const double *)h->scratch, (const int32_t *)h->inv_dev, h->na, n, r.comp);
+  HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(h->stream));
   return EVP1D_OK;
 }
@@ -531,14 +682,15 @@ int evp1d_get_field(evp1d_handle *h, int32_t field, double *dst, int32_t n) {
   if (n != r.len) return fail(EVP1D_EINVAL, "evp1d_get_field: field %d has length %d, got %d", field, r.len, n);
   if (n == 0) return EVP1D_OK;
   if (!dst) return fail(EVP1D_EINVAL, "evp1d_get_field: dst is NULL");
-  if (r.kind == FK_PLAIN || r.kind == FK_SIG) {
-    HIP_TRY(hipMemcpyAsync(dst, r.plain, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-  } else {
-    hipLaunchKernelGGL(k_get_comp, dim3(blocks_for(n)), dim3(kBlock), 0, h->stream, h->scratch,
-                       (const double2 *)r.pair, n, r.comp);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(dst, h->scratch, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-  }
+  RC_TRY(ensure(h));
+  if (r.kind == FK_PLAIN)
+    hipLaunchKernelGGL(k_gather_plain, dim3(blocks_for(n)), dim3(kBlock), 0, h->stream, h->scratch,
+                       (const double *)r.plain, (const int32_t *)h->inv_dev, h->na, n);
+  else
+    hipLaunchKernelGGL(k_gather_comp, dim3(blocks_for(n)), dim3(kBlock), 0, h->stream, h->scratch,
+                       (const double2 *)r.pair, (const int32_t *)h->inv_dev, h->na, n, r.comp);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(dst, h->scratch, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
   return EVP1D_OK;
 }
@@ -552,16 +704,12 @@ int evp1d_set_index(evp1d_handle *h, int32_t which, const int32_t *idx, int32_t 
   for (int i = 0; i < n; ++i) {
     const int32_t q = idx[i];
     if (q < 1 || q > h->navel)
-      return fail(EVP1D_EINDEX, "evp1d_set_index: list %d entry %d = %d outside 1..%d", which, i + 1, q,
-                  h->navel);
+      return fail(EVP1D_EINDEX, "evp1d_set_index: list %d entry %d = %d outside 1..%d", which, i + 1, q, h->navel);
     z[(size_t)i] = q - 1;
   }
-  if (n > 0) {
-    HIP_TRY(hipMemcpyAsync(h->idx_dev[which], z.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice,
-                           h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
-  }
+  h->idx_host[which].swap(z);
   h->idx_set[which] = true;
+  h->idx_dirty = true;
   return EVP1D_OK;
 }
 
@@ -571,14 +719,14 @@ int evp1d_set_skip(evp1d_handle *h, int32_t which, const int32_t *flags, int32_t
   if (n != h->na) return fail(EVP1D_EINVAL, "evp1d_set_skip: length %d != na %d", n, h->na);
   if (n > 0 && !flags) return fail(EVP1D_EINVAL, "evp1d_set_skip: flags is NULL");
   const uint8_t bit = (uint8_t)(1u << which);
+  bool changed = false;
   for (int i = 0; i < n; ++i) {
     uint8_t &b = h->skip_host[(size_t)i];
-    b = (uint8_t)(flags[i] ? (b | bit) : (b & ~bit));
+    const uint8_t nb = (uint8_t)(flags[i] ? (b | bit) : (b & ~bit));
+    changed |= nb != b;
+    b = nb;
   }
-  if (n > 0) {
-    HIP_TRY(hipMemcpyAsync(h->skip_dev, h->skip_host.data(), (size_t)n, hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
-  }
+  if (changed) h->layout_dirty = true;
   return EVP1D_OK;
 }
 
@@ -586,7 +734,7 @@ int evp1d_set_halo(evp1d_handle *h, int32_t n, const int32_t *dst, const int32_t
   if (!h) return fail(EVP1D_EINVAL, "evp1d_set_halo: NULL handle");
   if (n < 0) return fail(EVP1D_EINVAL, "evp1d_set_halo: n=%d < 0", n);
   if (n > 0 && (!dst || !src)) return fail(EVP1D_EINVAL, "evp1d_set_halo: NULL list");
-  std::vector<int32_t> buf((size_t)2 * n);
+  std::vector<int32_t> d((size_t)n), s((size_t)n);
   std::vector<char> seen((size_t)(h->navel - h->na), 0);
   for (int k = 0; k < n; ++k) {
     if (dst[k] <= h->na || dst[k] > h->navel)
@@ -594,43 +742,42 @@ int evp1d_set_halo(evp1d_handle *h, int32_t n, const int32_t *dst, const int32_t
                   h->navel);
     if (src[k] < 0 || src[k] > h->na)
       return fail(EVP1D_EINDEX, "evp1d_set_halo: src[%d]=%d outside 0..%d", k + 1, src[k], h->na);
-    char &s = seen[(size_t)(dst[k] - h->na - 1)];
-    if (s) return fail(EVP1D_EINDEX, "evp1d_set_halo: ghost slot %d listed twice", dst[k]);
-    s = 1;
-    buf[(size_t)k] = dst[k] - 1;
-    buf[(size_t)n + k] = src[k] - 1;
+    char &seen_k = seen[(size_t)(dst[k] - h->na - 1)];
+    if (seen_k) return fail(EVP1D_EINDEX, "evp1d_set_halo: ghost slot %d listed twice", dst[k]);
+    seen_k = 1;
+    d[(size_t)k] = dst[k] - 1;
+    s[(size_t)k] = src[k] - 1;   // -1: fill with zero
   }
   if (n > h->halo_cap) {
-    int rc = dev_alloc(h, (void **)&h->halo_dev, (size_t)2 * n * sizeof(int32_t));
-    if (rc) return rc;
+    RC_TRY(dev_alloc(h, (void **)&h->halo_dev, (size_t)2 * n * sizeof(int32_t)));
     h->halo_cap = n;
   }
-  if (n > 0) {
-    HIP_TRY(hipMemcpyAsync(h->halo_dev, buf.data(), (size_t)2 * n * sizeof(int32_t), hipMemcpyHostToDevice,
-                           h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
-  }
+  h->halo_dst_host.swap(d);
+  h->halo_src_host.swap(s);
   h->v.nhalo = n;
   h->v.halo_dst = h->halo_dev;
   h->v.halo_src = h->halo_dev ? h->halo_dev + n : nullptr;
-  h->graph_dirty = true;
+  h->idx_dirty = true;
   return EVP1D_OK;
 }
 
 int evp1d_stress(evp1d_handle *h) {
   if (!h) return fail(EVP1D_EINVAL, "evp1d_stress: NULL handle");
   if (!ready(h)) return fail(EVP1D_ESTATE, "evp1d_stress: neighbour lists not set");
+  RC_TRY(ensure(h));
   return launch_stress(h);
 }
 
 int evp1d_stepu(evp1d_handle *h) {
   if (!h) return fail(EVP1D_EINVAL, "evp1d_stepu: NULL handle");
   if (!ready(h)) return fail(EVP1D_ESTATE, "evp1d_stepu: neighbour lists not set");
+  RC_TRY(ensure(h));
   return launch_stepu(h, true);
 }
 
 int evp1d_halo(evp1d_handle *h) {
   if (!h) return fail(EVP1D_EINVAL, "evp1d_halo: NULL handle");
+  RC_TRY(ensure(h));
   return launch_halo(h);
 }
 
@@ -651,6 +798,7 @@ int evp1d_run(evp1d_handle *h, int32_t nsub) {
   if (!h) return fail(EVP1D_EINVAL, "evp1d_run: NULL handle");
   if (nsub < 0) return fail(EVP1D_EINVAL, "evp1d_run: nsub=%d < 0", nsub);
   if (!ready(h)) return fail(EVP1D_ESTATE, "evp1d_run: neighbour lists not set");
+  RC_TRY(ensure(h));
   if (nsub == 0 || h->na == 0) return EVP1D_OK;
   switch (h->mode) {
     case EVP1D_MODE_LAUNCH: return issue_subcycles(h, nsub);
@@ -665,10 +813,8 @@ int evp1d_run_individual(evp1d_handle *h, int32_t which, int32_t nsub) {
   if (!h) return fail(EVP1D_EINVAL, "evp1d_run_individual: NULL handle");
   if (!ready(h)) return fail(EVP1D_ESTATE, "evp1d_run_individual: neighbour lists not set");
   if (which != 0 && which != 1) return fail(EVP1D_EINVAL, "evp1d_run_individual: which=%d", which);
-  for (int k = 0; k < nsub; ++k) {
-    const int rc = which == 0 ? launch_stress(h) : launch_stepu(h, k == nsub - 1);
-    if (rc) return rc;
-  }
+  RC_TRY(ensure(h));
+  for (int k = 0; k < nsub; ++k) RC_TRY(which == 0 ? launch_stress(h) : launch_stepu(h, k == nsub - 1));
   return EVP1D_OK;
 }
 

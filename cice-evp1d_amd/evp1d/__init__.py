@@ -83,12 +83,15 @@ def lib_path(fma: bool = False) -> Path:
     return LIBDIR / ("libevp1d_fma.so" if fma else "libevp1d.so")
 
 
-def load(fma: bool = False) -> ctypes.CDLL:
-    """Load libevp1d(.so|_fma.so) from the package's lib/ (raises if absent)."""
+def load(fma: bool = False, path: str | os.PathLike | None = None) -> ctypes.CDLL:
+    """Load libevp1d(.so|_fma.so) from the package's lib/ (raises if absent).
+
+    `path` binds another build of the same ABI (e.g. a tuning variant) in
+    place of the default one for this process."""
     key = bool(fma)
-    if key in _libs:
+    if path is None and key in _libs:
         return _libs[key]
-    path = lib_path(fma)
+    path = Path(path) if path is not None else lib_path(fma)
     if not path.exists():
         raise ImportError(f"{path} is not built: run __graft_entry__.build() or python -m evp1d.build")
     L = ctypes.CDLL(str(path))

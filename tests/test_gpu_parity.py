@@ -164,3 +164,38 @@ def test_bad_inputs_rejected(gpu):
             eng.set_halo([1], [1])                   # slot 1 is not a ghost
     with pytest.raises(evp1d.Evp1dError):
         evp1d.Engine(10, 5, p)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_skip_flags_change_between_runs(gpu, mode):
+    """New skip flags (a new model step's ice masks, set_skipMe) re-partition
+    the device slots; results stay bit-identical to the oracle."""
+    st = synthetic_state(80, 60, seed=51)
+    lib, p = oracle_params(st, 20)
+    want = st.copy()
+    lib.run(p, want, 10)
+    rng = np.random.default_rng(9)
+    newT = (rng.random(st.na) < 0.4).astype(np.int32)
+    newU = (rng.random(st.na) < 0.2).astype(np.int32)
+    want.skipT[:] = newT
+    want.skipU[:] = newU
+    lib.run(p, want, 10)
+    with make_engine(st, 20, mode) as eng:
+        eng.run(10)
+        eng.set_skip("skipT", newT)
+        eng.set_skip("skipU", newU)
+        eng.run(10)
+        got = fetch_state(eng, want)
+    assert_state_bitexact(got, want)
+
+
+def test_fields_roundtrip_through_layout(gpu):
+    """set_field/get_field return the caller's order under any skip pattern."""
+    st = synthetic_state(40, 30, seed=61)
+    with make_engine(st, 3) as eng:
+        rng = np.random.default_rng(1)
+        for name in ("stressm_3", "uvel", "str6", "forcexU", "Cb"):
+            a = rng.standard_normal(eng.length(name))
+            eng.set_field(name, a)
+            eng.set_skip("skipT", (rng.random(st.na) < 0.5).astype(np.int32))
+            assert np.array_equal(eng.get_field(name), a), name
