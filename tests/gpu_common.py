@@ -56,9 +56,12 @@ def bits(a: np.ndarray) -> np.ndarray:
 
 
 def assert_state_bitexact(got: O.State1D, want: O.State1D, fields=None) -> None:
+    """Every value bit-identical; a NaN matches any NaN (IEEE 754 leaves the
+    sign and payload of a generated NaN to the hardware: x86 and CDNA differ)."""
     for k in fields or want.f:
         g, w = bits(got.f[k]), bits(want.f[k])
-        bad = np.nonzero(g != w)[0]
+        both_nan = np.isnan(got.f[k]) & np.isnan(want.f[k])
+        bad = np.nonzero((g != w) & ~both_nan)[0]
         assert bad.size == 0, (f"{k}: {bad.size} of {w.size} values differ; first at {bad[0]}: "
                                f"got {got.f[k][bad[0]]!r} want {want.f[k][bad[0]]!r}")
 
