@@ -99,6 +99,21 @@ def cpu_baseline(workload: str, s: dict, ndte_sample: int, threads: int) -> dict
             "kind": kind, "sample": what, "seconds": round(secs, 3)}
 
 
+def calibrate_copy(mib: int) -> int:
+    """Plain device-to-device copy of known size through torch (plumbing only):
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this command gives the factor
+    between the counters and the bytes a kernel really moves."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(20):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes_each_way": n * 8, "launches": 20}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,7 +125,12 @@ def main() -> int:
     ap.add_argument("--kernel-reps", type=int, default=200, help="launches per individual-kernel timing")
     ap.add_argument("--cpu-subcycles", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB 20 times (the reference kernel for "
+                         "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        return calibrate_copy(args.calibrate_copy)
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design "
