@@ -153,17 +153,52 @@ __device__ __forceinline__ void halo_at(const DevView &d, int k) {
   d.vel[dst] = (src >= 0) ? d.vel[src] : make_double2(0.0, 0.0);
 }
 
-constexpr int kBlock = EVP1D_BLOCK;
+constexpr int kBlock = EVP1D_BLOCK;   // block size of the helper kernels and the persistent form
 
-__global__ __launch_bounds__(kBlock) void k_stress(DevView d) {
-  const int iw = blockIdx.x * kBlock + threadIdx.x;
+// Workgroups of one launch are dispatched round-robin over the 8 XCDs of the
+// chip (workgroup b runs on XCD b % 8), and each XCD has its own L2.  With
+// `xcd` set, the workgroup with hardware index b processes logical tile L(b),
+// a bijection that hands each XCD one contiguous eighth of the slot range, so
+// the neighbour rows a tile gathers from (the previous row for stress, the
+// next row for stepu) were streamed through the same L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int tile_of(int b, int n, bool xcd) {
+  if (!xcd) return b;
+  const int q = n / kXcds, r = n % kXcds;
+  const int x = b % kXcds, k = b / kXcds;
+  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+template <int B, bool X>
+__global__ __launch_bounds__(B) void k_stress(DevView d) {
+  const int iw = tile_of(blockIdx.x, gridDim.x, X) * B + threadIdx.x;
   if (iw < d.t_end) stress_at(d, iw);
 }
 
-__global__ __launch_bounds__(kBlock) void k_stepu(DevView d, int write_diag) {
-  const int iw = d.u_begin + blockIdx.x * kBlock + threadIdx.x;
+template <int B, bool X>
+__global__ __launch_bounds__(B) void k_stepu(DevView d, int write_diag) {
+  const int iw = d.u_begin + tile_of(blockIdx.x, gridDim.x, X) * B + threadIdx.x;
   if (iw < d.u_end) stepu_at(d, iw, write_diag != 0);
 }
+
+// launch configurations the autotuner chooses from
+constexpr int kNumBlocks = 5;
+constexpr int kBlocks[kNumBlocks] = {64, 128, 256, 512, 1024};
+using StressFn = void (*)(DevView);
+using StepuFn = void (*)(DevView, int);
+static const StressFn kStressFns[kNumBlocks][2] = {
+    {k_stress<64, false>, k_stress<64, true>},     {k_stress<128, false>, k_stress<128, true>},
+    {k_stress<256, false>, k_stress<256, true>},   {k_stress<512, false>, k_stress<512, true>},
+    {k_stress<1024, false>, k_stress<1024, true>}};
+static const StepuFn kStepuFns[kNumBlocks][2] = {
+    {k_stepu<64, false>, k_stepu<64, true>},       {k_stepu<128, false>, k_stepu<128, true>},
+    {k_stepu<256, false>, k_stepu<256, true>},     {k_stepu<512, false>, k_stepu<512, true>},
+    {k_stepu<1024, false>, k_stepu<1024, true>}};
+
+struct LaunchCfg {
+  int bi = 2;      // index into kBlocks
+  int xcd = 0;
+};
 
 __global__ __launch_bounds__(kBlock) void k_halo(DevView d) {
   const int k = blockIdx.x * kBlock + threadIdx.x;
@@ -189,28 +224,37 @@ __global__ __launch_bounds__(kBlock) void k_persistent(DevView d, int nsub) {
   }
 }
 
-// data movement between caller order (q) and slot order (s = inv[q]);
-// positions q >= na are ghost slots and map to themselves.
-__global__ void k_scatter_plain(double *dst, const double *src, const int32_t *inv, int na, int n) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < n) dst[q < na ? inv[q] : q] = src[q];
+// data movement between caller order and slot order.  Element t of a host
+// array of member k sits at caller position q = a_off + t (t < m_na, active
+// points) or g_off + t - m_na (ghost slots); caller position q < na lives in
+// slot inv[q], ghost positions q >= na are their own slot.
+struct Span {
+  int a_off, m_na, g_off, n;
+};
+__device__ __forceinline__ int slot_of_t(const Span &sp, const int32_t *inv, int na, int t) {
+  const int q = t < sp.m_na ? sp.a_off + t : sp.g_off + (t - sp.m_na);
+  return q < na ? inv[q] : q;
 }
-__global__ void k_gather_plain(double *dst, const double *src, const int32_t *inv, int na, int n) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < n) dst[q] = src[q < na ? inv[q] : q];
+__global__ void k_scatter_plain(double *dst, const double *src, const int32_t *inv, int na, Span sp) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < sp.n) dst[slot_of_t(sp, inv, na, t)] = src[t];
 }
-__global__ void k_scatter_comp(double2 *dst, const double *src, const int32_t *inv, int na, int n, int comp) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < n) {
-    const int s = q < na ? inv[q] : q;
-    if (comp == 0) dst[s].x = src[q]; else dst[s].y = src[q];
+__global__ void k_gather_plain(double *dst, const double *src, const int32_t *inv, int na, Span sp) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < sp.n) dst[t] = src[slot_of_t(sp, inv, na, t)];
+}
+__global__ void k_scatter_comp(double2 *dst, const double *src, const int32_t *inv, int na, Span sp, int comp) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < sp.n) {
+    const int s = slot_of_t(sp, inv, na, t);
+    if (comp == 0) dst[s].x = src[t]; else dst[s].y = src[t];
   }
 }
-__global__ void k_gather_comp(double *dst, const double2 *src, const int32_t *inv, int na, int n, int comp) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < n) {
-    const int s = q < na ? inv[q] : q;
-    dst[q] = (comp == 0) ? src[s].x : src[s].y;
+__global__ void k_gather_comp(double *dst, const double2 *src, const int32_t *inv, int na, Span sp, int comp) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < sp.n) {
+    const int s = slot_of_t(sp, inv, na, t);
+    dst[t] = (comp == 0) ? src[s].x : src[s].y;
   }
 }
 // relayout: new slot s takes old slot map[s] (s < na)
@@ -251,6 +295,11 @@ struct evp1d_handle {
   bool layout_dirty = false;       // skip flags changed since the last relayout
   bool idx_dirty = false;          // slot-space lists need re-upload
   int mode = EVP1D_MODE_AUTO;
+  LaunchCfg cfg_stress, cfg_stepu;
+  // ensemble members: member k owns caller positions [a_off[k], a_off[k]+m_na[k])
+  // and ghost positions [g_off[k], g_off[k]+m_navel[k]-m_na[k])
+  std::vector<int32_t> m_na, m_navel, a_off, g_off;
+  std::vector<uint8_t> idx_mset[6];   // per member: list set
   int coop_blocks = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -362,6 +411,20 @@ static bool ready(const evp1d_handle *h) {
   return true;
 }
 
+// caller-position span of member k (k < 0: the whole handle)
+struct MemberSpan {
+  int a_off, m_na, g_off, m_navel;
+};
+static bool member_span(const evp1d_handle *h, int k, MemberSpan &m) {
+  if (k < 0) {
+    m = MemberSpan{0, h->na, h->na, h->navel};
+    return true;
+  }
+  if (k >= (int)h->m_na.size()) return false;
+  m = MemberSpan{h->a_off[(size_t)k], h->m_na[(size_t)k], h->g_off[(size_t)k], h->m_navel[(size_t)k]};
+  return true;
+}
+
 // every na-length device array (slot order), for relayout
 static void plain_arrays(evp1d_handle *h, std::vector<double *> &out) {
   DevView &v = h->v;
@@ -461,7 +524,9 @@ static int ensure(evp1d_handle *h) {
 // ---------------------------------------------------------------------------
 static int launch_stress(evp1d_handle *h) {
   if (h->v.t_end == 0) return EVP1D_OK;
-  hipLaunchKernelGGL(k_stress, dim3(blocks_for(h->v.t_end)), dim3(kBlock), 0, h->stream, h->v);
+  const LaunchCfg c = h->cfg_stress;
+  const int B = kBlocks[c.bi];
+  hipLaunchKernelGGL(kStressFns[c.bi][c.xcd], dim3((h->v.t_end + B - 1) / B), dim3(B), 0, h->stream, h->v);
   HIP_TRY(hipGetLastError());
   return EVP1D_OK;
 }
@@ -469,7 +534,10 @@ static int launch_stress(evp1d_handle *h) {
 static int launch_stepu(evp1d_handle *h, bool write_diag) {
   const int n = h->v.u_end - h->v.u_begin;
   if (n == 0) return EVP1D_OK;
-  hipLaunchKernelGGL(k_stepu, dim3(blocks_for(n)), dim3(kBlock), 0, h->stream, h->v, write_diag ? 1 : 0);
+  const LaunchCfg c = h->cfg_stepu;
+  const int B = kBlocks[c.bi];
+  hipLaunchKernelGGL(kStepuFns[c.bi][c.xcd], dim3((n + B - 1) / B), dim3(B), 0, h->stream, h->v,
+                     write_diag ? 1 : 0);
   HIP_TRY(hipGetLastError());
   return EVP1D_OK;
 }
@@ -561,16 +629,38 @@ int evp1d_set_evp_parameters(int32_t ndte, double dt, evp1d_params *p) {
   return EVP1D_OK;
 }
 
-int evp1d_create(int32_t na, int32_t navel, const evp1d_params *p, evp1d_handle **out) {
+static int create_impl(int32_t nmem, const int32_t *mna, const int32_t *mnavel, const evp1d_params *p,
+                       evp1d_handle **out) {
   if (!out) return fail(EVP1D_EINVAL, "evp1d_create: out is NULL");
   *out = nullptr;
   if (!p) return fail(EVP1D_EINVAL, "evp1d_create: params is NULL");
-  if (na < 0 || navel < na)
-    return fail(EVP1D_EINVAL, "evp1d_create: need 0 <= na <= navel (na=%d navel=%d)", na, navel);
+  if (nmem < 1 || !mna || !mnavel) return fail(EVP1D_EINVAL, "evp1d_create: need nmem >= 1 and size lists");
+  long na_l = 0, navel_l = 0;
+  for (int k = 0; k < nmem; ++k) {
+    if (mna[k] < 0 || mnavel[k] < mna[k])
+      return fail(EVP1D_EINVAL, "evp1d_create: member %d needs 0 <= na <= navel (na=%d navel=%d)", k, mna[k],
+                  mnavel[k]);
+    na_l += mna[k];
+    navel_l += mnavel[k];
+  }
+  if (navel_l > 0x7fffffffL) return fail(EVP1D_EINVAL, "evp1d_create: %ld slots exceed int32", navel_l);
+  const int na = (int)na_l, navel = (int)navel_l;
   evp1d_handle *h = new (std::nothrow) evp1d_handle();
   if (!h) return fail(EVP1D_ENOMEM, "evp1d_create: host allocation failed");
   h->na = na;
   h->navel = navel;
+  {
+    int a = 0, g = na;
+    for (int k = 0; k < nmem; ++k) {
+      h->m_na.push_back(mna[k]);
+      h->m_navel.push_back(mnavel[k]);
+      h->a_off.push_back(a);
+      h->g_off.push_back(g);
+      a += mna[k];
+      g += mnavel[k] - mna[k];
+    }
+    for (int w = 0; w < 6; ++w) h->idx_mset[w].assign((size_t)nmem, 0);
+  }
   h->p = *p;
   h->ldsig = ((long)na + 63) / 64 * 64;
   int rc = EVP1D_OK;
@@ -629,6 +719,33 @@ int evp1d_create(int32_t na, int32_t navel, const evp1d_params *p, evp1d_handle 
   return EVP1D_OK;
 }
 
+int evp1d_create(int32_t na, int32_t navel, const evp1d_params *p, evp1d_handle **out) {
+  return create_impl(1, &na, &navel, p, out);
+}
+
+int evp1d_create_batch(int32_t nmem, const int32_t *na, const int32_t *navel, const evp1d_params *p,
+                       evp1d_handle **out) {
+  return create_impl(nmem, na, navel, p, out);
+}
+
+int evp1d_member_info(const evp1d_handle *h, int32_t k, int32_t *na, int32_t *navel, int32_t *active_offset,
+                      int32_t *ghost_offset) {
+  if (!h) return fail(EVP1D_EINVAL, "evp1d_member_info: NULL handle");
+  MemberSpan m;
+  if (k < 0 || !member_span(h, k, m)) return fail(EVP1D_EINVAL, "evp1d_member_info: no member %d", k);
+  if (na) *na = m.m_na;
+  if (navel) *navel = m.m_navel;
+  if (active_offset) *active_offset = m.a_off;
+  if (ghost_offset) *ghost_offset = m.g_off;
+  return EVP1D_OK;
+}
+
+int evp1d_get_members(const evp1d_handle *h, int32_t *nmem) {
+  if (!h || !nmem) return fail(EVP1D_EINVAL, "evp1d_get_members: NULL argument");
+  *nmem = (int32_t)h->m_na.size();
+  return EVP1D_OK;
+}
+
 int evp1d_destroy(evp1d_handle *h) {
   release(h);
   return EVP1D_OK;
@@ -655,79 +772,125 @@ int evp1d_get_sizes(const evp1d_handle *h, int32_t *na, int32_t *navel) {
   return EVP1D_OK;
 }
 
-int evp1d_set_field(evp1d_handle *h, int32_t field, const double *src, int32_t n) {
+static int set_field_impl(evp1d_handle *h, int k, int32_t field, const double *src, int32_t n) {
   if (!h) return fail(EVP1D_EINVAL, "evp1d_set_field: NULL handle");
   FieldRef r;
+  MemberSpan m;
   if (!field_ref(h, field, r)) return fail(EVP1D_EINVAL, "evp1d_set_field: unknown field %d", field);
-  if (n != r.len) return fail(EVP1D_EINVAL, "evp1d_set_field: field %d has length %d, got %d", field, r.len, n);
+  if (!member_span(h, k, m)) return fail(EVP1D_EINVAL, "evp1d_set_field: no member %d", k);
+  const int len = r.len == h->navel && r.kind == FK_PAIR ? m.m_navel : m.m_na;
+  if (n != len) return fail(EVP1D_EINVAL, "evp1d_set_field: field %d has length %d, got %d", field, len, n);
   if (n == 0) return EVP1D_OK;
   if (!src) return fail(EVP1D_EINVAL, "evp1d_set_field: src is NULL");
   RC_TRY(ensure(h));
+  const Span sp{m.a_off, m.m_na, m.g_off, n};
   HIP_TRY(hipMemcpyAsync(h->scratch, src, (size_t)n * sizeof(double), hipMemcpyHostToDevice, h->stream));
   if (r.kind == FK_PLAIN)
     hipLaunchKernelGGL(k_scatter_plain, dim3(blocks_for(n)), dim3(kBlock), 0, h->stream, r.plain,
-                       (const double *)h->scratch, (const int32_t *)h->inv_dev, h->na, n);
+                       (const double *)h->scratch, (const int32_t *)h->inv_dev, h->na, sp);
   else
     hipLaunchKernelGGL(k_scatter_comp, dim3(blocks_for(n)), dim3(kBlock), 0, h->stream, r.pair,
-                       (const double *)h->scratch, (const int32_t *)h->inv_dev, h->na, n, r.comp);
+                       (const double *)h->scratch, (const int32_t *)h->inv_dev, h->na, sp, r.comp);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(h->stream));
   return EVP1D_OK;
 }
 
-int evp1d_get_field(evp1d_handle *h, int32_t field, double *dst, int32_t n) {
+static int get_field_impl(evp1d_handle *h, int k, int32_t field, double *dst, int32_t n) {
   if (!h) return fail(EVP1D_EINVAL, "evp1d_get_field: NULL handle");
   FieldRef r;
+  MemberSpan m;
   if (!field_ref(h, field, r)) return fail(EVP1D_EINVAL, "evp1d_get_field: unknown field %d", field);
-  if (n != r.len) return fail(EVP1D_EINVAL, "evp1d_get_field: field %d has length %d, got %d", field, r.len, n);
+  if (!member_span(h, k, m)) return fail(EVP1D_EINVAL, "evp1d_get_field: no member %d", k);
+  const int len = r.len == h->navel && r.kind == FK_PAIR ? m.m_navel : m.m_na;
+  if (n != len) return fail(EVP1D_EINVAL, "evp1d_get_field: field %d has length %d, got %d", field, len, n);
   if (n == 0) return EVP1D_OK;
   if (!dst) return fail(EVP1D_EINVAL, "evp1d_get_field: dst is NULL");
   RC_TRY(ensure(h));
+  const Span sp{m.a_off, m.m_na, m.g_off, n};
   if (r.kind == FK_PLAIN)
     hipLaunchKernelGGL(k_gather_plain, dim3(blocks_for(n)), dim3(kBlock), 0, h->stream, h->scratch,
-                       (const double *)r.plain, (const int32_t *)h->inv_dev, h->na, n);
+                       (const double *)r.plain, (const int32_t *)h->inv_dev, h->na, sp);
   else
     hipLaunchKernelGGL(k_gather_comp, dim3(blocks_for(n)), dim3(kBlock), 0, h->stream, h->scratch,
-                       (const double2 *)r.pair, (const int32_t *)h->inv_dev, h->na, n, r.comp);
+                       (const double2 *)r.pair, (const int32_t *)h->inv_dev, h->na, sp, r.comp);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(dst, h->scratch, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
   return EVP1D_OK;
 }
 
-int evp1d_set_index(evp1d_handle *h, int32_t which, const int32_t *idx, int32_t n) {
+static int set_index_impl(evp1d_handle *h, int k, int32_t which, const int32_t *idx, int32_t n) {
   if (!h) return fail(EVP1D_EINVAL, "evp1d_set_index: NULL handle");
+  MemberSpan m;
+  if (!member_span(h, k, m)) return fail(EVP1D_EINVAL, "evp1d_set_index: no member %d", k);
   if (which < EVP1D_EE || which > EVP1D_SSE) return fail(EVP1D_EINVAL, "evp1d_set_index: unknown list %d", which);
-  if (n != h->na) return fail(EVP1D_EINVAL, "evp1d_set_index: length %d != na %d", n, h->na);
+  if (n != m.m_na) return fail(EVP1D_EINVAL, "evp1d_set_index: length %d != na %d", n, m.m_na);
   if (n > 0 && !idx) return fail(EVP1D_EINVAL, "evp1d_set_index: idx is NULL");
-  std::vector<int32_t> z((size_t)n);
+  std::vector<int32_t> &dst = h->idx_host[which];
+  if (dst.size() != (size_t)h->na) dst.assign((size_t)h->na, 0);
   for (int i = 0; i < n; ++i) {
     const int32_t q = idx[i];
-    if (q < 1 || q > h->navel)
-      return fail(EVP1D_EINDEX, "evp1d_set_index: list %d entry %d = %d outside 1..%d", which, i + 1, q, h->navel);
-    z[(size_t)i] = q - 1;
+    if (q < 1 || q > m.m_navel)
+      return fail(EVP1D_EINDEX, "evp1d_set_index: list %d entry %d = %d outside 1..%d", which, i + 1, q, m.m_navel);
   }
-  h->idx_host[which].swap(z);
-  h->idx_set[which] = true;
+  for (int i = 0; i < n; ++i) {
+    const int32_t q = idx[i];   // member-local, 1-based
+    dst[(size_t)m.a_off + i] = q <= m.m_na ? m.a_off + q - 1 : m.g_off + (q - m.m_na - 1);
+  }
+  std::vector<uint8_t> &ms = h->idx_mset[which];
+  if (k < 0) std::fill(ms.begin(), ms.end(), (uint8_t)1); else ms[(size_t)k] = 1;
+  h->idx_set[which] = std::all_of(ms.begin(), ms.end(), [](uint8_t b) { return b != 0; });
   h->idx_dirty = true;
   return EVP1D_OK;
 }
 
-int evp1d_set_skip(evp1d_handle *h, int32_t which, const int32_t *flags, int32_t n) {
+static int set_skip_impl(evp1d_handle *h, int k, int32_t which, const int32_t *flags, int32_t n) {
   if (!h) return fail(EVP1D_EINVAL, "evp1d_set_skip: NULL handle");
+  MemberSpan m;
+  if (!member_span(h, k, m)) return fail(EVP1D_EINVAL, "evp1d_set_skip: no member %d", k);
   if (which != EVP1D_SKIPT && which != EVP1D_SKIPU) return fail(EVP1D_EINVAL, "evp1d_set_skip: unknown flag %d", which);
-  if (n != h->na) return fail(EVP1D_EINVAL, "evp1d_set_skip: length %d != na %d", n, h->na);
+  if (n != m.m_na) return fail(EVP1D_EINVAL, "evp1d_set_skip: length %d != na %d", n, m.m_na);
   if (n > 0 && !flags) return fail(EVP1D_EINVAL, "evp1d_set_skip: flags is NULL");
   const uint8_t bit = (uint8_t)(1u << which);
   bool changed = false;
   for (int i = 0; i < n; ++i) {
-    uint8_t &b = h->skip_host[(size_t)i];
+    uint8_t &b = h->skip_host[(size_t)m.a_off + i];
     const uint8_t nb = (uint8_t)(flags[i] ? (b | bit) : (b & ~bit));
     changed |= nb != b;
     b = nb;
   }
   if (changed) h->layout_dirty = true;
   return EVP1D_OK;
+}
+
+int evp1d_set_field(evp1d_handle *h, int32_t field, const double *src, int32_t n) {
+  return set_field_impl(h, -1, field, src, n);
+}
+int evp1d_get_field(evp1d_handle *h, int32_t field, double *dst, int32_t n) {
+  return get_field_impl(h, -1, field, dst, n);
+}
+int evp1d_set_index(evp1d_handle *h, int32_t which, const int32_t *idx, int32_t n) {
+  return set_index_impl(h, -1, which, idx, n);
+}
+int evp1d_set_skip(evp1d_handle *h, int32_t which, const int32_t *flags, int32_t n) {
+  return set_skip_impl(h, -1, which, flags, n);
+}
+int evp1d_set_field_member(evp1d_handle *h, int32_t k, int32_t field, const double *src, int32_t n) {
+  if (k < 0) return fail(EVP1D_EINVAL, "evp1d_set_field_member: member %d < 0", k);
+  return set_field_impl(h, k, field, src, n);
+}
+int evp1d_get_field_member(evp1d_handle *h, int32_t k, int32_t field, double *dst, int32_t n) {
+  if (k < 0) return fail(EVP1D_EINVAL, "evp1d_get_field_member: member %d < 0", k);
+  return get_field_impl(h, k, field, dst, n);
+}
+int evp1d_set_index_member(evp1d_handle *h, int32_t k, int32_t which, const int32_t *idx, int32_t n) {
+  if (k < 0) return fail(EVP1D_EINVAL, "evp1d_set_index_member: member %d < 0", k);
+  return set_index_impl(h, k, which, idx, n);
+}
+int evp1d_set_skip_member(evp1d_handle *h, int32_t k, int32_t which, const int32_t *flags, int32_t n) {
+  if (k < 0) return fail(EVP1D_EINVAL, "evp1d_set_skip_member: member %d < 0", k);
+  return set_skip_impl(h, k, which, flags, n);
 }
 
 int evp1d_set_halo(evp1d_handle *h, int32_t n, const int32_t *dst, const int32_t *src) {
@@ -816,6 +979,95 @@ int evp1d_run_individual(evp1d_handle *h, int32_t which, int32_t nsub) {
   RC_TRY(ensure(h));
   for (int k = 0; k < nsub; ++k) RC_TRY(which == 0 ? launch_stress(h) : launch_stepu(h, k == nsub - 1));
   return EVP1D_OK;
+}
+
+static int block_index(int32_t block) {
+  for (int i = 0; i < kNumBlocks; ++i)
+    if (kBlocks[i] == block) return i;
+  return -1;
+}
+
+int evp1d_set_tuning(evp1d_handle *h, int32_t stress_block, int32_t stress_xcd, int32_t stepu_block,
+                     int32_t stepu_xcd) {
+  if (!h) return fail(EVP1D_EINVAL, "evp1d_set_tuning: NULL handle");
+  const int a = block_index(stress_block), b = block_index(stepu_block);
+  if (a < 0 || b < 0) return fail(EVP1D_EINVAL, "evp1d_set_tuning: block sizes must be 64, 128, 256, 512 or 1024");
+  h->cfg_stress = LaunchCfg{a, stress_xcd ? 1 : 0};
+  h->cfg_stepu = LaunchCfg{b, stepu_xcd ? 1 : 0};
+  h->graph_dirty = true;
+  return EVP1D_OK;
+}
+
+int evp1d_get_tuning(const evp1d_handle *h, int32_t *stress_block, int32_t *stress_xcd, int32_t *stepu_block,
+                     int32_t *stepu_xcd) {
+  if (!h) return fail(EVP1D_EINVAL, "evp1d_get_tuning: NULL handle");
+  if (stress_block) *stress_block = kBlocks[h->cfg_stress.bi];
+  if (stress_xcd) *stress_xcd = h->cfg_stress.xcd;
+  if (stepu_block) *stepu_block = kBlocks[h->cfg_stepu.bi];
+  if (stepu_xcd) *stepu_xcd = h->cfg_stepu.xcd;
+  return EVP1D_OK;
+}
+
+// Time every launch configuration of k_stress and k_stepu on this handle's
+// own data (reps launches each, HIP events) and keep the fastest.  The state
+// the trial launches change is saved first and restored afterwards, so the
+// handle's results are unaffected.
+int evp1d_autotune(evp1d_handle *h, int32_t reps, float *stress_us, float *stepu_us) {
+  if (!h) return fail(EVP1D_EINVAL, "evp1d_autotune: NULL handle");
+  if (!ready(h)) return fail(EVP1D_ESTATE, "evp1d_autotune: neighbour lists not set");
+  if (reps < 1) return fail(EVP1D_EINVAL, "evp1d_autotune: reps=%d < 1", reps);
+  RC_TRY(ensure(h));
+  struct Saved { void *dev; void *bak; size_t bytes; };
+  std::vector<Saved> saved;
+  auto save = [&](void *p, size_t bytes) -> int {
+    if (bytes == 0) return EVP1D_OK;
+    void *b = nullptr;
+    const hipError_t e = hipMalloc(&b, bytes);
+    if (e != hipSuccess) return fail(EVP1D_ENOMEM, "evp1d_autotune: hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    saved.push_back(Saved{p, b, bytes});
+    HIP_TRY(hipMemcpyAsync(b, p, bytes, hipMemcpyDeviceToDevice, h->stream));
+    return EVP1D_OK;
+  };
+  const size_t nV = (size_t)h->navel * sizeof(double2), nA = (size_t)h->na * sizeof(double);
+  int rc = save(h->v.sig, (size_t)12 * h->ldsig * sizeof(double));
+  void *pairs[5] = {h->v.vel, h->v.s15, h->v.s27, h->v.s36, h->v.s48};
+  for (void *q : pairs) if (!rc) rc = save(q, nV);
+  void *diag[3] = {h->v.strintx, h->v.strinty, h->v.cb};
+  for (void *q : diag) if (!rc) rc = save(q, nA);
+  float best_s = 1e30f, best_u = 1e30f;
+  const LaunchCfg keep_s = h->cfg_stress, keep_u = h->cfg_stepu;
+  LaunchCfg win_s = keep_s, win_u = keep_u;
+  for (int which = 0; which < 2 && !rc; ++which) {
+    for (int bi = 0; bi < kNumBlocks && !rc; ++bi) {
+      for (int x = 0; x < 2 && !rc; ++x) {
+        (which == 0 ? h->cfg_stress : h->cfg_stepu) = LaunchCfg{bi, x};
+        rc = which == 0 ? launch_stress(h) : launch_stepu(h, true);    // warm
+        if (!rc && hipEventRecord(h->ev0, h->stream) != hipSuccess) rc = fail(EVP1D_EHIP, "hipEventRecord");
+        for (int r = 0; r < reps && !rc; ++r) rc = which == 0 ? launch_stress(h) : launch_stepu(h, true);
+        float ms = 0.f;
+        if (!rc && (hipEventRecord(h->ev1, h->stream) != hipSuccess || hipEventSynchronize(h->ev1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, h->ev0, h->ev1) != hipSuccess))
+          rc = fail(EVP1D_EHIP, "evp1d_autotune: event timing failed");
+        const float us = ms * 1000.f / (float)reps;
+        if (!rc && which == 0 && us < best_s) { best_s = us; win_s = h->cfg_stress; }
+        if (!rc && which == 1 && us < best_u) { best_u = us; win_u = h->cfg_stepu; }
+      }
+    }
+  }
+  for (const Saved &sv : saved) {
+    if (hipMemcpyAsync(sv.dev, sv.bak, sv.bytes, hipMemcpyDeviceToDevice, h->stream) != hipSuccess && !rc)
+      rc = fail(EVP1D_EHIP, "evp1d_autotune: restore failed");
+  }
+  if (hipStreamSynchronize(h->stream) != hipSuccess && !rc) rc = fail(EVP1D_EHIP, "evp1d_autotune: sync failed");
+  for (const Saved &sv : saved) (void)hipFree(sv.bak);
+  h->cfg_stress = rc ? keep_s : win_s;
+  h->cfg_stepu = rc ? keep_u : win_u;
+  h->graph_dirty = true;
+  if (!rc) {
+    if (stress_us) *stress_us = best_s;
+    if (stepu_us) *stepu_us = best_u;
+  }
+  return rc;
 }
 
 int evp1d_synchronize(evp1d_handle *h) {

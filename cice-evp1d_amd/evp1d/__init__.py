@@ -51,6 +51,9 @@ EXPORTS = (
     "evp1d_halo", "evp1d_run", "evp1d_set_mode", "evp1d_get_mode", "evp1d_run_individual",
     "evp1d_synchronize", "evp1d_timer_start", "evp1d_timer_stop", "evp1d_index_count",
     "evp1d_index_build", "evp1d_last_error", "evp1d_build_info",
+    "evp1d_create_batch", "evp1d_get_members", "evp1d_member_info", "evp1d_set_field_member",
+    "evp1d_get_field_member", "evp1d_set_index_member", "evp1d_set_skip_member",
+    "evp1d_autotune", "evp1d_set_tuning", "evp1d_get_tuning",
 )
 
 
@@ -120,6 +123,16 @@ def load(fma: bool = False, path: str | os.PathLike | None = None) -> ctypes.CDL
         "evp1d_index_build": [i32, i32, i32, _IP, i32, i32] + [_IP] * 9,
         "evp1d_last_error": [ctypes.c_char_p, i32],
         "evp1d_build_info": [_IP, _IP],
+        "evp1d_create_batch": [i32, _IP, _IP, ctypes.POINTER(Params), ctypes.POINTER(_H)],
+        "evp1d_get_members": [_H, _IP],
+        "evp1d_member_info": [_H, i32, _IP, _IP, _IP, _IP],
+        "evp1d_set_field_member": [_H, i32, i32, _DP, i32],
+        "evp1d_get_field_member": [_H, i32, i32, _DP, i32],
+        "evp1d_set_index_member": [_H, i32, i32, _IP, i32],
+        "evp1d_set_skip_member": [_H, i32, i32, _IP, i32],
+        "evp1d_autotune": [_H, i32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
+        "evp1d_set_tuning": [_H, i32, i32, i32, i32],
+        "evp1d_get_tuning": [_H, _IP, _IP, _IP, _IP],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -187,13 +200,74 @@ def build_index(tmask: np.ndarray, nghost: int = 1, fma: bool = False) -> dict:
 class Engine:
     """One evp1d handle (device-resident 1D state of one domain or batch)."""
 
-    def __init__(self, na: int, navel: int, params: Params, fma: bool = False):
+    def __init__(self, na: int, navel: int, params: Params, fma: bool = False, members=None):
+        """members: optional list of (na_k, navel_k) -> evp1d_create_batch."""
         self.L = load(fma)
-        self.na, self.navel = int(na), int(navel)
         self.fma = fma
         h = _H()
-        _check(self.L, self.L.evp1d_create(self.na, self.navel, ctypes.byref(params), ctypes.byref(h)))
+        if members is None:
+            self.na, self.navel = int(na), int(navel)
+            _check(self.L, self.L.evp1d_create(self.na, self.navel, ctypes.byref(params), ctypes.byref(h)))
+            self.members = [(self.na, self.navel)]
+        else:
+            mna = np.ascontiguousarray([m[0] for m in members], dtype=np.int32)
+            mnv = np.ascontiguousarray([m[1] for m in members], dtype=np.int32)
+            self.na, self.navel = int(mna.sum()), int(mnv.sum())
+            _check(self.L, self.L.evp1d_create_batch(mna.size, _ip(mna), _ip(mnv), ctypes.byref(params),
+                                                     ctypes.byref(h)))
+            self.members = [(int(a), int(b)) for a, b in zip(mna, mnv)]
         self.h = h
+
+    @classmethod
+    def batch(cls, members, params: Params, fma: bool = False) -> "Engine":
+        return cls(0, 0, params, fma=fma, members=members)
+
+    def member_info(self, k: int) -> dict:
+        a = [ctypes.c_int32() for _ in range(4)]
+        _check(self.L, self.L.evp1d_member_info(self.h, k, *[ctypes.byref(x) for x in a]))
+        return dict(zip(("na", "navel", "active_offset", "ghost_offset"), (x.value for x in a)))
+
+    def set_field_member(self, k: int, name: str, a) -> None:
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        _check(self.L, self.L.evp1d_set_field_member(self.h, k, FIELD_ID[name], _dp(a), a.size))
+
+    def get_field_member(self, k: int, name: str) -> np.ndarray:
+        na_k, navel_k = self.members[k]
+        out = np.empty(navel_k if name in NAVEL_FIELDS else na_k)
+        _check(self.L, self.L.evp1d_get_field_member(self.h, k, FIELD_ID[name], _dp(out), out.size))
+        return out
+
+    def set_index_member(self, k: int, name: str, idx) -> None:
+        a = np.ascontiguousarray(idx, dtype=np.int32)
+        _check(self.L, self.L.evp1d_set_index_member(self.h, k, NBR.index(name), _ip(a), a.size))
+
+    def set_skip_member(self, k: int, which: str, flags) -> None:
+        a = np.ascontiguousarray(flags, dtype=np.int32)
+        _check(self.L, self.L.evp1d_set_skip_member(self.h, k, {"skipT": 0, "skipU": 1}[which], _ip(a), a.size))
+
+    def load_member(self, k: int, fields: dict, idx: dict, skipT, skipU) -> None:
+        for n in NBR:
+            self.set_index_member(k, n, idx[n])
+        self.set_skip_member(k, "skipT", skipT)
+        self.set_skip_member(k, "skipU", skipU)
+        for n in FIELDS:
+            if n in fields:
+                self.set_field_member(k, n, fields[n])
+
+    def autotune(self, reps: int = 20) -> dict:
+        a, b = ctypes.c_float(), ctypes.c_float()
+        _check(self.L, self.L.evp1d_autotune(self.h, reps, ctypes.byref(a), ctypes.byref(b)))
+        out = self.get_tuning()
+        out.update(stress_us=a.value, stepu_us=b.value)
+        return out
+
+    def set_tuning(self, stress_block=256, stress_xcd=0, stepu_block=256, stepu_xcd=0) -> None:
+        _check(self.L, self.L.evp1d_set_tuning(self.h, stress_block, stress_xcd, stepu_block, stepu_xcd))
+
+    def get_tuning(self) -> dict:
+        a = [ctypes.c_int32() for _ in range(4)]
+        _check(self.L, self.L.evp1d_get_tuning(self.h, *[ctypes.byref(x) for x in a]))
+        return dict(zip(("stress_block", "stress_xcd", "stepu_block", "stepu_xcd"), (x.value for x in a)))
 
     # lifetime ------------------------------------------------------------
     def close(self) -> None:

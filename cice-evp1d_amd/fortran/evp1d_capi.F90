@@ -49,7 +49,10 @@ module evp1d_capi
             evp1d_halo, evp1d_run, evp1d_set_mode, evp1d_get_mode,                      &
             evp1d_run_individual, evp1d_synchronize, evp1d_timer_start,                 &
             evp1d_timer_stop, evp1d_index_count, evp1d_index_build, evp1d_last_error,   &
-            evp1d_build_info, evp1d_check
+            evp1d_build_info, evp1d_check, evp1d_create_batch, evp1d_get_members,       &
+            evp1d_member_info, evp1d_set_field_member, evp1d_get_field_member,          &
+            evp1d_set_index_member, evp1d_set_skip_member, evp1d_autotune,              &
+            evp1d_set_tuning, evp1d_get_tuning
 
   interface
     integer(c_int) function evp1d_set_evp_parameters(ndte, dt, p) bind(C, name='evp1d_set_evp_parameters')
@@ -180,6 +183,64 @@ module evp1d_capi
     integer(c_int) function evp1d_build_info(abi, fma) bind(C, name='evp1d_build_info')
       import :: c_int, c_int32_t
       integer(c_int32_t), intent(out) :: abi, fma
+    end function
+    integer(c_int) function evp1d_create_batch(nmem, na, navel, p, h) bind(C, name='evp1d_create_batch')
+      import :: c_int, c_int32_t, c_ptr, evp1d_params_t
+      integer(c_int32_t), value :: nmem
+      integer(c_int32_t), intent(in) :: na(*), navel(*)
+      type(evp1d_params_t), intent(in) :: p
+      type(c_ptr), intent(out) :: h
+    end function
+    integer(c_int) function evp1d_get_members(h, nmem) bind(C, name='evp1d_get_members')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(out) :: nmem
+    end function
+    integer(c_int) function evp1d_member_info(h, k, na, navel, aoff, goff) bind(C, name='evp1d_member_info')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: k
+      integer(c_int32_t), intent(out) :: na, navel, aoff, goff
+    end function
+    integer(c_int) function evp1d_set_field_member(h, k, field, src, n) bind(C, name='evp1d_set_field_member')
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: k, field, n
+      real(c_double), intent(in) :: src(*)
+    end function
+    integer(c_int) function evp1d_get_field_member(h, k, field, dst, n) bind(C, name='evp1d_get_field_member')
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: k, field, n
+      real(c_double), intent(out) :: dst(*)
+    end function
+    integer(c_int) function evp1d_set_index_member(h, k, which, idx, n) bind(C, name='evp1d_set_index_member')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: k, which, n
+      integer(c_int32_t), intent(in) :: idx(*)
+    end function
+    integer(c_int) function evp1d_set_skip_member(h, k, which, flags, n) bind(C, name='evp1d_set_skip_member')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: k, which, n
+      integer(c_int32_t), intent(in) :: flags(*)
+    end function
+    integer(c_int) function evp1d_autotune(h, reps, stress_us, stepu_us) bind(C, name='evp1d_autotune')
+      import :: c_int, c_int32_t, c_float, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: reps
+      real(c_float), intent(out) :: stress_us, stepu_us
+    end function
+    integer(c_int) function evp1d_set_tuning(h, sb, sx, ub, ux) bind(C, name='evp1d_set_tuning')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: sb, sx, ub, ux
+    end function
+    integer(c_int) function evp1d_get_tuning(h, sb, sx, ub, ux) bind(C, name='evp1d_get_tuning')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(out) :: sb, sx, ub, ux
     end function
   end interface
 

@@ -151,6 +151,41 @@ int evp1d_get_mode(const evp1d_handle *h, int32_t *mode);
 int evp1d_run_individual(evp1d_handle *h, int32_t which, int32_t nsub);
 int evp1d_synchronize(evp1d_handle *h);
 
+/* ---- ensembles: many domains in one handle ------------------------------
+ * evp1d_create_batch concatenates nmem independent 1D lists (member k with
+ * na[k] points and navel[k] slots) into one handle whose stress/stepu
+ * launches cover every member at once.  Member k's active points occupy
+ * caller positions a_off[k] .. a_off[k]+na[k]-1 (a_off = prefix sums of na),
+ * its ghost slots follow all active points at positions ghost_off[k] ..
+ * (ghost_off = sum(na) + prefix sums of navel-na), so the whole-handle calls
+ * above see one list of sum(na) points and sum(navel) slots.  The *_member
+ * calls take member-local arrays and 1-based member-local neighbour
+ * positions (1..navel[k]) and apply the offsets; members never reference
+ * each other's slots.  All members share the handle's scalar parameters.
+ * This is the weak-scaling device of src/vars.F90:1794-2053 (replicate),
+ * with the offsets applied consistently to active and ghost slots.         */
+int evp1d_create_batch(int32_t nmem, const int32_t *na, const int32_t *navel, const evp1d_params *p,
+                       evp1d_handle **out);
+int evp1d_get_members(const evp1d_handle *h, int32_t *nmem);
+int evp1d_member_info(const evp1d_handle *h, int32_t k, int32_t *na, int32_t *navel,
+                      int32_t *active_offset, int32_t *ghost_offset);
+int evp1d_set_field_member(evp1d_handle *h, int32_t k, int32_t field, const double *src, int32_t n);
+int evp1d_get_field_member(evp1d_handle *h, int32_t k, int32_t field, double *dst, int32_t n);
+int evp1d_set_index_member(evp1d_handle *h, int32_t k, int32_t which, const int32_t *idx, int32_t n);
+int evp1d_set_skip_member(evp1d_handle *h, int32_t k, int32_t which, const int32_t *flags, int32_t n);
+
+/* ---- launch tuning (no counterpart in the reference) ---------------------
+ * Block size (64, 128, 256, 512 or 1024 threads) and XCD-aware tile order of
+ * the stress and stepu launches.  evp1d_autotune times every configuration
+ * on the handle's own data (reps launches each; the state is saved and
+ * restored) and keeps the fastest; it reports their average durations in
+ * microseconds.  Results never depend on the tuning.                        */
+int evp1d_autotune(evp1d_handle *h, int32_t reps, float *stress_us, float *stepu_us);
+int evp1d_set_tuning(evp1d_handle *h, int32_t stress_block, int32_t stress_xcd, int32_t stepu_block,
+                     int32_t stepu_xcd);
+int evp1d_get_tuning(const evp1d_handle *h, int32_t *stress_block, int32_t *stress_xcd,
+                     int32_t *stepu_block, int32_t *stepu_xcd);
+
 /* Device timing on the handle's stream (HIP events): start, then stop
  * returns the elapsed milliseconds between the two after synchronising.     */
 int evp1d_timer_start(evp1d_handle *h);

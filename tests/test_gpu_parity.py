@@ -201,3 +201,51 @@ def test_fields_roundtrip_through_layout(gpu):
             eng.set_field(name, a)
             eng.set_skip("skipT", (rng.random(st.na) < 0.5).astype(np.int32))
             assert np.array_equal(eng.get_field(name), a), name
+
+
+def test_batch_members_match_independent_oracle_runs(gpu):
+    """evp1d_create_batch: uneven members in one handle; each member's result
+    is bit-identical to its own oracle run (member-local inputs and indices)."""
+    sizes = [(30, 22, 81), (64, 40, 82), (12, 9, 83), (50, 70, 84), (5, 4, 85)]
+    states = [synthetic_state(nx, ny, seed, force_perturb=0.02) for nx, ny, seed in sizes]
+    ndte = 17
+    lib, p = oracle_params(states[0], ndte)
+    wants = []
+    for st in states:
+        w = st.copy()
+        lib.run(p, w, ndte)
+        wants.append(w)
+    params = evp1d.set_evp_parameters(ndte)
+    for mode in MODES:
+        with evp1d.Engine.batch([(st.na, st.navel) for st in states], params) as eng:
+            eng.set_mode(mode)
+            for k, st in enumerate(states):
+                eng.load_member(k, st.f, st.idx, st.skipT, st.skipU)
+            info = eng.member_info(3)
+            assert info["active_offset"] == sum(s.na for s in states[:3])
+            assert info["ghost_offset"] == eng.na + sum(s.navel - s.na for s in states[:3])
+            eng.run(ndte)
+            for k, st in enumerate(states):
+                got = st.copy()
+                for n in got.f:
+                    got.f[n][:] = eng.get_field_member(k, n)
+                assert_state_bitexact(got, wants[k])
+
+
+def test_autotune_keeps_results_and_state(gpu):
+    st = synthetic_state(120, 90, seed=91)
+    lib, p = oracle_params(st, 12)
+    want = st.copy()
+    lib.run(p, want, 12)
+    with make_engine(st, 12) as eng:
+        eng.run(5)
+        t = eng.autotune(reps=3)
+        assert t["stress_block"] in (64, 128, 256, 512, 1024) and t["stress_us"] > 0
+        eng.run(7)
+        got = fetch_state(eng, st)
+    assert_state_bitexact(got, want)
+    for cfg in [(64, 1, 1024, 0), (512, 0, 128, 1), (1024, 1, 64, 1)]:
+        with make_engine(st, 12) as eng:
+            eng.set_tuning(*cfg)
+            eng.run(12)
+            assert_state_bitexact(fetch_state(eng, st), want)
