@@ -125,6 +125,10 @@ def main() -> int:
     ap.add_argument("--kernel-reps", type=int, default=200, help="launches per individual-kernel timing")
     ap.add_argument("--cpu-subcycles", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--autotune", type=int, default=0, metavar="REPS",
+                    help="run evp1d_autotune with REPS launches per configuration before timing")
+    ap.add_argument("--tuning", default=None, metavar="SB,SX,UB,UX",
+                    help="fixed launch tuning: stress block, stress xcd order, stepu block, stepu xcd order")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB 20 times (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
@@ -143,6 +147,12 @@ def main() -> int:
     eng = evp1d.Engine(na, s["navel"], p, fma=args.fma)
     eng.set_mode(args.mode)
     eng.load_state(s["fields"], s["idx"], s["skipT"], s["skipU"])
+    if args.tuning:
+        eng.set_tuning(*[int(x) for x in args.tuning.split(",")])
+    if args.autotune:
+        t = eng.autotune(args.autotune)
+        log(f"[bench] autotune: {t}")
+    tuning = eng.get_tuning()
 
     for k in range(args.warmup):
         eng.run(ndte)
@@ -206,7 +216,7 @@ def main() -> int:
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{args.workload} {w['nx']}x{w['ny']} ndte={ndte}", "nx": w["nx"],
                    "ny": w["ny"], "seed": w["seed"], "na": na, "navel": s["navel"], "nT": nT, "nU": nU,
-                   "ndte": ndte, "mode": args.mode, "build": "fma" if args.fma else "exact",
+                   "ndte": ndte, "mode": args.mode, "build": "fma" if args.fma else "exact", "tuning": tuning,
                    "us_per_subcycle": round(step_us, 3), "loop_GBps_algorithmic": round(whole, 1)},
         "kernels": kernels,
         "roofline": roofline,

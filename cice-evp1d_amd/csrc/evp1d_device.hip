@@ -39,6 +39,7 @@
 #include <vector>
 
 #include "../../include/evp1d.h"
+#include "evp1d_adjoint.hpp"
 #include "evp1d_point.hpp"
 
 #ifndef EVP1D_FMA
@@ -222,6 +223,194 @@ __global__ __launch_bounds__(kBlock) void k_persistent(DevView d, int nsub) {
       grid.sync();
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// adjoint kernels (reverse mode of one subcycle, see evp1d_adjoint.hpp)
+// ---------------------------------------------------------------------------
+struct AdjView {
+  double2 *ub;            // [navel] adjoint of (uvel, vvel)
+  double *sigb;           // [12][ldsig] adjoint of the stresses
+  double2 *g;             // [navel] uarear * adjoint of (cc1, cc2); zero off the stepu range
+  double2 *vc;            // [4][na] stress-adjoint velocity contributions: cc, ee, se, ne
+  double *gstr, *gcw, *gfx, *gfy;   // [na] gradient accumulators
+};
+
+// str1..8 of subcycle k from the state before it (stresses are not stored)
+__global__ __launch_bounds__(256) void k_str_recompute(DevView d) {
+  const int iw = blockIdx.x * 256 + threadIdx.x;
+  if (iw >= d.t_end) return;
+  const double2 cc = d.vel[iw];
+  const double2 ve = d.vel[d.ee[iw]];
+  const double2 vn = d.vel[d.ne[iw]];
+  const double2 vs = d.vel[d.se[iw]];
+  double sig[12];
+#pragma unroll
+  for (int c = 0; c < 12; ++c) sig[c] = d.sig[c * d.ldsig + iw];
+  evp1d::StressOut o;
+  evp1d::stress_point(d.s, cc.x, cc.y, ve.x, ve.y, vs.x, vs.y, vn.x, vn.y, d.dxT[iw], d.dyT[iw],
+                      d.strength[iw], d.hte[iw], d.htn[iw], d.htem1[iw], d.htnm1[iw], sig, o);
+  d.s15[iw] = make_double2(o.str[0], o.str[4]);
+  d.s27[iw] = make_double2(o.str[1], o.str[6]);
+  d.s36[iw] = make_double2(o.str[2], o.str[5]);
+  d.s48[iw] = make_double2(o.str[3], o.str[7]);
+}
+
+__global__ __launch_bounds__(256) void k_stepu_adj(DevView d, AdjView a) {
+  const int iw = d.u_begin + blockIdx.x * 256 + threadIdx.x;
+  if (iw >= d.u_end) return;
+  const double2 v = d.vel[iw];
+  const double2 s15 = d.s15[iw];
+  const double2 s27 = d.s27[d.nw[iw]];
+  const double2 s36 = d.s36[d.sse[iw]];
+  const double2 s48 = d.s48[d.sw[iw]];
+  const double2 ub = a.ub[iw];
+  evp1d::StepuAdj o;
+  evp1d::stepu_adjoint(d.s, v.x, v.y, d.cw[iw], d.aiu[iw], d.uocn[iw], d.vocn[iw], d.waterx[iw],
+                       d.watery[iw], d.forcex[iw], d.forcey[iw], d.umassdti[iw], d.fm[iw], d.uarear[iw],
+                       d.uvel_init[iw], d.vvel_init[iw], d.tbu[iw], s15.x, s27.x, s36.x, s48.x,
+                       s15.y, s36.y, s27.y, s48.y, ub.x, ub.y, o);
+  a.ub[iw] = make_double2(o.uold_b, o.vold_b);
+  a.g[iw] = make_double2(o.gx, o.gy);
+  a.gfx[iw] += o.fx_b;
+  a.gfy[iw] += o.fy_b;
+  a.gcw[iw] += o.cw_b;
+}
+
+// str_b of point q: own terms and the terms stepu took from q as the nw, sse
+// and sw neighbour of the points ee(q), se(q) and ne(q) (the inverse maps).
+__global__ __launch_bounds__(256) void k_stress_adj(DevView d, AdjView a) {
+  const int iw = blockIdx.x * 256 + threadIdx.x;
+  if (iw >= d.t_end) return;
+  const int ie = d.ee[iw], in = d.ne[iw], is = d.se[iw];
+  const double2 cc = d.vel[iw], ve = d.vel[ie], vn = d.vel[in], vs = d.vel[is];
+  const double2 g0 = a.g[iw], ge = a.g[ie], gs = a.g[is], gn = a.g[in];
+  const double str_b[8] = {g0.x, ge.x, gs.x, gn.x, g0.y, gs.y, ge.y, gn.y};
+  double sb[12];
+#pragma unroll
+  for (int c = 0; c < 12; ++c) sb[c] = a.sigb[c * d.ldsig + iw];
+  evp1d::StressAdj o;
+  evp1d::stress_adjoint(d.s, cc.x, cc.y, ve.x, ve.y, vs.x, vs.y, vn.x, vn.y, d.dxT[iw], d.dyT[iw],
+                        d.strength[iw], d.hte[iw], d.htn[iw], d.htem1[iw], d.htnm1[iw], sb, str_b, o);
+#pragma unroll
+  for (int c = 0; c < 12; ++c) a.sigb[c * d.ldsig + iw] = o.sig_b[c];
+  const long na = d.na;
+  a.vc[iw] = make_double2(o.vel_b[0], o.vel_b[1]);
+  a.vc[na + iw] = make_double2(o.vel_b[2], o.vel_b[3]);
+  a.vc[2 * na + iw] = make_double2(o.vel_b[4], o.vel_b[5]);
+  a.vc[3 * na + iw] = make_double2(o.vel_b[6], o.vel_b[7]);
+  a.gstr[iw] += o.strength_b;
+}
+
+// u_b(q) += contributions of the stress adjoints that read q: its own (cc), and
+// those of nw(q), sw(q), sse(q), which read q as their ee, ne, se neighbour.
+__global__ __launch_bounds__(256) void k_vel_gather(DevView d, AdjView a) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= d.na) return;
+  const long na = d.na;
+  double2 acc = a.ub[q];
+  if (q < d.t_end) { const double2 t = a.vc[q]; acc.x += t.x; acc.y += t.y; }
+  const int pw = d.nw[q], ps = d.sw[q], pe = d.sse[q];
+  if (pw < d.t_end) { const double2 t = a.vc[na + pw]; acc.x += t.x; acc.y += t.y; }
+  if (ps < d.t_end) { const double2 t = a.vc[3 * na + ps]; acc.x += t.x; acc.y += t.y; }
+  if (pe < d.t_end) { const double2 t = a.vc[2 * na + pe]; acc.x += t.x; acc.y += t.y; }
+  a.ub[q] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// tangent-linear kernels: the forward point functions on Dual numbers
+// ---------------------------------------------------------------------------
+struct TanView {
+  double *dsig;                         // [12][ldsig]
+  double2 *dvel;                        // [navel]
+  double2 *ds15, *ds27, *ds36, *ds48;   // [navel]
+  const double *dstrength, *dcw, *dfx, *dfy;   // [na] direction
+};
+
+__device__ __forceinline__ evp1d::Dual D0(double v) { return evp1d::Dual{v, 0.0}; }
+__device__ __forceinline__ evp1d::Dual DD(double v, double d) { return evp1d::Dual{v, d}; }
+
+__global__ __launch_bounds__(256) void k_stress_tl(DevView d, TanView t) {
+  using evp1d::Dual;
+  const int iw = blockIdx.x * 256 + threadIdx.x;
+  if (iw >= d.t_end) return;
+  const int ie = d.ee[iw], in = d.ne[iw], is = d.se[iw];
+  const double2 cc = d.vel[iw], ve = d.vel[ie], vn = d.vel[in], vs = d.vel[is];
+  const double2 tc = t.dvel[iw], te = t.dvel[ie], tn = t.dvel[in], ts = t.dvel[is];
+  Dual sig[12];
+#pragma unroll
+  for (int c = 0; c < 12; ++c) sig[c] = DD(d.sig[c * d.ldsig + iw], t.dsig[c * d.ldsig + iw]);
+  evp1d::StressOutT<Dual> o;
+  evp1d::stress_point<Dual>(d.s, DD(cc.x, tc.x), DD(cc.y, tc.y), DD(ve.x, te.x), DD(ve.y, te.y),
+                            DD(vs.x, ts.x), DD(vs.y, ts.y), DD(vn.x, tn.x), DD(vn.y, tn.y),
+                            D0(d.dxT[iw]), D0(d.dyT[iw]), DD(d.strength[iw], t.dstrength[iw]),
+                            D0(d.hte[iw]), D0(d.htn[iw]), D0(d.htem1[iw]), D0(d.htnm1[iw]), sig, o);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    d.sig[c * d.ldsig + iw] = o.sp[c].v;       t.dsig[c * d.ldsig + iw] = o.sp[c].d;
+    d.sig[(4 + c) * d.ldsig + iw] = o.sm[c].v; t.dsig[(4 + c) * d.ldsig + iw] = o.sm[c].d;
+    d.sig[(8 + c) * d.ldsig + iw] = o.s12[c].v; t.dsig[(8 + c) * d.ldsig + iw] = o.s12[c].d;
+  }
+  d.s15[iw] = make_double2(o.str[0].v, o.str[4].v); t.ds15[iw] = make_double2(o.str[0].d, o.str[4].d);
+  d.s27[iw] = make_double2(o.str[1].v, o.str[6].v); t.ds27[iw] = make_double2(o.str[1].d, o.str[6].d);
+  d.s36[iw] = make_double2(o.str[2].v, o.str[5].v); t.ds36[iw] = make_double2(o.str[2].d, o.str[5].d);
+  d.s48[iw] = make_double2(o.str[3].v, o.str[7].v); t.ds48[iw] = make_double2(o.str[3].d, o.str[7].d);
+}
+
+__global__ __launch_bounds__(256) void k_stepu_tl(DevView d, TanView t, int write_diag) {
+  using evp1d::Dual;
+  const int iw = d.u_begin + blockIdx.x * 256 + threadIdx.x;
+  if (iw >= d.u_end) return;
+  const int jw = d.nw[iw], js = d.sse[iw], jsw = d.sw[iw];
+  const double2 v = d.vel[iw], tv = t.dvel[iw];
+  const double2 a = d.s15[iw], ta = t.ds15[iw];
+  const double2 b = d.s27[jw], tb = t.ds27[jw];
+  const double2 c = d.s36[js], tcs = t.ds36[js];
+  const double2 e = d.s48[jsw], te = t.ds48[jsw];
+  evp1d::StepuOutT<Dual> o;
+  evp1d::stepu_point<Dual>(d.s, DD(v.x, tv.x), DD(v.y, tv.y), DD(d.cw[iw], t.dcw[iw]), D0(d.aiu[iw]),
+                           D0(d.uocn[iw]), D0(d.vocn[iw]), D0(d.waterx[iw]), D0(d.watery[iw]),
+                           DD(d.forcex[iw], t.dfx[iw]), DD(d.forcey[iw], t.dfy[iw]), D0(d.umassdti[iw]),
+                           D0(d.fm[iw]), D0(d.uarear[iw]), D0(d.uvel_init[iw]), D0(d.vvel_init[iw]),
+                           D0(d.tbu[iw]), DD(a.x, ta.x), DD(b.x, tb.x), DD(c.x, tcs.x), DD(e.x, te.x),
+                           DD(a.y, ta.y), DD(c.y, tcs.y), DD(b.y, tb.y), DD(e.y, te.y), o);
+  d.vel[iw] = make_double2(o.uvel.v, o.vvel.v);
+  t.dvel[iw] = make_double2(o.uvel.d, o.vvel.d);
+  if (write_diag) {
+    d.strintx[iw] = o.strintx.v;
+    d.strinty[iw] = o.strinty.v;
+    d.cb[iw] = o.Cb.v;
+  }
+}
+
+// loss and its directional derivative terms at the current state
+__global__ void k_loss_tl(DevView d, TanView t, const double *ut, const double *vt, double *terms) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= d.na) return;
+  double l = 0.0, dl = 0.0;
+  if (q >= d.u_begin && q < d.u_end) {
+    const double2 v = d.vel[q], tv = t.dvel[q];
+    const double ex = v.x - ut[q], ey = v.y - vt[q];
+    l = 0.5 * (ex * ex + ey * ey);
+    dl = ex * tv.x + ey * tv.y;
+  }
+  terms[q] = l;
+  terms[d.na + q] = dl;
+}
+
+// loss terms and adjoint seed at the final state (active U slots)
+__global__ void k_seed(DevView d, AdjView a, const double *ut, const double *vt, double *terms) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= d.na) return;
+  double t = 0.0;
+  double2 s = make_double2(0.0, 0.0);
+  if (q >= d.u_begin && q < d.u_end) {
+    const double2 v = d.vel[q];
+    s = make_double2(v.x - ut[q], v.y - vt[q]);
+    t = 0.5 * (s.x * s.x + s.y * s.y);
+  }
+  if (a.ub) a.ub[q] = s;
+  terms[q] = t;
 }
 
 // data movement between caller order and slot order.  Element t of a host
@@ -1068,6 +1257,365 @@ int evp1d_autotune(evp1d_handle *h, int32_t reps, float *stress_us, float *stepu
     if (stepu_us) *stepu_us = best_u;
   }
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// adjoint driver: binomial checkpointing over the subcycles
+// ---------------------------------------------------------------------------
+namespace {
+
+// Optimal schedule for reversing n steps with s free snapshot slots when the
+// state at the segment start is already stored: split at m, advance m steps,
+// store, reverse the right part with s-1 slots, then the left part with s.
+// cost(n, s) = min_m m + cost(n-m, s-1) + cost(m, s); cost(1, s) = 0,
+// cost(n, 0) = n(n-1)/2 (restart from the stored state for every step).
+struct Schedule {
+  int N = 0, S = 0;
+  std::vector<long> cost;
+  std::vector<int> split;
+  long &C(int n, int s) { return cost[(size_t)n * (S + 1) + s]; }
+  int &M(int n, int s) { return split[(size_t)n * (S + 1) + s]; }
+  void build(int n_max, int s_max) {
+    N = n_max;
+    S = s_max;
+    cost.assign((size_t)(N + 1) * (S + 1), 0);
+    split.assign((size_t)(N + 1) * (S + 1), 0);
+    for (int n = 1; n <= N; ++n) C(n, 0) = (long)n * (n - 1) / 2;
+    for (int s = 1; s <= S; ++s) {
+      C(1, s) = 0;
+      for (int n = 2; n <= N; ++n) {
+        long best = -1;
+        int bm = 1;
+        for (int m = 1; m < n; ++m) {
+          const long c = m + C(n - m, s - 1) + C(m, s);
+          if (best < 0 || c < best) { best = c; bm = m; }
+        }
+        C(n, s) = best;
+        M(n, s) = bm;
+      }
+    }
+  }
+};
+
+struct StateBuf {
+  double *sig = nullptr;    // [12][ldsig]
+  double2 *vel = nullptr;   // [navel]
+};
+
+}  // namespace
+
+struct AdjointRun {
+  evp1d_handle *h;
+  int nsub;
+  AdjView a{};
+  std::vector<StateBuf> slots;
+  std::vector<char> used;
+  StateBuf final_state;
+  double2 *str_save = nullptr;   // [4][navel]
+  double *ut = nullptr, *vt = nullptr, *terms = nullptr;
+  std::vector<void *> mem;
+  Schedule sched;
+  int working = -1;              // index i of the state x_i held by the working arrays
+  double loss = 0.0;
+
+  int alloc(void **p, size_t bytes) {
+    const hipError_t e = hipMalloc(p, bytes ? bytes : 256);
+    if (e != hipSuccess) return fail(EVP1D_ENOMEM, "evp1d_adjoint_run: hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    mem.push_back(*p);
+    HIP_TRY(hipMemsetAsync(*p, 0, bytes ? bytes : 256, h->stream));
+    return EVP1D_OK;
+  }
+  ~AdjointRun() {
+    if (h && h->stream) (void)hipStreamSynchronize(h->stream);
+    for (void *p : mem) (void)hipFree(p);
+  }
+  size_t sig_bytes() const { return (size_t)12 * h->ldsig * sizeof(double); }
+  int copy_state(const StateBuf &dst, const StateBuf &src, bool ghosts) {
+    HIP_TRY(hipMemcpyAsync(dst.sig, src.sig, sig_bytes(), hipMemcpyDeviceToDevice, h->stream));
+    const size_t n = (size_t)(ghosts ? h->navel : h->na) * sizeof(double2);
+    HIP_TRY(hipMemcpyAsync(dst.vel, src.vel, n, hipMemcpyDeviceToDevice, h->stream));
+    return EVP1D_OK;
+  }
+  StateBuf work() const { return StateBuf{h->v.sig, h->v.vel}; }
+  int restore(int slot, int index) {
+    if (working == index) return EVP1D_OK;
+    RC_TRY(copy_state(work(), slots[(size_t)slot], false));
+    working = index;
+    return EVP1D_OK;
+  }
+  int advance(int n) {
+    for (int k = 0; k < n; ++k) {
+      RC_TRY(launch_stress(h));
+      RC_TRY(launch_stepu(h, false));
+      RC_TRY(launch_halo(h));
+    }
+    working += n;
+    return EVP1D_OK;
+  }
+  int take(int index) {
+    for (size_t k = 0; k < used.size(); ++k)
+      if (!used[k]) {
+        used[k] = 1;
+        RC_TRY(copy_state(slots[k], work(), false));
+        return (int)k;
+      }
+    return -fail(EVP1D_ESTATE, "evp1d_adjoint_run: out of snapshot slots at step %d", index);
+  }
+  static int blocks(int n) { return (n + 255) / 256; }
+  // the first reverse step: x_N from x_{N-1} (kept as the handle's final
+  // state, with its str1..8, strint and Cb), the loss and the adjoint seed
+  int final_step() {
+    RC_TRY(copy_state(final_state, work(), true));
+    DevView fv = h->v;
+    fv.sig = final_state.sig;
+    fv.vel = final_state.vel;
+    DevView keep = h->v;
+    h->v = fv;
+    int rc = launch_stress(h);
+    if (!rc) rc = launch_stepu(h, true);
+    if (!rc) rc = launch_halo(h);
+    h->v = keep;
+    RC_TRY(rc);
+    const size_t nV = (size_t)h->navel * sizeof(double2);
+    double2 *pairs[4] = {h->v.s15, h->v.s27, h->v.s36, h->v.s48};
+    for (int k = 0; k < 4; ++k)
+      HIP_TRY(hipMemcpyAsync(str_save + (size_t)k * h->navel, pairs[k], nV, hipMemcpyDeviceToDevice, h->stream));
+    if (h->na > 0) {
+      hipLaunchKernelGGL(k_seed, dim3(blocks(h->na)), dim3(256), 0, h->stream, fv, a, (const double *)ut,
+                         (const double *)vt, terms);
+      HIP_TRY(hipGetLastError());
+    }
+    std::vector<double> t((size_t)h->na);
+    if (h->na > 0)
+      HIP_TRY(hipMemcpyAsync(t.data(), terms, (size_t)h->na * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    loss = 0.0;
+    for (double x : t) loss += x;
+    return EVP1D_OK;
+  }
+  // reverse subcycle k (x_{k-1} -> x_k) with the working arrays holding x_{k-1}
+  int reverse_step(int k) {
+    if (k == nsub) RC_TRY(final_step());
+    const DevView &d = h->v;
+    if (d.t_end > 0) {
+      hipLaunchKernelGGL(k_str_recompute, dim3(blocks(d.t_end)), dim3(256), 0, h->stream, d);
+      HIP_TRY(hipGetLastError());
+    }
+    if (d.u_end > d.u_begin) {
+      hipLaunchKernelGGL(k_stepu_adj, dim3(blocks(d.u_end - d.u_begin)), dim3(256), 0, h->stream, d, a);
+      HIP_TRY(hipGetLastError());
+    }
+    if (d.t_end > 0) {
+      hipLaunchKernelGGL(k_stress_adj, dim3(blocks(d.t_end)), dim3(256), 0, h->stream, d, a);
+      HIP_TRY(hipGetLastError());
+    }
+    if (d.na > 0) {
+      hipLaunchKernelGGL(k_vel_gather, dim3(blocks(d.na)), dim3(256), 0, h->stream, d, a);
+      HIP_TRY(hipGetLastError());
+    }
+    return EVP1D_OK;
+  }
+  // reverse steps a+1 .. a+n; x_a is stored in slot sa; s further free slots
+  int rev(int a0, int n, int s, int sa) {
+    if (n == 1) {
+      RC_TRY(restore(sa, a0));
+      return reverse_step(a0 + 1);
+    }
+    if (s == 0) {
+      for (int i = n - 1; i >= 0; --i) {
+        RC_TRY(restore(sa, a0));
+        RC_TRY(advance(i));
+        RC_TRY(reverse_step(a0 + i + 1));
+      }
+      return EVP1D_OK;
+    }
+    const int m = sched.M(n, s);
+    RC_TRY(restore(sa, a0));
+    RC_TRY(advance(m));
+    const int sb = take(a0 + m);
+    if (sb < 0) return -sb;
+    RC_TRY(rev(a0 + m, n - m, s - 1, sb));
+    used[(size_t)sb] = 0;
+    return rev(a0, m, s, sa);
+  }
+};
+
+static int upload_caller_order(evp1d_handle *h, double *dst, const double *src) {
+  if (h->na == 0) return EVP1D_OK;
+  HIP_TRY(hipMemcpyAsync(h->scratch, src, (size_t)h->na * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  const Span sp{0, h->na, h->na, h->na};
+  hipLaunchKernelGGL(k_scatter_plain, dim3(blocks_for(h->na)), dim3(kBlock), 0, h->stream, dst,
+                     (const double *)h->scratch, (const int32_t *)h->inv_dev, h->na, sp);
+  HIP_TRY(hipGetLastError());
+  return EVP1D_OK;
+}
+
+static int download_caller_order(evp1d_handle *h, double *dst, const double *src) {
+  if (!dst || h->na == 0) return EVP1D_OK;
+  const Span sp{0, h->na, h->na, h->na};
+  hipLaunchKernelGGL(k_gather_plain, dim3(blocks_for(h->na)), dim3(kBlock), 0, h->stream, h->scratch, src,
+                     (const int32_t *)h->inv_dev, h->na, sp);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(dst, h->scratch, (size_t)h->na * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return EVP1D_OK;
+}
+
+int evp1d_adjoint_run(evp1d_handle *h, int32_t nsub, int32_t nsnap, const double *u_target, const double *v_target,
+                      double *loss, double *g_strength, double *g_cdn_ocn, double *g_forcex, double *g_forcey) {
+  if (!h) return fail(EVP1D_EINVAL, "evp1d_adjoint_run: NULL handle");
+  if (!ready(h)) return fail(EVP1D_ESTATE, "evp1d_adjoint_run: neighbour lists not set");
+  if (nsub < 1) return fail(EVP1D_EINVAL, "evp1d_adjoint_run: nsub=%d < 1", nsub);
+  if (nsnap < 1) return fail(EVP1D_EINVAL, "evp1d_adjoint_run: nsnap=%d < 1", nsnap);
+  if (h->na > 0 && (!u_target || !v_target)) return fail(EVP1D_EINVAL, "evp1d_adjoint_run: NULL target");
+  if (h->v.nhalo > 0)
+    return fail(EVP1D_ESTATE, "evp1d_adjoint_run: a ghost copy list is set; its reverse is not implemented");
+  RC_TRY(ensure(h));
+  AdjointRun R;
+  R.h = h;
+  R.nsub = nsub;
+  const int S = std::min(nsnap, nsub);       // more slots than steps are never used
+  const size_t nA = (size_t)h->na * sizeof(double), nV = (size_t)h->navel * sizeof(double2);
+  RC_TRY(R.alloc((void **)&R.a.ub, nV));
+  RC_TRY(R.alloc((void **)&R.a.sigb, R.sig_bytes()));
+  RC_TRY(R.alloc((void **)&R.a.g, nV));
+  RC_TRY(R.alloc((void **)&R.a.vc, 4 * (size_t)h->na * sizeof(double2)));
+  RC_TRY(R.alloc((void **)&R.a.gstr, nA));
+  RC_TRY(R.alloc((void **)&R.a.gcw, nA));
+  RC_TRY(R.alloc((void **)&R.a.gfx, nA));
+  RC_TRY(R.alloc((void **)&R.a.gfy, nA));
+  RC_TRY(R.alloc((void **)&R.ut, nA));
+  RC_TRY(R.alloc((void **)&R.vt, nA));
+  RC_TRY(R.alloc((void **)&R.terms, nA));
+  RC_TRY(R.alloc((void **)&R.str_save, 4 * nV));
+  RC_TRY(R.alloc((void **)&R.final_state.sig, R.sig_bytes()));
+  RC_TRY(R.alloc((void **)&R.final_state.vel, nV));
+  R.slots.resize((size_t)S);
+  R.used.assign((size_t)S, 0);
+  for (int k = 0; k < S; ++k) {
+    RC_TRY(R.alloc((void **)&R.slots[(size_t)k].sig, R.sig_bytes()));
+    RC_TRY(R.alloc((void **)&R.slots[(size_t)k].vel, (size_t)h->na * sizeof(double2)));
+  }
+  RC_TRY(upload_caller_order(h, R.ut, u_target));
+  RC_TRY(upload_caller_order(h, R.vt, v_target));
+  R.sched.build(nsub, S - 1);
+  R.working = 0;
+  const int s0 = R.take(0);
+  if (s0 < 0) return -s0;
+  RC_TRY(R.rev(0, nsub, S - 1, s0));
+  // leave the handle in the forward final state x_N
+  RC_TRY(R.copy_state(R.work(), R.final_state, false));
+  double2 *pairs[4] = {h->v.s15, h->v.s27, h->v.s36, h->v.s48};
+  for (int k = 0; k < 4; ++k)
+    HIP_TRY(hipMemcpyAsync(pairs[k], R.str_save + (size_t)k * h->navel, nV, hipMemcpyDeviceToDevice, h->stream));
+  RC_TRY(download_caller_order(h, g_strength, R.a.gstr));
+  RC_TRY(download_caller_order(h, g_cdn_ocn, R.a.gcw));
+  RC_TRY(download_caller_order(h, g_forcex, R.a.gfx));
+  RC_TRY(download_caller_order(h, g_forcey, R.a.gfy));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (loss) *loss = R.loss;
+  return EVP1D_OK;
+}
+
+int evp1d_tangent_run(evp1d_handle *h, int32_t nsub, const double *d_strength, const double *d_cdn_ocn,
+                      const double *d_forcex, const double *d_forcey, const double *u_target,
+                      const double *v_target, double *loss, double *dloss) {
+  if (!h) return fail(EVP1D_EINVAL, "evp1d_tangent_run: NULL handle");
+  if (!ready(h)) return fail(EVP1D_ESTATE, "evp1d_tangent_run: neighbour lists not set");
+  if (nsub < 0) return fail(EVP1D_EINVAL, "evp1d_tangent_run: nsub=%d < 0", nsub);
+  if (h->na > 0 && (!u_target || !v_target)) return fail(EVP1D_EINVAL, "evp1d_tangent_run: NULL target");
+  if (h->v.nhalo > 0) return fail(EVP1D_ESTATE, "evp1d_tangent_run: a ghost copy list is set");
+  RC_TRY(ensure(h));
+  const size_t nA = (size_t)h->na * sizeof(double), nV = (size_t)h->navel * sizeof(double2);
+  std::vector<void *> mem;
+  struct Free {
+    std::vector<void *> &m;
+    hipStream_t s;
+    ~Free() {
+      (void)hipStreamSynchronize(s);
+      for (void *p : m) (void)hipFree(p);
+    }
+  } guard{mem, h->stream};
+  auto A = [&](void **p, size_t bytes) -> int {
+    const hipError_t e = hipMalloc(p, bytes ? bytes : 256);
+    if (e != hipSuccess) return fail(EVP1D_ENOMEM, "evp1d_tangent_run: hipMalloc(%zu)", bytes);
+    mem.push_back(*p);
+    HIP_TRY(hipMemsetAsync(*p, 0, bytes ? bytes : 256, h->stream));
+    return EVP1D_OK;
+  };
+  TanView t{};
+  double *dir[4] = {nullptr, nullptr, nullptr, nullptr}, *ut = nullptr, *vt = nullptr, *terms = nullptr;
+  RC_TRY(A((void **)&t.dsig, (size_t)12 * h->ldsig * sizeof(double)));
+  RC_TRY(A((void **)&t.dvel, nV));
+  RC_TRY(A((void **)&t.ds15, nV));
+  RC_TRY(A((void **)&t.ds27, nV));
+  RC_TRY(A((void **)&t.ds36, nV));
+  RC_TRY(A((void **)&t.ds48, nV));
+  const double *src[4] = {d_strength, d_cdn_ocn, d_forcex, d_forcey};
+  for (int k = 0; k < 4; ++k) {
+    RC_TRY(A((void **)&dir[k], nA));
+    if (src[k]) RC_TRY(upload_caller_order(h, dir[k], src[k]));
+  }
+  t.dstrength = dir[0];
+  t.dcw = dir[1];
+  t.dfx = dir[2];
+  t.dfy = dir[3];
+  RC_TRY(A((void **)&ut, nA));
+  RC_TRY(A((void **)&vt, nA));
+  RC_TRY(A((void **)&terms, 2 * nA));
+  RC_TRY(upload_caller_order(h, ut, u_target));
+  RC_TRY(upload_caller_order(h, vt, v_target));
+  const DevView &d = h->v;
+  for (int k = 0; k < nsub; ++k) {
+    if (d.t_end > 0) hipLaunchKernelGGL(k_stress_tl, dim3((d.t_end + 255) / 256), dim3(256), 0, h->stream, d, t);
+    if (d.u_end > d.u_begin)
+      hipLaunchKernelGGL(k_stepu_tl, dim3((d.u_end - d.u_begin + 255) / 256), dim3(256), 0, h->stream, d, t,
+                         k == nsub - 1 ? 1 : 0);
+    HIP_TRY(hipGetLastError());
+  }
+  double l = 0.0, dl = 0.0;
+  if (h->na > 0) {
+    hipLaunchKernelGGL(k_loss_tl, dim3(blocks_for(h->na)), dim3(kBlock), 0, h->stream, d, t, (const double *)ut,
+                       (const double *)vt, terms);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> buf(2 * (size_t)h->na);
+    HIP_TRY(hipMemcpyAsync(buf.data(), terms, 2 * nA, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (int q = 0; q < h->na; ++q) {
+      l += buf[(size_t)q];
+      dl += buf[(size_t)h->na + q];
+    }
+  }
+  if (loss) *loss = l;
+  if (dloss) *dloss = dl;
+  return EVP1D_OK;
+}
+
+int evp1d_loss(evp1d_handle *h, const double *u_target, const double *v_target, double *loss) {
+  if (!h || !loss) return fail(EVP1D_EINVAL, "evp1d_loss: NULL argument");
+  if (h->na > 0 && (!u_target || !v_target)) return fail(EVP1D_EINVAL, "evp1d_loss: NULL target");
+  RC_TRY(ensure(h));
+  *loss = 0.0;
+  if (h->na == 0) return EVP1D_OK;
+  double *buf = nullptr;
+  HIP_TRY(hipMalloc(&buf, 3 * (size_t)h->na * sizeof(double)));
+  int rc = upload_caller_order(h, buf, u_target);
+  if (!rc) rc = upload_caller_order(h, buf + h->na, v_target);
+  if (!rc) {
+    AdjView none{};
+    hipLaunchKernelGGL(k_seed, dim3(blocks_for(h->na)), dim3(kBlock), 0, h->stream, h->v, none,
+                       (const double *)buf, (const double *)(buf + h->na), buf + 2 * h->na);
+    if (hipGetLastError() != hipSuccess) rc = fail(EVP1D_EHIP, "evp1d_loss: launch failed");
+  }
+  std::vector<double> t((size_t)h->na);
+  if (!rc && hipMemcpyAsync(t.data(), buf + 2 * h->na, (size_t)h->na * sizeof(double), hipMemcpyDeviceToHost,
+                            h->stream) != hipSuccess)
+    rc = fail(EVP1D_EHIP, "evp1d_loss: copy failed");
+  if (hipStreamSynchronize(h->stream) != hipSuccess && !rc) rc = fail(EVP1D_EHIP, "evp1d_loss: sync failed");
+  (void)hipFree(buf);
+  if (rc) return rc;
+  for (double x : t) *loss += x;
+  return EVP1D_OK;
 }
 
 int evp1d_synchronize(evp1d_handle *h) {

@@ -54,6 +54,7 @@ EXPORTS = (
     "evp1d_create_batch", "evp1d_get_members", "evp1d_member_info", "evp1d_set_field_member",
     "evp1d_get_field_member", "evp1d_set_index_member", "evp1d_set_skip_member",
     "evp1d_autotune", "evp1d_set_tuning", "evp1d_get_tuning",
+    "evp1d_adjoint_run", "evp1d_loss", "evp1d_tangent_run",
 )
 
 
@@ -133,6 +134,9 @@ def load(fma: bool = False, path: str | os.PathLike | None = None) -> ctypes.CDL
         "evp1d_autotune": [_H, i32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
         "evp1d_set_tuning": [_H, i32, i32, i32, i32],
         "evp1d_get_tuning": [_H, _IP, _IP, _IP, _IP],
+        "evp1d_adjoint_run": [_H, i32, i32, _DP, _DP, _DP, _DP, _DP, _DP, _DP],
+        "evp1d_loss": [_H, _DP, _DP, _DP],
+        "evp1d_tangent_run": [_H, i32, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -260,6 +264,41 @@ class Engine:
         out = self.get_tuning()
         out.update(stress_us=a.value, stepu_us=b.value)
         return out
+
+    # gradients ----------------------------------------------------------
+    def loss(self, u_target, v_target) -> float:
+        ut = np.ascontiguousarray(u_target, dtype=np.float64)
+        vt = np.ascontiguousarray(v_target, dtype=np.float64)
+        out = ctypes.c_double()
+        _check(self.L, self.L.evp1d_loss(self.h, _dp(ut), _dp(vt), ctypes.byref(out)))
+        return out.value
+
+    def adjoint_run(self, nsub: int, u_target, v_target, nsnap: int = 32) -> dict:
+        """Loss after nsub subcycles and its gradient w.r.t. strength, cdn_ocn
+        (per point), forcex, forcey (evp1d_adjoint_run)."""
+        ut = np.ascontiguousarray(u_target, dtype=np.float64)
+        vt = np.ascontiguousarray(v_target, dtype=np.float64)
+        g = {k: np.zeros(self.na) for k in ("strength", "cdn_ocn", "forcex", "forcey")}
+        loss = ctypes.c_double()
+        _check(self.L, self.L.evp1d_adjoint_run(self.h, int(nsub), int(nsnap), _dp(ut), _dp(vt),
+                                                ctypes.byref(loss), _dp(g["strength"]), _dp(g["cdn_ocn"]),
+                                                _dp(g["forcex"]), _dp(g["forcey"])))
+        g["loss"] = loss.value
+        return g
+
+    def tangent_run(self, nsub: int, u_target, v_target, d_strength=None, d_cdn_ocn=None,
+                    d_forcex=None, d_forcey=None) -> tuple:
+        """(loss, directional derivative) along the given perturbation."""
+        ut = np.ascontiguousarray(u_target, dtype=np.float64)
+        vt = np.ascontiguousarray(v_target, dtype=np.float64)
+        dirs = []
+        for a in (d_strength, d_cdn_ocn, d_forcex, d_forcey):
+            dirs.append(None if a is None else np.ascontiguousarray(a, dtype=np.float64))
+        ptr = [(_dp(a) if a is not None else ctypes.cast(None, _DP)) for a in dirs]
+        l, dl = ctypes.c_double(), ctypes.c_double()
+        _check(self.L, self.L.evp1d_tangent_run(self.h, int(nsub), *ptr, _dp(ut), _dp(vt), ctypes.byref(l),
+                                                ctypes.byref(dl)))
+        return l.value, dl.value
 
     def set_tuning(self, stress_block=256, stress_xcd=0, stepu_block=256, stepu_xcd=0) -> None:
         _check(self.L, self.L.evp1d_set_tuning(self.h, stress_block, stress_xcd, stepu_block, stepu_xcd))

@@ -186,6 +186,32 @@ int evp1d_set_tuning(evp1d_handle *h, int32_t stress_block, int32_t stress_xcd, 
 int evp1d_get_tuning(const evp1d_handle *h, int32_t *stress_block, int32_t *stress_xcd,
                      int32_t *stepu_block, int32_t *stepu_xcd);
 
+/* ---- gradients (no counterpart in the reference) -------------------------
+ * Loss L = 1/2 sum over stepu-active points (skipU = 0) of
+ * (uvel - u_target)^2 + (vvel - v_target)^2 after nsub subcycles started from
+ * the handle's current state.  evp1d_adjoint_run returns L and its gradient
+ * with respect to strength, cdn_ocn (per point; the gradient with respect to
+ * one scalar drag coefficient is the sum), forcex and forcey (caller order,
+ * na each; any output may be NULL).  The reverse sweep uses hand-written
+ * adjoint stress and stepu kernels and binomial checkpointing with nsnap
+ * stored subcycle states (nsnap >= nsub stores them all: no recomputation).
+ * Afterwards the handle holds the forward state after nsub subcycles, exactly
+ * as evp1d_run(h, nsub) leaves it.  evp1d_loss evaluates L at the current
+ * state.                                                                    */
+int evp1d_adjoint_run(evp1d_handle *h, int32_t nsub, int32_t nsnap, const double *u_target,
+                      const double *v_target, double *loss, double *g_strength, double *g_cdn_ocn,
+                      double *g_forcex, double *g_forcey);
+int evp1d_loss(evp1d_handle *h, const double *u_target, const double *v_target, double *loss);
+/* Tangent-linear run: nsub subcycles from the current state propagating the
+ * perturbation (d_strength, d_cdn_ocn, d_forcex, d_forcey) (caller order, na
+ * each, NULL = zero) by forward-mode differentiation of the same point
+ * functions; returns L and its directional derivative dL.  The handle ends
+ * in the same state as evp1d_run(h, nsub).  With the adjoint gradient g,
+ * dL = <g, d> (the dot-product test).                                       */
+int evp1d_tangent_run(evp1d_handle *h, int32_t nsub, const double *d_strength, const double *d_cdn_ocn,
+                      const double *d_forcex, const double *d_forcey, const double *u_target,
+                      const double *v_target, double *loss, double *dloss);
+
 /* Device timing on the handle's stream (HIP events): start, then stop
  * returns the elapsed milliseconds between the two after synchronising.     */
 int evp1d_timer_start(evp1d_handle *h);
