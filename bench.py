@@ -3,16 +3,27 @@
 
 One step = one model time step's EVP solve: ndte subcycles of
 (stress; stepu; ghost update) over the whole 1D point list, inputs resident in
-HBM.  Default workload: 1280x1520 grid, ndte = 1000 (the operational-size
-single-GPU configuration).  Prints ONE JSON line:
+HBM.  Default workload: the 1280x1520 grid with ndte = 1000 (the operational-
+size single-GPU configuration).  Prints ONE JSON line:
 
-  value      ice-points x subcycles per second (na * ndte * steps / time)
+  value      ice-points x subcycles per second (na * ndte * steps / time),
+             exact (bit-identical) FP64 build
   roofline   dominant kernel: algorithmic bytes per launch / average launch
              duration (HIP events on the library's stream) vs 8 TB/s
+  adjoint    time of one gradient evaluation (hand-written adjoint kernels,
+             binomial checkpointing with 32 stored subcycle states) over the
+             time of one forward step, with central finite-difference and
+             tangent-linear dot-product checks of the gradient
   cpu_baseline  the reference's own OpenMP v2b benchmark (flang build under
-             oracle/_ref) timed on this host's cores on a bounded sample
-             (same grid, fewer subcycles); the oracle's OpenMP C
-             restatement if the reference build is absent.
+             oracle/_ref) on this host's cores on a bounded sample (same
+             grid, fewer subcycles); the oracle's OpenMP C restatement if the
+             reference build is absent.
+
+Other measurements (one JSON line each):
+  --workload evp_c2 [--mode persistent]   2560x3040 grid, graph vs persistent
+  --ensemble 64      64 members in one batched handle vs 64 sequential handles
+  --fma              the contraction-on build
+  --calibrate-copy MIB   plain copy for scaling rocprofv3 FETCH/WRITE_SIZE
 
 Algorithmic bytes per subcycle (reference data model, f64 = 8 B,
 int32/logical = 4 B): stress 4*na + 340*nT, stepu 4*na + 244*nU.
@@ -23,11 +34,12 @@ import argparse
 import json
 import os
 import re
-import subprocess
 import sys
 import tempfile
 import time
 from pathlib import Path
+
+import numpy as np
 
 REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO / "cice-evp1d_amd"))
@@ -45,21 +57,48 @@ def algorithmic_bytes(na: int, nT: int, nU: int) -> dict:
     return {"stress": stress, "stepu": stepu, "subcycle": stress + stepu}
 
 
-def make_state(workload: str):
-    import numpy as np
+def make_state(nx: int, ny: int, seed: int, **kw):
     import evp1d
     from evp1d import layout, synth
+    g = synth.make_grid(nx, ny, seed, **kw)
+    ix = evp1d.build_index(g.tmask, g.nghost)
+    s = layout.pack(g, ix)
+    s["nT"] = int((s["skipT"] == 0).sum())
+    s["nU"] = int((s["skipU"] == 0).sum())
+    return s
+
+
+def workload_state(workload: str):
+    from evp1d import synth
     w = dict(synth.WORKLOADS[workload])
     ndte = w.pop("ndte")
     t = time.time()
-    g = synth.make_grid(**w)
-    ix = evp1d.build_index(g.tmask, g.nghost)
-    s = layout.pack(g, ix)
-    nT = int((s["skipT"] == 0).sum())
-    nU = int((s["skipU"] == 0).sum())
-    log(f"[bench] {workload}: {w['nx']}x{w['ny']} na={s['na']} navel={s['navel']} nT={nT} nU={nU} "
-        f"ndte={ndte} (generated in {time.time() - t:.1f}s)")
-    return w, ndte, g, ix, s, nT, nU
+    s = make_state(**w)
+    log(f"[bench] {workload}: {w['nx']}x{w['ny']} na={s['na']} navel={s['navel']} nT={s['nT']} "
+        f"nU={s['nU']} ndte={ndte} (generated in {time.time() - t:.1f}s)")
+    return w, ndte, s
+
+
+def new_engine(s: dict, ndte: int, mode: str = "auto", fma: bool = False):
+    import evp1d
+    p = evp1d.set_evp_parameters(ndte, fma=fma)
+    eng = evp1d.Engine(s["na"], s["navel"], p, fma=fma)
+    eng.set_mode(mode)
+    eng.load_state(s["fields"], s["idx"], s["skipT"], s["skipU"])
+    return eng
+
+
+DYNAMIC = ("stressp_1", "stressp_2", "stressp_3", "stressp_4", "stressm_1", "stressm_2", "stressm_3",
+           "stressm_4", "stress12_1", "stress12_2", "stress12_3", "stress12_4", "uvel", "vvel",
+           "strintxU", "strintyU", "Cb", "str1", "str2", "str3", "str4", "str5", "str6", "str7", "str8")
+
+
+def reset_state(eng, s: dict, overrides: dict | None = None) -> None:
+    """Put the initial dynamic state (and optionally perturbed parameters) back."""
+    for n in DYNAMIC:
+        eng.set_field(n, s["fields"][n])
+    for n, a in (overrides or {}).items():
+        eng.set_field(n, a)
 
 
 def cpu_baseline(workload: str, s: dict, ndte_sample: int, threads: int) -> dict | None:
@@ -114,6 +153,123 @@ def calibrate_copy(mib: int) -> int:
     return 0
 
 
+def time_steps(eng, ndte: int, steps: int, warmup: int) -> float:
+    for k in range(warmup):
+        eng.run(ndte)
+        eng.synchronize()
+        log(f"[bench] warmup step {k + 1}/{warmup} done")
+    eng.synchronize()
+    t0 = time.perf_counter()
+    eng.timer_start()
+    for _ in range(steps):
+        eng.run(ndte)
+    ev_ms = eng.timer_stop()
+    eng.synchronize()
+    wall = time.perf_counter() - t0
+    log(f"[bench] timed {steps} steps: wall {wall:.3f}s, events {ev_ms / 1e3:.3f}s")
+    return max(wall, ev_ms / 1e3)
+
+
+def adjoint_measure(s: dict, ndte: int, t_fwd_step: float, nsnap: int, ndir: int, ndot: int) -> dict:
+    """Gradient of L = 1/2 sum_{skipU=0} |u_N - u*|^2 w.r.t. strength,
+    cdn_ocn, forcex, forcey; u* from a run with strength x 1.2."""
+    eng = new_engine(s, ndte)
+    reset_state(eng, s, {"strength": 1.2 * s["fields"]["strength"]})
+    eng.run(ndte)
+    ut = eng.get_field("uvel")[:s["na"]]
+    vt = eng.get_field("vvel")[:s["na"]]
+    reset_state(eng, s, {"strength": s["fields"]["strength"]})
+    eng.run(ndte)                      # warm the forward graph and kernels
+    reset_state(eng, s)
+    eng.synchronize()
+    t0 = time.perf_counter()
+    g = eng.adjoint_run(ndte, ut, vt, nsnap=nsnap)
+    t_adj = time.perf_counter() - t0
+    log(f"[bench] adjoint ({nsnap} stored states): {t_adj:.3f}s, forward step {t_fwd_step:.3f}s, "
+        f"ratio {t_adj / t_fwd_step:.2f}")
+    rng = np.random.default_rng(1234)
+    f = s["fields"]
+
+    def direction():
+        return {"strength": rng.standard_normal(s["na"]) * f["strength"],
+                "cdn_ocn": np.full(s["na"], rng.standard_normal()) * f["cdn_ocn"],
+                "forcexU": rng.standard_normal(s["na"]) * np.abs(f["forcexU"]),
+                "forceyU": rng.standard_normal(s["na"]) * np.abs(f["forceyU"])}
+
+    def dot(d):
+        return (np.dot(g["strength"], d["strength"]) + np.dot(g["cdn_ocn"], d["cdn_ocn"]) +
+                np.dot(g["forcex"], d["forcexU"]) + np.dot(g["forcey"], d["forceyU"]))
+
+    h = 1e-6
+    fd_err = 0.0
+    for k in range(ndir):
+        d = direction()
+        vals = []
+        for sign in (1.0, -1.0):
+            reset_state(eng, s, {n: f[n] + sign * h * d[n] for n in d})
+            eng.run(ndte)
+            vals.append(eng.loss(ut, vt))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        ad = dot(d)
+        fd_err = max(fd_err, abs(fd - ad) / max(abs(fd), abs(ad)))
+    dot_err = 0.0
+    for k in range(ndot):
+        d = direction()
+        reset_state(eng, s, {n: f[n] for n in d})
+        _, dl = eng.tangent_run(ndte, ut, vt, d["strength"], d["cdn_ocn"], d["forcexU"], d["forceyU"])
+        ad = dot(d)
+        dot_err = max(dot_err, abs(dl - ad) / max(abs(dl), abs(ad)))
+    eng.close()
+    log(f"[bench] adjoint checks: FD max rel err {fd_err:.2e} ({ndir} directions, eps 1e-6 relative), "
+        f"tangent-linear dot product max rel err {dot_err:.2e} ({ndot} directions)")
+    return {"t_adj_over_t_fwd": round(t_adj / t_fwd_step, 3), "t_adj_s": round(t_adj, 4),
+            "t_fwd_s": round(t_fwd_step, 4), "stored_states": nsnap, "loss": g["loss"],
+            "fd_directions": ndir, "fd_max_rel_err": fd_err, "dot_directions": ndot,
+            "dot_max_rel_err": dot_err,
+            "grad_cdn_ocn_scalar": float(np.sum(g["cdn_ocn"]))}
+
+
+def ensemble_measure(nmem: int, steps: int, autotune: int) -> dict:
+    import evp1d
+    from evp1d import synth
+    t = time.time()
+    members = []
+    for k in range(nmem):
+        m = synth.ensemble_member(k)
+        members.append(make_state(m["nx"], m["ny"], m["seed"], force_perturb=0.02))
+    sizes = [(s["na"], s["navel"]) for s in members]
+    tot = sum(a for a, _ in sizes)
+    log(f"[bench] ensemble: {nmem} members, na from {min(a for a, _ in sizes)} to "
+        f"{max(a for a, _ in sizes)}, total {tot} (generated in {time.time() - t:.1f}s)")
+    ndte = 1000
+    p = evp1d.set_evp_parameters(ndte)
+    batch = evp1d.Engine.batch(sizes, p)
+    for k, s in enumerate(members):
+        batch.load_member(k, s["fields"], s["idx"], s["skipT"], s["skipU"])
+    tuning = batch.autotune(autotune) if autotune else batch.get_tuning()
+    t_batch = time_steps(batch, ndte, steps, 1) / steps
+    batch.close()
+    singles = [new_engine(s, ndte) for s in members]
+    for e in singles:
+        e.run(ndte)
+    for e in singles:
+        e.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for e in singles:
+            e.run(ndte)
+            e.synchronize()
+    t_seq = (time.perf_counter() - t0) / steps
+    for e in singles:
+        e.close()
+    out = {"members": nmem, "total_points": tot, "ndte": ndte,
+           "batched_s": round(t_batch, 4), "sequential_s": round(t_seq, 4),
+           "batched_pts_sub_per_s": tot * ndte / t_batch, "sequential_pts_sub_per_s": tot * ndte / t_seq,
+           "speedup": round(t_seq / t_batch, 3), "tuning": tuning}
+    log(f"[bench] ensemble: batched {t_batch:.3f}s, sequential {t_seq:.3f}s, speedup {t_seq / t_batch:.2f}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,8 +279,14 @@ def main() -> int:
     ap.add_argument("--mode", default="auto", choices=("auto", "graph", "persistent", "launch"))
     ap.add_argument("--fma", action="store_true", help="time libevp1d_fma.so instead of the exact build")
     ap.add_argument("--kernel-reps", type=int, default=200, help="launches per individual-kernel timing")
-    ap.add_argument("--cpu-subcycles", type=int, default=100)
+    ap.add_argument("--cpu-subcycles", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-snapshots", type=int, default=32)
+    ap.add_argument("--fd-directions", type=int, default=3)
+    ap.add_argument("--dot-directions", type=int, default=2)
+    ap.add_argument("--ensemble", type=int, default=0, metavar="N",
+                    help="only measure an N-member ensemble, batched vs sequential")
     ap.add_argument("--autotune", type=int, default=0, metavar="REPS",
                     help="run evp1d_autotune with REPS launches per configuration before timing")
     ap.add_argument("--tuning", default=None, metavar="SB,SX,UB,UX",
@@ -135,39 +297,26 @@ def main() -> int:
     args = ap.parse_args()
     if args.calibrate_copy:
         return calibrate_copy(args.calibrate_copy)
-
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design "
                                    "note (DESIGN.md), not a measured path", "n_gpus": args.gpus}))
         return 2
-    import evp1d
-    w, ndte, g, ix, s, nT, nU = make_state(args.workload)
-    na = s["na"]
-    p = evp1d.set_evp_parameters(ndte, fma=args.fma)
-    eng = evp1d.Engine(na, s["navel"], p, fma=args.fma)
-    eng.set_mode(args.mode)
-    eng.load_state(s["fields"], s["idx"], s["skipT"], s["skipU"])
+    if args.ensemble:
+        out = ensemble_measure(args.ensemble, args.steps, args.autotune)
+        out.update({"metric": "ensemble per-member throughput, batched vs sequential", "n_gpus": 1,
+                    "dtype": "f64", "data": "synthetic"})
+        print(json.dumps(out), flush=True)
+        return 0
+
+    w, ndte, s = workload_state(args.workload)
+    na, nT, nU = s["na"], s["nT"], s["nU"]
+    eng = new_engine(s, ndte, args.mode, args.fma)
     if args.tuning:
         eng.set_tuning(*[int(x) for x in args.tuning.split(",")])
     if args.autotune:
-        t = eng.autotune(args.autotune)
-        log(f"[bench] autotune: {t}")
+        log(f"[bench] autotune: {eng.autotune(args.autotune)}")
     tuning = eng.get_tuning()
-
-    for k in range(args.warmup):
-        eng.run(ndte)
-        eng.synchronize()
-        log(f"[bench] warmup step {k + 1}/{args.warmup} done")
-    eng.synchronize()
-    t0 = time.perf_counter()
-    eng.timer_start()
-    for k in range(args.steps):
-        eng.run(ndte)
-    ev_ms = eng.timer_stop()
-    eng.synchronize()
-    wall = time.perf_counter() - t0
-    log(f"[bench] timed {args.steps} steps: wall {wall:.3f}s, events {ev_ms / 1e3:.3f}s")
-    secs = max(wall, ev_ms / 1e3)
+    secs = time_steps(eng, ndte, args.steps, args.warmup)
     value = na * ndte * args.steps / secs
 
     # per-kernel average launch durations (HIP events, the library's stream)
@@ -182,6 +331,7 @@ def main() -> int:
         kernels[which] = {"avg_us": round(avg_us, 3), "bytes": B[which],
                           "GBps": round(B[which] / (avg_us * 1e-6) / 1e9, 1)}
         log(f"[bench] {which}: {avg_us:.2f} us/launch, {kernels[which]['GBps']} GB/s algorithmic")
+    eng.close()
     step_us = secs / (args.steps * ndte) * 1e6
     if args.mode == "persistent":
         dom = {"name": "k_persistent", "bytes": B["subcycle"] * ndte, "avg_us": secs / args.steps * 1e6}
@@ -197,7 +347,15 @@ def main() -> int:
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom["name"],
                 "bytes_per_launch": dom["bytes"]}
     whole = B["subcycle"] / (step_us * 1e-6) / 1e9
-    eng.close()
+
+    adjoint = None
+    if not args.no_adjoint and not args.fma:
+        try:
+            adjoint = adjoint_measure(s, ndte, secs / args.steps, args.adjoint_snapshots,
+                                      args.fd_directions, args.dot_directions)
+        except Exception as e:   # reported, never fatal for the forward figure
+            log(f"[bench] adjoint measurement failed: {e!r}")
+            adjoint = {"error": repr(e)[:200]}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -205,21 +363,23 @@ def main() -> int:
         try:
             cpu = cpu_baseline(args.workload, s, args.cpu_subcycles, threads)
             log(f"[bench] cpu baseline: {cpu}")
-        except Exception as e:  # reported, never fatal for the GPU figure
+        except Exception as e:
             log(f"[bench] cpu baseline failed: {e!r}")
             cpu = {"value": None, "error": repr(e)[:200]}
 
     out = {
-        "metric": "ice-pts*subcycles/s + % HBM roofline (bit-exact FP64)",
+        "metric": "ice-pts*subcycles/s + % HBM roofline (bit-exact FP64); adjoint/forward time ratio",
         "value": value, "unit": "ice-pts*subcycles/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": secs / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{args.workload} {w['nx']}x{w['ny']} ndte={ndte}", "nx": w["nx"],
                    "ny": w["ny"], "seed": w["seed"], "na": na, "navel": s["navel"], "nT": nT, "nU": nU,
                    "ndte": ndte, "mode": args.mode, "build": "fma" if args.fma else "exact", "tuning": tuning,
-                   "us_per_subcycle": round(step_us, 3), "loop_GBps_algorithmic": round(whole, 1)},
+                   "us_per_subcycle": round(step_us, 3), "loop_GBps_algorithmic": round(whole, 1),
+                   "loop_frac_of_peak": round(whole / HBM_PEAK_GBS, 4)},
         "kernels": kernels,
         "roofline": roofline,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

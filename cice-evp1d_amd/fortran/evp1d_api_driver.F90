@@ -34,16 +34,16 @@ program evp1d_api_driver
 
   open(newunit=lun, file=trim(fin), form='unformatted', access='stream', action='read', &
        status='old', iostat=ios)
-  if (ios /= 0) stop 'evp1d_api_driver: cannot open input'
+  if (ios /= 0) error stop 'evp1d_api_driver: cannot open input'
   read(lun, iostat=ios) hdr
-  if (ios /= 0) stop 'evp1d_api_driver: short header'
+  if (ios /= 0) error stop 'evp1d_api_driver: short header'
   nxg = hdr(1); nyg = hdr(2); ng = hdr(3); ndte = hdr(4); nrun = hdr(5)
   NX = nxg + 2*ng
   NY = nyg + 2*ng
   allocate(im(NX,NY,3), st(NX,NY,5), dyn(NX,NY,1,29), G_HTE(NX,NY), G_HTN(NX,NY))
   allocate(tmask(NX,NY,1), icet(NX,NY,1), iceu(NX,NY,1))
   read(lun, iostat=ios) im, st, dyn
-  if (ios /= 0) stop 'evp1d_api_driver: short input'
+  if (ios /= 0) error stop 'evp1d_api_driver: short input'
   close(lun)
   tmask(:,:,1) = im(:,:,1) /= 0
   icet(:,:,1) = im(:,:,2) /= 0
@@ -78,7 +78,7 @@ program evp1d_api_driver
 
   open(newunit=lun, file=trim(fout), form='unformatted', access='stream', action='write', &
        status='replace', iostat=ios)
-  if (ios /= 0) stop 'evp1d_api_driver: cannot open output'
+  if (ios /= 0) error stop 'evp1d_api_driver: cannot open output'
   do k = 1, 29
     write(lun) dyn(:,:,1,k)
   enddo

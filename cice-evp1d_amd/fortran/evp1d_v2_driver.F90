@@ -49,12 +49,12 @@ program evp1d_v2_driver
   nx_block = -1; ny_block = -1; testscale = 1; lconvert21d = .false.
   lindividual = .false.; binoutput = .true.; max_blocks = 1; nghost = 1
   open(newunit=lun, file='kernel.nml', status='old', iostat=ios)
-  if (ios /= 0) stop 'ERROR: kernel.nml not found'
+  if (ios /= 0) error stop 'ERROR: kernel.nml not found'
   read(lun, nml=kernel_nml, iostat=ios)
-  if (ios /= 0) stop 'ERROR: Can not read namelist: kernel_nml from file kernel.nml'
+  if (ios /= 0) error stop 'ERROR: Can not read namelist: kernel_nml from file kernel.nml'
   close(lun)
-  if (testscale /= 1) stop 'ERROR: testscale /= 1 is not supported'
-  if (na < 0 .or. navel < na) stop 'ERROR: kernel.nml needs 0 <= na <= navel'
+  if (testscale /= 1) error stop 'ERROR: testscale /= 1 is not supported'
+  if (na < 0 .or. navel < na) error stop 'ERROR: kernel.nml needs 0 <= na <= navel'
   write(*,'(a,i0,a,i0,a,i0)') ' na = ', na, '  navel = ', navel, '  ndte = ', ndte
 
   mode = MODE_AUTO
@@ -64,7 +64,7 @@ program evp1d_v2_driver
     case ('graph');      mode = MODE_GRAPH
     case ('persistent'); mode = MODE_PERSISTENT
     case ('launch');     mode = MODE_LAUNCH
-    case default;        stop 'usage: evp1d_v2_driver [graph|persistent|launch]'
+    case default;        error stop 'usage: evp1d_v2_driver [graph|persistent|launch]'
     end select
   endif
 
@@ -72,21 +72,21 @@ program evp1d_v2_driver
   allocate(out(na,12), vout(max(na,1)))
   open(newunit=lun, file='input_double_1d_v2.bin', form='unformatted', access='stream', &
        action='read', status='old', iostat=ios)
-  if (ios /= 0) stop 'ERROR: cannot open input_double_1d_v2.bin'
+  if (ios /= 0) error stop 'ERROR: cannot open input_double_1d_v2.bin'
   read(lun, iostat=ios) vout(1:na), uvel, vvel, f, scal, ht
-  if (ios /= 0) stop 'ERROR: input_double_1d_v2.bin is short or unreadable'
+  if (ios /= 0) error stop 'ERROR: input_double_1d_v2.bin is short or unreadable'
   close(lun)
   open(newunit=lun, file='input_integer_1d.bin', form='unformatted', access='stream', &
        action='read', status='old', iostat=ios)
-  if (ios /= 0) stop 'ERROR: cannot open input_integer_1d.bin'
+  if (ios /= 0) error stop 'ERROR: cannot open input_integer_1d.bin'
   read(lun, iostat=ios) idx
-  if (ios /= 0) stop 'ERROR: input_integer_1d.bin is short or unreadable'
+  if (ios /= 0) error stop 'ERROR: input_integer_1d.bin is short or unreadable'
   close(lun)
   open(newunit=lun, file='input_logical_1d.bin', form='unformatted', access='stream', &
        action='read', status='old', iostat=ios)
-  if (ios /= 0) stop 'ERROR: cannot open input_logical_1d.bin'
+  if (ios /= 0) error stop 'ERROR: cannot open input_logical_1d.bin'
   read(lun, iostat=ios) skipU, skipT
-  if (ios /= 0) stop 'ERROR: input_logical_1d.bin is short or unreadable'
+  if (ios /= 0) error stop 'ERROR: input_logical_1d.bin is short or unreadable'
   close(lun)
 
   ! set_evp_parameters(dt=300) then the file's scalars (src/vars.F90:2116)
@@ -125,15 +125,15 @@ program evp1d_v2_driver
     enddo
     open(newunit=lun, file='output_stress_v2g_1d.bin', form='unformatted', access='stream', &
          action='write', status='replace', iostat=ios)
-    if (ios /= 0) stop 'Failed open output_stress_v2g_1d.bin'
+    if (ios /= 0) error stop 'Failed open output_stress_v2g_1d.bin'
     write(lun, iostat=ios) out
-    if (ios /= 0) stop 'Failed write output_stress_v2g_1d.bin'
+    if (ios /= 0) error stop 'Failed write output_stress_v2g_1d.bin'
     close(lun)
     nbb = count(skipU == 0)
     if (nb >= 0 .and. nb /= nbb) write(*,'(a,i0,a,i0)') ' note: nb = ', nb, ' but active U points = ', nbb
     open(newunit=lun, file='output_stepu_v2g_1d.bin', form='unformatted', access='stream', &
          action='write', status='replace', iostat=ios)
-    if (ios /= 0) stop 'Failed open output_stepu_v2g_1d.bin'
+    if (ios /= 0) error stop 'Failed open output_stepu_v2g_1d.bin'
     do k = 1, 4
       select case (k)
       case (1); call evp1d_check(evp1d_get_field(h, F_STRINTX, vout, int(na,c_int32_t)), 'get')
@@ -149,9 +149,9 @@ program evp1d_v2_driver
         i = i + 1
         out(i,1) = vout(iw)
       enddo
-      if (i /= nbb) stop 'PROBLEM in generating output_stepu_v2g_1d.bin'
+      if (i /= nbb) error stop 'PROBLEM in generating output_stepu_v2g_1d.bin'
       if (nbb > 0) write(lun, iostat=ios) out(1:nbb,1)
-      if (ios /= 0) stop 'Failed write output_stepu_v2g_1d.bin'
+      if (ios /= 0) error stop 'Failed write output_stepu_v2g_1d.bin'
     enddo
     close(lun)
   endif

@@ -74,7 +74,7 @@ contains
       read(lun, nml=evp1d_nml, iostat=ios)
       close(lun)
     endif
-    if (buf1d /= 0) stop 'ice_dyn_evp1d: buf1d /= 0 is not supported (static point list)'
+    if (buf1d /= 0) error stop 'ice_dyn_evp1d: buf1d /= 0 is not supported (static point list)'
   end subroutine read_nml
 
   !=============================================================================
@@ -95,7 +95,7 @@ contains
     integer(kind=int_kind) :: k, ios
 
     open(newunit=iu06, file='evp1d.log', status='replace', iostat=ios)
-    if (ios /= 0) stop 'Unable to open evp1d.log'
+    if (ios /= 0) error stop 'Unable to open evp1d.log'
     write(iu06,'(a)') 'Initializing evp1d (MI355X engine)'
     call read_nml()
     if (c_associated(handle)) call dyn_evp1d_finalize_handle()
@@ -217,7 +217,7 @@ contains
     type(evp1d_params_t) :: p
     integer(kind=int_kind) :: iw
 
-    if (.not. c_associated(handle)) stop 'dyn_evp1d_run: dyn_evp1d_init was not called'
+    if (.not. c_associated(handle)) error stop 'dyn_evp1d_run: dyn_evp1d_init was not called'
     allocate(G(nx,ny), GT(nx,ny), GU(nx,ny))
 
     ! skip flags (set_skipMe, ice_dyn_evp1d.F90:521-536)
@@ -327,7 +327,7 @@ contains
     allocate(G(nx,ny))
     write(fname,'(a,a,i6.6,a)') trim(mydebugF), '_', ts - 1, '.bin'
     open(newunit=lun, file=trim(fname), form='unformatted', access='stream', status='replace', iostat=ios)
-    if (ios /= 0) stop 'dyn_evp2d_dump: cannot open dump file'
+    if (ios /= 0) error stop 'dyn_evp2d_dump: cannot open dump file'
     call w(L_stressp_1);  call w(L_stressp_2);  call w(L_stressp_3);  call w(L_stressp_4)
     call w(L_stressm_1);  call w(L_stressm_2);  call w(L_stressm_3);  call w(L_stressm_4)
     call w(L_stress12_1); call w(L_stress12_2); call w(L_stress12_3); call w(L_stress12_4)
@@ -338,7 +338,7 @@ contains
     call w(L_vvel)
     close(lun)
     deallocate(G)
-    if (count(L_iceTmask) < 0 .or. count(L_iceUmask) < 0) stop 'unreachable'
+    if (count(L_iceTmask) < 0 .or. count(L_iceUmask) < 0) error stop 'unreachable'
   contains
     subroutine w(L)
       real(kind=dbl_kind), intent(in) :: L(:,:,:)
