@@ -180,13 +180,20 @@ def adjoint_measure(s: dict, ndte: int, t_fwd_step: float, nsnap: int, ndir: int
     vt = eng.get_field("vvel")[:s["na"]]
     reset_state(eng, s, {"strength": s["fields"]["strength"]})
     eng.run(ndte)                      # warm the forward graph and kernels
-    reset_state(eng, s)
-    eng.synchronize()
-    t0 = time.perf_counter()
-    g = eng.adjoint_run(ndte, ut, vt, nsnap=nsnap)
-    t_adj = time.perf_counter() - t0
-    log(f"[bench] adjoint ({nsnap} stored states): {t_adj:.3f}s, forward step {t_fwd_step:.3f}s, "
-        f"ratio {t_adj / t_fwd_step:.2f}")
+    timings = {}
+    grads = {}
+    for ns in sorted({min(nsnap, ndte), ndte}):
+        reset_state(eng, s)
+        eng.synchronize()
+        t0 = time.perf_counter()
+        grads[ns] = eng.adjoint_run(ndte, ut, vt, nsnap=ns)
+        timings[ns] = time.perf_counter() - t0
+        log(f"[bench] adjoint ({ns} stored states{', whole trajectory' if ns >= ndte else ''}): "
+            f"{timings[ns]:.3f}s, forward step {t_fwd_step:.3f}s, ratio {timings[ns] / t_fwd_step:.2f}")
+    g = grads[ndte]
+    same = all(np.array_equal(grads[ns][k], g[k]) for ns in grads
+               for k in ("strength", "cdn_ocn", "forcex", "forcey"))
+    t_adj = timings[ndte]
     rng = np.random.default_rng(1234)
     f = s["fields"]
 
@@ -223,7 +230,10 @@ def adjoint_measure(s: dict, ndte: int, t_fwd_step: float, nsnap: int, ndir: int
     log(f"[bench] adjoint checks: FD max rel err {fd_err:.2e} ({ndir} directions, eps 1e-6 relative), "
         f"tangent-linear dot product max rel err {dot_err:.2e} ({ndot} directions)")
     return {"t_adj_over_t_fwd": round(t_adj / t_fwd_step, 3), "t_adj_s": round(t_adj, 4),
-            "t_fwd_s": round(t_fwd_step, 4), "stored_states": nsnap, "loss": g["loss"],
+            "t_fwd_s": round(t_fwd_step, 4), "stored_states": ndte, "loss": g["loss"],
+            "checkpointed": {"stored_states": min(nsnap, ndte),
+                             "t_adj_over_t_fwd": round(timings[min(nsnap, ndte)] / t_fwd_step, 3)},
+            "gradients_identical": same,
             "fd_directions": ndir, "fd_max_rel_err": fd_err, "dot_directions": ndot,
             "dot_max_rel_err": dot_err,
             "grad_cdn_ocn_scalar": float(np.sum(g["cdn_ocn"]))}

@@ -93,6 +93,13 @@ struct DevView {
   double2 *vel;
   double *sig;
   double2 *s15, *s27, *s36, *s48;
+  // Split addressing (gradient trajectories): vel holds only the stepu slots
+  // [u_begin, u_end) and the str pairs only the stress slots [0, t_end); the
+  // other slots, whose values never change during a run, are read from velc
+  // and s15c..s48c.  vel_out receives stepu's result.  Unsplit kernels use
+  // vel for everything and vel_out == vel.
+  double2 *vel_out;
+  const double2 *velc, *s15c, *s27c, *s36c, *s48c;
   const double *dxT, *dyT, *strength, *hte, *htn, *htem1, *htnm1;
   const double *cw, *aiu, *uocn, *vocn, *waterx, *watery, *forcex, *forcey;
   const double *umassdti, *fm, *uarear, *uvel_init, *vvel_init, *tbu;
@@ -102,11 +109,26 @@ struct DevView {
   Scalars s;
 };
 
+template <bool SPLIT>
+struct Ld {
+  __device__ static __forceinline__ double2 vel(const DevView &d, int j) {
+    if (!SPLIT) return d.vel[j];
+    return (j >= d.u_begin && j < d.u_end) ? d.vel[j - d.u_begin] : d.velc[j];
+  }
+  __device__ static __forceinline__ double2 pair(const double2 *a, const double2 *c, const DevView &d, int j) {
+    if (!SPLIT) return a[j];
+    return j < d.t_end ? a[j] : c[j];
+  }
+  __device__ static __forceinline__ int vslot(const DevView &d, int j) { return SPLIT ? j - d.u_begin : j; }
+};
+
+template <bool SPLIT = false>
 __device__ __forceinline__ void stress_at(const DevView &d, int iw) {
-  const double2 cc = d.vel[iw];
-  const double2 ve = d.vel[d.ee[iw]];
-  const double2 vn = d.vel[d.ne[iw]];
-  const double2 vs = d.vel[d.se[iw]];
+  using L = Ld<SPLIT>;
+  const double2 cc = L::vel(d, iw);
+  const double2 ve = L::vel(d, d.ee[iw]);
+  const double2 vn = L::vel(d, d.ne[iw]);
+  const double2 vs = L::vel(d, d.se[iw]);
   double sig[12];
 #pragma unroll
   for (int c = 0; c < 12; ++c) sig[c] = d.sig[c * d.ldsig + iw];
@@ -126,19 +148,21 @@ __device__ __forceinline__ void stress_at(const DevView &d, int iw) {
   d.s48[iw] = make_double2(o.str[3], o.str[7]);
 }
 
+template <bool SPLIT = false>
 __device__ __forceinline__ void stepu_at(const DevView &d, int iw, bool write_diag) {
-  const double2 v = d.vel[iw];
-  const double2 a = d.s15[iw];
-  const double2 b = d.s27[d.nw[iw]];
-  const double2 c = d.s36[d.sse[iw]];
-  const double2 e = d.s48[d.sw[iw]];
+  using L = Ld<SPLIT>;
+  const double2 v = L::vel(d, iw);
+  const double2 a = L::pair(d.s15, d.s15c, d, iw);
+  const double2 b = L::pair(d.s27, d.s27c, d, d.nw[iw]);
+  const double2 c = L::pair(d.s36, d.s36c, d, d.sse[iw]);
+  const double2 e = L::pair(d.s48, d.s48c, d, d.sw[iw]);
   evp1d::StepuOut o;
   evp1d::stepu_point(d.s, v.x, v.y, d.cw[iw], d.aiu[iw], d.uocn[iw], d.vocn[iw],
                      d.waterx[iw], d.watery[iw], d.forcex[iw], d.forcey[iw],
                      d.umassdti[iw], d.fm[iw], d.uarear[iw], d.uvel_init[iw],
                      d.vvel_init[iw], d.tbu[iw],
                      a.x, b.x, c.x, e.x, a.y, c.y, b.y, e.y, o);
-  d.vel[iw] = make_double2(o.uvel, o.vvel);
+  d.vel_out[L::vslot(d, iw)] = make_double2(o.uvel, o.vvel);
   // strintx/strinty/Cb are overwritten every subcycle and read by nothing
   // inside the loop; only the values of the final subcycle are observable.
   if (write_diag) {
@@ -180,6 +204,16 @@ template <int B, bool X>
 __global__ __launch_bounds__(B) void k_stepu(DevView d, int write_diag) {
   const int iw = d.u_begin + tile_of(blockIdx.x, gridDim.x, X) * B + threadIdx.x;
   if (iw < d.u_end) stepu_at(d, iw, write_diag != 0);
+}
+
+// split-addressed forward kernels of the stored-trajectory gradient sweep
+__global__ __launch_bounds__(256) void k_stress_split(DevView d) {
+  const int iw = blockIdx.x * 256 + threadIdx.x;
+  if (iw < d.t_end) stress_at<true>(d, iw);
+}
+__global__ __launch_bounds__(256) void k_stepu_split(DevView d, int write_diag) {
+  const int iw = d.u_begin + blockIdx.x * 256 + threadIdx.x;
+  if (iw < d.u_end) stepu_at<true>(d, iw, write_diag != 0);
 }
 
 // launch configurations the autotuner chooses from
@@ -256,15 +290,29 @@ __global__ __launch_bounds__(256) void k_str_recompute(DevView d) {
   d.s48[iw] = make_double2(o.str[3], o.str[7]);
 }
 
+// Reverse of stepu for subcycle k.  The adjoint of u_k at q is the stepu
+// part stored in ub (from subcycle k+1's stepu adjoint, or the seed) plus
+// the contributions of subcycle k+1's stress adjoints that read q: its own
+// (cc), and those of nw(q), sw(q), sse(q), which read q as their ee, ne, se
+// neighbour (the inverse maps).  Only stepu slots need it: nothing else reads
+// the velocity adjoint.
+template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_stepu_adj(DevView d, AdjView a) {
+  using L = Ld<SPLIT>;
   const int iw = d.u_begin + blockIdx.x * 256 + threadIdx.x;
   if (iw >= d.u_end) return;
-  const double2 v = d.vel[iw];
-  const double2 s15 = d.s15[iw];
-  const double2 s27 = d.s27[d.nw[iw]];
-  const double2 s36 = d.s36[d.sse[iw]];
-  const double2 s48 = d.s48[d.sw[iw]];
-  const double2 ub = a.ub[iw];
+  const int jw = d.nw[iw], js = d.sse[iw], jsw = d.sw[iw];
+  const long na = d.na;
+  double2 ub = a.ub[iw];
+  if (iw < d.t_end) { const double2 t = a.vc[iw]; ub.x += t.x; ub.y += t.y; }
+  if (jw < d.t_end) { const double2 t = a.vc[na + jw]; ub.x += t.x; ub.y += t.y; }
+  if (jsw < d.t_end) { const double2 t = a.vc[3 * na + jsw]; ub.x += t.x; ub.y += t.y; }
+  if (js < d.t_end) { const double2 t = a.vc[2 * na + js]; ub.x += t.x; ub.y += t.y; }
+  const double2 v = L::vel(d, iw);
+  const double2 s15 = L::pair(d.s15, d.s15c, d, iw);
+  const double2 s27 = L::pair(d.s27, d.s27c, d, jw);
+  const double2 s36 = L::pair(d.s36, d.s36c, d, js);
+  const double2 s48 = L::pair(d.s48, d.s48c, d, jsw);
   evp1d::StepuAdj o;
   evp1d::stepu_adjoint(d.s, v.x, v.y, d.cw[iw], d.aiu[iw], d.uocn[iw], d.vocn[iw], d.waterx[iw],
                        d.watery[iw], d.forcex[iw], d.forcey[iw], d.umassdti[iw], d.fm[iw], d.uarear[iw],
@@ -277,13 +325,16 @@ __global__ __launch_bounds__(256) void k_stepu_adj(DevView d, AdjView a) {
   a.gcw[iw] += o.cw_b;
 }
 
-// str_b of point q: own terms and the terms stepu took from q as the nw, sse
-// and sw neighbour of the points ee(q), se(q) and ne(q) (the inverse maps).
+// Reverse of stress for subcycle k.  str_b of point q: its own terms and the
+// terms stepu took from q as the nw, sse and sw neighbour of the points
+// ee(q), se(q) and ne(q).
+template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_stress_adj(DevView d, AdjView a) {
+  using L = Ld<SPLIT>;
   const int iw = blockIdx.x * 256 + threadIdx.x;
   if (iw >= d.t_end) return;
   const int ie = d.ee[iw], in = d.ne[iw], is = d.se[iw];
-  const double2 cc = d.vel[iw], ve = d.vel[ie], vn = d.vel[in], vs = d.vel[is];
+  const double2 cc = L::vel(d, iw), ve = L::vel(d, ie), vn = L::vel(d, in), vs = L::vel(d, is);
   const double2 g0 = a.g[iw], ge = a.g[ie], gs = a.g[is], gn = a.g[in];
   const double str_b[8] = {g0.x, ge.x, gs.x, gn.x, g0.y, gs.y, ge.y, gn.y};
   double sb[12];
@@ -300,21 +351,6 @@ __global__ __launch_bounds__(256) void k_stress_adj(DevView d, AdjView a) {
   a.vc[2 * na + iw] = make_double2(o.vel_b[4], o.vel_b[5]);
   a.vc[3 * na + iw] = make_double2(o.vel_b[6], o.vel_b[7]);
   a.gstr[iw] += o.strength_b;
-}
-
-// u_b(q) += contributions of the stress adjoints that read q: its own (cc), and
-// those of nw(q), sw(q), sse(q), which read q as their ee, ne, se neighbour.
-__global__ __launch_bounds__(256) void k_vel_gather(DevView d, AdjView a) {
-  const int q = blockIdx.x * 256 + threadIdx.x;
-  if (q >= d.na) return;
-  const long na = d.na;
-  double2 acc = a.ub[q];
-  if (q < d.t_end) { const double2 t = a.vc[q]; acc.x += t.x; acc.y += t.y; }
-  const int pw = d.nw[q], ps = d.sw[q], pe = d.sse[q];
-  if (pw < d.t_end) { const double2 t = a.vc[na + pw]; acc.x += t.x; acc.y += t.y; }
-  if (ps < d.t_end) { const double2 t = a.vc[3 * na + ps]; acc.x += t.x; acc.y += t.y; }
-  if (pe < d.t_end) { const double2 t = a.vc[2 * na + pe]; acc.x += t.x; acc.y += t.y; }
-  a.ub[q] = acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -883,6 +919,9 @@ static int create_impl(int32_t nmem, const int32_t *mna, const int32_t *mnavel, 
   A((void **)&h->scratch, (size_t)navel * sizeof(double));
   A((void **)&h->ptmp, (size_t)na * sizeof(double2));
   if (rc) { release(h); return rc; }
+  v.vel_out = v.vel;
+  v.velc = v.vel;
+  v.s15c = v.s15; v.s27c = v.s27; v.s36c = v.s36; v.s48c = v.s48;
   v.ee = h->idx_dev[0]; v.ne = h->idx_dev[1]; v.se = h->idx_dev[2];
   v.nw = h->idx_dev[3]; v.sw = h->idx_dev[4]; v.sse = h->idx_dev[5];
   v.nhalo = 0;
@@ -1369,6 +1408,7 @@ struct AdjointRun {
     DevView fv = h->v;
     fv.sig = final_state.sig;
     fv.vel = final_state.vel;
+    fv.vel_out = final_state.vel;
     DevView keep = h->v;
     h->v = fv;
     int rc = launch_stress(h);
@@ -1402,15 +1442,69 @@ struct AdjointRun {
       HIP_TRY(hipGetLastError());
     }
     if (d.u_end > d.u_begin) {
-      hipLaunchKernelGGL(k_stepu_adj, dim3(blocks(d.u_end - d.u_begin)), dim3(256), 0, h->stream, d, a);
+      hipLaunchKernelGGL(k_stepu_adj<false>, dim3(blocks(d.u_end - d.u_begin)), dim3(256), 0, h->stream, d, a);
       HIP_TRY(hipGetLastError());
     }
     if (d.t_end > 0) {
-      hipLaunchKernelGGL(k_stress_adj, dim3(blocks(d.t_end)), dim3(256), 0, h->stream, d, a);
+      hipLaunchKernelGGL(k_stress_adj<false>, dim3(blocks(d.t_end)), dim3(256), 0, h->stream, d, a);
       HIP_TRY(hipGetLastError());
     }
-    if (d.na > 0) {
-      hipLaunchKernelGGL(k_vel_gather, dim3(blocks(d.na)), dim3(256), 0, h->stream, d, a);
+return EVP1D_OK;
+  }
+  // All nsub states kept (as much HBM as the trajectory needs): the forward
+  // sweep writes each subcycle's velocities and str1..8 out of place into
+  // per-subcycle buffers (split addressing: only the slots a subcycle changes),
+  // the reverse sweep reads them back; no recomputation, no state copies.
+  int run_stored() {
+    const DevView W = h->v;
+    const int N = nsub, nu = W.u_end - W.u_begin, nt = W.t_end;
+    const size_t ub = (size_t)N * nu * sizeof(double2), sb = (size_t)(N > 1 ? N - 1 : 1) * nt * sizeof(double2);
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    if (ub + 4 * sb + ((size_t)1 << 30) > free_b)
+      return fail(EVP1D_ENOMEM, "evp1d_adjoint_run: storing %d states needs %.1f GB, %.1f GB free; use fewer", N,
+                  (ub + 4 * sb) / 1e9, free_b / 1e9);
+    double2 *U = nullptr, *S[4] = {nullptr, nullptr, nullptr, nullptr};
+    RC_TRY(alloc((void **)&U, ub));
+    for (int c = 0; c < 4; ++c) RC_TRY(alloc((void **)&S[c], sb));
+    if (nu > 0)
+      HIP_TRY(hipMemcpyAsync(U, W.vel + W.u_begin, (size_t)nu * sizeof(double2), hipMemcpyDeviceToDevice, h->stream));
+    auto view = [&](int k) {   // the split view of subcycle k (x_{k-1} -> x_k)
+      DevView dk = W;
+      dk.vel = U + (size_t)(k - 1) * nu;
+      dk.velc = W.vel;
+      dk.s15c = W.s15; dk.s27c = W.s27; dk.s36c = W.s36; dk.s48c = W.s48;
+      if (k < N) {
+        const size_t o = (size_t)(k - 1) * nt;
+        dk.s15 = S[0] + o; dk.s27 = S[1] + o; dk.s36 = S[2] + o; dk.s48 = S[3] + o;
+        dk.vel_out = U + (size_t)k * nu;
+      } else {
+        dk.vel_out = W.vel + W.u_begin;
+      }
+      return dk;
+    };
+    for (int k = 1; k <= N; ++k) {
+      const DevView dk = view(k);
+      if (nt > 0) hipLaunchKernelGGL(k_stress_split, dim3(blocks(nt)), dim3(256), 0, h->stream, dk);
+      if (nu > 0) hipLaunchKernelGGL(k_stepu_split, dim3(blocks(nu)), dim3(256), 0, h->stream, dk, k == N ? 1 : 0);
+      HIP_TRY(hipGetLastError());
+    }
+    // the handle now holds x_N (stresses, velocities, str1..8, strint, Cb)
+    if (h->na > 0) {
+      hipLaunchKernelGGL(k_seed, dim3(blocks(h->na)), dim3(256), 0, h->stream, W, a, (const double *)ut,
+                         (const double *)vt, terms);
+      HIP_TRY(hipGetLastError());
+    }
+    std::vector<double> t((size_t)h->na);
+    if (h->na > 0)
+      HIP_TRY(hipMemcpyAsync(t.data(), terms, (size_t)h->na * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    loss = 0.0;
+    for (double x : t) loss += x;
+    for (int k = N; k >= 1; --k) {
+      const DevView dk = view(k);
+      if (nu > 0) hipLaunchKernelGGL(k_stepu_adj<true>, dim3(blocks(nu)), dim3(256), 0, h->stream, dk, a);
+      if (nt > 0) hipLaunchKernelGGL(k_stress_adj<true>, dim3(blocks(nt)), dim3(256), 0, h->stream, dk, a);
       HIP_TRY(hipGetLastError());
     }
     return EVP1D_OK;
@@ -1487,27 +1581,31 @@ int evp1d_adjoint_run(evp1d_handle *h, int32_t nsub, int32_t nsnap, const double
   RC_TRY(R.alloc((void **)&R.ut, nA));
   RC_TRY(R.alloc((void **)&R.vt, nA));
   RC_TRY(R.alloc((void **)&R.terms, nA));
-  RC_TRY(R.alloc((void **)&R.str_save, 4 * nV));
-  RC_TRY(R.alloc((void **)&R.final_state.sig, R.sig_bytes()));
-  RC_TRY(R.alloc((void **)&R.final_state.vel, nV));
-  R.slots.resize((size_t)S);
-  R.used.assign((size_t)S, 0);
-  for (int k = 0; k < S; ++k) {
-    RC_TRY(R.alloc((void **)&R.slots[(size_t)k].sig, R.sig_bytes()));
-    RC_TRY(R.alloc((void **)&R.slots[(size_t)k].vel, (size_t)h->na * sizeof(double2)));
-  }
   RC_TRY(upload_caller_order(h, R.ut, u_target));
   RC_TRY(upload_caller_order(h, R.vt, v_target));
-  R.sched.build(nsub, S - 1);
-  R.working = 0;
-  const int s0 = R.take(0);
-  if (s0 < 0) return -s0;
-  RC_TRY(R.rev(0, nsub, S - 1, s0));
-  // leave the handle in the forward final state x_N
-  RC_TRY(R.copy_state(R.work(), R.final_state, false));
-  double2 *pairs[4] = {h->v.s15, h->v.s27, h->v.s36, h->v.s48};
-  for (int k = 0; k < 4; ++k)
-    HIP_TRY(hipMemcpyAsync(pairs[k], R.str_save + (size_t)k * h->navel, nV, hipMemcpyDeviceToDevice, h->stream));
+  if (nsnap >= nsub) {
+    RC_TRY(R.run_stored());
+  } else {
+    RC_TRY(R.alloc((void **)&R.str_save, 4 * nV));
+    RC_TRY(R.alloc((void **)&R.final_state.sig, R.sig_bytes()));
+    RC_TRY(R.alloc((void **)&R.final_state.vel, nV));
+    R.slots.resize((size_t)S);
+    R.used.assign((size_t)S, 0);
+    for (int k = 0; k < S; ++k) {
+      RC_TRY(R.alloc((void **)&R.slots[(size_t)k].sig, R.sig_bytes()));
+      RC_TRY(R.alloc((void **)&R.slots[(size_t)k].vel, (size_t)h->na * sizeof(double2)));
+    }
+    R.sched.build(nsub, S - 1);
+    R.working = 0;
+    const int s0 = R.take(0);
+    if (s0 < 0) return -s0;
+    RC_TRY(R.rev(0, nsub, S - 1, s0));
+    // leave the handle in the forward final state x_N
+    RC_TRY(R.copy_state(R.work(), R.final_state, false));
+    double2 *pairs[4] = {h->v.s15, h->v.s27, h->v.s36, h->v.s48};
+    for (int k = 0; k < 4; ++k)
+      HIP_TRY(hipMemcpyAsync(pairs[k], R.str_save + (size_t)k * h->navel, nV, hipMemcpyDeviceToDevice, h->stream));
+  }
   RC_TRY(download_caller_order(h, g_strength, R.a.gstr));
   RC_TRY(download_caller_order(h, g_cdn_ocn, R.a.gcw));
   RC_TRY(download_caller_order(h, g_forcex, R.a.gfx));

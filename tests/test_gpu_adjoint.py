@@ -10,7 +10,8 @@ Checks:
     to 1e-10 relative;
   * central finite differences (eps = 1e-6 relative per component) agree
     with <grad, d> to 1e-6 relative;
-  * the gradient does not depend on the number of stored states.
+  * the gradient does not depend on the number of stored states, from one
+    stored state (recompute everything) to all of them (no recomputation).
 """
 import numpy as np
 import pytest
@@ -67,11 +68,12 @@ def test_tangent_and_adjoint_leave_exact_forward_state(problem):
     with make_engine(st, NDTE) as eng:
         eng.tangent_run(NDTE, ut, vt, d["strength"], d["cdn_ocn"], d["forcexU"], d["forceyU"])
         assert_state_bitexact(fetch_state(eng, st), want)
-    with make_engine(st, NDTE) as eng:
-        g = eng.adjoint_run(NDTE, ut, vt, nsnap=6)
-        assert_state_bitexact(fetch_state(eng, st), want)
-    with make_engine(want, NDTE) as eng:
-        assert abs(eng.loss(ut, vt) - g["loss"]) <= 1e-15 * max(1.0, g["loss"])
+    for nsnap in (6, NDTE):     # checkpointed; every state stored
+        with make_engine(st, NDTE) as eng:
+            g = eng.adjoint_run(NDTE, ut, vt, nsnap=nsnap)
+            assert_state_bitexact(fetch_state(eng, st), want)
+        with make_engine(want, NDTE) as eng:
+            assert abs(eng.loss(ut, vt) - g["loss"]) <= 1e-15 * max(1.0, g["loss"])
 
 
 def test_dot_product_tangent_vs_adjoint(problem):
