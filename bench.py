@@ -286,7 +286,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="evp_c1")
-    ap.add_argument("--mode", default="auto", choices=("auto", "graph", "persistent", "launch"))
+    ap.add_argument("--mode", default="auto", choices=("auto", "graph", "persistent", "launch", "fused"))
+    ap.add_argument("--tiling", default=None, metavar="W,H,BLOCK,XCD",
+                    help="fused-form tile (grid cells) and XCD-aware tile order")
     ap.add_argument("--fma", action="store_true", help="time libevp1d_fma.so instead of the exact build")
     ap.add_argument("--kernel-reps", type=int, default=200, help="launches per individual-kernel timing")
     ap.add_argument("--cpu-subcycles", type=int, default=1000)
@@ -323,16 +325,23 @@ def main() -> int:
     eng = new_engine(s, ndte, args.mode, args.fma)
     if args.tuning:
         eng.set_tuning(*[int(x) for x in args.tuning.split(",")])
+    if args.tiling:
+        eng.set_tiling(*[int(x) for x in args.tiling.split(",")])
     if args.autotune:
         log(f"[bench] autotune: {eng.autotune(args.autotune)}")
     tuning = eng.get_tuning()
+    tiling = eng.get_tiling()
+    # the form evp1d_run takes: auto is the fused form whenever the lists tile
+    fused = args.mode == "fused" or (args.mode == "auto" and tiling["ntiles"] > 0)
+    log(f"[bench] mode {args.mode} ({'fused' if fused else 'unfused'}), tiling {tiling}, tuning {tuning}")
     secs = time_steps(eng, ndte, args.steps, args.warmup)
     value = na * ndte * args.steps / secs
 
     # per-kernel average launch durations (HIP events, the library's stream)
     B = algorithmic_bytes(na, nT, nU)
+    B["fused"] = B["subcycle"]
     kernels = {}
-    for which in ("stress", "stepu"):
+    for which in ("stress", "stepu") + (("fused",) if fused else ()):
         eng.synchronize()
         eng.timer_start()
         eng.run_individual(which, args.kernel_reps)
@@ -345,6 +354,8 @@ def main() -> int:
     step_us = secs / (args.steps * ndte) * 1e6
     if args.mode == "persistent":
         dom = {"name": "k_persistent", "bytes": B["subcycle"] * ndte, "avg_us": secs / args.steps * 1e6}
+    elif fused:
+        dom = {"name": "k_fused", "bytes": B["fused"], "avg_us": kernels["fused"]["avg_us"]}
     else:
         name = max(kernels, key=lambda k: kernels[k]["avg_us"])
         dom = {"name": "k_" + name, "bytes": B[name], "avg_us": kernels[name]["avg_us"]}
@@ -384,7 +395,8 @@ def main() -> int:
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{args.workload} {w['nx']}x{w['ny']} ndte={ndte}", "nx": w["nx"],
                    "ny": w["ny"], "seed": w["seed"], "na": na, "navel": s["navel"], "nT": nT, "nU": nU,
-                   "ndte": ndte, "mode": args.mode, "build": "fma" if args.fma else "exact", "tuning": tuning,
+                   "ndte": ndte, "mode": args.mode, "form": "fused" if fused else "unfused", "tiling": tiling,
+                   "build": "fma" if args.fma else "exact", "tuning": tuning,
                    "us_per_subcycle": round(step_us, 3), "loop_GBps_algorithmic": round(whole, 1),
                    "loop_frac_of_peak": round(whole / HBM_PEAK_GBS, 4)},
         "kernels": kernels,

@@ -36,10 +36,12 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "../../include/evp1d.h"
 #include "evp1d_adjoint.hpp"
+#include "evp1d_tiles.hpp"
 #include "evp1d_point.hpp"
 
 #ifndef EVP1D_FMA
@@ -109,6 +111,38 @@ struct DevView {
   Scalars s;
 };
 
+// Cache policy of the forward kernels' streams.  Each switch, when 1, marks
+// that stream's accesses nontemporal (streamed past the caches):
+//   EVP1D_NT_FIELDS  loads of the per-point constants (geometry, forcing)
+//   EVP1D_NT_SIG     loads and stores of the 12 stresses
+//   EVP1D_NT_STR     stores of str1..8 by stress
+#ifndef EVP1D_NT_FIELDS
+#define EVP1D_NT_FIELDS 0
+#endif
+#ifndef EVP1D_NT_SIG
+#define EVP1D_NT_SIG 0
+#endif
+#ifndef EVP1D_NT_STR
+#define EVP1D_NT_STR 0
+#endif
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+template <bool NT>
+__device__ __forceinline__ double ld(const double *p, long i) {
+  if (NT) return __builtin_nontemporal_load(p + i);
+  return p[i];
+}
+template <bool NT>
+__device__ __forceinline__ void st(double *p, long i, double v) {
+  if (NT) __builtin_nontemporal_store(v, p + i);
+  else p[i] = v;
+}
+template <bool NT>
+__device__ __forceinline__ void st2(double2 *p, long i, double x, double y) {
+  if (NT) __builtin_nontemporal_store(dvec2{x, y}, reinterpret_cast<dvec2 *>(p) + i);
+  else p[i] = make_double2(x, y);
+}
+#define LDF(a, i) ld<EVP1D_NT_FIELDS != 0>(a, i)
+
 template <bool SPLIT>
 struct Ld {
   __device__ static __forceinline__ double2 vel(const DevView &d, int j) {
@@ -122,30 +156,38 @@ struct Ld {
   __device__ static __forceinline__ int vslot(const DevView &d, int j) { return SPLIT ? j - d.u_begin : j; }
 };
 
+// stress at slot iw from the view's velocities and stresses (no stores)
 template <bool SPLIT = false>
-__device__ __forceinline__ void stress_at(const DevView &d, int iw) {
+__device__ __forceinline__ void stress_eval(const DevView &d, int iw, evp1d::StressOut &o) {
   using L = Ld<SPLIT>;
   const double2 cc = L::vel(d, iw);
   const double2 ve = L::vel(d, d.ee[iw]);
   const double2 vn = L::vel(d, d.ne[iw]);
   const double2 vs = L::vel(d, d.se[iw]);
+  constexpr bool NS = EVP1D_NT_SIG != 0;
   double sig[12];
 #pragma unroll
-  for (int c = 0; c < 12; ++c) sig[c] = d.sig[c * d.ldsig + iw];
-  evp1d::StressOut o;
+  for (int c = 0; c < 12; ++c) sig[c] = ld<NS>(d.sig, c * d.ldsig + iw);
   evp1d::stress_point(d.s, cc.x, cc.y, ve.x, ve.y, vs.x, vs.y, vn.x, vn.y,
-                      d.dxT[iw], d.dyT[iw], d.strength[iw], d.hte[iw], d.htn[iw],
-                      d.htem1[iw], d.htnm1[iw], sig, o);
+                      LDF(d.dxT, iw), LDF(d.dyT, iw), LDF(d.strength, iw), LDF(d.hte, iw), LDF(d.htn, iw),
+                      LDF(d.htem1, iw), LDF(d.htnm1, iw), sig, o);
+}
+
+template <bool SPLIT = false>
+__device__ __forceinline__ void stress_at(const DevView &d, int iw) {
+  constexpr bool NS = EVP1D_NT_SIG != 0, NR = EVP1D_NT_STR != 0;
+  evp1d::StressOut o;
+  stress_eval<SPLIT>(d, iw, o);
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    d.sig[c * d.ldsig + iw] = o.sp[c];
-    d.sig[(4 + c) * d.ldsig + iw] = o.sm[c];
-    d.sig[(8 + c) * d.ldsig + iw] = o.s12[c];
+    st<NS>(d.sig, c * d.ldsig + iw, o.sp[c]);
+    st<NS>(d.sig, (4 + c) * d.ldsig + iw, o.sm[c]);
+    st<NS>(d.sig, (8 + c) * d.ldsig + iw, o.s12[c]);
   }
-  d.s15[iw] = make_double2(o.str[0], o.str[4]);
-  d.s27[iw] = make_double2(o.str[1], o.str[6]);
-  d.s36[iw] = make_double2(o.str[2], o.str[5]);
-  d.s48[iw] = make_double2(o.str[3], o.str[7]);
+  st2<NR>(d.s15, iw, o.str[0], o.str[4]);
+  st2<NR>(d.s27, iw, o.str[1], o.str[6]);
+  st2<NR>(d.s36, iw, o.str[2], o.str[5]);
+  st2<NR>(d.s48, iw, o.str[3], o.str[7]);
 }
 
 template <bool SPLIT = false>
@@ -157,10 +199,10 @@ __device__ __forceinline__ void stepu_at(const DevView &d, int iw, bool write_di
   const double2 c = L::pair(d.s36, d.s36c, d, d.sse[iw]);
   const double2 e = L::pair(d.s48, d.s48c, d, d.sw[iw]);
   evp1d::StepuOut o;
-  evp1d::stepu_point(d.s, v.x, v.y, d.cw[iw], d.aiu[iw], d.uocn[iw], d.vocn[iw],
-                     d.waterx[iw], d.watery[iw], d.forcex[iw], d.forcey[iw],
-                     d.umassdti[iw], d.fm[iw], d.uarear[iw], d.uvel_init[iw],
-                     d.vvel_init[iw], d.tbu[iw],
+  evp1d::stepu_point(d.s, v.x, v.y, LDF(d.cw, iw), LDF(d.aiu, iw), LDF(d.uocn, iw), LDF(d.vocn, iw),
+                     LDF(d.waterx, iw), LDF(d.watery, iw), LDF(d.forcex, iw), LDF(d.forcey, iw),
+                     LDF(d.umassdti, iw), LDF(d.fm, iw), LDF(d.uarear, iw), LDF(d.uvel_init, iw),
+                     LDF(d.vvel_init, iw), LDF(d.tbu, iw),
                      a.x, b.x, c.x, e.x, a.y, c.y, b.y, e.y, o);
   d.vel_out[L::vslot(d, iw)] = make_double2(o.uvel, o.vvel);
   // strintx/strinty/Cb are overwritten every subcycle and read by nothing
@@ -215,6 +257,107 @@ __global__ __launch_bounds__(256) void k_stepu_split(DevView d, int write_diag) 
   const int iw = d.u_begin + blockIdx.x * 256 + threadIdx.x;
   if (iw < d.u_end) stepu_at<true>(d, iw, write_diag != 0);
 }
+
+// ---------------------------------------------------------------------------
+// fused subcycle: stress and stepu of one tile of points per workgroup
+// ---------------------------------------------------------------------------
+// Tile t owns the stress slots [a0,a1) and [b0,b1) and the stepu slots
+// [b0,b1) and [c0,c1).  Its LDS table holds str1..8 of [owned | halo |
+// constant] entries: owned and halo entries are evaluated here from the
+// input velocities and stresses (halo entries belong to other tiles and are
+// not stored), constant entries (points whose str1..8 stress never writes)
+// come from cstr.  Stepu then reads its own, nw, sse and sw entries from LDS.
+// Velocities and stresses ping-pong between two buffers (in: d.vel, d.sig;
+// out: f.vel_out, f.sig_out), so neighbour tiles never see a half-updated
+// subcycle.
+struct FusedView {
+  const evp1d::TileDesc *tiles;
+  const int32_t *hlist;     // halo slots of every tile
+  const uint2 *uloc;        // per stepu slot (slot - u_begin): own | nw << 16, sse | sw << 16
+  const double2 *cstr;      // [4][nk]: s15, s27, s36, s48 of the constant entries
+  long nk;
+  double2 *vel_out;
+  double *sig_out;
+  int cap;                  // entries per LDS table
+};
+
+template <int B, bool X>
+__global__ __launch_bounds__(B) void k_fused(DevView d, FusedView f, int last) {
+  extern __shared__ double2 lds[];
+  const int cap = f.cap;
+  const evp1d::TileDesc t = f.tiles[tile_of(blockIdx.x, gridDim.x, X)];
+  const int nA = t.a1 - t.a0, nB = t.b1 - t.b0, nO = nA + nB, nS = nO + t.nh, nE = nS + t.nk;
+  for (int k = threadIdx.x; k < nE; k += B) {
+    double2 p15, p27, p36, p48;
+    if (k < nS) {
+      const int g = k < nA ? t.a0 + k : (k < nO ? t.b0 + (k - nA) : f.hlist[t.h0 + (k - nO)]);
+      evp1d::StressOut o;
+      stress_eval(d, g, o);
+      p15 = make_double2(o.str[0], o.str[4]);
+      p27 = make_double2(o.str[1], o.str[6]);
+      p36 = make_double2(o.str[2], o.str[5]);
+      p48 = make_double2(o.str[3], o.str[7]);
+      if (k < nO) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          f.sig_out[c * d.ldsig + g] = o.sp[c];
+          f.sig_out[(4 + c) * d.ldsig + g] = o.sm[c];
+          f.sig_out[(8 + c) * d.ldsig + g] = o.s12[c];
+        }
+        if (last) {   // str1..8 are observable after the loop
+          d.s15[g] = p15;
+          d.s27[g] = p27;
+          d.s36[g] = p36;
+          d.s48[g] = p48;
+        }
+      }
+    } else {
+      const long j = t.k0 + (k - nS);
+      p15 = f.cstr[j];
+      p27 = f.cstr[f.nk + j];
+      p36 = f.cstr[2 * f.nk + j];
+      p48 = f.cstr[3 * f.nk + j];
+    }
+    lds[k] = p15;
+    lds[cap + k] = p27;
+    lds[2 * cap + k] = p36;
+    lds[3 * cap + k] = p48;
+  }
+  __syncthreads();
+  const int nU = nB + (t.c1 - t.c0);
+  for (int k = threadIdx.x; k < nU; k += B) {
+    const int q = k < nB ? t.b0 + k : t.c0 + (k - nB);
+    const uint2 w = f.uloc[q - d.u_begin];
+    const double2 a = lds[w.x & 0xffffu];
+    const double2 b = lds[cap + (w.x >> 16)];
+    const double2 c = lds[2 * cap + (w.y & 0xffffu)];
+    const double2 e = lds[3 * cap + (w.y >> 16)];
+    const double2 v = d.vel[q];
+    evp1d::StepuOut o;
+    evp1d::stepu_point(d.s, v.x, v.y, d.cw[q], d.aiu[q], d.uocn[q], d.vocn[q], d.waterx[q], d.watery[q],
+                       d.forcex[q], d.forcey[q], d.umassdti[q], d.fm[q], d.uarear[q], d.uvel_init[q],
+                       d.vvel_init[q], d.tbu[q], a.x, b.x, c.x, e.x, a.y, c.y, b.y, e.y, o);
+    f.vel_out[q] = make_double2(o.uvel, o.vvel);
+    if (last) {
+      d.strintx[q] = o.strintx;
+      d.strinty[q] = o.strinty;
+      d.cb[q] = o.Cb;
+    }
+  }
+}
+
+// str1..8 of the constant entries, gathered once per run
+__global__ void k_fill_const(DevView d, const int32_t *klist, double2 *cstr, long nk) {
+  const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nk) return;
+  const int g = klist[j];
+  cstr[j] = d.s15[g];
+  cstr[nk + j] = d.s27[g];
+  cstr[2 * nk + j] = d.s36[g];
+  cstr[3 * nk + j] = d.s48[g];
+}
+
+constexpr int kFusedMaxEntries = 1024;   // 64 KiB of LDS per workgroup
 
 // launch configurations the autotuner chooses from
 constexpr int kNumBlocks = 5;
@@ -526,6 +669,19 @@ struct evp1d_handle {
   std::vector<int32_t> m_na, m_navel, a_off, g_off;
   std::vector<uint8_t> idx_mset[6];   // per member: list set
   int coop_blocks = 0;
+  // fused form: tiling and its device tables (owned here, replaced on relayout)
+  int tile_w = 32, tile_h = 16, tile_block = 256, tile_xcd = 1;
+  bool fused_ok = false;
+  std::string fused_why = "lists not set";
+  int fused_ntiles = 0, fused_cap = 0;
+  long nk_total = 0;
+  evp1d::TileDesc *tiles_dev = nullptr;
+  int32_t *hlist_dev = nullptr, *klist_dev = nullptr;
+  uint2 *uloc_dev = nullptr;
+  double2 *cstr_dev = nullptr;
+  double2 *vel_alt = nullptr;      // second velocity buffer (navel)
+  double *sig_alt = nullptr;       // second stress buffer (12 x ldsig)
+  int graph_mode = -1;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int graph_nsub = -1;
@@ -559,10 +715,26 @@ static void drop_graph(evp1d_handle *h) {
   h->graph_nsub = -1;
 }
 
+static void free_tiling(evp1d_handle *h) {
+  void *bufs[5] = {h->tiles_dev, h->hlist_dev, h->klist_dev, h->uloc_dev, h->cstr_dev};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  h->tiles_dev = nullptr;
+  h->hlist_dev = h->klist_dev = nullptr;
+  h->uloc_dev = nullptr;
+  h->cstr_dev = nullptr;
+  h->fused_ok = false;
+  h->fused_ntiles = h->fused_cap = 0;
+  h->nk_total = 0;
+}
+
 static void release(evp1d_handle *h) {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   drop_graph(h);
+  free_tiling(h);
+  if (h->vel_alt) (void)hipFree(h->vel_alt);
+  if (h->sig_alt) (void)hipFree(h->sig_alt);
   for (void *p : h->allocs) (void)hipFree(p);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -664,26 +836,56 @@ static void plain_arrays(evp1d_handle *h, std::vector<double *> &out) {
   for (int c = 0; c < 12; ++c) out.push_back(v.sig + (long)c * v.ldsig);
 }
 
-// Rebuild the class permutation from the skip flags and move every resident
-// array into the new slot order.
+// Upload the fused form's tables for a plan made on the current slot order.
+template <class T>
+static int upload_table(evp1d_handle *h, T **dst, const T *src, size_t n) {
+  HIP_TRY(hipMalloc((void **)dst, std::max<size_t>(n, 1) * sizeof(T)));
+  if (n) HIP_TRY(hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyHostToDevice, h->stream));
+  return EVP1D_OK;
+}
+
+static int install_tiling(evp1d_handle *h, const evp1d::TilePlan &plan) {
+  HIP_TRY(hipStreamSynchronize(h->stream));   // no launch may still read the old tables
+  drop_graph(h);
+  free_tiling(h);
+  h->fused_why = plan.why;
+  if (!plan.ok) return EVP1D_OK;
+  RC_TRY(upload_table(h, &h->tiles_dev, plan.tiles.data(), plan.tiles.size()));
+  RC_TRY(upload_table(h, &h->hlist_dev, plan.list.data(), plan.list.size()));
+  RC_TRY(upload_table(h, &h->klist_dev, plan.klist.data(), plan.klist.size()));
+  static_assert(sizeof(uint2) == 4 * sizeof(uint16_t), "uloc packing");
+  RC_TRY(upload_table(h, &h->uloc_dev, reinterpret_cast<const uint2 *>(plan.uloc.data()), plan.uloc.size() / 4));
+  h->nk_total = (long)plan.klist.size();
+  HIP_TRY(hipMalloc((void **)&h->cstr_dev, std::max<size_t>((size_t)4 * h->nk_total, 1) * sizeof(double2)));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->fused_ntiles = (int)plan.tiles.size();
+  h->fused_cap = std::max(plan.max_entries, 1);
+  h->fused_ok = true;
+  return EVP1D_OK;
+}
+
+// Rebuild the slot order (class, then fused tile, then caller position) from
+// the skip flags and the neighbour lists, and move every resident array into
+// it.  Without a complete set of lists the order is class, caller position.
 static int relayout(evp1d_handle *h) {
   const int na = h->na;
-  std::vector<int32_t> nperm((size_t)na), ninv((size_t)na);
-  int cnt[4] = {0, 0, 0, 0};
-  auto cls = [](uint8_t b) {
-    const bool t = !(b & 1), u = !(b & 2);
-    return t && !u ? 0 : (t && u ? 1 : (u ? 2 : 3));
-  };
-  for (int q = 0; q < na; ++q) ++cnt[cls(h->skip_host[(size_t)q])];
-  int start[4] = {0, cnt[0], cnt[0] + cnt[1], cnt[0] + cnt[1] + cnt[2]};
-  for (int q = 0; q < na; ++q) {
-    const int s = start[cls(h->skip_host[(size_t)q])]++;
-    nperm[(size_t)s] = q;
-    ninv[(size_t)q] = s;
+  const int32_t *idx[6];
+  bool all = na > 0;
+  for (int k = 0; k < 6; ++k) {
+    all = all && h->idx_set[k] && h->idx_host[k].size() == (size_t)na;
+    idx[k] = all ? h->idx_host[k].data() : nullptr;
   }
+  const evp1d::TilePlan plan = evp1d::plan_tiles(na, h->navel, all ? idx : nullptr, h->skip_host.data(), h->tile_w,
+                                                 h->tile_h, kFusedMaxEntries);
+  const std::string bad = evp1d::check_plan(plan, na, h->navel, idx, h->skip_host.data());
+  if (!bad.empty()) return fail(EVP1D_EINDEX, "slot layout: %s", bad.c_str());
+  const std::vector<int32_t> &nperm = plan.perm;
   if (na > 0 && nperm != h->perm) {
-    std::vector<int32_t> map((size_t)na);
-    for (int s = 0; s < na; ++s) map[(size_t)s] = h->inv[(size_t)nperm[(size_t)s]];
+    std::vector<int32_t> map((size_t)na), ninv((size_t)na);
+    for (int s = 0; s < na; ++s) {
+      map[(size_t)s] = h->inv[(size_t)nperm[(size_t)s]];
+      ninv[(size_t)nperm[(size_t)s]] = s;
+    }
     HIP_TRY(hipMemcpyAsync(h->map_dev, map.data(), (size_t)na * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
     std::vector<double *> arrs;
     plain_arrays(h, arrs);
@@ -701,12 +903,13 @@ static int relayout(evp1d_handle *h) {
     }
     HIP_TRY(hipMemcpyAsync(h->inv_dev, ninv.data(), (size_t)na * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
-    h->perm.swap(nperm);
+    h->perm = nperm;
     h->inv.swap(ninv);
   }
-  h->v.t_end = cnt[0] + cnt[1];
-  h->v.u_begin = cnt[0];
-  h->v.u_end = cnt[0] + cnt[1] + cnt[2];
+  RC_TRY(install_tiling(h, plan));
+  h->v.t_end = plan.cnt[0] + plan.cnt[1];
+  h->v.u_begin = plan.cnt[0];
+  h->v.u_end = plan.cnt[0] + plan.cnt[1] + plan.cnt[2];
   h->layout_dirty = false;
   h->idx_dirty = true;
   h->graph_dirty = true;
@@ -783,11 +986,60 @@ static int issue_subcycles(evp1d_handle *h, int nsub) {
   return EVP1D_OK;
 }
 
-static int run_graph(evp1d_handle *h, int nsub) {
-  if (h->graph_dirty || h->graph_nsub != nsub || !h->graph_exec) {
+static int ensure_fused_buffers(evp1d_handle *h) {
+  if (!h->vel_alt) HIP_TRY(hipMalloc((void **)&h->vel_alt, (size_t)h->navel * sizeof(double2)));
+  if (!h->sig_alt) HIP_TRY(hipMalloc((void **)&h->sig_alt, (size_t)12 * h->ldsig * sizeof(double)));
+  return EVP1D_OK;
+}
+
+typedef void (*FusedFn)(DevView, FusedView, int);
+static const FusedFn kFusedFns[2][2] = {{k_fused<256, false>, k_fused<256, true>},
+                                        {k_fused<512, false>, k_fused<512, true>}};
+
+// nsub fused subcycles.  The state starts in the handle's buffers (X), is
+// copied to the second set (Y), and the subcycles alternate so that the last
+// one writes X: subcycle k writes X when nsub - k is even.
+static int issue_fused(evp1d_handle *h, int nsub) {
+  const DevView X = h->v;
+  DevView Y = h->v;
+  Y.vel = Y.vel_out = h->vel_alt;
+  Y.velc = h->vel_alt;
+  Y.sig = h->sig_alt;
+  HIP_TRY(hipMemcpyAsync(Y.vel, X.vel, (size_t)h->navel * sizeof(double2), hipMemcpyDeviceToDevice, h->stream));
+  HIP_TRY(hipMemcpyAsync(Y.sig, X.sig, (size_t)12 * h->ldsig * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  if (h->nk_total > 0) {
+    hipLaunchKernelGGL(k_fill_const, dim3((unsigned)((h->nk_total + 255) / 256)), dim3(256), 0, h->stream, X,
+                       (const int32_t *)h->klist_dev, h->cstr_dev, h->nk_total);
+    HIP_TRY(hipGetLastError());
+  }
+  FusedView f{h->tiles_dev, h->hlist_dev, h->uloc_dev, h->cstr_dev, h->nk_total, nullptr, nullptr, h->fused_cap};
+  const size_t lds = (size_t)4 * h->fused_cap * sizeof(double2);
+  for (int k = 1; k <= nsub; ++k) {
+    const bool to_x = ((nsub - k) % 2) == 0;
+    DevView in = to_x ? Y : X;
+    DevView out = to_x ? X : Y;
+    f.vel_out = out.vel;
+    f.sig_out = out.sig;
+    if (h->fused_ntiles > 0) {
+      hipLaunchKernelGGL(kFusedFns[h->tile_block == 512 ? 1 : 0][h->tile_xcd ? 1 : 0], dim3(h->fused_ntiles),
+                         dim3(h->tile_block), lds, h->stream, in, f, k == nsub ? 1 : 0);
+      HIP_TRY(hipGetLastError());
+    }
+    if (h->v.nhalo > 0) {
+      hipLaunchKernelGGL(k_halo, dim3(blocks_for(h->v.nhalo)), dim3(kBlock), 0, h->stream, out);
+      HIP_TRY(hipGetLastError());
+    }
+  }
+  return EVP1D_OK;
+}
+
+static int run_graph(evp1d_handle *h, int nsub, bool fused) {
+  const int gmode = fused ? EVP1D_MODE_FUSED : EVP1D_MODE_GRAPH;
+  if (h->graph_dirty || h->graph_nsub != nsub || h->graph_mode != gmode || !h->graph_exec) {
     drop_graph(h);
+    if (fused) RC_TRY(ensure_fused_buffers(h));
     HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = issue_subcycles(h, nsub);
+    const int rc = fused ? issue_fused(h, nsub) : issue_subcycles(h, nsub);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(h->stream, &g);
     if (rc) {
@@ -798,6 +1050,7 @@ static int run_graph(evp1d_handle *h, int nsub) {
     h->graph = g;
     HIP_TRY(hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
     h->graph_nsub = nsub;
+    h->graph_mode = gmode;
     h->graph_dirty = false;
   }
   HIP_TRY(hipGraphLaunch(h->graph_exec, h->stream));
@@ -1010,7 +1263,8 @@ static int set_field_impl(evp1d_handle *h, int k, int32_t field, const double *s
   if (n != len) return fail(EVP1D_EINVAL, "evp1d_set_field: field %d has length %d, got %d", field, len, n);
   if (n == 0) return EVP1D_OK;
   if (!src) return fail(EVP1D_EINVAL, "evp1d_set_field: src is NULL");
-  RC_TRY(ensure(h));
+  // the caller->slot map always matches the data, so no relayout is needed
+  
   const Span sp{m.a_off, m.m_na, m.g_off, n};
   HIP_TRY(hipMemcpyAsync(h->scratch, src, (size_t)n * sizeof(double), hipMemcpyHostToDevice, h->stream));
   if (r.kind == FK_PLAIN)
@@ -1034,7 +1288,8 @@ static int get_field_impl(evp1d_handle *h, int k, int32_t field, double *dst, in
   if (n != len) return fail(EVP1D_EINVAL, "evp1d_get_field: field %d has length %d, got %d", field, len, n);
   if (n == 0) return EVP1D_OK;
   if (!dst) return fail(EVP1D_EINVAL, "evp1d_get_field: dst is NULL");
-  RC_TRY(ensure(h));
+  // the caller->slot map always matches the data, so no relayout is needed
+  
   const Span sp{m.a_off, m.m_na, m.g_off, n};
   if (r.kind == FK_PLAIN)
     hipLaunchKernelGGL(k_gather_plain, dim3(blocks_for(n)), dim3(kBlock), 0, h->stream, h->scratch,
@@ -1070,6 +1325,7 @@ static int set_index_impl(evp1d_handle *h, int k, int32_t which, const int32_t *
   if (k < 0) std::fill(ms.begin(), ms.end(), (uint8_t)1); else ms[(size_t)k] = 1;
   h->idx_set[which] = std::all_of(ms.begin(), ms.end(), [](uint8_t b) { return b != 0; });
   h->idx_dirty = true;
+  h->layout_dirty = true;   // the tiled slot order follows the lists
   return EVP1D_OK;
 }
 
@@ -1174,7 +1430,7 @@ int evp1d_halo(evp1d_handle *h) {
 
 int evp1d_set_mode(evp1d_handle *h, int32_t mode) {
   if (!h) return fail(EVP1D_EINVAL, "evp1d_set_mode: NULL handle");
-  if (mode < EVP1D_MODE_LAUNCH || mode > EVP1D_MODE_AUTO) return fail(EVP1D_EINVAL, "evp1d_set_mode: bad mode %d", mode);
+  if (mode < EVP1D_MODE_LAUNCH || mode > EVP1D_MODE_FUSED) return fail(EVP1D_EINVAL, "evp1d_set_mode: bad mode %d", mode);
   h->mode = mode;
   return EVP1D_OK;
 }
@@ -1194,17 +1450,27 @@ int evp1d_run(evp1d_handle *h, int32_t nsub) {
   switch (h->mode) {
     case EVP1D_MODE_LAUNCH: return issue_subcycles(h, nsub);
     case EVP1D_MODE_PERSISTENT: return run_persistent(h, nsub);
-    case EVP1D_MODE_GRAPH:
+    case EVP1D_MODE_FUSED:
+      if (!h->fused_ok) return fail(EVP1D_ESTATE, "evp1d_run: fused form unavailable (%s)", h->fused_why.c_str());
+      return run_graph(h, nsub, true);
+    case EVP1D_MODE_GRAPH: return run_graph(h, nsub, false);
     case EVP1D_MODE_AUTO:
-    default: return run_graph(h, nsub);
+    default: return run_graph(h, nsub, h->fused_ok);
   }
 }
 
 int evp1d_run_individual(evp1d_handle *h, int32_t which, int32_t nsub) {
   if (!h) return fail(EVP1D_EINVAL, "evp1d_run_individual: NULL handle");
   if (!ready(h)) return fail(EVP1D_ESTATE, "evp1d_run_individual: neighbour lists not set");
-  if (which != 0 && which != 1) return fail(EVP1D_EINVAL, "evp1d_run_individual: which=%d", which);
+  if (which < 0 || which > 2) return fail(EVP1D_EINVAL, "evp1d_run_individual: which=%d", which);
   RC_TRY(ensure(h));
+  if (which == 2) {
+    if (!h->fused_ok)
+      return fail(EVP1D_ESTATE, "evp1d_run_individual: fused form unavailable (%s)", h->fused_why.c_str());
+    if (nsub <= 0) return EVP1D_OK;
+    RC_TRY(ensure_fused_buffers(h));
+    return issue_fused(h, nsub);
+  }
   for (int k = 0; k < nsub; ++k) RC_TRY(which == 0 ? launch_stress(h) : launch_stepu(h, k == nsub - 1));
   return EVP1D_OK;
 }
@@ -1240,6 +1506,34 @@ int evp1d_get_tuning(const evp1d_handle *h, int32_t *stress_block, int32_t *stre
 // own data (reps launches each, HIP events) and keep the fastest.  The state
 // the trial launches change is saved first and restored afterwards, so the
 // handle's results are unaffected.
+int evp1d_set_tiling(evp1d_handle *h, int32_t tile_w, int32_t tile_h, int32_t block, int32_t xcd) {
+  if (!h) return fail(EVP1D_EINVAL, "evp1d_set_tiling: NULL handle");
+  if (tile_w < 1 || tile_h < 1 || (long)(tile_w + 1) * (tile_h + 1) > kFusedMaxEntries)
+    return fail(EVP1D_EINVAL, "evp1d_set_tiling: tile %d x %d: need w, h >= 1 and (w+1)(h+1) <= %d", tile_w, tile_h,
+                kFusedMaxEntries);
+  if (xcd != 0 && xcd != 1) return fail(EVP1D_EINVAL, "evp1d_set_tiling: xcd must be 0 or 1");
+  if (block != 256 && block != 512) return fail(EVP1D_EINVAL, "evp1d_set_tiling: block must be 256 or 512");
+  h->tile_block = block;
+  if (tile_w != h->tile_w || tile_h != h->tile_h) h->layout_dirty = true;
+  h->tile_w = tile_w;
+  h->tile_h = tile_h;
+  h->tile_xcd = xcd;
+  h->graph_dirty = true;
+  return EVP1D_OK;
+}
+
+int evp1d_get_tiling(evp1d_handle *h, int32_t *tile_w, int32_t *tile_h, int32_t *block, int32_t *xcd,
+                     int32_t *ntiles) {
+  if (!h) return fail(EVP1D_EINVAL, "evp1d_get_tiling: NULL handle");
+  if (ready(h)) RC_TRY(ensure(h));
+  if (tile_w) *tile_w = h->tile_w;
+  if (tile_h) *tile_h = h->tile_h;
+  if (block) *block = h->tile_block;
+  if (xcd) *xcd = h->tile_xcd;
+  if (ntiles) *ntiles = h->fused_ok ? h->fused_ntiles : 0;
+  return EVP1D_OK;
+}
+
 int evp1d_autotune(evp1d_handle *h, int32_t reps, float *stress_us, float *stepu_us) {
   if (!h) return fail(EVP1D_EINVAL, "evp1d_autotune: NULL handle");
   if (!ready(h)) return fail(EVP1D_ESTATE, "evp1d_autotune: neighbour lists not set");

@@ -1,0 +1,327 @@
+// Tiling of the 1D point list for the fused subcycle kernel (host only).
+// The layout and the roles of halo and constant entries are described in
+// evp1d_tiles.hpp.
+#include "evp1d_tiles.hpp"
+
+#include <algorithm>
+#include <cstdio>
+
+namespace evp1d {
+
+namespace {
+
+// stress only | stress and stepu | stepu only | neither
+inline int point_class(uint8_t b) {
+  const bool t = !(b & 1), u = !(b & 2);
+  return t && !u ? 0 : (t && u ? 1 : (u ? 2 : 3));
+}
+
+// grid offset of each neighbour list: ee ne se nw sw sse
+constexpr int kDx[6] = {-1, -1, 0, 1, 1, 0};
+constexpr int kDy[6] = {0, -1, -1, 0, 1, 1};
+
+// class order with caller order inside each class (the untiled layout)
+void class_order(int na, const uint8_t *skip, TilePlan &p) {
+  std::fill(p.cnt, p.cnt + 4, 0);
+  for (int q = 0; q < na; ++q) ++p.cnt[point_class(skip[q])];
+  int start[4] = {0, p.cnt[0], p.cnt[0] + p.cnt[1], p.cnt[0] + p.cnt[1] + p.cnt[2]};
+  p.perm.assign((size_t)na, 0);
+  for (int q = 0; q < na; ++q) p.perm[(size_t)start[point_class(skip[q])]++] = q;
+}
+
+// Grid coordinates of every point from the neighbour lists, one frame per
+// connected component, components placed side by side.  Returns false when
+// the lists are not those of a grid (a point reached at two positions).
+bool grid_coordinates(int na, const int32_t *const idx[6], std::vector<int64_t> &gx, std::vector<int64_t> &gy,
+                      std::string &why) {
+  std::vector<int64_t> x((size_t)na), y((size_t)na);
+  std::vector<int32_t> comp((size_t)na, -1), queue;
+  queue.reserve((size_t)na);
+  gx.assign((size_t)na, 0);
+  gy.assign((size_t)na, 0);
+  int64_t frame_x = 0;
+  int ncomp = 0;
+  for (int r = 0; r < na; ++r) {
+    if (comp[(size_t)r] >= 0) continue;
+    queue.clear();
+    queue.push_back(r);
+    comp[(size_t)r] = ncomp;
+    x[(size_t)r] = 0;
+    y[(size_t)r] = 0;
+    int64_t xmin = 0, xmax = 0, ymin = 0;
+    for (size_t h = 0; h < queue.size(); ++h) {
+      const int p = queue[h];
+      for (int k = 0; k < 6; ++k) {
+        const int32_t q = idx[k][p];
+        if (q < 0 || q >= na) continue;   // ghost slot: no coordinates needed
+        const int64_t qx = x[(size_t)p] + kDx[k], qy = y[(size_t)p] + kDy[k];
+        if (comp[(size_t)q] < 0) {
+          comp[(size_t)q] = ncomp;
+          x[(size_t)q] = qx;
+          y[(size_t)q] = qy;
+          xmin = std::min(xmin, qx);
+          xmax = std::max(xmax, qx);
+          ymin = std::min(ymin, qy);
+          queue.push_back(q);
+        } else if (x[(size_t)q] != qx || y[(size_t)q] != qy) {
+          char buf[160];
+          snprintf(buf, sizeof buf, "point %d is reached from point %d at two grid positions", q + 1, p + 1);
+          why = buf;
+          return false;
+        }
+      }
+    }
+    for (int p : queue) {
+      gx[(size_t)p] = x[(size_t)p] - xmin + frame_x;
+      gy[(size_t)p] = y[(size_t)p] - ymin;
+    }
+    frame_x += xmax - xmin + 2;
+    ++ncomp;
+  }
+  return true;
+}
+
+}  // namespace
+
+TilePlan plan_tiles(int na, int navel, const int32_t *const idx[6], const uint8_t *skip, int tile_w, int tile_h,
+                    int max_entries) {
+  TilePlan p;
+  p.tile_w = tile_w;
+  p.tile_h = tile_h;
+  class_order(na, skip, p);
+  if (na == 0) {
+    p.why = "empty list";
+    return p;
+  }
+  if (!idx) {
+    p.why = "neighbour lists not set";
+    return p;
+  }
+  for (int k = 0; k < 6; ++k)
+    if (!idx[k]) {
+      p.why = "neighbour lists not set";
+      return p;
+    }
+  if (tile_w < 1 || tile_h < 1) {
+    p.why = "tile size < 1";
+    return p;
+  }
+  for (int k = 0; k < 6; ++k)
+    for (int q = 0; q < na; ++q)
+      if (idx[k][q] < 0 || idx[k][q] >= navel) {
+        p.why = "neighbour index out of range";
+        return p;
+      }
+  std::vector<int64_t> gx, gy;
+  if (!grid_coordinates(na, idx, gx, gy, p.why)) return p;
+
+  // dense tile ids in (row of tiles, column of tiles) order
+  int64_t width = 0;
+  for (int q = 0; q < na; ++q) width = std::max(width, gx[(size_t)q] + 1);
+  const int64_t ntx = (width + tile_w - 1) / tile_w;
+  std::vector<int64_t> key((size_t)na);
+  for (int q = 0; q < na; ++q) key[(size_t)q] = (gy[(size_t)q] / tile_h) * ntx + gx[(size_t)q] / tile_w;
+  std::vector<int64_t> keys(key);
+  std::sort(keys.begin(), keys.end());
+  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+  const int nt = (int)keys.size();
+  std::vector<int32_t> tid((size_t)na);
+  for (int q = 0; q < na; ++q)
+    tid[(size_t)q] = (int32_t)(std::lower_bound(keys.begin(), keys.end(), key[(size_t)q]) - keys.begin());
+
+  // counting sort by (class, tile), caller order inside
+  std::vector<int32_t> start((size_t)4 * nt + 1, 0);
+  for (int q = 0; q < na; ++q) ++start[(size_t)point_class(skip[q]) * nt + tid[(size_t)q] + 1];
+  for (size_t b = 1; b < start.size(); ++b) start[b] += start[b - 1];
+  std::vector<int32_t> fill(start.begin(), start.end() - 1);
+  for (int q = 0; q < na; ++q) p.perm[(size_t)fill[(size_t)point_class(skip[q]) * nt + tid[(size_t)q]]++] = q;
+  std::vector<int32_t> inv((size_t)na);
+  for (int s = 0; s < na; ++s) inv[(size_t)p.perm[(size_t)s]] = s;
+  auto slot = [&](int32_t q) { return q < na ? inv[(size_t)q] : q; };
+
+  const int t_end = p.cnt[0] + p.cnt[1], u_begin = p.cnt[0];
+  const int nu = p.cnt[1] + p.cnt[2];
+  p.uloc.assign((size_t)4 * nu, 0);
+  std::vector<int32_t> mark((size_t)navel, -1), lidx((size_t)navel, -1), halo, konst;
+  for (int t = 0; t < nt; ++t) {
+    TileDesc d{};
+    d.a0 = start[(size_t)0 * nt + t];
+    d.a1 = start[(size_t)0 * nt + t + 1];
+    d.b0 = start[(size_t)1 * nt + t];
+    d.b1 = start[(size_t)1 * nt + t + 1];
+    d.c0 = start[(size_t)2 * nt + t];
+    d.c1 = start[(size_t)2 * nt + t + 1];
+    const int nA = d.a1 - d.a0, nB = d.b1 - d.b0, nO = nA + nB;
+    if (nO + (d.c1 - d.c0) == 0) continue;
+    for (int s = d.a0; s < d.a1; ++s) { mark[(size_t)s] = t; lidx[(size_t)s] = s - d.a0; }
+    for (int s = d.b0; s < d.b1; ++s) { mark[(size_t)s] = t; lidx[(size_t)s] = nA + (s - d.b0); }
+    halo.clear();
+    konst.clear();
+    auto visit = [&](int32_t r) {
+      if (mark[(size_t)r] == t) return;
+      mark[(size_t)r] = t;
+      (r < t_end ? halo : konst).push_back(r);
+    };
+    auto refs = [&](int s, int32_t out[4]) {
+      const int32_t q = p.perm[(size_t)s];
+      out[0] = s;
+      out[1] = slot(idx[3][q]);   // nw: str2, str7
+      out[2] = slot(idx[5][q]);   // sse: str3, str6
+      out[3] = slot(idx[4][q]);   // sw: str4, str8
+    };
+    int32_t r[4];
+    for (int s = d.b0; s < d.b1; ++s) { refs(s, r); for (int k = 0; k < 4; ++k) visit(r[k]); }
+    for (int s = d.c0; s < d.c1; ++s) { refs(s, r); for (int k = 0; k < 4; ++k) visit(r[k]); }
+    const int nH = (int)halo.size(), nK = (int)konst.size();
+    for (int k = 0; k < nH; ++k) lidx[(size_t)halo[(size_t)k]] = nO + k;
+    for (int k = 0; k < nK; ++k) lidx[(size_t)konst[(size_t)k]] = nO + nH + k;
+    const int entries = nO + nH + nK;
+    if (entries > max_entries || entries > 65535) {
+      char buf[160];
+      snprintf(buf, sizeof buf, "tile %d needs %d LDS entries (limit %d)", t, entries, max_entries);
+      p.why = buf;
+      p.tiles.clear();
+      return p;
+    }
+    p.max_entries = std::max(p.max_entries, entries);
+    auto put = [&](int s) {
+      refs(s, r);
+      for (int k = 0; k < 4; ++k) p.uloc[(size_t)4 * (s - u_begin) + k] = (uint16_t)lidx[(size_t)r[k]];
+    };
+    for (int s = d.b0; s < d.b1; ++s) put(s);
+    for (int s = d.c0; s < d.c1; ++s) put(s);
+    d.h0 = (int32_t)p.list.size();
+    d.nh = nH;
+    d.k0 = (int32_t)p.klist.size();
+    d.nk = nK;
+    p.list.insert(p.list.end(), halo.begin(), halo.end());
+    p.klist.insert(p.klist.end(), konst.begin(), konst.end());
+    p.tiles.push_back(d);
+  }
+  p.ok = true;
+  return p;
+}
+
+std::string check_plan(const TilePlan &p, int na, int navel, const int32_t *const idx[6], const uint8_t *skip) {
+  char buf[200];
+  if ((int)p.perm.size() != na) return "perm has the wrong length";
+  std::vector<int32_t> inv((size_t)na, -1);
+  for (int s = 0; s < na; ++s) {
+    const int32_t q = p.perm[(size_t)s];
+    if (q < 0 || q >= na || inv[(size_t)q] >= 0) return "perm is not a permutation";
+    inv[(size_t)q] = s;
+  }
+  int lo = 0;
+  for (int c = 0; c < 4; ++c) {
+    for (int s = lo; s < lo + p.cnt[c]; ++s)
+      if (point_class(skip[p.perm[(size_t)s]]) != c) return "slot outside its class range";
+    lo += p.cnt[c];
+  }
+  if (lo != na) return "class sizes do not add up to na";
+  if (!p.ok) return "";
+  const int t_end = p.cnt[0] + p.cnt[1], u_begin = p.cnt[0], u_end = p.cnt[0] + p.cnt[1] + p.cnt[2];
+  if ((int)p.uloc.size() != 4 * (u_end - u_begin)) return "uloc has the wrong length";
+  auto slot = [&](int32_t q) { return q < na ? inv[(size_t)q] : q; };
+  std::vector<char> owned((size_t)na, 0), stepped((size_t)na, 0);
+  std::vector<int32_t> emap;
+  for (size_t t = 0; t < p.tiles.size(); ++t) {
+    const TileDesc &d = p.tiles[t];
+    if (!(0 <= d.a0 && d.a0 <= d.a1 && d.a1 <= p.cnt[0])) return "stress-only range outside its class";
+    if (!(p.cnt[0] <= d.b0 && d.b0 <= d.b1 && d.b1 <= t_end)) return "stress-and-stepu range outside its class";
+    if (!(t_end <= d.c0 && d.c0 <= d.c1 && d.c1 <= u_end)) return "stepu-only range outside its class";
+    if (d.h0 < 0 || d.nh < 0 || (size_t)d.h0 + d.nh > p.list.size()) return "halo list range";
+    if (d.k0 < 0 || d.nk < 0 || (size_t)d.k0 + d.nk > p.klist.size()) return "constant list range";
+    const int nA = d.a1 - d.a0, nO = nA + (d.b1 - d.b0), ne = nO + d.nh + d.nk;
+    if (ne > p.max_entries) return "tile exceeds max_entries";
+    emap.assign((size_t)ne, -1);
+    for (int s = d.a0; s < d.a1; ++s) { if (owned[(size_t)s]++) return "T slot owned twice"; emap[(size_t)(s - d.a0)] = s; }
+    for (int s = d.b0; s < d.b1; ++s) { if (owned[(size_t)s]++) return "T slot owned twice"; emap[(size_t)(nA + s - d.b0)] = s; }
+    for (int k = 0; k < d.nh; ++k) {
+      const int32_t g = p.list[(size_t)d.h0 + k];
+      if (g < 0 || g >= t_end) return "halo entry is not an active T slot";
+      if ((g >= d.a0 && g < d.a1) || (g >= d.b0 && g < d.b1)) return "halo entry owned by its own tile";
+      emap[(size_t)(nO + k)] = g;
+    }
+    for (int k = 0; k < d.nk; ++k) {
+      const int32_t g = p.klist[(size_t)d.k0 + k];
+      if (g < t_end || g >= navel) return "constant entry is an active T slot or out of range";
+      emap[(size_t)(nO + d.nh + k)] = g;
+    }
+    auto check_u = [&](int s) -> bool {
+      if (stepped[(size_t)s]++) return false;
+      const int32_t q = p.perm[(size_t)s];
+      const int32_t want[4] = {s, slot(idx[3][q]), slot(idx[5][q]), slot(idx[4][q])};
+      for (int k = 0; k < 4; ++k) {
+        const int e = p.uloc[(size_t)4 * (s - u_begin) + k];
+        if (e >= ne || emap[(size_t)e] != want[k]) return false;
+      }
+      return true;
+    };
+    for (int s = d.b0; s < d.b1; ++s)
+      if (!check_u(s)) { snprintf(buf, sizeof buf, "stepu slot %d: bad LDS entry", s); return buf; }
+    for (int s = d.c0; s < d.c1; ++s)
+      if (!check_u(s)) { snprintf(buf, sizeof buf, "stepu slot %d: bad LDS entry", s); return buf; }
+  }
+  for (int s = 0; s < t_end; ++s)
+    if (!owned[(size_t)s]) return "active T slot owned by no tile";
+  for (int s = u_begin; s < u_end; ++s)
+    if (!stepped[(size_t)s]) return "active U slot in no tile";
+  return "";
+}
+
+}  // namespace evp1d
+
+#include "../../include/evp1d.h"
+
+extern "C" int evp1d__set_error(int code, const char *msg);
+
+extern "C" int evp1d_tile_plan(int32_t na, int32_t navel, const int32_t *ee, const int32_t *ne, const int32_t *se,
+                               const int32_t *nw, const int32_t *sw, const int32_t *sse, const int32_t *skipT,
+                               const int32_t *skipU, int32_t tile_w, int32_t tile_h, int32_t max_entries,
+                               int32_t *perm, int32_t *counts, int32_t *ntiles, int32_t *max_used,
+                               int32_t *tiles, int32_t tiles_cap, int32_t *list, int32_t list_cap, int32_t *nlist,
+                               int32_t *klist, int32_t klist_cap, int32_t *nklist, int32_t *uloc) {
+  if (na < 0 || navel < na) return evp1d__set_error(EVP1D_EINVAL, "evp1d_tile_plan: need 0 <= na <= navel");
+  const int32_t *in[6] = {ee, ne, se, nw, sw, sse};
+  for (int k = 0; k < 6; ++k)
+    if (na > 0 && !in[k]) return evp1d__set_error(EVP1D_EINVAL, "evp1d_tile_plan: NULL neighbour list");
+  if (na > 0 && (!skipT || !skipU)) return evp1d__set_error(EVP1D_EINVAL, "evp1d_tile_plan: NULL flags");
+  std::vector<int32_t> z[6];
+  const int32_t *zp[6];
+  for (int k = 0; k < 6; ++k) {
+    z[k].resize((size_t)na);
+    for (int q = 0; q < na; ++q) {
+      const int32_t v = in[k][q];
+      if (v < 1 || v > navel) return evp1d__set_error(EVP1D_EINDEX, "evp1d_tile_plan: neighbour index out of range");
+      z[k][(size_t)q] = v - 1;
+    }
+    zp[k] = z[k].data();
+  }
+  std::vector<uint8_t> skip((size_t)na);
+  for (int q = 0; q < na; ++q) skip[(size_t)q] = (uint8_t)((skipT[q] ? 1 : 0) | (skipU[q] ? 2 : 0));
+  const evp1d::TilePlan p = evp1d::plan_tiles(na, navel, zp, skip.data(), tile_w, tile_h, max_entries);
+  const std::string bad = evp1d::check_plan(p, na, navel, zp, skip.data());
+  if (!bad.empty()) return evp1d__set_error(EVP1D_EINDEX, ("evp1d_tile_plan: " + bad).c_str());
+  if (perm)
+    for (int s = 0; s < na; ++s) perm[s] = p.perm[(size_t)s] + 1;
+  if (counts)
+    for (int c = 0; c < 4; ++c) counts[c] = p.cnt[c];
+  const int nt = p.ok ? (int)p.tiles.size() : 0;
+  if (ntiles) *ntiles = nt;
+  if (max_used) *max_used = p.max_entries;
+  if (nlist) *nlist = (int32_t)p.list.size();
+  if (nklist) *nklist = (int32_t)p.klist.size();
+  if (tiles && tiles_cap >= nt)
+    for (int t = 0; t < nt; ++t) {
+      const evp1d::TileDesc &d = p.tiles[(size_t)t];
+      const int32_t v[10] = {d.a0, d.a1, d.b0, d.b1, d.c0, d.c1, d.h0, d.nh, d.k0, d.nk};
+      for (int k = 0; k < 10; ++k) tiles[10 * t + k] = v[k];
+    }
+  if (list && list_cap >= (int32_t)p.list.size())
+    for (size_t k = 0; k < p.list.size(); ++k) list[k] = p.list[k];
+  if (klist && klist_cap >= (int32_t)p.klist.size())
+    for (size_t k = 0; k < p.klist.size(); ++k) klist[k] = p.klist[k];
+  if (uloc && p.ok)
+    for (size_t k = 0; k < p.uloc.size(); ++k) uloc[k] = p.uloc[k];
+  return EVP1D_OK;
+}

@@ -1,0 +1,59 @@
+// Tiling of the 1D point list for the fused subcycle kernel (host only).
+//
+// One subcycle is stress on every active T point, then stepu on every active
+// U point, where stepu at q reads str1..8 at q and at its neighbours nw(q),
+// sse(q), sw(q) (bench_v2b.F90:526-531).  The fused kernel runs both halves
+// for one tile in one workgroup: it evaluates stress for the tile's own T
+// points plus the "halo" T points its U points reference, keeps str1..8 in
+// LDS, synchronises the workgroup, then runs stepu for the tile's U points.
+// Halo stresses are evaluated twice (by their owner and by the neighbour
+// tile) but written only by their owner, so every output is produced once.
+//
+// Slot order.  Points are ordered by class (stress only | stress and stepu |
+// stepu only | neither), then by tile, then by caller position, so a tile's
+// owned T points and its U points are each two contiguous slot ranges and
+// the class ranges that the unfused kernels iterate stay contiguous.  Tiles
+// are W x H boxes of grid coordinates recovered from the neighbour lists:
+// ee=(i-1,j) ne=(i-1,j-1) se=(i,j-1) nw=(i+1,j) sw=(i+1,j+1) sse=(i,j+1)
+// (ice_dyn_evp1d.F90:766-772).  Connected components get disjoint frames.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace evp1d {
+
+// Slot ranges of one tile; entries of its LDS table are numbered
+// [owned stress-only | owned stress-and-stepu | halo stress | constants].
+struct TileDesc {
+  int32_t a0, a1;   // owned stress-only slots
+  int32_t b0, b1;   // owned stress-and-stepu slots
+  int32_t c0, c1;   // stepu-only slots
+  int32_t h0, nh;   // halo stress slots: TilePlan::list[h0, h0 + nh)
+  int32_t k0, nk;   // constant slots (str1..8 never written): TilePlan::klist[k0, k0 + nk)
+};
+
+struct TilePlan {
+  bool ok = false;          // false: no tiling (perm is still the class order)
+  std::string why;          // reason when !ok
+  int tile_w = 0, tile_h = 0;
+  int cnt[4] = {0, 0, 0, 0};          // class sizes
+  std::vector<int32_t> perm;          // slot -> caller position, slots [0, na)
+  std::vector<TileDesc> tiles;        // tiles with work, in launch order
+  std::vector<int32_t> list;          // halo stress slots, per tile
+  std::vector<int32_t> klist;         // constant slots, per tile
+  std::vector<uint16_t> uloc;         // 4 per stepu slot (slot - cnt[0]): own, nw, sse, sw entries
+  int max_entries = 0;
+};
+
+// idx: ee, ne, se, nw, sw, sse as 0-based caller positions (values in
+// [0, navel); positions >= na are ghost slots).  skip: bit0 skipT, bit1 skipU.
+// idx may be null (no tiling, class order only).
+TilePlan plan_tiles(int na, int navel, const int32_t *const idx[6], const uint8_t *skip, int tile_w, int tile_h,
+                    int max_entries);
+
+// Re-check every bound the fused kernel relies on; returns "" when the plan
+// is consistent, else the first violation.
+std::string check_plan(const TilePlan &p, int na, int navel, const int32_t *const idx[6], const uint8_t *skip);
+
+}  // namespace evp1d

@@ -41,7 +41,7 @@ FIELD_ID = {n: k for k, n in enumerate(FIELDS)}
 NAVEL_FIELDS = frozenset({"uvel", "vvel", "str1", "str2", "str3", "str4", "str5", "str6", "str7", "str8"})
 STRESS_FIELDS = FIELDS[19:31]
 NBR = ("ee", "ne", "se", "nw", "sw", "sse")
-MODES = {"launch": 0, "graph": 1, "persistent": 2, "auto": 3}
+MODES = {"launch": 0, "graph": 1, "persistent": 2, "auto": 3, "fused": 4}
 
 # every symbol include/evp1d.h declares
 EXPORTS = (
@@ -55,6 +55,7 @@ EXPORTS = (
     "evp1d_get_field_member", "evp1d_set_index_member", "evp1d_set_skip_member",
     "evp1d_autotune", "evp1d_set_tuning", "evp1d_get_tuning",
     "evp1d_adjoint_run", "evp1d_loss", "evp1d_tangent_run",
+    "evp1d_set_tiling", "evp1d_get_tiling", "evp1d_tile_plan",
 )
 
 
@@ -137,6 +138,10 @@ def load(fma: bool = False, path: str | os.PathLike | None = None) -> ctypes.CDL
         "evp1d_adjoint_run": [_H, i32, i32, _DP, _DP, _DP, _DP, _DP, _DP, _DP],
         "evp1d_loss": [_H, _DP, _DP, _DP],
         "evp1d_tangent_run": [_H, i32, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP],
+        "evp1d_set_tiling": [_H, i32, i32, i32, i32],
+        "evp1d_get_tiling": [_H, _IP, _IP, _IP, _IP, _IP],
+        "evp1d_tile_plan": [i32, i32] + [_IP] * 8 + [i32, i32, i32] + [_IP] * 5 + [i32, _IP, i32, _IP, _IP, i32,
+                                                                                _IP, _IP],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -199,6 +204,35 @@ def build_index(tmask: np.ndarray, nghost: int = 1, fma: bool = False) -> dict:
     out = dict(na=na, navel=navel, indxti=ti[:na], indxtj=tj[:na], indxtij=tij[:navel])
     out.update({n: nb[n][:na] for n in NBR})
     return out
+
+
+def tile_plan(idx: dict, skipT, skipU, navel: int, tile_w: int = 32, tile_h: int = 16,
+              max_entries: int = 1024, fma: bool = False) -> dict:
+    """Host-only fused-subcycle tiling of a 1D list (evp1d_tile_plan).
+
+    idx: the six 1-based neighbour lists.  Returns perm (slot -> 1-based
+    point), counts, tiles (n x 10: a0,a1,b0,b1,c0,c1,h0,nh,k0,nk), list
+    (halo slots), klist (constant slots), uloc (stepu slot x 4: own, nw, sse,
+    sw LDS entries) and max_entries; ntiles 0 when the lists are no grid."""
+    L = load(fma)
+    skipT = np.ascontiguousarray(skipT, dtype=np.int32)
+    skipU = np.ascontiguousarray(skipU, dtype=np.int32)
+    na = len(skipT)
+    lists = [np.ascontiguousarray(idx[n], dtype=np.int32) for n in NBR]
+    perm = np.zeros(max(na, 1), np.int32)
+    counts = np.zeros(4, np.int32)
+    nt, mx, nl, nk = (ctypes.c_int32() for _ in range(4))
+    args = [na, navel, *[_ip(a) for a in lists], _ip(skipT), _ip(skipU), tile_w, tile_h, max_entries, _ip(perm),
+            _ip(counts), ctypes.byref(nt), ctypes.byref(mx)]
+    _check(L, L.evp1d_tile_plan(*args, None, 0, None, 0, ctypes.byref(nl), None, 0, ctypes.byref(nk), None))
+    tiles = np.zeros((max(nt.value, 1), 10), np.int32)
+    hl = np.zeros(max(nl.value, 1), np.int32)
+    kl = np.zeros(max(nk.value, 1), np.int32)
+    uloc = np.zeros((max(int(counts[1] + counts[2]), 1), 4), np.int32)
+    _check(L, L.evp1d_tile_plan(*args, _ip(tiles), nt.value, _ip(hl), nl.value, ctypes.byref(nl), _ip(kl),
+                                nk.value, ctypes.byref(nk), _ip(uloc)))
+    return dict(perm=perm[:na], counts=counts, ntiles=nt.value, max_entries=mx.value, tiles=tiles[:nt.value],
+                list=hl[:nl.value], klist=kl[:nk.value], uloc=uloc[:int(counts[1] + counts[2])])
 
 
 class Engine:
@@ -308,6 +342,14 @@ class Engine:
         _check(self.L, self.L.evp1d_get_tuning(self.h, *[ctypes.byref(x) for x in a]))
         return dict(zip(("stress_block", "stress_xcd", "stepu_block", "stepu_xcd"), (x.value for x in a)))
 
+    def set_tiling(self, tile_w: int = 32, tile_h: int = 16, block: int = 256, xcd: int = 1) -> None:
+        _check(self.L, self.L.evp1d_set_tiling(self.h, tile_w, tile_h, block, xcd))
+
+    def get_tiling(self) -> dict:
+        a = [ctypes.c_int32() for _ in range(5)]
+        _check(self.L, self.L.evp1d_get_tiling(self.h, *[ctypes.byref(x) for x in a]))
+        return dict(zip(("tile_w", "tile_h", "block", "xcd", "ntiles"), (x.value for x in a)))
+
     # lifetime ------------------------------------------------------------
     def close(self) -> None:
         if getattr(self, "h", None):
@@ -389,7 +431,8 @@ class Engine:
         _check(self.L, self.L.evp1d_run(self.h, int(nsub)))
 
     def run_individual(self, which: str, nsub: int) -> None:
-        _check(self.L, self.L.evp1d_run_individual(self.h, {"stress": 0, "stepu": 1}[which], int(nsub)))
+        _check(self.L, self.L.evp1d_run_individual(self.h, {"stress": 0, "stepu": 1, "fused": 2}[which],
+                                                   int(nsub)))
 
     def synchronize(self) -> None:
         _check(self.L, self.L.evp1d_synchronize(self.h))

@@ -41,7 +41,7 @@ module evp1d_capi
                                            I_SW = 4, I_SSE = 5
   integer(c_int32_t), parameter, public :: S_SKIPT = 0, S_SKIPU = 1
   integer(c_int32_t), parameter, public :: MODE_LAUNCH = 0, MODE_GRAPH = 1, &
-                                           MODE_PERSISTENT = 2, MODE_AUTO = 3
+                                           MODE_PERSISTENT = 2, MODE_AUTO = 3, MODE_FUSED = 4
 
   public :: evp1d_set_evp_parameters, evp1d_create, evp1d_destroy, evp1d_set_params,   &
             evp1d_get_params, evp1d_get_sizes, evp1d_set_field, evp1d_get_field,        &
@@ -52,7 +52,7 @@ module evp1d_capi
             evp1d_build_info, evp1d_check, evp1d_create_batch, evp1d_get_members,       &
             evp1d_member_info, evp1d_set_field_member, evp1d_get_field_member,          &
             evp1d_set_index_member, evp1d_set_skip_member, evp1d_autotune,              &
-            evp1d_set_tuning, evp1d_get_tuning
+            evp1d_set_tuning, evp1d_get_tuning, evp1d_set_tiling, evp1d_get_tiling
 
   interface
     integer(c_int) function evp1d_set_evp_parameters(ndte, dt, p) bind(C, name='evp1d_set_evp_parameters')
@@ -241,6 +241,16 @@ module evp1d_capi
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: h
       integer(c_int32_t), intent(out) :: sb, sx, ub, ux
+    end function
+    integer(c_int) function evp1d_set_tiling(h, tw, th, block, xcd) bind(C, name='evp1d_set_tiling')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: tw, th, block, xcd
+    end function
+    integer(c_int) function evp1d_get_tiling(h, tw, th, block, xcd, ntiles) bind(C, name='evp1d_get_tiling')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(out) :: tw, th, block, xcd, ntiles
     end function
   end interface
 

@@ -17,7 +17,7 @@
 ! must be 1 (the reference's replicate() is a timing-only device whose copies
 ! do not line up, src/vars.F90:1801-1838).
 !
-! Usage:  evp1d_v2_driver [graph|persistent|launch]
+! Usage:  evp1d_v2_driver [graph|persistent|launch|fused]
 !===============================================================================
 program evp1d_v2_driver
   use, intrinsic :: iso_c_binding

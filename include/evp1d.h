@@ -105,7 +105,9 @@ enum evp1d_mode {
   EVP1D_MODE_LAUNCH = 0,      /* one stress and one stepu launch per subcycle */
   EVP1D_MODE_GRAPH = 1,       /* the nsub x (stress, stepu, halo) sequence captured once as a hipGraph */
   EVP1D_MODE_PERSISTENT = 2,  /* one cooperative kernel, grid barrier between phases */
-  EVP1D_MODE_AUTO = 3         /* library default (fastest measured form) */
+  EVP1D_MODE_AUTO = 3,        /* library default: FUSED when the lists tile, else GRAPH */
+  EVP1D_MODE_FUSED = 4        /* one launch per subcycle: stress and stepu per tile of points, str1..8
+                                 kept on chip (hipGraph); needs lists that describe a grid */
 };
 
 typedef struct evp1d_handle evp1d_handle;
@@ -147,7 +149,9 @@ int evp1d_run(evp1d_handle *h, int32_t nsub);
 int evp1d_set_mode(evp1d_handle *h, int32_t mode);
 int evp1d_get_mode(const evp1d_handle *h, int32_t *mode);
 /* nsub launches of stress only (which = 0) or stepu only (which = 1): the
- * lindividual timing loops of src/evp_v2b.F90:128-155.                       */
+ * lindividual timing loops of src/evp_v2b.F90:128-155.  which = 2: nsub full
+ * subcycles of the fused form, one launch each, without a graph (for timing
+ * the fused kernel; needs the fused form, see evp1d_set_tiling).             */
 int evp1d_run_individual(evp1d_handle *h, int32_t which, int32_t nsub);
 int evp1d_synchronize(evp1d_handle *h);
 
@@ -185,6 +189,42 @@ int evp1d_set_tuning(evp1d_handle *h, int32_t stress_block, int32_t stress_xcd, 
                      int32_t stepu_xcd);
 int evp1d_get_tuning(const evp1d_handle *h, int32_t *stress_block, int32_t *stress_xcd,
                      int32_t *stepu_block, int32_t *stepu_xcd);
+
+/* ---- fused-subcycle tiling (no counterpart in the reference) -------------
+ * The fused form runs stress and stepu of one subcycle for a tile of grid
+ * cells in one workgroup: stress for the tile's own T points plus the T
+ * points its U points read (evaluated again, never written), str1..8 kept in
+ * LDS, then stepu.  Tiles are tile_w x tile_h boxes of grid coordinates that
+ * the library recovers from the neighbour lists.  evp1d_set_tiling chooses
+ * the tile ((tile_w+1)(tile_h+1) <= 1024), the workgroup size (256 or 512)
+ * and the XCD-aware tile order (xcd = 0/1); a new tile re-lays the points out
+ * at the next run.  evp1d_get_tiling reports them and the number of tiles (0
+ * when the lists do not describe a grid: the fused form is then unavailable).
+ * Results never depend on the tiling.                                       */
+int evp1d_set_tiling(evp1d_handle *h, int32_t tile_w, int32_t tile_h, int32_t block, int32_t xcd);
+int evp1d_get_tiling(evp1d_handle *h, int32_t *tile_w, int32_t *tile_h, int32_t *block, int32_t *xcd,
+                     int32_t *ntiles);
+
+/* Host-only tiling of a 1D list (no device use; for inspection and tests).
+ * ee..sse: na each, 1-based slots as for evp1d_set_index; skipT, skipU: na
+ * flags.  Outputs, each optional: perm[na] (slot -> 1-based point), counts[4]
+ * (points that are stress only, stress and stepu, stepu only, neither),
+ * *ntiles, *max_entries (largest LDS table), and, when the capacities
+ * suffice, tiles[10 * ntiles] (a0,a1,b0,b1,c0,c1,h0,nh,k0,nk: 0-based slot
+ * ranges of owned stress-only, owned stress-and-stepu and stepu-only points,
+ * then offset and length in list and in klist), list[*nlist] (halo slots:
+ * stress evaluated by the tile, written by their owner), klist[*nklist]
+ * (constant slots: str1..8 read, never written) and uloc[4 * (counts[1] +
+ * counts[2])] (LDS entries of own, nw, sse, sw for each stepu slot in slot
+ * order; the LDS table is [owned stress-only | owned stress-and-stepu | halo |
+ * constant]).  *ntiles = 0 when the lists do not describe a grid.  Returns
+ * EVP1D_EINDEX if the plan fails its own consistency check.                  */
+int evp1d_tile_plan(int32_t na, int32_t navel, const int32_t *ee, const int32_t *ne, const int32_t *se,
+                    const int32_t *nw, const int32_t *sw, const int32_t *sse, const int32_t *skipT,
+                    const int32_t *skipU, int32_t tile_w, int32_t tile_h, int32_t max_entries,
+                    int32_t *perm, int32_t *counts, int32_t *ntiles, int32_t *max_used,
+                    int32_t *tiles, int32_t tiles_cap, int32_t *list, int32_t list_cap, int32_t *nlist,
+                    int32_t *klist, int32_t klist_cap, int32_t *nklist, int32_t *uloc);
 
 /* ---- gradients (no counterpart in the reference) -------------------------
  * Loss L = 1/2 sum over stepu-active points (skipU = 0) of

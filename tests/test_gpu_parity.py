@@ -18,7 +18,7 @@ from gpu_common import (GOLD, assert_state_bitexact, fetch_state, golden_cases, 
                         make_engine, oracle_params, synthetic_state)
 
 pytestmark = pytest.mark.gpu
-MODES = ("launch", "graph", "persistent")
+MODES = ("launch", "graph", "persistent", "fused")
 
 
 def _log(msg):
@@ -249,3 +249,55 @@ def test_autotune_keeps_results_and_state(gpu):
             eng.set_tuning(*cfg)
             eng.run(12)
             assert_state_bitexact(fetch_state(eng, st), want)
+
+
+@pytest.mark.parametrize("tile", [(32, 16, 256, 1), (1, 1, 256, 0), (5, 7, 512, 1), (64, 8, 512, 0), (200, 2, 256, 1)])
+@pytest.mark.parametrize("nsub", [1, 2, 7])
+def test_fused_tilings_match_oracle(gpu, tile, nsub):
+    """Any tile shape and XCD order, odd and even subcycle counts (the state
+    ping-pongs between two buffers and must end in the handle's own)."""
+    st = synthetic_state(70, 50, seed=61)
+    lib, p = oracle_params(st, nsub)
+    want = st.copy()
+    lib.run(p, want, nsub)
+    _log(f"fused tile={tile} nsub={nsub}")
+    with make_engine(st, nsub, "fused") as eng:
+        eng.set_tiling(*tile)
+        assert eng.get_tiling()["ntiles"] > 0
+        eng.run(nsub)
+        assert_state_bitexact(fetch_state(eng, st), want)
+        eng.run(nsub)                 # a second run from the end state
+        lib.run(p, want, nsub)
+        assert_state_bitexact(fetch_state(eng, st), want)
+
+
+def test_fused_sees_fields_set_between_runs(gpu):
+    """str1..8 at points stress never writes (read by stepu from memory) and
+    ghost velocities may be changed by the caller between runs."""
+    st = synthetic_state(50, 40, seed=62)
+    lib, p = oracle_params(st, 12)
+    want = st.copy()
+    lib.run(p, want, 5)
+    rng = np.random.default_rng(7)
+    stale = np.concatenate([st.skipT != 0, np.ones(st.navel - st.na, bool)])
+    for k in ("str2", "str3", "str7"):
+        want.f[k][stale] = rng.standard_normal(stale.sum())
+    want.f["uvel"][st.na:] = rng.standard_normal(st.navel - st.na) * 0.1
+    with make_engine(st, 12, "fused") as eng:
+        eng.run(5)
+        for k in ("str2", "str3", "str7", "uvel"):
+            eng.set_field(k, want.f[k])
+        eng.run(7)
+        lib.run(p, want, 7)
+        assert_state_bitexact(fetch_state(eng, st), want)
+
+
+def test_fused_unavailable_without_a_grid(gpu):
+    st = synthetic_state(30, 20, seed=63)
+    bad = st.copy()
+    bad.idx["nw"] = st.idx["nw"].copy()
+    bad.idx["nw"][[3, 10]] = bad.idx["nw"][[10, 3]]
+    with make_engine(bad, 3, "fused") as eng:
+        assert eng.get_tiling()["ntiles"] == 0
+        with pytest.raises(evp1d.Evp1dError):
+            eng.run(3)

@@ -1,0 +1,162 @@
+"""CPU tests of the fused-subcycle tiling (evp1d_tile_plan, host only).
+
+The fused kernel trusts three things about a plan, each re-derived here in
+numpy from the neighbour lists alone: (1) the slot order is class-major
+(stress only | stress and stepu | stepu only | neither), so the unfused
+kernels' ranges stay valid; (2) every active T point is owned (stress stored)
+by exactly one tile and every active U point is stepped by exactly one tile;
+(3) each stepu read goes to an LDS entry that holds the right slot: its own,
+nw, sse and sw neighbour (bench_v2b.F90:526-531), evaluated by the tile when
+that neighbour is an active T point, read from memory otherwise.
+"""
+import numpy as np
+import pytest
+
+import evp1d
+from evp1d import synth
+
+
+def _lists(nx, ny, seed, frac, pT=0.7):
+    rng = np.random.default_rng(seed)
+    m = np.zeros((nx, ny), dtype=bool, order="F")
+    m[1:-1, 1:-1] = rng.random((nx - 2, ny - 2)) < frac
+    ix = evp1d.build_index(m)
+    na = ix["na"]
+    skipT = (rng.random(na) > pT).astype(np.int32)
+    skipU = (rng.random(na) > pT).astype(np.int32)
+    return ix, skipT, skipU
+
+
+def _classes(skipT, skipU):
+    t, u = skipT == 0, skipU == 0
+    return np.where(t & ~u, 0, np.where(t & u, 1, np.where(u, 2, 3)))
+
+
+def check_plan(plan, ix, skipT, skipU, tile_w, tile_h):
+    """Independent check of a plan; raises AssertionError on the first fault."""
+    na, navel = ix["na"], ix["navel"]
+    perm = plan["perm"] - 1
+    assert np.array_equal(np.sort(perm), np.arange(na))
+    cls = _classes(skipT, skipU)
+    cnt = np.bincount(cls, minlength=4)
+    assert np.array_equal(plan["counts"], cnt)
+    assert np.all(np.diff(cls[perm]) >= 0), "slot order is not class-major"
+    inv = np.empty(na, np.int64)
+    inv[perm] = np.arange(na)
+
+    def slot(q1):          # 1-based list entry -> slot
+        q = np.asarray(q1, np.int64) - 1
+        return np.where(q < na, inv[np.minimum(q, na - 1)], q)
+
+    t_end, u_begin, u_end = cnt[0] + cnt[1], cnt[0], cnt[0] + cnt[1] + cnt[2]
+    owned = np.zeros(na, np.int32)
+    stepped = np.zeros(na, np.int32)
+    nw, sse, sw = slot(ix["nw"]), slot(ix["sse"]), slot(ix["sw"])
+    for a0, a1, b0, b1, c0, c1, h0, nh, k0, nk in plan["tiles"]:
+        assert 0 <= a0 <= a1 <= cnt[0] and cnt[0] <= b0 <= b1 <= t_end and t_end <= c0 <= c1 <= u_end
+        halo = plan["list"][h0:h0 + nh]
+        kon = plan["klist"][k0:k0 + nk]
+        table = np.concatenate([np.arange(a0, a1), np.arange(b0, b1), halo, kon])
+        assert len(table) <= plan["max_entries"] <= (tile_w + 1) * (tile_h + 1)
+        assert len(np.unique(table)) == len(table)
+        owned[a0:a1] += 1
+        owned[b0:b1] += 1
+        assert np.all(halo < t_end) and not np.any((halo >= a0) & (halo < a1) | (halo >= b0) & (halo < b1))
+        assert np.all((kon >= t_end) & (kon < navel))
+        us = np.concatenate([np.arange(b0, b1), np.arange(c0, c1)])
+        stepped[us] += 1
+        if len(us) == 0:
+            continue
+        p = perm[us]
+        loc = plan["uloc"][us - u_begin]
+        assert np.all(loc < len(table))
+        assert np.array_equal(table[loc[:, 0]], us)
+        assert np.array_equal(table[loc[:, 1]], nw[p])
+        assert np.array_equal(table[loc[:, 2]], sse[p])
+        assert np.array_equal(table[loc[:, 3]], sw[p])
+    assert np.all(owned[:t_end] == 1) and np.all(owned[t_end:] == 0)
+    assert np.all(stepped[u_begin:u_end] == 1) and np.all(stepped[:u_begin] == 0) and np.all(stepped[u_end:] == 0)
+
+
+@pytest.mark.parametrize("nx,ny,seed,frac", [(40, 30, 1, 0.8), (70, 20, 2, 0.6), (25, 60, 3, 1.0), (50, 50, 4, 0.3)])
+@pytest.mark.parametrize("tile", [(32, 16), (8, 4), (5, 7), (1, 1), (64, 1)])
+def test_plan_is_consistent(nx, ny, seed, frac, tile):
+    ix, skipT, skipU = _lists(nx, ny, seed, frac)
+    plan = evp1d.tile_plan(ix, skipT, skipU, ix["navel"], *tile)
+    assert plan["ntiles"] > 0
+    check_plan(plan, ix, skipT, skipU, *tile)
+
+
+def test_tiles_are_grid_boxes_on_a_connected_domain():
+    nx, ny, (tw, th) = 70, 45, (16, 8)
+    m = np.zeros((nx, ny), dtype=bool, order="F")
+    m[1:-1, 1:-1] = True
+    ix = evp1d.build_index(m)
+    z = np.zeros(ix["na"], np.int32)
+    plan = evp1d.tile_plan(ix, z, z, ix["navel"], tw, th)
+    check_plan(plan, ix, z, z, tw, th)
+    perm = plan["perm"] - 1
+    for a0, a1, b0, b1, *_ in plan["tiles"]:
+        pts = perm[b0:b1]
+        i, j = ix["indxti"][pts], ix["indxtj"][pts]
+        assert i.max() - i.min() < tw and j.max() - j.min() < th
+    # interior tiles are full boxes; halo per tile is one row and one column
+    full = [t for t in plan["tiles"] if t[3] - t[2] == tw * th]
+    assert full and all(t[7] == tw + th + 1 for t in full)
+
+
+def test_plan_follows_a_permuted_list():
+    ix, skipT, skipU = _lists(45, 35, 7, 0.75)
+    na = ix["na"]
+    rng = np.random.default_rng(8)
+    sigma = rng.permutation(na)                 # new position k holds old point sigma[k]
+    pos = np.empty(na, np.int64)
+    pos[sigma] = np.arange(na)
+    ix2 = dict(ix)
+    for n in evp1d.NBR:
+        v = ix[n][sigma].astype(np.int64) - 1
+        ix2[n] = (np.where(v < na, pos[np.minimum(v, na - 1)], v) + 1).astype(np.int32)
+    a = evp1d.tile_plan(ix, skipT, skipU, ix["navel"], 8, 8)
+    b = evp1d.tile_plan(ix2, skipT[sigma], skipU[sigma], ix["navel"], 8, 8)
+    check_plan(b, ix2, skipT[sigma], skipU[sigma], 8, 8)
+    assert b["ntiles"] > 0 and abs(b["ntiles"] - a["ntiles"]) <= a["ntiles"] // 4
+
+
+def test_lists_that_are_no_grid_give_no_tiles():
+    ix, skipT, skipU = _lists(30, 30, 9, 0.9)
+    bad = dict(ix)
+    bad["nw"] = ix["nw"].copy()
+    bad["nw"][[3, 10]] = bad["nw"][[10, 3]]      # two points now disagree about their east neighbour
+    plan = evp1d.tile_plan(bad, skipT, skipU, ix["navel"], 8, 8)
+    assert plan["ntiles"] == 0
+    cls = _classes(skipT, skipU)
+    assert np.all(np.diff(cls[plan["perm"] - 1]) >= 0)
+
+
+def test_tile_too_large_for_lds_gives_no_tiles():
+    ix, skipT, skipU = _lists(60, 60, 10, 1.0, pT=1.0)
+    plan = evp1d.tile_plan(ix, skipT, skipU, ix["navel"], 40, 40, max_entries=1024)
+    assert plan["ntiles"] == 0
+
+
+def test_checker_catches_a_planted_mistake():
+    """A wrong LDS entry (an nw read sent to the sse neighbour) must fail."""
+    ix, skipT, skipU = _lists(40, 30, 11, 0.8)
+    plan = evp1d.tile_plan(ix, skipT, skipU, ix["navel"], 8, 8)
+    check_plan(plan, ix, skipT, skipU, 8, 8)
+    bad = dict(plan)
+    bad["uloc"] = plan["uloc"].copy()
+    bad["uloc"][:, [1, 2]] = bad["uloc"][:, [2, 1]]
+    with pytest.raises(AssertionError):
+        check_plan(bad, ix, skipT, skipU, 8, 8)
+
+
+def test_plan_on_the_bench_grid_has_small_halo():
+    g = synth.make_grid(200, 160, 1001)
+    ix = evp1d.build_index(g.tmask, g.nghost)
+    from evp1d import layout
+    s = layout.pack(g, ix)
+    plan = evp1d.tile_plan(s["idx"], s["skipT"], s["skipU"], s["navel"], 32, 16)
+    check_plan(plan, ix, s["skipT"], s["skipU"], 32, 16)
+    nT = plan["counts"][0] + plan["counts"][1]
+    assert plan["tiles"][:, 7].sum() < 0.2 * nT
