@@ -264,19 +264,35 @@ def ensemble_measure(nmem: int, steps: int, autotune: int) -> dict:
         e.run(ndte)
     for e in singles:
         e.synchronize()
+    # member after member, each its own ndte-subcycle graph
     t0 = time.perf_counter()
     for _ in range(steps):
         for e in singles:
             e.run(ndte)
             e.synchronize()
     t_seq = (time.perf_counter() - t0) / steps
+    # lockstep: every subcycle launches each member in turn (64 launches per subcycle)
+    for e in singles:
+        e.run(1)
+    for e in singles:
+        e.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(ndte):
+        for e in singles:
+            e.run(1)
+    for e in singles:
+        e.synchronize()
+    t_lock = time.perf_counter() - t0
     for e in singles:
         e.close()
     out = {"members": nmem, "total_points": tot, "ndte": ndte,
-           "batched_s": round(t_batch, 4), "sequential_s": round(t_seq, 4),
+           "batched_s": round(t_batch, 4), "sequential_s": round(t_seq, 4), "lockstep_s": round(t_lock, 4),
            "batched_pts_sub_per_s": tot * ndte / t_batch, "sequential_pts_sub_per_s": tot * ndte / t_seq,
-           "speedup": round(t_seq / t_batch, 3), "tuning": tuning}
-    log(f"[bench] ensemble: batched {t_batch:.3f}s, sequential {t_seq:.3f}s, speedup {t_seq / t_batch:.2f}")
+           "lockstep_pts_sub_per_s": tot * ndte / t_lock,
+           "speedup": round(t_seq / t_batch, 3), "speedup_vs_lockstep": round(t_lock / t_batch, 3),
+           "tuning": tuning}
+    log(f"[bench] ensemble: batched {t_batch:.3f}s, member after member {t_seq:.3f}s (x{t_seq / t_batch:.2f}), "
+        f"lockstep {t_lock:.3f}s (x{t_lock / t_batch:.2f})")
     return out
 
 
@@ -331,8 +347,8 @@ def main() -> int:
         log(f"[bench] autotune: {eng.autotune(args.autotune)}")
     tuning = eng.get_tuning()
     tiling = eng.get_tiling()
-    # the form evp1d_run takes: auto is the fused form whenever the lists tile
-    fused = args.mode == "fused" or (args.mode == "auto" and tiling["ntiles"] > 0)
+    # the form evp1d_run takes: auto is the fused form whenever the lists tile (and ndte >= 4)
+    fused = args.mode == "fused" or (args.mode == "auto" and tiling["ntiles"] > 0 and ndte >= 4)
     log(f"[bench] mode {args.mode} ({'fused' if fused else 'unfused'}), tiling {tiling}, tuning {tuning}")
     secs = time_steps(eng, ndte, args.steps, args.warmup)
     value = na * ndte * args.steps / secs

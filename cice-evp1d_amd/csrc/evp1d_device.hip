@@ -275,6 +275,7 @@ struct FusedView {
   const int32_t *hlist;     // halo slots of every tile
   const uint2 *uloc;        // per stepu slot (slot - u_begin): own | nw << 16, sse | sw << 16
   const double2 *cstr;      // [4][nk]: s15, s27, s36, s48 of the constant entries
+  const int *cnz;           // nonzero when some constant entry has a bit set (else all are +0.0)
   long nk;
   double2 *vel_out;
   double *sig_out;
@@ -287,6 +288,7 @@ __global__ __launch_bounds__(B) void k_fused(DevView d, FusedView f, int last) {
   const int cap = f.cap;
   const evp1d::TileDesc t = f.tiles[tile_of(blockIdx.x, gridDim.x, X)];
   const int nA = t.a1 - t.a0, nB = t.b1 - t.b0, nO = nA + nB, nS = nO + t.nh, nE = nS + t.nk;
+  const bool cz = *f.cnz == 0;   // constant entries all +0.0: no loads
   for (int k = threadIdx.x; k < nE; k += B) {
     double2 p15, p27, p36, p48;
     if (k < nS) {
@@ -311,6 +313,8 @@ __global__ __launch_bounds__(B) void k_fused(DevView d, FusedView f, int last) {
           d.s48[g] = p48;
         }
       }
+    } else if (cz) {
+      p15 = p27 = p36 = p48 = make_double2(0.0, 0.0);
     } else {
       const long j = t.k0 + (k - nS);
       p15 = f.cstr[j];
@@ -346,18 +350,24 @@ __global__ __launch_bounds__(B) void k_fused(DevView d, FusedView f, int last) {
   }
 }
 
-// str1..8 of the constant entries, gathered once per run
-__global__ void k_fill_const(DevView d, const int32_t *klist, double2 *cstr, long nk) {
+// str1..8 of the constant entries, gathered once per run; *cnz (cleared by
+// k_clear_flag first) becomes 1 if any of them has a bit set
+__global__ void k_clear_flag(int *flag) { *flag = 0; }
+__global__ void k_fill_const(DevView d, const int32_t *klist, double2 *cstr, long nk, int *cnz) {
   const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nk) return;
   const int g = klist[j];
-  cstr[j] = d.s15[g];
-  cstr[nk + j] = d.s27[g];
-  cstr[2 * nk + j] = d.s36[g];
-  cstr[3 * nk + j] = d.s48[g];
+  const double2 v[4] = {d.s15[g], d.s27[g], d.s36[g], d.s48[g]};
+  unsigned long long bits = 0;
+  for (int c = 0; c < 4; ++c) {
+    cstr[c * nk + j] = v[c];
+    bits |= (unsigned long long)__double_as_longlong(v[c].x) | (unsigned long long)__double_as_longlong(v[c].y);
+  }
+  if (bits) *cnz = 1;   // every writer stores the same value
 }
 
-constexpr int kFusedMaxEntries = 1024;   // 64 KiB of LDS per workgroup
+constexpr int kFusedMaxEntries = 2560;   // 160 KiB of LDS per workgroup (gfx950)
+constexpr int kFusedMinSub = 4;          // evp1d_run in AUTO mode: fused from this many subcycles
 
 // launch configurations the autotuner chooses from
 constexpr int kNumBlocks = 5;
@@ -670,7 +680,7 @@ struct evp1d_handle {
   std::vector<uint8_t> idx_mset[6];   // per member: list set
   int coop_blocks = 0;
   // fused form: tiling and its device tables (owned here, replaced on relayout)
-  int tile_w = 32, tile_h = 16, tile_block = 256, tile_xcd = 1;
+  int tile_w = 64, tile_h = 16, tile_block = 512, tile_xcd = 1;
   bool fused_ok = false;
   std::string fused_why = "lists not set";
   int fused_ntiles = 0, fused_cap = 0;
@@ -679,6 +689,7 @@ struct evp1d_handle {
   int32_t *hlist_dev = nullptr, *klist_dev = nullptr;
   uint2 *uloc_dev = nullptr;
   double2 *cstr_dev = nullptr;
+  int *cnz_dev = nullptr;          // constant entries not all +0.0
   double2 *vel_alt = nullptr;      // second velocity buffer (navel)
   double *sig_alt = nullptr;       // second stress buffer (12 x ldsig)
   int graph_mode = -1;
@@ -734,6 +745,7 @@ static void release(evp1d_handle *h) {
   drop_graph(h);
   free_tiling(h);
   if (h->vel_alt) (void)hipFree(h->vel_alt);
+  if (h->cnz_dev) (void)hipFree(h->cnz_dev);
   if (h->sig_alt) (void)hipFree(h->sig_alt);
   for (void *p : h->allocs) (void)hipFree(p);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -986,15 +998,32 @@ static int issue_subcycles(evp1d_handle *h, int nsub) {
   return EVP1D_OK;
 }
 
+typedef void (*FusedFn)(DevView, FusedView, int);
+static const int kFusedBlocks[4] = {64, 128, 256, 512};
+static const FusedFn kFusedFns[4][2] = {{k_fused<64, false>, k_fused<64, true>},
+                                        {k_fused<128, false>, k_fused<128, true>},
+                                        {k_fused<256, false>, k_fused<256, true>},
+                                        {k_fused<512, false>, k_fused<512, true>}};
+static int fused_block_index(int b) {
+  for (int i = 0; i < 4; ++i)
+    if (kFusedBlocks[i] == b) return i;
+  return -1;
+}
+
 static int ensure_fused_buffers(evp1d_handle *h) {
+  if (!h->cnz_dev) {
+    HIP_TRY(hipMalloc((void **)&h->cnz_dev, 256));
+    // tables above 64 KiB of LDS
+    for (auto &row : kFusedFns)
+      for (FusedFn fn : row)
+        HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    kFusedMaxEntries * 4 * (int)sizeof(double2)));
+  }
   if (!h->vel_alt) HIP_TRY(hipMalloc((void **)&h->vel_alt, (size_t)h->navel * sizeof(double2)));
   if (!h->sig_alt) HIP_TRY(hipMalloc((void **)&h->sig_alt, (size_t)12 * h->ldsig * sizeof(double)));
   return EVP1D_OK;
 }
 
-typedef void (*FusedFn)(DevView, FusedView, int);
-static const FusedFn kFusedFns[2][2] = {{k_fused<256, false>, k_fused<256, true>},
-                                        {k_fused<512, false>, k_fused<512, true>}};
 
 // nsub fused subcycles.  The state starts in the handle's buffers (X), is
 // copied to the second set (Y), and the subcycles alternate so that the last
@@ -1007,12 +1036,15 @@ static int issue_fused(evp1d_handle *h, int nsub) {
   Y.sig = h->sig_alt;
   HIP_TRY(hipMemcpyAsync(Y.vel, X.vel, (size_t)h->navel * sizeof(double2), hipMemcpyDeviceToDevice, h->stream));
   HIP_TRY(hipMemcpyAsync(Y.sig, X.sig, (size_t)12 * h->ldsig * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, h->stream, h->cnz_dev);
+  HIP_TRY(hipGetLastError());
   if (h->nk_total > 0) {
     hipLaunchKernelGGL(k_fill_const, dim3((unsigned)((h->nk_total + 255) / 256)), dim3(256), 0, h->stream, X,
-                       (const int32_t *)h->klist_dev, h->cstr_dev, h->nk_total);
+                       (const int32_t *)h->klist_dev, h->cstr_dev, h->nk_total, h->cnz_dev);
     HIP_TRY(hipGetLastError());
   }
-  FusedView f{h->tiles_dev, h->hlist_dev, h->uloc_dev, h->cstr_dev, h->nk_total, nullptr, nullptr, h->fused_cap};
+  FusedView f{h->tiles_dev, h->hlist_dev, h->uloc_dev, h->cstr_dev, h->cnz_dev, h->nk_total, nullptr, nullptr,
+              h->fused_cap};
   const size_t lds = (size_t)4 * h->fused_cap * sizeof(double2);
   for (int k = 1; k <= nsub; ++k) {
     const bool to_x = ((nsub - k) % 2) == 0;
@@ -1021,7 +1053,7 @@ static int issue_fused(evp1d_handle *h, int nsub) {
     f.vel_out = out.vel;
     f.sig_out = out.sig;
     if (h->fused_ntiles > 0) {
-      hipLaunchKernelGGL(kFusedFns[h->tile_block == 512 ? 1 : 0][h->tile_xcd ? 1 : 0], dim3(h->fused_ntiles),
+      hipLaunchKernelGGL(kFusedFns[fused_block_index(h->tile_block)][h->tile_xcd ? 1 : 0], dim3(h->fused_ntiles),
                          dim3(h->tile_block), lds, h->stream, in, f, k == nsub ? 1 : 0);
       HIP_TRY(hipGetLastError());
     }
@@ -1455,7 +1487,10 @@ int evp1d_run(evp1d_handle *h, int32_t nsub) {
       return run_graph(h, nsub, true);
     case EVP1D_MODE_GRAPH: return run_graph(h, nsub, false);
     case EVP1D_MODE_AUTO:
-    default: return run_graph(h, nsub, h->fused_ok);
+    default:
+      // the fused form copies the state once per run (two buffers), which
+      // only pays off over several subcycles
+      return run_graph(h, nsub, h->fused_ok && nsub >= kFusedMinSub);
   }
 }
 
@@ -1512,7 +1547,7 @@ int evp1d_set_tiling(evp1d_handle *h, int32_t tile_w, int32_t tile_h, int32_t bl
     return fail(EVP1D_EINVAL, "evp1d_set_tiling: tile %d x %d: need w, h >= 1 and (w+1)(h+1) <= %d", tile_w, tile_h,
                 kFusedMaxEntries);
   if (xcd != 0 && xcd != 1) return fail(EVP1D_EINVAL, "evp1d_set_tiling: xcd must be 0 or 1");
-  if (block != 256 && block != 512) return fail(EVP1D_EINVAL, "evp1d_set_tiling: block must be 256 or 512");
+  if (fused_block_index(block) < 0) return fail(EVP1D_EINVAL, "evp1d_set_tiling: block must be 64, 128, 256 or 512");
   h->tile_block = block;
   if (tile_w != h->tile_w || tile_h != h->tile_h) h->layout_dirty = true;
   h->tile_w = tile_w;

@@ -207,7 +207,7 @@ def build_index(tmask: np.ndarray, nghost: int = 1, fma: bool = False) -> dict:
 
 
 def tile_plan(idx: dict, skipT, skipU, navel: int, tile_w: int = 32, tile_h: int = 16,
-              max_entries: int = 1024, fma: bool = False) -> dict:
+              max_entries: int = 2560, fma: bool = False) -> dict:
     """Host-only fused-subcycle tiling of a 1D list (evp1d_tile_plan).
 
     idx: the six 1-based neighbour lists.  Returns perm (slot -> 1-based

@@ -105,7 +105,7 @@ enum evp1d_mode {
   EVP1D_MODE_LAUNCH = 0,      /* one stress and one stepu launch per subcycle */
   EVP1D_MODE_GRAPH = 1,       /* the nsub x (stress, stepu, halo) sequence captured once as a hipGraph */
   EVP1D_MODE_PERSISTENT = 2,  /* one cooperative kernel, grid barrier between phases */
-  EVP1D_MODE_AUTO = 3,        /* library default: FUSED when the lists tile, else GRAPH */
+  EVP1D_MODE_AUTO = 3,        /* library default: FUSED when the lists tile and nsub >= 4, else GRAPH */
   EVP1D_MODE_FUSED = 4        /* one launch per subcycle: stress and stepu per tile of points, str1..8
                                  kept on chip (hipGraph); needs lists that describe a grid */
 };
@@ -196,7 +196,7 @@ int evp1d_get_tuning(const evp1d_handle *h, int32_t *stress_block, int32_t *stre
  * points its U points read (evaluated again, never written), str1..8 kept in
  * LDS, then stepu.  Tiles are tile_w x tile_h boxes of grid coordinates that
  * the library recovers from the neighbour lists.  evp1d_set_tiling chooses
- * the tile ((tile_w+1)(tile_h+1) <= 1024), the workgroup size (256 or 512)
+ * the tile ((tile_w+1)(tile_h+1) <= 2560), the workgroup size (64 to 512)
  * and the XCD-aware tile order (xcd = 0/1); a new tile re-lays the points out
  * at the next run.  evp1d_get_tiling reports them and the number of tiles (0
  * when the lists do not describe a grid: the fused form is then unavailable).

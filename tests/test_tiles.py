@@ -134,8 +134,8 @@ def test_lists_that_are_no_grid_give_no_tiles():
 
 
 def test_tile_too_large_for_lds_gives_no_tiles():
-    ix, skipT, skipU = _lists(60, 60, 10, 1.0, pT=1.0)
-    plan = evp1d.tile_plan(ix, skipT, skipU, ix["navel"], 40, 40, max_entries=1024)
+    ix, skipT, skipU = _lists(80, 80, 10, 1.0, pT=1.0)
+    plan = evp1d.tile_plan(ix, skipT, skipU, ix["navel"], 60, 60, max_entries=2560)
     assert plan["ntiles"] == 0
 
 
