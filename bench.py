@@ -183,6 +183,9 @@ def adjoint_measure(s: dict, ndte: int, t_fwd_step: float, nsnap: int, ndir: int
     timings = {}
     grads = {}
     for ns in sorted({min(nsnap, ndte), ndte}):
+        if ns >= ndte:   # first call allocates the trajectory storage (kept by the handle)
+            reset_state(eng, s)
+            eng.adjoint_run(ndte, ut, vt, nsnap=ns)
         reset_state(eng, s)
         eng.synchronize()
         t0 = time.perf_counter()

@@ -282,7 +282,11 @@ struct FusedView {
   int cap;                  // entries per LDS table
 };
 
-template <int B, bool X>
+// SPLIT: the gradient trajectory's forward sweep.  Velocities are split
+// addressed (see Ld), stepu writes f.vel_out[q - u_begin], and str1..8 of the
+// owned points are stored every subcycle (d.s15..s48 then hold one subcycle's
+// t_end entries of the trajectory).
+template <int B, bool X, bool SPLIT = false>
 __global__ __launch_bounds__(B) void k_fused(DevView d, FusedView f, int last) {
   extern __shared__ double2 lds[];
   const int cap = f.cap;
@@ -294,7 +298,7 @@ __global__ __launch_bounds__(B) void k_fused(DevView d, FusedView f, int last) {
     if (k < nS) {
       const int g = k < nA ? t.a0 + k : (k < nO ? t.b0 + (k - nA) : f.hlist[t.h0 + (k - nO)]);
       evp1d::StressOut o;
-      stress_eval(d, g, o);
+      stress_eval<SPLIT>(d, g, o);
       p15 = make_double2(o.str[0], o.str[4]);
       p27 = make_double2(o.str[1], o.str[6]);
       p36 = make_double2(o.str[2], o.str[5]);
@@ -306,7 +310,7 @@ __global__ __launch_bounds__(B) void k_fused(DevView d, FusedView f, int last) {
           f.sig_out[(4 + c) * d.ldsig + g] = o.sm[c];
           f.sig_out[(8 + c) * d.ldsig + g] = o.s12[c];
         }
-        if (last) {   // str1..8 are observable after the loop
+        if (last || SPLIT) {   // str1..8 are observable after the loop
           d.s15[g] = p15;
           d.s27[g] = p27;
           d.s36[g] = p36;
@@ -336,12 +340,12 @@ __global__ __launch_bounds__(B) void k_fused(DevView d, FusedView f, int last) {
     const double2 b = lds[cap + (w.x >> 16)];
     const double2 c = lds[2 * cap + (w.y & 0xffffu)];
     const double2 e = lds[3 * cap + (w.y >> 16)];
-    const double2 v = d.vel[q];
+    const double2 v = Ld<SPLIT>::vel(d, q);
     evp1d::StepuOut o;
     evp1d::stepu_point(d.s, v.x, v.y, d.cw[q], d.aiu[q], d.uocn[q], d.vocn[q], d.waterx[q], d.watery[q],
                        d.forcex[q], d.forcey[q], d.umassdti[q], d.fm[q], d.uarear[q], d.uvel_init[q],
                        d.vvel_init[q], d.tbu[q], a.x, b.x, c.x, e.x, a.y, c.y, b.y, e.y, o);
-    f.vel_out[q] = make_double2(o.uvel, o.vvel);
+    f.vel_out[Ld<SPLIT>::vslot(d, q)] = make_double2(o.uvel, o.vvel);
     if (last) {
       d.strintx[q] = o.strintx;
       d.strinty[q] = o.strinty;
@@ -692,6 +696,8 @@ struct evp1d_handle {
   int *cnz_dev = nullptr;          // constant entries not all +0.0
   double2 *vel_alt = nullptr;      // second velocity buffer (navel)
   double *sig_alt = nullptr;       // second stress buffer (12 x ldsig)
+  void *traj = nullptr;            // gradient trajectory storage, kept between calls
+  size_t traj_bytes = 0;
   int graph_mode = -1;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -746,6 +752,7 @@ static void release(evp1d_handle *h) {
   free_tiling(h);
   if (h->vel_alt) (void)hipFree(h->vel_alt);
   if (h->cnz_dev) (void)hipFree(h->cnz_dev);
+  if (h->traj) (void)hipFree(h->traj);
   if (h->sig_alt) (void)hipFree(h->sig_alt);
   for (void *p : h->allocs) (void)hipFree(p);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -1004,6 +1011,8 @@ static const FusedFn kFusedFns[4][2] = {{k_fused<64, false>, k_fused<64, true>},
                                         {k_fused<128, false>, k_fused<128, true>},
                                         {k_fused<256, false>, k_fused<256, true>},
                                         {k_fused<512, false>, k_fused<512, true>}};
+static const FusedFn kFusedSplitFns[4] = {k_fused<64, true, true>, k_fused<128, true, true>,
+                                         k_fused<256, true, true>, k_fused<512, true, true>};
 static int fused_block_index(int b) {
   for (int i = 0; i < 4; ++i)
     if (kFusedBlocks[i] == b) return i;
@@ -1018,6 +1027,9 @@ static int ensure_fused_buffers(evp1d_handle *h) {
       for (FusedFn fn : row)
         HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     kFusedMaxEntries * 4 * (int)sizeof(double2)));
+    for (FusedFn fn : kFusedSplitFns)
+      HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  kFusedMaxEntries * 4 * (int)sizeof(double2)));
   }
   if (!h->vel_alt) HIP_TRY(hipMalloc((void **)&h->vel_alt, (size_t)h->navel * sizeof(double2)));
   if (!h->sig_alt) HIP_TRY(hipMalloc((void **)&h->sig_alt, (size_t)12 * h->ldsig * sizeof(double)));
@@ -1686,11 +1698,11 @@ struct AdjointRun {
   int working = -1;              // index i of the state x_i held by the working arrays
   double loss = 0.0;
 
-  int alloc(void **p, size_t bytes) {
+  int alloc(void **p, size_t bytes, bool zero = true) {
     const hipError_t e = hipMalloc(p, bytes ? bytes : 256);
     if (e != hipSuccess) return fail(EVP1D_ENOMEM, "evp1d_adjoint_run: hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
     mem.push_back(*p);
-    HIP_TRY(hipMemsetAsync(*p, 0, bytes ? bytes : 256, h->stream));
+    if (zero) HIP_TRY(hipMemsetAsync(*p, 0, bytes ? bytes : 256, h->stream));
     return EVP1D_OK;
   }
   ~AdjointRun() {
@@ -1788,14 +1800,31 @@ return EVP1D_OK;
     const DevView W = h->v;
     const int N = nsub, nu = W.u_end - W.u_begin, nt = W.t_end;
     const size_t ub = (size_t)N * nu * sizeof(double2), sb = (size_t)(N > 1 ? N - 1 : 1) * nt * sizeof(double2);
-    size_t free_b = 0, total_b = 0;
-    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
-    if (ub + 4 * sb + ((size_t)1 << 30) > free_b)
-      return fail(EVP1D_ENOMEM, "evp1d_adjoint_run: storing %d states needs %.1f GB, %.1f GB free; use fewer", N,
-                  (ub + 4 * sb) / 1e9, free_b / 1e9);
-    double2 *U = nullptr, *S[4] = {nullptr, nullptr, nullptr, nullptr};
-    RC_TRY(alloc((void **)&U, ub));
-    for (int c = 0; c < 4; ++c) RC_TRY(alloc((void **)&S[c], sb));
+    // The trajectory storage stays with the handle (large allocations and
+    // frees are slow) until it is destroyed or a checkpointed run releases
+    // it.  Every entry is written before it is read.
+    const size_t ua = (ub + 255) / 256 * 256, sa = (sb + 255) / 256 * 256, need = ua + 4 * sa;
+    if (h->traj_bytes < need) {
+      HIP_TRY(hipStreamSynchronize(h->stream));
+      if (h->traj) HIP_TRY(hipFree(h->traj));
+      h->traj = nullptr;
+      h->traj_bytes = 0;
+      size_t free_b = 0, total_b = 0;
+      HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+      if (need + ((size_t)1 << 30) > free_b)
+        return fail(EVP1D_ENOMEM, "evp1d_adjoint_run: storing %d states needs %.1f GB, %.1f GB free; use fewer", N,
+                    need / 1e9, free_b / 1e9);
+      const hipError_t e = hipMalloc(&h->traj, need);
+      if (e != hipSuccess) {
+        h->traj = nullptr;
+        return fail(EVP1D_ENOMEM, "evp1d_adjoint_run: hipMalloc(%zu): %s", need, hipGetErrorString(e));
+      }
+      h->traj_bytes = need;
+    }
+    char *base = static_cast<char *>(h->traj);
+    double2 *U = reinterpret_cast<double2 *>(base);
+    double2 *S[4];
+    for (int c = 0; c < 4; ++c) S[c] = reinterpret_cast<double2 *>(base + ua + (size_t)c * sa);
     if (nu > 0)
       HIP_TRY(hipMemcpyAsync(U, W.vel + W.u_begin, (size_t)nu * sizeof(double2), hipMemcpyDeviceToDevice, h->stream));
     auto view = [&](int k) {   // the split view of subcycle k (x_{k-1} -> x_k)
@@ -1812,11 +1841,40 @@ return EVP1D_OK;
       }
       return dk;
     };
-    for (int k = 1; k <= N; ++k) {
-      const DevView dk = view(k);
-      if (nt > 0) hipLaunchKernelGGL(k_stress_split, dim3(blocks(nt)), dim3(256), 0, h->stream, dk);
-      if (nu > 0) hipLaunchKernelGGL(k_stepu_split, dim3(blocks(nu)), dim3(256), 0, h->stream, dk, k == N ? 1 : 0);
+    if (h->fused_ok) {
+      // fused forward sweep; the stresses ping-pong between the handle's two
+      // buffers and end in the handle's own (subcycle k writes it when N - k
+      // is even)
+      RC_TRY(ensure_fused_buffers(h));
+      HIP_TRY(hipMemcpyAsync(h->sig_alt, W.sig, (size_t)12 * h->ldsig * sizeof(double), hipMemcpyDeviceToDevice,
+                             h->stream));
+      hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, h->stream, h->cnz_dev);
+      if (h->nk_total > 0)
+        hipLaunchKernelGGL(k_fill_const, dim3((unsigned)((h->nk_total + 255) / 256)), dim3(256), 0, h->stream, W,
+                           (const int32_t *)h->klist_dev, h->cstr_dev, h->nk_total, h->cnz_dev);
       HIP_TRY(hipGetLastError());
+      FusedView f{h->tiles_dev, h->hlist_dev, h->uloc_dev, h->cstr_dev, h->cnz_dev, h->nk_total, nullptr, nullptr,
+                  h->fused_cap};
+      const size_t lds = (size_t)4 * h->fused_cap * sizeof(double2);
+      for (int k = 1; k <= N; ++k) {
+        DevView dk = view(k);
+        const bool to_w = ((N - k) % 2) == 0;
+        dk.sig = to_w ? h->sig_alt : W.sig;
+        f.sig_out = to_w ? W.sig : h->sig_alt;
+        f.vel_out = dk.vel_out;
+        if (h->fused_ntiles > 0)
+          hipLaunchKernelGGL(kFusedSplitFns[fused_block_index(h->tile_block)], dim3(h->fused_ntiles),
+                             dim3(h->tile_block), lds, h->stream, dk, f, k == N ? 1 : 0);
+        HIP_TRY(hipGetLastError());
+      }
+    } else {
+      for (int k = 1; k <= N; ++k) {
+        const DevView dk = view(k);
+        if (nt > 0) hipLaunchKernelGGL(k_stress_split, dim3(blocks(nt)), dim3(256), 0, h->stream, dk);
+        if (nu > 0)
+          hipLaunchKernelGGL(k_stepu_split, dim3(blocks(nu)), dim3(256), 0, h->stream, dk, k == N ? 1 : 0);
+        HIP_TRY(hipGetLastError());
+      }
     }
     // the handle now holds x_N (stresses, velocities, str1..8, strint, Cb)
     if (h->na > 0) {
@@ -1915,6 +1973,12 @@ int evp1d_adjoint_run(evp1d_handle *h, int32_t nsub, int32_t nsnap, const double
   if (nsnap >= nsub) {
     RC_TRY(R.run_stored());
   } else {
+    if (h->traj) {   // a checkpointed run gives the stored trajectory's memory back
+      HIP_TRY(hipStreamSynchronize(h->stream));
+      HIP_TRY(hipFree(h->traj));
+      h->traj = nullptr;
+      h->traj_bytes = 0;
+    }
     RC_TRY(R.alloc((void **)&R.str_save, 4 * nV));
     RC_TRY(R.alloc((void **)&R.final_state.sig, R.sig_bytes()));
     RC_TRY(R.alloc((void **)&R.final_state.vel, nV));

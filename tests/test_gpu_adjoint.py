@@ -130,3 +130,18 @@ def test_gradient_is_zero_off_ice(problem):
     for k in ("cdn_ocn", "forcex", "forcey"):
         assert np.all(g[k][st.skipU != 0] == 0.0)
     assert np.any(g["strength"] != 0.0) and np.any(g["forcex"] != 0.0)
+
+
+def test_repeated_gradient_calls_on_one_handle(problem):
+    """The stored trajectory's memory is kept and reused by later calls and
+    released by a checkpointed one; every call starts from the state it finds."""
+    st, ut, vt = problem
+    with make_engine(st, NDTE) as eng:
+        ref = eng.adjoint_run(NDTE, ut, vt, nsnap=NDTE)
+    with make_engine(st, NDTE) as eng:
+        for nsnap in (NDTE, NDTE, 4, NDTE):
+            eng.load_state(st.f, st.idx, st.skipT, st.skipU)
+            g = eng.adjoint_run(NDTE, ut, vt, nsnap=nsnap)
+            for k in ("strength", "cdn_ocn", "forcex", "forcey"):
+                assert np.array_equal(g[k], ref[k]), (nsnap, k)
+            assert g["loss"] == ref["loss"]
