@@ -64,7 +64,8 @@ program evp1d_v2_driver
     case ('graph');      mode = MODE_GRAPH
     case ('persistent'); mode = MODE_PERSISTENT
     case ('launch');     mode = MODE_LAUNCH
-    case default;        error stop 'usage: evp1d_v2_driver [graph|persistent|launch]'
+    case ('fused');      mode = MODE_FUSED
+    case default;        error stop 'usage: evp1d_v2_driver [graph|persistent|launch|fused]'
     end select
   endif
 
