@@ -652,6 +652,28 @@ __global__ void k_permute_pair(double2 *dst, const double2 *src, const int32_t *
 // ---------------------------------------------------------------------------
 // handle
 // ---------------------------------------------------------------------------
+// 2D <-> 1D through the slot -> cell map (convert_2d_1d_dyn / convert_1d_2d_dyn)
+__global__ void k_pack2d_plain(double *dst, const double *g, const int32_t *pos, int na) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < na) dst[s] = g[pos[s]];
+}
+__global__ void k_pack2d_comp(double2 *dst, const double *g, const int32_t *pos, int na, int navel, int comp,
+                              int zero_ghosts) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= navel || (s >= na && !zero_ghosts)) return;
+  const double v = s < na ? g[pos[s]] : 0.0;
+  double *e = reinterpret_cast<double *>(dst + s);
+  e[comp] = v;
+}
+__global__ void k_unpack2d_plain(double *g, const double *src, const int32_t *pos, int na) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < na) g[pos[s]] = src[s];
+}
+__global__ void k_unpack2d_comp(double *g, const double2 *src, const int32_t *pos, int na, int comp) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < na) g[pos[s]] = reinterpret_cast<const double *>(src + s)[comp];
+}
+
 struct evp1d_handle {
   int na = 0, navel = 0;
   long ldsig = 0;
@@ -698,6 +720,12 @@ struct evp1d_handle {
   double *sig_alt = nullptr;       // second stress buffer (12 x ldsig)
   void *traj = nullptr;            // gradient trajectory storage, kept between calls
   size_t traj_bytes = 0;
+  // 2D grid of the caller's arrays (device-side pack/unpack)
+  int grid_nx = 0, grid_ny = 0;
+  std::vector<int32_t> grid_ij;    // caller position < na -> 0-based linear 2D index
+  int32_t *pos2d_dev = nullptr;    // slot < na -> linear 2D index in the current slot order
+  bool pos2d_dirty = true;
+  double *g2d[EVP1D_NFIELDS] = {}; // device copy of each field's 2D array
   int graph_mode = -1;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -753,6 +781,9 @@ static void release(evp1d_handle *h) {
   if (h->vel_alt) (void)hipFree(h->vel_alt);
   if (h->cnz_dev) (void)hipFree(h->cnz_dev);
   if (h->traj) (void)hipFree(h->traj);
+  if (h->pos2d_dev) (void)hipFree(h->pos2d_dev);
+  for (double *g : h->g2d)
+    if (g) (void)hipFree(g);
   if (h->sig_alt) (void)hipFree(h->sig_alt);
   for (void *p : h->allocs) (void)hipFree(p);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -924,6 +955,7 @@ static int relayout(evp1d_handle *h) {
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->perm = nperm;
     h->inv.swap(ninv);
+    h->pos2d_dirty = true;
   }
   RC_TRY(install_tiling(h, plan));
   h->v.t_end = plan.cnt[0] + plan.cnt[1];
@@ -2106,6 +2138,121 @@ int evp1d_loss(evp1d_handle *h, const double *u_target, const double *v_target, 
   (void)hipFree(buf);
   if (rc) return rc;
   for (double x : t) *loss += x;
+  return EVP1D_OK;
+}
+
+static int ensure_pos2d(evp1d_handle *h) {
+  if (!h->pos2d_dirty && h->pos2d_dev) return EVP1D_OK;
+  if (!h->pos2d_dev) HIP_TRY(hipMalloc((void **)&h->pos2d_dev, (size_t)std::max(h->na, 1) * sizeof(int32_t)));
+  std::vector<int32_t> pos((size_t)h->na);
+  for (int s = 0; s < h->na; ++s) pos[(size_t)s] = h->grid_ij[(size_t)h->perm[(size_t)s]];
+  if (h->na > 0)
+    HIP_TRY(hipMemcpyAsync(h->pos2d_dev, pos.data(), (size_t)h->na * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->pos2d_dirty = false;
+  return EVP1D_OK;
+}
+
+int evp1d_set_grid(evp1d_handle *h, int32_t nx, int32_t ny, const int32_t *indxtij, int32_t n) {
+  if (!h) return fail(EVP1D_EINVAL, "evp1d_set_grid: NULL handle");
+  if (h->m_na.size() != 1) return fail(EVP1D_EINVAL, "evp1d_set_grid: only for single-domain handles");
+  if (nx < 1 || ny < 1 || (long)nx * ny > 0x7fffffffL) return fail(EVP1D_EINVAL, "evp1d_set_grid: bad nx=%d ny=%d", nx, ny);
+  if (n != h->navel) return fail(EVP1D_EINVAL, "evp1d_set_grid: length %d != navel %d", n, h->navel);
+  if (n > 0 && !indxtij) return fail(EVP1D_EINVAL, "evp1d_set_grid: indxtij is NULL");
+  std::vector<int32_t> ij((size_t)h->na);
+  for (int q = 0; q < h->na; ++q) {
+    const int32_t v = indxtij[q];
+    if (v < 1 || v > nx * ny)
+      return fail(EVP1D_EINDEX, "evp1d_set_grid: indxtij(%d) = %d outside 1..%d", q + 1, v, nx * ny);
+    ij[(size_t)q] = v - 1;
+  }
+  if (nx != h->grid_nx || ny != h->grid_ny)
+    for (double *&g : h->g2d) {
+      if (g) HIP_TRY(hipFree(g));
+      g = nullptr;
+    }
+  h->grid_nx = nx;
+  h->grid_ny = ny;
+  h->grid_ij.swap(ij);
+  h->pos2d_dirty = true;
+  return EVP1D_OK;
+}
+
+int evp1d_put_2d(evp1d_handle *h, int32_t nfield, const int32_t *fields, const double *arrays) {
+  if (!h) return fail(EVP1D_EINVAL, "evp1d_put_2d: NULL handle");
+  if (h->grid_nx == 0) return fail(EVP1D_ESTATE, "evp1d_put_2d: evp1d_set_grid was not called");
+  if (nfield < 0 || (nfield > 0 && (!fields || !arrays))) return fail(EVP1D_EINVAL, "evp1d_put_2d: bad arguments");
+  const size_t n2 = (size_t)h->grid_nx * h->grid_ny;
+  FieldRef r;
+  for (int k = 0; k < nfield; ++k)
+    if (!field_ref(h, fields[k], r)) return fail(EVP1D_EINVAL, "evp1d_put_2d: unknown field %d", fields[k]);
+  RC_TRY(ensure_pos2d(h));
+  const int na = h->na, navel = h->navel;
+  for (int k = 0; k < nfield; ++k) {
+    const int f = fields[k];
+    field_ref(h, f, r);
+    if (!h->g2d[f]) HIP_TRY(hipMalloc((void **)&h->g2d[f], n2 * sizeof(double)));
+    HIP_TRY(hipMemcpyAsync(h->g2d[f], arrays + (size_t)k * n2, n2 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if (r.kind == FK_PLAIN) {
+      if (na > 0)
+        hipLaunchKernelGGL(k_pack2d_plain, dim3(blocks_for(na)), dim3(kBlock), 0, h->stream, r.plain,
+                           (const double *)h->g2d[f], (const int32_t *)h->pos2d_dev, na);
+    } else {
+      const int zg = (f == EVP1D_UVEL || f == EVP1D_VVEL) ? 1 : 0;   // ice_dyn_evp1d.F90:880-881
+      if (navel > 0)
+        hipLaunchKernelGGL(k_pack2d_comp, dim3(blocks_for(navel)), dim3(kBlock), 0, h->stream, r.pair,
+                           (const double *)h->g2d[f], (const int32_t *)h->pos2d_dev, na, navel, r.comp, zg);
+    }
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream));   // the caller may reuse its arrays
+  return EVP1D_OK;
+}
+
+int evp1d_get_2d(evp1d_handle *h, int32_t nfield, const int32_t *fields, double *arrays) {
+  if (!h) return fail(EVP1D_EINVAL, "evp1d_get_2d: NULL handle");
+  if (h->grid_nx == 0) return fail(EVP1D_ESTATE, "evp1d_get_2d: evp1d_set_grid was not called");
+  if (nfield < 0 || (nfield > 0 && (!fields || !arrays))) return fail(EVP1D_EINVAL, "evp1d_get_2d: bad arguments");
+  const size_t n2 = (size_t)h->grid_nx * h->grid_ny;
+  FieldRef r;
+  for (int k = 0; k < nfield; ++k) {
+    if (!field_ref(h, fields[k], r)) return fail(EVP1D_EINVAL, "evp1d_get_2d: unknown field %d", fields[k]);
+    if (!h->g2d[fields[k]])
+      return fail(EVP1D_ESTATE, "evp1d_get_2d: field %d was never put with evp1d_put_2d", fields[k]);
+  }
+  RC_TRY(ensure_pos2d(h));
+  const int na = h->na;
+  for (int k = 0; k < nfield; ++k) {
+    const int f = fields[k];
+    field_ref(h, f, r);
+    if (na > 0) {
+      if (r.kind == FK_PLAIN)
+        hipLaunchKernelGGL(k_unpack2d_plain, dim3(blocks_for(na)), dim3(kBlock), 0, h->stream, h->g2d[f],
+                           (const double *)r.plain, (const int32_t *)h->pos2d_dev, na);
+      else
+        hipLaunchKernelGGL(k_unpack2d_comp, dim3(blocks_for(na)), dim3(kBlock), 0, h->stream, h->g2d[f],
+                           (const double2 *)r.pair, (const int32_t *)h->pos2d_dev, na, r.comp);
+      HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipMemcpyAsync(arrays + (size_t)k * n2, h->g2d[f], n2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return EVP1D_OK;
+}
+
+int evp1d_host_alloc(int64_t bytes, void **ptr) {
+  if (!ptr || bytes < 0) return fail(EVP1D_EINVAL, "evp1d_host_alloc: bad arguments");
+  *ptr = nullptr;
+  const hipError_t e = hipHostMalloc(ptr, (size_t)std::max<int64_t>(bytes, 1), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    return fail(EVP1D_ENOMEM, "evp1d_host_alloc(%lld): %s", (long long)bytes, hipGetErrorString(e));
+  }
+  return EVP1D_OK;
+}
+
+int evp1d_host_free(void *ptr) {
+  if (ptr) HIP_TRY(hipHostFree(ptr));
   return EVP1D_OK;
 }
 

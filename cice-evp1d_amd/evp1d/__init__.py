@@ -56,6 +56,7 @@ EXPORTS = (
     "evp1d_autotune", "evp1d_set_tuning", "evp1d_get_tuning",
     "evp1d_adjoint_run", "evp1d_loss", "evp1d_tangent_run",
     "evp1d_set_tiling", "evp1d_get_tiling", "evp1d_tile_plan",
+    "evp1d_set_grid", "evp1d_put_2d", "evp1d_get_2d", "evp1d_host_alloc", "evp1d_host_free",
 )
 
 
@@ -139,6 +140,11 @@ def load(fma: bool = False, path: str | os.PathLike | None = None) -> ctypes.CDL
         "evp1d_loss": [_H, _DP, _DP, _DP],
         "evp1d_tangent_run": [_H, i32, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP],
         "evp1d_set_tiling": [_H, i32, i32, i32, i32],
+        "evp1d_set_grid": [_H, i32, i32, _IP, i32],
+        "evp1d_put_2d": [_H, i32, _IP, _DP],
+        "evp1d_get_2d": [_H, i32, _IP, _DP],
+        "evp1d_host_alloc": [ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)],
+        "evp1d_host_free": [ctypes.c_void_p],
         "evp1d_get_tiling": [_H, _IP, _IP, _IP, _IP, _IP],
         "evp1d_tile_plan": [i32, i32] + [_IP] * 8 + [i32, i32, i32] + [_IP] * 5 + [i32, _IP, i32, _IP, _IP, i32,
                                                                                 _IP, _IP],
@@ -342,7 +348,27 @@ class Engine:
         _check(self.L, self.L.evp1d_get_tuning(self.h, *[ctypes.byref(x) for x in a]))
         return dict(zip(("stress_block", "stress_xcd", "stepu_block", "stepu_xcd"), (x.value for x in a)))
 
-    def set_tiling(self, tile_w: int = 32, tile_h: int = 16, block: int = 256, xcd: int = 1) -> None:
+    def set_grid(self, nx: int, ny: int, indxtij) -> None:
+        ij = np.ascontiguousarray(indxtij, dtype=np.int32)
+        _check(self.L, self.L.evp1d_set_grid(self.h, nx, ny, _ip(ij), len(ij)))
+        self._grid = (nx, ny)
+
+    def put_2d(self, fields: dict) -> None:
+        """fields: name -> (nx, ny) array (any order; sent in Fortran order)."""
+        names = list(fields)
+        ids = np.array([FIELD_ID[n] for n in names], dtype=np.int32)
+        buf = np.stack([np.asfortranarray(fields[n], dtype=np.float64).ravel(order="F") for n in names])
+        _check(self.L, self.L.evp1d_put_2d(self.h, len(names), _ip(ids), _dp(buf)))
+
+    def get_2d(self, names) -> dict:
+        nx, ny = self._grid
+        names = list(names)
+        ids = np.array([FIELD_ID[n] for n in names], dtype=np.int32)
+        buf = np.zeros((len(names), nx * ny))
+        _check(self.L, self.L.evp1d_get_2d(self.h, len(names), _ip(ids), _dp(buf)))
+        return {n: buf[k].reshape((nx, ny), order="F") for k, n in enumerate(names)}
+
+    def set_tiling(self, tile_w: int = 64, tile_h: int = 16, block: int = 512, xcd: int = 1) -> None:
         _check(self.L, self.L.evp1d_set_tiling(self.h, tile_w, tile_h, block, xcd))
 
     def get_tiling(self) -> dict:

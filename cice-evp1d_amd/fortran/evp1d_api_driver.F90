@@ -12,6 +12,8 @@
 !                 f64   the 29 dynamic fields in dyn_evp1d_run argument order
 !   api_out.bin : f64   the 29 dynamic fields after nrun calls of dyn_evp1d_run
 ! with NX = nx_global + 2*nghost, NY likewise, one block (max_block = 1).
+! Prints the wall time of every dyn_evp1d_run call (2D<->1D data movement and
+! the ndte subcycles).
 !
 ! Usage:  evp1d_api_driver [input [output]]
 !===============================================================================
@@ -22,6 +24,7 @@ program evp1d_api_driver
   implicit none
   integer(4) :: hdr(5)
   integer(kind=int_kind) :: nxg, nyg, ng, NX, NY, nrun, r, lun, ios, k
+  integer(8) :: t0, t1, rate
   integer(4), allocatable :: im(:,:,:)
   real(kind=dbl_kind), allocatable :: st(:,:,:), dyn(:,:,:,:), G_HTE(:,:), G_HTN(:,:)
   logical(kind=log_kind), allocatable :: tmask(:,:,:), icet(:,:,:), iceu(:,:,:)
@@ -55,6 +58,7 @@ program evp1d_api_driver
   call dyn_evp1d_init(nxg, nyg, NX, NY, 1, ng,                                   &
                       st(:,:,2:2), st(:,:,1:1), st(:,:,3:3), tmask, G_HTE, G_HTN)
   do r = 1, nrun
+    call system_clock(t0, rate)
     call dyn_evp1d_run(dyn(:,:,:,1), dyn(:,:,:,2), dyn(:,:,:,3), dyn(:,:,:,4),    &
                        dyn(:,:,:,5), dyn(:,:,:,6), dyn(:,:,:,7), dyn(:,:,:,8),    &
                        dyn(:,:,:,9), dyn(:,:,:,10), dyn(:,:,:,11), dyn(:,:,:,12), &
@@ -64,6 +68,9 @@ program evp1d_api_driver
                        dyn(:,:,:,22), dyn(:,:,:,23), dyn(:,:,:,24), dyn(:,:,:,25), &
                        dyn(:,:,:,26), dyn(:,:,:,27), dyn(:,:,:,28), dyn(:,:,:,29), &
                        icet, iceu)
+    call system_clock(t1)
+    write(*,'(a,i0,a,f10.3,a)') ' dyn_evp1d_run ', r, ': ', 1.0e3_dbl_kind * real(t1 - t0, dbl_kind) / &
+                                real(rate, dbl_kind), ' ms'
   enddo
   call dyn_evp2d_dump(dyn(:,:,:,1), dyn(:,:,:,2), dyn(:,:,:,3), dyn(:,:,:,4),     &
                       dyn(:,:,:,5), dyn(:,:,:,6), dyn(:,:,:,7), dyn(:,:,:,8),     &

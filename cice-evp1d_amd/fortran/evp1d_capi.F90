@@ -52,7 +52,8 @@ module evp1d_capi
             evp1d_build_info, evp1d_check, evp1d_create_batch, evp1d_get_members,       &
             evp1d_member_info, evp1d_set_field_member, evp1d_get_field_member,          &
             evp1d_set_index_member, evp1d_set_skip_member, evp1d_autotune,              &
-            evp1d_set_tuning, evp1d_get_tuning, evp1d_set_tiling, evp1d_get_tiling
+            evp1d_set_tuning, evp1d_get_tuning, evp1d_set_tiling, evp1d_get_tiling,       &
+            evp1d_set_grid, evp1d_put_2d, evp1d_get_2d, evp1d_host_alloc, evp1d_host_free
 
   interface
     integer(c_int) function evp1d_set_evp_parameters(ndte, dt, p) bind(C, name='evp1d_set_evp_parameters')
@@ -174,6 +175,35 @@ module evp1d_capi
       integer(c_int32_t), intent(in) :: tmask(*)
       integer(c_int32_t), intent(out) :: indxti(*), indxtj(*), indxtij(*), ee(*), ne(*), se(*), &
                                          nw(*), sw(*), sse(*)
+    end function
+    integer(c_int) function evp1d_set_grid(h, nx, ny, indxtij, navel) bind(C, name='evp1d_set_grid')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nx, ny, navel
+      integer(c_int32_t), intent(in) :: indxtij(*)
+    end function
+    integer(c_int) function evp1d_put_2d(h, nfield, fields, arrays) bind(C, name='evp1d_put_2d')
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nfield
+      integer(c_int32_t), intent(in) :: fields(*)
+      real(c_double), intent(in) :: arrays(*)
+    end function
+    integer(c_int) function evp1d_get_2d(h, nfield, fields, arrays) bind(C, name='evp1d_get_2d')
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nfield
+      integer(c_int32_t), intent(in) :: fields(*)
+      real(c_double), intent(inout) :: arrays(*)
+    end function
+    integer(c_int) function evp1d_host_alloc(bytes, ptr) bind(C, name='evp1d_host_alloc')
+      import :: c_int, c_int64_t, c_ptr
+      integer(c_int64_t), value :: bytes
+      type(c_ptr), intent(out) :: ptr
+    end function
+    integer(c_int) function evp1d_host_free(ptr) bind(C, name='evp1d_host_free')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ptr
     end function
     integer(c_int) function evp1d_last_error(buf, n) bind(C, name='evp1d_last_error')
       import :: c_int, c_int32_t, c_char

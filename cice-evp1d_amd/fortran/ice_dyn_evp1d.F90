@@ -21,7 +21,11 @@
 !   * wet cells in the west/south ghost rim stop with a message instead of
 !     leaving zero indices (ice_dyn_evp1d.F90:503-517 vs 579-580).
 ! The subcycle loop, the 1D list construction and the neighbour search run in
-! libevp1d.so (include/evp1d.h) through module evp1d_capi.
+! libevp1d.so (include/evp1d.h) through module evp1d_capi.  With ldevpack
+! (namelist evp1d_nml, default .true.) the 2D <-> 1D copies of the dynamic
+! fields (convert_2d_1d_dyn / convert_1d_2d_dyn, ice_dyn_evp1d.F90:818-947)
+! also run on the device: whole 2D arrays cross PCIe and the library gathers
+! and scatters them; .false. keeps the host loops.
 !===============================================================================
 module ice_dyn_evp1d
   use, intrinsic :: iso_c_binding
@@ -34,12 +38,24 @@ module ice_dyn_evp1d
   public :: dyn_evp1d_init, dyn_evp1d_run, dyn_evp1d_finalize, dyn_evp2d_dump
 
   type(c_ptr), save :: handle = c_null_ptr
-  logical(kind=log_kind), save :: ldebug = .false., linfo = .true.
+  logical(kind=log_kind), save :: ldebug = .false., linfo = .true., ldevpack = .true.
   integer(kind=int_kind), save :: ts = 0, iu06 = -1, buf1d = 0
   integer(kind=int_kind), save :: nx, ny, nActive, navel, nghost, nx_block, ny_block, max_block
   integer(kind=int_kind), allocatable, save :: indxti(:), indxtj(:)
   real(kind=dbl_kind), allocatable, save :: buf(:)
   integer(c_int32_t), allocatable, save :: ibuf(:)
+  real(kind=dbl_kind), pointer, save :: gall(:,:,:) => null()   ! 2D staging (page-locked)
+  type(c_ptr), save :: gall_mem = c_null_ptr
+  ! the fields dyn_evp1d_run packs (argument order, then uvel/vvel as _init)
+  ! and the fields it unpacks
+  integer(c_int32_t), parameter :: fin(31) = [F_STRESSP_1, F_STRESSP_2, F_STRESSP_3, F_STRESSP_4, &
+      F_STRESSM_1, F_STRESSM_2, F_STRESSM_3, F_STRESSM_4, F_STRESS12_1, F_STRESS12_2, F_STRESS12_3, &
+      F_STRESS12_4, F_STRENGTH, F_CDN_OCN, F_AIU, F_UOCN, F_VOCN, F_WATERX, F_WATERY, F_FORCEX,    &
+      F_FORCEY, F_UMASSDTI, F_FM, F_STRINTX, F_STRINTY, F_TBU, F_CB, F_UVEL, F_VVEL,               &
+      F_UVEL_INIT, F_VVEL_INIT]
+  integer(c_int32_t), parameter :: fout(17) = [F_STRESSP_1, F_STRESSP_2, F_STRESSP_3, F_STRESSP_4, &
+      F_STRESSM_1, F_STRESSM_2, F_STRESSM_3, F_STRESSM_4, F_STRESS12_1, F_STRESS12_2, F_STRESS12_3, &
+      F_STRESS12_4, F_STRINTX, F_STRINTY, F_CB, F_UVEL, F_VVEL]
 
 contains
 
@@ -68,7 +84,7 @@ contains
   !=============================================================================
   subroutine read_nml()
     integer(kind=int_kind) :: lun, ios
-    namelist /evp1d_nml/ ldebug, linfo, buf1d
+    namelist /evp1d_nml/ ldebug, linfo, buf1d, ldevpack
     open(newunit=lun, file='evp_1d.nml', status='old', iostat=ios)
     if (ios == 0) then
       read(lun, nml=evp1d_nml, iostat=ios)
@@ -133,6 +149,13 @@ contains
     do k = 1, 6
       call evp1d_check(evp1d_set_index(handle, int(k-1,c_int32_t), nbr(:,k), na32), 'evp1d_set_index')
     enddo
+    call evp1d_check(evp1d_set_grid(handle, int(nx,c_int32_t), int(ny,c_int32_t), indxtij, navel32), &
+                     'evp1d_set_grid')
+    call free_gall()
+    if (ldevpack) then
+      call evp1d_check(evp1d_host_alloc(int(nx,c_int64_t) * ny * size(fin) * 8, gall_mem), 'evp1d_host_alloc')
+      call c_f_pointer(gall_mem, gall, [nx, ny, size(fin)])
+    endif
     ! static metrics (convert_2d_1d_init, ice_dyn_evp1d.F90:803-815)
     call put_static(G_uarear, 0, 0, F_UAREAR)
     call put_static(G_dxT, 0, 0, F_DXT)
@@ -234,41 +257,100 @@ contains
     if (ldebug) write(iu06,*) 'time step, active points, ice T, ice U: ', ts, nActive, &
                               count(GT), count(GU)
 
-    call gather_put(L_stressp_1, F_STRESSP_1);   call gather_put(L_stressp_2, F_STRESSP_2)
-    call gather_put(L_stressp_3, F_STRESSP_3);   call gather_put(L_stressp_4, F_STRESSP_4)
-    call gather_put(L_stressm_1, F_STRESSM_1);   call gather_put(L_stressm_2, F_STRESSM_2)
-    call gather_put(L_stressm_3, F_STRESSM_3);   call gather_put(L_stressm_4, F_STRESSM_4)
-    call gather_put(L_stress12_1, F_STRESS12_1); call gather_put(L_stress12_2, F_STRESS12_2)
-    call gather_put(L_stress12_3, F_STRESS12_3); call gather_put(L_stress12_4, F_STRESS12_4)
-    call gather_put(L_strength, F_STRENGTH)
-    call gather_put(L_cdn_ocn, F_CDN_OCN);       call gather_put(L_aiu, F_AIU)
-    call gather_put(L_uocn, F_UOCN);             call gather_put(L_vocn, F_VOCN)
-    call gather_put(L_waterxU, F_WATERX);        call gather_put(L_wateryU, F_WATERY)
-    call gather_put(L_forcexU, F_FORCEX);        call gather_put(L_forceyU, F_FORCEY)
-    call gather_put(L_umassdti, F_UMASSDTI);     call gather_put(L_fmU, F_FM)
-    call gather_put(L_strintxU, F_STRINTX);      call gather_put(L_strintyU, F_STRINTY)
-    call gather_put(L_Tbu, F_TBU);               call gather_put(L_Cb, F_CB)
-    call gather_global_ext(G, L_uvel, master_task, distrb_info)
-    call put_dyn(G, F_UVEL, .true.)
-    call put_dyn(G, F_UVEL_INIT, .false.)
-    call gather_global_ext(G, L_vvel, master_task, distrb_info)
-    call put_dyn(G, F_VVEL, .true.)
-    call put_dyn(G, F_VVEL_INIT, .false.)
+    if (ldevpack) then
+      ! 2D -> 1D on the device; fin's order is the argument order
+      call gather_global_ext(gall(:,:,1), L_stressp_1, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,2), L_stressp_2, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,3), L_stressp_3, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,4), L_stressp_4, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,5), L_stressm_1, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,6), L_stressm_2, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,7), L_stressm_3, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,8), L_stressm_4, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,9), L_stress12_1, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,10), L_stress12_2, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,11), L_stress12_3, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,12), L_stress12_4, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,13), L_strength, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,14), L_cdn_ocn, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,15), L_aiu, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,16), L_uocn, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,17), L_vocn, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,18), L_waterxU, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,19), L_wateryU, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,20), L_forcexU, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,21), L_forceyU, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,22), L_umassdti, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,23), L_fmU, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,24), L_strintxU, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,25), L_strintyU, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,26), L_Tbu, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,27), L_Cb, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,28), L_uvel, master_task, distrb_info)
+      call gather_global_ext(gall(:,:,29), L_vvel, master_task, distrb_info)
+      gall(:,:,30) = gall(:,:,28)
+      gall(:,:,31) = gall(:,:,29)
+      call evp1d_check(evp1d_put_2d(handle, int(size(fin),c_int32_t), fin, gall), 'evp1d_put_2d')
+    else
+      call gather_put(L_stressp_1, F_STRESSP_1);   call gather_put(L_stressp_2, F_STRESSP_2)
+      call gather_put(L_stressp_3, F_STRESSP_3);   call gather_put(L_stressp_4, F_STRESSP_4)
+      call gather_put(L_stressm_1, F_STRESSM_1);   call gather_put(L_stressm_2, F_STRESSM_2)
+      call gather_put(L_stressm_3, F_STRESSM_3);   call gather_put(L_stressm_4, F_STRESSM_4)
+      call gather_put(L_stress12_1, F_STRESS12_1); call gather_put(L_stress12_2, F_STRESS12_2)
+      call gather_put(L_stress12_3, F_STRESS12_3); call gather_put(L_stress12_4, F_STRESS12_4)
+      call gather_put(L_strength, F_STRENGTH)
+      call gather_put(L_cdn_ocn, F_CDN_OCN);       call gather_put(L_aiu, F_AIU)
+      call gather_put(L_uocn, F_UOCN);             call gather_put(L_vocn, F_VOCN)
+      call gather_put(L_waterxU, F_WATERX);        call gather_put(L_wateryU, F_WATERY)
+      call gather_put(L_forcexU, F_FORCEX);        call gather_put(L_forceyU, F_FORCEY)
+      call gather_put(L_umassdti, F_UMASSDTI);     call gather_put(L_fmU, F_FM)
+      call gather_put(L_strintxU, F_STRINTX);      call gather_put(L_strintyU, F_STRINTY)
+      call gather_put(L_Tbu, F_TBU);               call gather_put(L_Cb, F_CB)
+      call gather_global_ext(G, L_uvel, master_task, distrb_info)
+      call put_dyn(G, F_UVEL, .true.)
+      call put_dyn(G, F_UVEL_INIT, .false.)
+      call gather_global_ext(G, L_vvel, master_task, distrb_info)
+      call put_dyn(G, F_VVEL, .true.)
+      call put_dyn(G, F_VVEL_INIT, .false.)
+    endif
 
     ! the subcycles (ice_dyn_evp1d.F90:188-207), device resident
     call fill_params(p)
     call evp1d_check(evp1d_set_params(handle, p), 'evp1d_set_params')
     call evp1d_check(evp1d_run(handle, int(ndte,c_int32_t)), 'evp1d_run')
 
-    call get_scatter(L_stressp_1, F_STRESSP_1);   call get_scatter(L_stressp_2, F_STRESSP_2)
-    call get_scatter(L_stressp_3, F_STRESSP_3);   call get_scatter(L_stressp_4, F_STRESSP_4)
-    call get_scatter(L_stressm_1, F_STRESSM_1);   call get_scatter(L_stressm_2, F_STRESSM_2)
-    call get_scatter(L_stressm_3, F_STRESSM_3);   call get_scatter(L_stressm_4, F_STRESSM_4)
-    call get_scatter(L_stress12_1, F_STRESS12_1); call get_scatter(L_stress12_2, F_STRESS12_2)
-    call get_scatter(L_stress12_3, F_STRESS12_3); call get_scatter(L_stress12_4, F_STRESS12_4)
-    call get_scatter(L_strintxU, F_STRINTX);      call get_scatter(L_strintyU, F_STRINTY)
-    call get_scatter(L_Cb, F_CB)
-    call get_scatter(L_uvel, F_UVEL, .true.);     call get_scatter(L_vvel, F_VVEL, .true.)
+    if (ldevpack) then
+      ! 1D -> 2D on the device into gall(:,:,1:17) (fout is fin's first 12, 24,
+      ! 25, 27, 28, 29 in that order), then back to the caller's blocks
+      call evp1d_check(evp1d_get_2d(handle, int(size(fout),c_int32_t), fout, gall), 'evp1d_get_2d')
+      call scatter_global_ext(L_stressp_1, gall(:,:,1), master_task, distrb_info)
+      call scatter_global_ext(L_stressp_2, gall(:,:,2), master_task, distrb_info)
+      call scatter_global_ext(L_stressp_3, gall(:,:,3), master_task, distrb_info)
+      call scatter_global_ext(L_stressp_4, gall(:,:,4), master_task, distrb_info)
+      call scatter_global_ext(L_stressm_1, gall(:,:,5), master_task, distrb_info)
+      call scatter_global_ext(L_stressm_2, gall(:,:,6), master_task, distrb_info)
+      call scatter_global_ext(L_stressm_3, gall(:,:,7), master_task, distrb_info)
+      call scatter_global_ext(L_stressm_4, gall(:,:,8), master_task, distrb_info)
+      call scatter_global_ext(L_stress12_1, gall(:,:,9), master_task, distrb_info)
+      call scatter_global_ext(L_stress12_2, gall(:,:,10), master_task, distrb_info)
+      call scatter_global_ext(L_stress12_3, gall(:,:,11), master_task, distrb_info)
+      call scatter_global_ext(L_stress12_4, gall(:,:,12), master_task, distrb_info)
+      call scatter_global_ext(L_strintxU, gall(:,:,13), master_task, distrb_info)
+      call scatter_global_ext(L_strintyU, gall(:,:,14), master_task, distrb_info)
+      call scatter_global_ext(L_Cb, gall(:,:,15), master_task, distrb_info)
+      call scatter_global_ext(L_uvel, gall(:,:,16), master_task, distrb_info)
+      call scatter_global_ext(L_vvel, gall(:,:,17), master_task, distrb_info)
+    else
+      call get_scatter(L_stressp_1, F_STRESSP_1);   call get_scatter(L_stressp_2, F_STRESSP_2)
+      call get_scatter(L_stressp_3, F_STRESSP_3);   call get_scatter(L_stressp_4, F_STRESSP_4)
+      call get_scatter(L_stressm_1, F_STRESSM_1);   call get_scatter(L_stressm_2, F_STRESSM_2)
+      call get_scatter(L_stressm_3, F_STRESSM_3);   call get_scatter(L_stressm_4, F_STRESSM_4)
+      call get_scatter(L_stress12_1, F_STRESS12_1); call get_scatter(L_stress12_2, F_STRESS12_2)
+      call get_scatter(L_stress12_3, F_STRESS12_3); call get_scatter(L_stress12_4, F_STRESS12_4)
+      call get_scatter(L_strintxU, F_STRINTX);      call get_scatter(L_strintyU, F_STRINTY)
+      call get_scatter(L_Cb, F_CB)
+      call get_scatter(L_uvel, F_UVEL, .true.);     call get_scatter(L_vvel, F_VVEL, .true.)
+    endif
     deallocate(G, GT, GU)
     ts = ts + 1
 
@@ -348,6 +430,12 @@ contains
   end subroutine dyn_evp2d_dump
 
   !=============================================================================
+  subroutine free_gall()
+    if (c_associated(gall_mem)) call evp1d_check(evp1d_host_free(gall_mem), 'evp1d_host_free')
+    gall_mem = c_null_ptr
+    gall => null()
+  end subroutine free_gall
+
   subroutine dyn_evp1d_finalize_handle()
     if (c_associated(handle)) then
       call evp1d_check(evp1d_destroy(handle), 'evp1d_destroy')
@@ -358,6 +446,7 @@ contains
   subroutine dyn_evp1d_finalize()
     call dyn_evp1d_finalize_handle()
     if (allocated(indxti)) deallocate(indxti, indxtj, buf, ibuf)
+    call free_gall()
     write(*,'(a21)') 'Time to die for evp1d'
     if (iu06 >= 0) then
       write(iu06,'(a21)') 'Time to die for evp1d'

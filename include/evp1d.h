@@ -178,6 +178,28 @@ int evp1d_get_field_member(evp1d_handle *h, int32_t k, int32_t field, double *ds
 int evp1d_set_index_member(evp1d_handle *h, int32_t k, int32_t which, const int32_t *idx, int32_t n);
 int evp1d_set_skip_member(evp1d_handle *h, int32_t k, int32_t which, const int32_t *flags, int32_t n);
 
+/* ---- 2D <-> 1D on the device ----------------------------------------------
+ * Replace the host loops of convert_2d_1d_dyn and convert_1d_2d_dyn
+ * (integration/ice_dyn_evp1d.F90:818-882, 884-947) for a single-domain
+ * handle.  evp1d_set_grid gives the caller's 2D array size (nx x ny, Fortran
+ * order, rim included) and indxtij (navel, 1-based linear index i + (j-1)*nx
+ * of every 1D position, as evp1d_index_build returns it).
+ * evp1d_put_2d copies nfield fields from `arrays` (nfield consecutive nx*ny
+ * arrays) into the handle: position iw <= na takes G(indxti(iw), indxtj(iw));
+ * for EVP1D_UVEL and EVP1D_VVEL the ghost positions become 0, as in the
+ * reference (ice_dyn_evp1d.F90:880-881).  evp1d_get_2d writes nfield fields
+ * back: G(indxti(iw), indxtj(iw)) = value for iw <= na; every other cell
+ * takes the value that field's array had at its last evp1d_put_2d (so a
+ * caller that has not changed the array in between sees it unchanged).  Both
+ * return when the caller's arrays may be reused.                            */
+int evp1d_set_grid(evp1d_handle *h, int32_t nx, int32_t ny, const int32_t *indxtij, int32_t navel);
+/* Page-locked host memory for such arrays (transfers from it run at the full
+ * host-link rate); free with evp1d_host_free.                               */
+int evp1d_host_alloc(int64_t bytes, void **ptr);
+int evp1d_host_free(void *ptr);
+int evp1d_put_2d(evp1d_handle *h, int32_t nfield, const int32_t *fields, const double *arrays);
+int evp1d_get_2d(evp1d_handle *h, int32_t nfield, const int32_t *fields, double *arrays);
+
 /* ---- launch tuning (no counterpart in the reference) ---------------------
  * Block size (64, 128, 256, 512 or 1024 threads) and XCD-aware tile order of
  * the stress and stepu launches.  evp1d_autotune times every configuration
