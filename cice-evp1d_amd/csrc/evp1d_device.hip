@@ -674,6 +674,15 @@ __global__ void k_unpack2d_comp(double *g, const double2 *src, const int32_t *po
   if (s < na) g[pos[s]] = reinterpret_cast<const double *>(src + s)[comp];
 }
 
+// neighbour list in slot space: slot s of point perm[s] gets the slot of its
+// neighbour (ghost positions >= na are their own slot)
+__global__ void k_idx_to_slots(int32_t *dst, const int32_t *src, const int32_t *perm, const int32_t *inv, int na) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= na) return;
+  const int32_t q = src[perm[s]];
+  dst[s] = q < na ? inv[q] : q;
+}
+
 struct evp1d_handle {
   int na = 0, navel = 0;
   long ldsig = 0;
@@ -687,6 +696,10 @@ struct evp1d_handle {
   double2 *ptmp = nullptr;         // na double2: relayout staging
   int32_t *idx_dev[6] = {};
   int32_t *inv_dev = nullptr;      // caller position -> slot
+  int32_t *perm_dev = nullptr;     // slot -> caller position (slots < na)
+  bool perm_dev_dirty = true;
+  int32_t *idx_caller_dev[6] = {}; // neighbour lists in caller order (remapped to slots on the device)
+  bool idx_caller_dirty[6] = {true, true, true, true, true, true};
   int32_t *map_dev = nullptr;      // relayout map
   int32_t *halo_dev = nullptr;     // 2*halo_cap: dst then src (slot space)
   int halo_cap = 0;
@@ -707,6 +720,8 @@ struct evp1d_handle {
   int coop_blocks = 0;
   // fused form: tiling and its device tables (owned here, replaced on relayout)
   int tile_w = 64, tile_h = 16, tile_block = 512, tile_xcd = 1;
+  evp1d::TileIds tile_cache;       // tile of every point for the current lists and tile size
+  bool tile_cache_ok = false;
   bool fused_ok = false;
   std::string fused_why = "lists not set";
   int fused_ntiles = 0, fused_cap = 0;
@@ -925,8 +940,14 @@ static int relayout(evp1d_handle *h) {
     all = all && h->idx_set[k] && h->idx_host[k].size() == (size_t)na;
     idx[k] = all ? h->idx_host[k].data() : nullptr;
   }
-  const evp1d::TilePlan plan = evp1d::plan_tiles(na, h->navel, all ? idx : nullptr, h->skip_host.data(), h->tile_w,
-                                                 h->tile_h, kFusedMaxEntries);
+  // the tile of each point depends on the lists and the tile size only: it
+  // is kept while only skip flags change (a new model step's ice masks)
+  if (!h->tile_cache_ok || h->tile_cache.tile_w != h->tile_w || h->tile_cache.tile_h != h->tile_h) {
+    h->tile_cache = evp1d::tile_ids(na, h->navel, all ? idx : nullptr, h->tile_w, h->tile_h);
+    h->tile_cache_ok = true;
+  }
+  const evp1d::TilePlan plan = evp1d::plan_tiles(na, h->navel, all ? idx : nullptr, h->skip_host.data(),
+                                                 h->tile_cache, kFusedMaxEntries);
   const std::string bad = evp1d::check_plan(plan, na, h->navel, idx, h->skip_host.data());
   if (!bad.empty()) return fail(EVP1D_EINDEX, "slot layout: %s", bad.c_str());
   const std::vector<int32_t> &nperm = plan.perm;
@@ -956,6 +977,7 @@ static int relayout(evp1d_handle *h) {
     h->perm = nperm;
     h->inv.swap(ninv);
     h->pos2d_dirty = true;
+    h->perm_dev_dirty = true;
   }
   RC_TRY(install_tiling(h, plan));
   h->v.t_end = plan.cnt[0] + plan.cnt[1];
@@ -974,14 +996,23 @@ static int ensure(evp1d_handle *h) {
   if (h->layout_dirty) RC_TRY(relayout(h));
   if (!h->idx_dirty) return EVP1D_OK;
   const int na = h->na;
-  std::vector<int32_t> z((size_t)na);
+  if (na > 0 && h->perm_dev_dirty) {
+    HIP_TRY(hipMemcpyAsync(h->perm_dev, h->perm.data(), (size_t)na * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
+    h->perm_dev_dirty = false;
+  }
   for (int k = 0; k < 6; ++k) {
     if (!h->idx_set[k] || na == 0) continue;
-    const std::vector<int32_t> &src = h->idx_host[k];
-    for (int s = 0; s < na; ++s) z[(size_t)s] = slot_of(h, src[(size_t)h->perm[(size_t)s]]);
-    HIP_TRY(hipMemcpyAsync(h->idx_dev[k], z.data(), (size_t)na * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (h->idx_caller_dirty[k]) {   // uploaded once per evp1d_set_index
+      HIP_TRY(hipMemcpyAsync(h->idx_caller_dev[k], h->idx_host[k].data(), (size_t)na * sizeof(int32_t),
+                             hipMemcpyHostToDevice, h->stream));
+      h->idx_caller_dirty[k] = false;
+    }
+    hipLaunchKernelGGL(k_idx_to_slots, dim3(blocks_for(na)), dim3(kBlock), 0, h->stream, h->idx_dev[k],
+                       (const int32_t *)h->idx_caller_dev[k], (const int32_t *)h->perm_dev,
+                       (const int32_t *)h->inv_dev, na);
+    HIP_TRY(hipGetLastError());
   }
+  if (na > 0) HIP_TRY(hipStreamSynchronize(h->stream));
   const int n = (int)h->halo_dst_host.size();
   if (n > 0) {
     std::vector<int32_t> buf((size_t)2 * n);
@@ -1244,6 +1275,8 @@ static int create_impl(int32_t nmem, const int32_t *mna, const int32_t *mnavel, 
   A((void **)&v.strintx, nA); A((void **)&v.strinty, nA); A((void **)&v.cb, nA);
   for (int k = 0; k < 6; ++k) A((void **)&h->idx_dev[k], (size_t)na * sizeof(int32_t));
   A((void **)&h->inv_dev, (size_t)na * sizeof(int32_t));
+  A((void **)&h->perm_dev, (size_t)na * sizeof(int32_t));
+  for (int k = 0; k < 6; ++k) A((void **)&h->idx_caller_dev[k], (size_t)na * sizeof(int32_t));
   A((void **)&h->map_dev, (size_t)na * sizeof(int32_t));
   A((void **)&h->scratch, (size_t)navel * sizeof(double));
   A((void **)&h->ptmp, (size_t)na * sizeof(double2));
@@ -1310,9 +1343,13 @@ int evp1d_destroy(evp1d_handle *h) {
 
 int evp1d_set_params(evp1d_handle *h, const evp1d_params *p) {
   if (!h || !p) return fail(EVP1D_EINVAL, "evp1d_set_params: NULL argument");
+  // the captured graph holds the scalars: capture again only if one changed
+  // (CICE hands the same parameters to every time step)
+  evp1d_params a = h->p, b = *p;
+  a.reserved = b.reserved = 0;
+  if (std::memcmp(&a, &b, sizeof(evp1d_params)) != 0) h->graph_dirty = true;
   h->p = *p;
   h->v.s = scalars_from(*p);
-  h->graph_dirty = true;
   return EVP1D_OK;
 }
 
@@ -1402,6 +1439,8 @@ static int set_index_impl(evp1d_handle *h, int k, int32_t which, const int32_t *
   h->idx_set[which] = std::all_of(ms.begin(), ms.end(), [](uint8_t b) { return b != 0; });
   h->idx_dirty = true;
   h->layout_dirty = true;   // the tiled slot order follows the lists
+  h->tile_cache_ok = false;
+  h->idx_caller_dirty[which] = true;
   return EVP1D_OK;
 }
 

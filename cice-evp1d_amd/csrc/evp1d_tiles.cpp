@@ -83,51 +83,83 @@ bool grid_coordinates(int na, const int32_t *const idx[6], std::vector<int64_t> 
 
 }  // namespace
 
-TilePlan plan_tiles(int na, int navel, const int32_t *const idx[6], const uint8_t *skip, int tile_w, int tile_h,
-                    int max_entries) {
-  TilePlan p;
-  p.tile_w = tile_w;
-  p.tile_h = tile_h;
-  class_order(na, skip, p);
+TileIds tile_ids(int na, int navel, const int32_t *const idx[6], int tile_w, int tile_h) {
+  TileIds t;
+  t.tile_w = tile_w;
+  t.tile_h = tile_h;
   if (na == 0) {
-    p.why = "empty list";
-    return p;
+    t.why = "empty list";
+    return t;
   }
   if (!idx) {
-    p.why = "neighbour lists not set";
-    return p;
+    t.why = "neighbour lists not set";
+    return t;
   }
   for (int k = 0; k < 6; ++k)
     if (!idx[k]) {
-      p.why = "neighbour lists not set";
-      return p;
+      t.why = "neighbour lists not set";
+      return t;
     }
   if (tile_w < 1 || tile_h < 1) {
-    p.why = "tile size < 1";
-    return p;
+    t.why = "tile size < 1";
+    return t;
   }
   for (int k = 0; k < 6; ++k)
     for (int q = 0; q < na; ++q)
       if (idx[k][q] < 0 || idx[k][q] >= navel) {
-        p.why = "neighbour index out of range";
-        return p;
+        t.why = "neighbour index out of range";
+        return t;
       }
   std::vector<int64_t> gx, gy;
-  if (!grid_coordinates(na, idx, gx, gy, p.why)) return p;
+  if (!grid_coordinates(na, idx, gx, gy, t.why)) return t;
 
   // dense tile ids in (row of tiles, column of tiles) order
-  int64_t width = 0;
-  for (int q = 0; q < na; ++q) width = std::max(width, gx[(size_t)q] + 1);
-  const int64_t ntx = (width + tile_w - 1) / tile_w;
-  std::vector<int64_t> key((size_t)na);
-  for (int q = 0; q < na; ++q) key[(size_t)q] = (gy[(size_t)q] / tile_h) * ntx + gx[(size_t)q] / tile_w;
-  std::vector<int64_t> keys(key);
-  std::sort(keys.begin(), keys.end());
-  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
-  const int nt = (int)keys.size();
-  std::vector<int32_t> tid((size_t)na);
-  for (int q = 0; q < na; ++q)
-    tid[(size_t)q] = (int32_t)(std::lower_bound(keys.begin(), keys.end(), key[(size_t)q]) - keys.begin());
+  int64_t width = 0, height = 0;
+  for (int q = 0; q < na; ++q) {
+    width = std::max(width, gx[(size_t)q] + 1);
+    height = std::max(height, gy[(size_t)q] + 1);
+  }
+  const int64_t ntx = (width + tile_w - 1) / tile_w, nty = (height + tile_h - 1) / tile_h;
+  t.tid.resize((size_t)na);
+  if (ntx * nty <= 4 * (int64_t)na + 1024) {   // a table over all tile positions
+    std::vector<int32_t> dense((size_t)(ntx * nty), -1);
+    for (int q = 0; q < na; ++q) dense[(size_t)((gy[(size_t)q] / tile_h) * ntx + gx[(size_t)q] / tile_w)] = 0;
+    int32_t n = 0;
+    for (int32_t &d : dense)
+      if (d == 0) d = n++;
+    for (int q = 0; q < na; ++q)
+      t.tid[(size_t)q] = dense[(size_t)((gy[(size_t)q] / tile_h) * ntx + gx[(size_t)q] / tile_w)];
+    t.ntiles = n;
+  } else {   // sparse frames (many small components): sort the keys
+    std::vector<int64_t> key((size_t)na);
+    for (int q = 0; q < na; ++q) key[(size_t)q] = (gy[(size_t)q] / tile_h) * ntx + gx[(size_t)q] / tile_w;
+    std::vector<int64_t> keys(key);
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    for (int q = 0; q < na; ++q)
+      t.tid[(size_t)q] = (int32_t)(std::lower_bound(keys.begin(), keys.end(), key[(size_t)q]) - keys.begin());
+    t.ntiles = (int)keys.size();
+  }
+  return t;
+}
+
+TilePlan plan_tiles(int na, int navel, const int32_t *const idx[6], const uint8_t *skip, int tile_w, int tile_h,
+                    int max_entries) {
+  return plan_tiles(na, navel, idx, skip, tile_ids(na, navel, idx, tile_w, tile_h), max_entries);
+}
+
+TilePlan plan_tiles(int na, int navel, const int32_t *const idx[6], const uint8_t *skip, const TileIds &ids,
+                    int max_entries) {
+  TilePlan p;
+  p.tile_w = ids.tile_w;
+  p.tile_h = ids.tile_h;
+  class_order(na, skip, p);
+  if (ids.ntiles == 0 || (int)ids.tid.size() != na) {
+    p.why = ids.why.empty() ? "no tiles" : ids.why;
+    return p;
+  }
+  const int nt = ids.ntiles;
+  const std::vector<int32_t> &tid = ids.tid;
 
   // counting sort by (class, tile), caller order inside
   std::vector<int32_t> start((size_t)4 * nt + 1, 0);
@@ -142,7 +174,7 @@ TilePlan plan_tiles(int na, int navel, const int32_t *const idx[6], const uint8_
   const int t_end = p.cnt[0] + p.cnt[1], u_begin = p.cnt[0];
   const int nu = p.cnt[1] + p.cnt[2];
   p.uloc.assign((size_t)4 * nu, 0);
-  std::vector<int32_t> mark((size_t)navel, -1), lidx((size_t)navel, -1), halo, konst;
+  std::vector<int32_t> mark((size_t)navel, -1), lidx((size_t)navel, -1), halo, konst, rr;
   for (int t = 0; t < nt; ++t) {
     TileDesc d{};
     d.a0 = start[(size_t)0 * nt + t];
@@ -169,9 +201,13 @@ TilePlan plan_tiles(int na, int navel, const int32_t *const idx[6], const uint8_
       out[2] = slot(idx[5][q]);   // sse: str3, str6
       out[3] = slot(idx[4][q]);   // sw: str4, str8
     };
-    int32_t r[4];
-    for (int s = d.b0; s < d.b1; ++s) { refs(s, r); for (int k = 0; k < 4; ++k) visit(r[k]); }
-    for (int s = d.c0; s < d.c1; ++s) { refs(s, r); for (int k = 0; k < 4; ++k) visit(r[k]); }
+    // the four slots each stepu point reads, B then C points
+    const int nBt = d.b1 - d.b0, nUt = nBt + (d.c1 - d.c0);
+    rr.resize((size_t)4 * nUt);
+    for (int k = 0; k < nUt; ++k) {
+      refs(k < nBt ? d.b0 + k : d.c0 + (k - nBt), &rr[(size_t)4 * k]);
+      for (int c = 0; c < 4; ++c) visit(rr[(size_t)4 * k + c]);
+    }
     const int nH = (int)halo.size(), nK = (int)konst.size();
     for (int k = 0; k < nH; ++k) lidx[(size_t)halo[(size_t)k]] = nO + k;
     for (int k = 0; k < nK; ++k) lidx[(size_t)konst[(size_t)k]] = nO + nH + k;
@@ -184,12 +220,10 @@ TilePlan plan_tiles(int na, int navel, const int32_t *const idx[6], const uint8_
       return p;
     }
     p.max_entries = std::max(p.max_entries, entries);
-    auto put = [&](int s) {
-      refs(s, r);
-      for (int k = 0; k < 4; ++k) p.uloc[(size_t)4 * (s - u_begin) + k] = (uint16_t)lidx[(size_t)r[k]];
-    };
-    for (int s = d.b0; s < d.b1; ++s) put(s);
-    for (int s = d.c0; s < d.c1; ++s) put(s);
+    for (int k = 0; k < nUt; ++k) {
+      const int s = k < nBt ? d.b0 + k : d.c0 + (k - nBt);
+      for (int c = 0; c < 4; ++c) p.uloc[(size_t)4 * (s - u_begin) + c] = (uint16_t)lidx[(size_t)rr[(size_t)4 * k + c]];
+    }
     d.h0 = (int32_t)p.list.size();
     d.nh = nH;
     d.k0 = (int32_t)p.klist.size();

@@ -46,10 +46,24 @@ struct TilePlan {
   int max_entries = 0;
 };
 
+// Tile of every point: depends on the neighbour lists and the tile size only,
+// so a caller can keep it while skip flags change.  ntiles = 0 (and `why`
+// set) when the lists are not those of a grid.
+struct TileIds {
+  int tile_w = 0, tile_h = 0;
+  int ntiles = 0;
+  std::vector<int32_t> tid;   // caller position -> tile, in launch order
+  std::string why;
+};
+TileIds tile_ids(int na, int navel, const int32_t *const idx[6], int tile_w, int tile_h);
+
 // idx: ee, ne, se, nw, sw, sse as 0-based caller positions (values in
 // [0, navel); positions >= na are ghost slots).  skip: bit0 skipT, bit1 skipU.
 // idx may be null (no tiling, class order only).
 TilePlan plan_tiles(int na, int navel, const int32_t *const idx[6], const uint8_t *skip, int tile_w, int tile_h,
+                    int max_entries);
+// The same from tile ids made by tile_ids for these lists.
+TilePlan plan_tiles(int na, int navel, const int32_t *const idx[6], const uint8_t *skip, const TileIds &ids,
                     int max_entries);
 
 // Re-check every bound the fused kernel relies on; returns "" when the plan

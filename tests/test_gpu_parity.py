@@ -301,3 +301,26 @@ def test_fused_unavailable_without_a_grid(gpu):
         assert eng.get_tiling()["ntiles"] == 0
         with pytest.raises(evp1d.Evp1dError):
             eng.run(3)
+
+
+@pytest.mark.parametrize("mode", ["graph", "fused"])
+def test_parameters_change_between_runs(gpu, mode):
+    """The captured graph holds the solver scalars: a changed parameter set
+    must reach the next run, an unchanged one must not disturb it."""
+    st = synthetic_state(40, 30, seed=64)
+    lib = O.OracleLib()
+    p1 = lib.params(10)
+    p2 = lib.params(10, capping=0.5, e_factor=0.3)
+    want = st.copy()
+    lib.run(p1, want, 6)
+    lib.run(p2, want, 6)
+    lib.run(p2, want, 6)
+    with make_engine(st, 10, mode) as eng:
+        eng.run(6)
+        q = evp1d.set_evp_parameters(10)
+        q.capping, q.e_factor = 0.5, 0.3
+        eng.set_params(q)
+        eng.run(6)
+        eng.set_params(q)          # same values again
+        eng.run(6)
+        assert_state_bitexact(fetch_state(eng, st), want)
