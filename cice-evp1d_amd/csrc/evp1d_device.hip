@@ -720,6 +720,9 @@ struct evp1d_handle {
   int coop_blocks = 0;
   // fused form: tiling and its device tables (owned here, replaced on relayout)
   int tile_w = 64, tile_h = 16, tile_block = 512, tile_xcd = 1;
+  bool tiling_auto = true;         // choose the tile from the domain size (until evp1d_set_tiling)
+  evp1d::GridCoords coords;        // grid coordinates of the current lists
+  bool coords_ok = false;
   evp1d::TileIds tile_cache;       // tile of every point for the current lists and tile size
   bool tile_cache_ok = false;
   bool fused_ok = false;
@@ -942,8 +945,27 @@ static int relayout(evp1d_handle *h) {
   }
   // the tile of each point depends on the lists and the tile size only: it
   // is kept while only skip flags change (a new model step's ice masks)
+  if (!h->coords_ok) {
+    h->coords = evp1d::grid_coords(na, h->navel, all ? idx : nullptr);
+    h->coords_ok = true;
+    h->tile_cache_ok = false;
+  }
+  if (h->tiling_auto && !h->tile_cache_ok) {
+    // enough tiles to fill the GPU: the largest tile of the ladder that
+    // still gives min_tiles tiles (256 CUs)
+    struct Rung { int w, h, block, min_tiles; };
+    static const Rung ladder[3] = {{64, 16, 512, 1024}, {32, 8, 256, 512}, {16, 8, 128, 0}};
+    for (const Rung &r : ladder) {
+      h->tile_cache = evp1d::tile_ids(h->coords, r.w, r.h);
+      h->tile_w = r.w;
+      h->tile_h = r.h;
+      h->tile_block = r.block;
+      if (h->tile_cache.ntiles >= r.min_tiles) break;
+    }
+    h->tile_cache_ok = true;
+  }
   if (!h->tile_cache_ok || h->tile_cache.tile_w != h->tile_w || h->tile_cache.tile_h != h->tile_h) {
-    h->tile_cache = evp1d::tile_ids(na, h->navel, all ? idx : nullptr, h->tile_w, h->tile_h);
+    h->tile_cache = evp1d::tile_ids(h->coords, h->tile_w, h->tile_h);
     h->tile_cache_ok = true;
   }
   const evp1d::TilePlan plan = evp1d::plan_tiles(na, h->navel, all ? idx : nullptr, h->skip_host.data(),
@@ -1440,6 +1462,7 @@ static int set_index_impl(evp1d_handle *h, int k, int32_t which, const int32_t *
   h->idx_dirty = true;
   h->layout_dirty = true;   // the tiled slot order follows the lists
   h->tile_cache_ok = false;
+  h->coords_ok = false;
   h->idx_caller_dirty[which] = true;
   return EVP1D_OK;
 }
@@ -1632,7 +1655,11 @@ int evp1d_set_tiling(evp1d_handle *h, int32_t tile_w, int32_t tile_h, int32_t bl
   if (xcd != 0 && xcd != 1) return fail(EVP1D_EINVAL, "evp1d_set_tiling: xcd must be 0 or 1");
   if (fused_block_index(block) < 0) return fail(EVP1D_EINVAL, "evp1d_set_tiling: block must be 64, 128, 256 or 512");
   h->tile_block = block;
-  if (tile_w != h->tile_w || tile_h != h->tile_h) h->layout_dirty = true;
+  if (tile_w != h->tile_w || tile_h != h->tile_h || h->tiling_auto) {
+    h->layout_dirty = true;
+    h->tile_cache_ok = false;
+  }
+  h->tiling_auto = false;
   h->tile_w = tile_w;
   h->tile_h = tile_h;
   h->tile_xcd = xcd;

@@ -4,6 +4,7 @@
 #include "evp1d_tiles.hpp"
 
 #include <algorithm>
+#include <cstdint>
 #include <cstdio>
 
 namespace evp1d {
@@ -83,64 +84,82 @@ bool grid_coordinates(int na, const int32_t *const idx[6], std::vector<int64_t> 
 
 }  // namespace
 
-TileIds tile_ids(int na, int navel, const int32_t *const idx[6], int tile_w, int tile_h) {
-  TileIds t;
-  t.tile_w = tile_w;
-  t.tile_h = tile_h;
+GridCoords grid_coords(int na, int navel, const int32_t *const idx[6]) {
+  GridCoords c;
   if (na == 0) {
-    t.why = "empty list";
-    return t;
+    c.why = "empty list";
+    return c;
   }
   if (!idx) {
-    t.why = "neighbour lists not set";
-    return t;
+    c.why = "neighbour lists not set";
+    return c;
   }
   for (int k = 0; k < 6; ++k)
     if (!idx[k]) {
-      t.why = "neighbour lists not set";
-      return t;
+      c.why = "neighbour lists not set";
+      return c;
     }
+  for (int k = 0; k < 6; ++k)
+    for (int q = 0; q < na; ++q)
+      if (idx[k][q] < 0 || idx[k][q] >= navel) {
+        c.why = "neighbour index out of range";
+        return c;
+      }
+  std::vector<int64_t> gx, gy;
+  if (!grid_coordinates(na, idx, gx, gy, c.why)) return c;
+  for (int q = 0; q < na; ++q) {
+    c.width = std::max(c.width, gx[(size_t)q] + 1);
+    c.height = std::max(c.height, gy[(size_t)q] + 1);
+  }
+  if (c.width > INT32_MAX || c.height > INT32_MAX) {
+    c.why = "grid frame too large";
+    return c;
+  }
+  c.gx.assign(gx.begin(), gx.end());
+  c.gy.assign(gy.begin(), gy.end());
+  c.ok = true;
+  return c;
+}
+
+TileIds tile_ids(const GridCoords &c, int tile_w, int tile_h) {
+  TileIds t;
+  t.tile_w = tile_w;
+  t.tile_h = tile_h;
+  if (!c.ok) {
+    t.why = c.why;
+    return t;
+  }
   if (tile_w < 1 || tile_h < 1) {
     t.why = "tile size < 1";
     return t;
   }
-  for (int k = 0; k < 6; ++k)
-    for (int q = 0; q < na; ++q)
-      if (idx[k][q] < 0 || idx[k][q] >= navel) {
-        t.why = "neighbour index out of range";
-        return t;
-      }
-  std::vector<int64_t> gx, gy;
-  if (!grid_coordinates(na, idx, gx, gy, t.why)) return t;
-
+  const int na = (int)c.gx.size();
   // dense tile ids in (row of tiles, column of tiles) order
-  int64_t width = 0, height = 0;
-  for (int q = 0; q < na; ++q) {
-    width = std::max(width, gx[(size_t)q] + 1);
-    height = std::max(height, gy[(size_t)q] + 1);
-  }
-  const int64_t ntx = (width + tile_w - 1) / tile_w, nty = (height + tile_h - 1) / tile_h;
+  const int64_t ntx = (c.width + tile_w - 1) / tile_w, nty = (c.height + tile_h - 1) / tile_h;
+  auto key_of = [&](int q) { return (int64_t)(c.gy[(size_t)q] / tile_h) * ntx + c.gx[(size_t)q] / tile_w; };
   t.tid.resize((size_t)na);
   if (ntx * nty <= 4 * (int64_t)na + 1024) {   // a table over all tile positions
     std::vector<int32_t> dense((size_t)(ntx * nty), -1);
-    for (int q = 0; q < na; ++q) dense[(size_t)((gy[(size_t)q] / tile_h) * ntx + gx[(size_t)q] / tile_w)] = 0;
+    for (int q = 0; q < na; ++q) dense[(size_t)key_of(q)] = 0;
     int32_t n = 0;
     for (int32_t &d : dense)
       if (d == 0) d = n++;
-    for (int q = 0; q < na; ++q)
-      t.tid[(size_t)q] = dense[(size_t)((gy[(size_t)q] / tile_h) * ntx + gx[(size_t)q] / tile_w)];
+    for (int q = 0; q < na; ++q) t.tid[(size_t)q] = dense[(size_t)key_of(q)];
     t.ntiles = n;
   } else {   // sparse frames (many small components): sort the keys
-    std::vector<int64_t> key((size_t)na);
-    for (int q = 0; q < na; ++q) key[(size_t)q] = (gy[(size_t)q] / tile_h) * ntx + gx[(size_t)q] / tile_w;
-    std::vector<int64_t> keys(key);
+    std::vector<int64_t> keys((size_t)na);
+    for (int q = 0; q < na; ++q) keys[(size_t)q] = key_of(q);
     std::sort(keys.begin(), keys.end());
     keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
     for (int q = 0; q < na; ++q)
-      t.tid[(size_t)q] = (int32_t)(std::lower_bound(keys.begin(), keys.end(), key[(size_t)q]) - keys.begin());
+      t.tid[(size_t)q] = (int32_t)(std::lower_bound(keys.begin(), keys.end(), key_of(q)) - keys.begin());
     t.ntiles = (int)keys.size();
   }
   return t;
+}
+
+TileIds tile_ids(int na, int navel, const int32_t *const idx[6], int tile_w, int tile_h) {
+  return tile_ids(grid_coords(na, navel, idx), tile_w, tile_h);
 }
 
 TilePlan plan_tiles(int na, int navel, const int32_t *const idx[6], const uint8_t *skip, int tile_w, int tile_h,

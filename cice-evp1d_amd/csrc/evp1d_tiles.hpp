@@ -46,6 +46,17 @@ struct TilePlan {
   int max_entries = 0;
 };
 
+// Grid coordinates of every point, recovered from the neighbour lists (one
+// frame per connected component, components side by side).  ok = false (and
+// `why` set) when the lists are not those of a grid.
+struct GridCoords {
+  bool ok = false;
+  std::string why;
+  std::vector<int32_t> gx, gy;
+  int64_t width = 0, height = 0;
+};
+GridCoords grid_coords(int na, int navel, const int32_t *const idx[6]);
+
 // Tile of every point: depends on the neighbour lists and the tile size only,
 // so a caller can keep it while skip flags change.  ntiles = 0 (and `why`
 // set) when the lists are not those of a grid.
@@ -55,6 +66,7 @@ struct TileIds {
   std::vector<int32_t> tid;   // caller position -> tile, in launch order
   std::string why;
 };
+TileIds tile_ids(const GridCoords &c, int tile_w, int tile_h);
 TileIds tile_ids(int na, int navel, const int32_t *const idx[6], int tile_w, int tile_h);
 
 // idx: ee, ne, se, nw, sw, sse as 0-based caller positions (values in

@@ -220,8 +220,11 @@ int evp1d_get_tuning(const evp1d_handle *h, int32_t *stress_block, int32_t *stre
  * the library recovers from the neighbour lists.  evp1d_set_tiling chooses
  * the tile ((tile_w+1)(tile_h+1) <= 2560), the workgroup size (64 to 512)
  * and the XCD-aware tile order (xcd = 0/1); a new tile re-lays the points out
- * at the next run.  evp1d_get_tiling reports them and the number of tiles (0
- * when the lists do not describe a grid: the fused form is then unavailable).
+ * at the next run.  Until it is called the library chooses by domain size:
+ * 64x16 cells on 512 threads if that gives at least 1024 tiles, else 32x8 on
+ * 256 if at least 512, else 16x8 on 128 (XCD-aware order).
+ * evp1d_get_tiling reports the tiling and the number of tiles (0 when the
+ * lists do not describe a grid: the fused form is then unavailable).
  * Results never depend on the tiling.                                       */
 int evp1d_set_tiling(evp1d_handle *h, int32_t tile_w, int32_t tile_h, int32_t block, int32_t xcd);
 int evp1d_get_tiling(evp1d_handle *h, int32_t *tile_w, int32_t *tile_h, int32_t *block, int32_t *xcd,
