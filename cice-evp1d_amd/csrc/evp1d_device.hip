@@ -111,38 +111,6 @@ struct DevView {
   Scalars s;
 };
 
-// Cache policy of the forward kernels' streams.  Each switch, when 1, marks
-// that stream's accesses nontemporal (streamed past the caches):
-//   EVP1D_NT_FIELDS  loads of the per-point constants (geometry, forcing)
-//   EVP1D_NT_SIG     loads and stores of the 12 stresses
-//   EVP1D_NT_STR     stores of str1..8 by stress
-#ifndef EVP1D_NT_FIELDS
-#define EVP1D_NT_FIELDS 0
-#endif
-#ifndef EVP1D_NT_SIG
-#define EVP1D_NT_SIG 0
-#endif
-#ifndef EVP1D_NT_STR
-#define EVP1D_NT_STR 0
-#endif
-typedef double dvec2 __attribute__((ext_vector_type(2)));
-template <bool NT>
-__device__ __forceinline__ double ld(const double *p, long i) {
-  if (NT) return __builtin_nontemporal_load(p + i);
-  return p[i];
-}
-template <bool NT>
-__device__ __forceinline__ void st(double *p, long i, double v) {
-  if (NT) __builtin_nontemporal_store(v, p + i);
-  else p[i] = v;
-}
-template <bool NT>
-__device__ __forceinline__ void st2(double2 *p, long i, double x, double y) {
-  if (NT) __builtin_nontemporal_store(dvec2{x, y}, reinterpret_cast<dvec2 *>(p) + i);
-  else p[i] = make_double2(x, y);
-}
-#define LDF(a, i) ld<EVP1D_NT_FIELDS != 0>(a, i)
-
 template <bool SPLIT>
 struct Ld {
   __device__ static __forceinline__ double2 vel(const DevView &d, int j) {
@@ -164,30 +132,28 @@ __device__ __forceinline__ void stress_eval(const DevView &d, int iw, evp1d::Str
   const double2 ve = L::vel(d, d.ee[iw]);
   const double2 vn = L::vel(d, d.ne[iw]);
   const double2 vs = L::vel(d, d.se[iw]);
-  constexpr bool NS = EVP1D_NT_SIG != 0;
   double sig[12];
 #pragma unroll
-  for (int c = 0; c < 12; ++c) sig[c] = ld<NS>(d.sig, c * d.ldsig + iw);
+  for (int c = 0; c < 12; ++c) sig[c] = d.sig[c * d.ldsig + iw];
   evp1d::stress_point(d.s, cc.x, cc.y, ve.x, ve.y, vs.x, vs.y, vn.x, vn.y,
-                      LDF(d.dxT, iw), LDF(d.dyT, iw), LDF(d.strength, iw), LDF(d.hte, iw), LDF(d.htn, iw),
-                      LDF(d.htem1, iw), LDF(d.htnm1, iw), sig, o);
+                      d.dxT[iw], d.dyT[iw], d.strength[iw], d.hte[iw], d.htn[iw],
+                      d.htem1[iw], d.htnm1[iw], sig, o);
 }
 
 template <bool SPLIT = false>
 __device__ __forceinline__ void stress_at(const DevView &d, int iw) {
-  constexpr bool NS = EVP1D_NT_SIG != 0, NR = EVP1D_NT_STR != 0;
   evp1d::StressOut o;
   stress_eval<SPLIT>(d, iw, o);
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    st<NS>(d.sig, c * d.ldsig + iw, o.sp[c]);
-    st<NS>(d.sig, (4 + c) * d.ldsig + iw, o.sm[c]);
-    st<NS>(d.sig, (8 + c) * d.ldsig + iw, o.s12[c]);
+    d.sig[c * d.ldsig + iw] = o.sp[c];
+    d.sig[(4 + c) * d.ldsig + iw] = o.sm[c];
+    d.sig[(8 + c) * d.ldsig + iw] = o.s12[c];
   }
-  st2<NR>(d.s15, iw, o.str[0], o.str[4]);
-  st2<NR>(d.s27, iw, o.str[1], o.str[6]);
-  st2<NR>(d.s36, iw, o.str[2], o.str[5]);
-  st2<NR>(d.s48, iw, o.str[3], o.str[7]);
+  d.s15[iw] = make_double2(o.str[0], o.str[4]);
+  d.s27[iw] = make_double2(o.str[1], o.str[6]);
+  d.s36[iw] = make_double2(o.str[2], o.str[5]);
+  d.s48[iw] = make_double2(o.str[3], o.str[7]);
 }
 
 template <bool SPLIT = false>
@@ -199,10 +165,10 @@ __device__ __forceinline__ void stepu_at(const DevView &d, int iw, bool write_di
   const double2 c = L::pair(d.s36, d.s36c, d, d.sse[iw]);
   const double2 e = L::pair(d.s48, d.s48c, d, d.sw[iw]);
   evp1d::StepuOut o;
-  evp1d::stepu_point(d.s, v.x, v.y, LDF(d.cw, iw), LDF(d.aiu, iw), LDF(d.uocn, iw), LDF(d.vocn, iw),
-                     LDF(d.waterx, iw), LDF(d.watery, iw), LDF(d.forcex, iw), LDF(d.forcey, iw),
-                     LDF(d.umassdti, iw), LDF(d.fm, iw), LDF(d.uarear, iw), LDF(d.uvel_init, iw),
-                     LDF(d.vvel_init, iw), LDF(d.tbu, iw),
+  evp1d::stepu_point(d.s, v.x, v.y, d.cw[iw], d.aiu[iw], d.uocn[iw], d.vocn[iw],
+                     d.waterx[iw], d.watery[iw], d.forcex[iw], d.forcey[iw],
+                     d.umassdti[iw], d.fm[iw], d.uarear[iw], d.uvel_init[iw],
+                     d.vvel_init[iw], d.tbu[iw],
                      a.x, b.x, c.x, e.x, a.y, c.y, b.y, e.y, o);
   d.vel_out[L::vslot(d, iw)] = make_double2(o.uvel, o.vvel);
   // strintx/strinty/Cb are overwritten every subcycle and read by nothing
