@@ -1748,6 +1748,15 @@ struct StateBuf {
 
 }  // namespace
 
+// device bytes of a stored gradient trajectory of nsub subcycles: the
+// stepu-range velocities of every subcycle and the stress-range str1..8 of
+// all but the last (AdjointRun::run_stored)
+static size_t trajectory_bytes(const evp1d_handle *h, int nsub) {
+  const size_t nu = (size_t)(h->v.u_end - h->v.u_begin), nt = (size_t)h->v.t_end;
+  const size_t ub = (size_t)nsub * nu * sizeof(double2), sb = (size_t)(nsub > 1 ? nsub - 1 : 1) * nt * sizeof(double2);
+  return (ub + 255) / 256 * 256 + 4 * ((sb + 255) / 256 * 256);
+}
+
 struct AdjointRun {
   evp1d_handle *h;
   int nsub;
@@ -1864,6 +1873,7 @@ return EVP1D_OK;
     const DevView W = h->v;
     const int N = nsub, nu = W.u_end - W.u_begin, nt = W.t_end;
     const size_t ub = (size_t)N * nu * sizeof(double2), sb = (size_t)(N > 1 ? N - 1 : 1) * nt * sizeof(double2);
+    // (trajectory_bytes() restates this layout)
     // The trajectory storage stays with the handle (large allocations and
     // frees are slow) until it is destroyed or a checkpointed run releases
     // it.  Every entry is written before it is read.
@@ -2011,7 +2021,6 @@ int evp1d_adjoint_run(evp1d_handle *h, int32_t nsub, int32_t nsnap, const double
   if (!h) return fail(EVP1D_EINVAL, "evp1d_adjoint_run: NULL handle");
   if (!ready(h)) return fail(EVP1D_ESTATE, "evp1d_adjoint_run: neighbour lists not set");
   if (nsub < 1) return fail(EVP1D_EINVAL, "evp1d_adjoint_run: nsub=%d < 1", nsub);
-  if (nsnap < 1) return fail(EVP1D_EINVAL, "evp1d_adjoint_run: nsnap=%d < 1", nsnap);
   if (h->na > 0 && (!u_target || !v_target)) return fail(EVP1D_EINVAL, "evp1d_adjoint_run: NULL target");
   if (h->v.nhalo > 0)
     return fail(EVP1D_ESTATE, "evp1d_adjoint_run: a ghost copy list is set; its reverse is not implemented");
@@ -2019,7 +2028,6 @@ int evp1d_adjoint_run(evp1d_handle *h, int32_t nsub, int32_t nsnap, const double
   AdjointRun R;
   R.h = h;
   R.nsub = nsub;
-  const int S = std::min(nsnap, nsub);       // more slots than steps are never used
   const size_t nA = (size_t)h->na * sizeof(double), nV = (size_t)h->navel * sizeof(double2);
   RC_TRY(R.alloc((void **)&R.a.ub, nV));
   RC_TRY(R.alloc((void **)&R.a.sigb, R.sig_bytes()));
@@ -2034,6 +2042,23 @@ int evp1d_adjoint_run(evp1d_handle *h, int32_t nsub, int32_t nsnap, const double
   RC_TRY(R.alloc((void **)&R.terms, nA));
   RC_TRY(upload_caller_order(h, R.ut, u_target));
   RC_TRY(upload_caller_order(h, R.vt, v_target));
+  if (nsnap <= 0) {
+    // automatic: the whole trajectory if it fits in free device memory, else
+    // binomial checkpointing with as many stored states as fit
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    const size_t reserve = (size_t)1 << 30;
+    const size_t traj = trajectory_bytes(h, nsub);
+    const size_t have = h->traj_bytes >= traj ? traj : 0;   // storage the handle already holds
+    if (have > 0 || traj + reserve <= free_b) {
+      nsnap = nsub;
+    } else {
+      const size_t slot = R.sig_bytes() + (size_t)h->na * sizeof(double2);
+      const size_t room = free_b + h->traj_bytes > reserve ? (free_b + h->traj_bytes - reserve) / slot : 0;
+      nsnap = (int)std::max<size_t>(1, std::min<size_t>(room / 2, (size_t)nsub - 1));
+    }
+  }
+  const int S = std::min(nsnap, nsub);       // more slots than steps are never used
   if (nsnap >= nsub) {
     RC_TRY(R.run_stored());
   } else {

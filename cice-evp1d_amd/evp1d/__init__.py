@@ -313,9 +313,10 @@ class Engine:
         _check(self.L, self.L.evp1d_loss(self.h, _dp(ut), _dp(vt), ctypes.byref(out)))
         return out.value
 
-    def adjoint_run(self, nsub: int, u_target, v_target, nsnap: int = 32) -> dict:
+    def adjoint_run(self, nsub: int, u_target, v_target, nsnap: int = 0) -> dict:
         """Loss after nsub subcycles and its gradient w.r.t. strength, cdn_ocn
-        (per point), forcex, forcey (evp1d_adjoint_run)."""
+        (per point), forcex, forcey (evp1d_adjoint_run).  nsnap: stored
+        states (>= nsub: the whole trajectory; 0: whatever fits)."""
         ut = np.ascontiguousarray(u_target, dtype=np.float64)
         vt = np.ascontiguousarray(v_target, dtype=np.float64)
         g = {k: np.zeros(self.na) for k in ("strength", "cdn_ocn", "forcex", "forcey")}
