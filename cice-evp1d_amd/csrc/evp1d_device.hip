@@ -124,9 +124,11 @@ struct Ld {
   __device__ static __forceinline__ int vslot(const DevView &d, int j) { return SPLIT ? j - d.u_begin : j; }
 };
 
-// stress at slot iw from the view's velocities and stresses (no stores)
+// stress at slot iw from the view's velocities and the stresses at sp[c * ss],
+// c = 0..11 (no stores)
 template <bool SPLIT = false>
-__device__ __forceinline__ void stress_eval(const DevView &d, int iw, evp1d::StressOut &o) {
+__device__ __forceinline__ void stress_eval_from(const DevView &d, int iw, const double *sp, long ss,
+                                                 evp1d::StressOut &o) {
   using L = Ld<SPLIT>;
   const double2 cc = L::vel(d, iw);
   const double2 ve = L::vel(d, d.ee[iw]);
@@ -134,10 +136,16 @@ __device__ __forceinline__ void stress_eval(const DevView &d, int iw, evp1d::Str
   const double2 vs = L::vel(d, d.se[iw]);
   double sig[12];
 #pragma unroll
-  for (int c = 0; c < 12; ++c) sig[c] = d.sig[c * d.ldsig + iw];
+  for (int c = 0; c < 12; ++c) sig[c] = sp[c * ss];
   evp1d::stress_point(d.s, cc.x, cc.y, ve.x, ve.y, vs.x, vs.y, vn.x, vn.y,
                       d.dxT[iw], d.dyT[iw], d.strength[iw], d.hte[iw], d.htn[iw],
                       d.htem1[iw], d.htnm1[iw], sig, o);
+}
+
+// stress at slot iw from the view's velocities and stresses (no stores)
+template <bool SPLIT = false>
+__device__ __forceinline__ void stress_eval(const DevView &d, int iw, evp1d::StressOut &o) {
+  stress_eval_from<SPLIT>(d, iw, d.sig + iw, d.ldsig, o);
 }
 
 template <bool SPLIT = false>
@@ -233,18 +241,32 @@ __global__ __launch_bounds__(256) void k_stepu_split(DevView d, int write_diag) 
 // input velocities and stresses (halo entries belong to other tiles and are
 // not stored), constant entries (points whose str1..8 stress never writes)
 // come from cstr.  Stepu then reads its own, nw, sse and sw entries from LDS.
-// Velocities and stresses ping-pong between two buffers (in: d.vel, d.sig;
-// out: f.vel_out, f.sig_out), so neighbour tiles never see a half-updated
-// subcycle.
+// Velocities ping-pong between two buffers (in: d.vel, out: f.vel_out), so
+// neighbour tiles never see a half-updated subcycle.  Stresses either do the
+// same (in: d.sig, out: f.sig_out) or, with f.inplace, are updated in place
+// (f.sig_out == d.sig): only their owner reads them, except for the "shared"
+// points that other tiles evaluate as halo.  Those live during an in-place
+// run in a double-buffered copy indexed by shared id: every evaluation reads
+// hsig_in (the previous subcycle), their owner writes hsig_out, and
+// k_scatter_shared puts the last copy back into d.sig at the end of the run.
+#ifndef EVP1D_SH_STRIDE
+#define EVP1D_SH_STRIDE 12
+#endif
+constexpr int kShStride = EVP1D_SH_STRIDE;   // doubles per shared point (12 stresses, point-major)
 struct FusedView {
   const evp1d::TileDesc *tiles;
   const int32_t *hlist;     // halo slots of every tile
+  const int32_t *hid;       // shared id of every halo entry (parallel to hlist)
   const uint2 *uloc;        // per stepu slot (slot - u_begin): own | nw << 16, sse | sw << 16
   const double2 *cstr;      // [4][nk]: s15, s27, s36, s48 of the constant entries
   const int *cnz;           // nonzero when some constant entry has a bit set (else all are +0.0)
   long nk;
+  int inplace;              // stresses in place (shared points through hsig)
+  long nsh;                 // shared points
+  const double *hsig_in;    // [nsh][kShStride]
+  double *hsig_out;         // [nsh][kShStride]
   double2 *vel_out;
-  double *sig_out;
+  double *sig_out;          // == d.sig when inplace
   int cap;                  // entries per LDS table
 };
 
@@ -262,19 +284,34 @@ __global__ __launch_bounds__(B) void k_fused(DevView d, FusedView f, int last) {
   for (int k = threadIdx.x; k < nE; k += B) {
     double2 p15, p27, p36, p48;
     if (k < nS) {
-      const int g = k < nA ? t.a0 + k : (k < nO ? t.b0 + (k - nA) : f.hlist[t.h0 + (k - nO)]);
+      // stresses from d.sig, shared points of an in-place run (owned ones
+      // lead their range) from copy in; written to f.sig_out or copy out
+      int g, id;
+      if (k < nO) {
+        const bool a = k < nA;
+        const int r = a ? k : k - nA;
+        g = (a ? t.a0 : t.b0) + r;
+        id = f.inplace && r < (a ? t.sa : t.sb) ? (a ? t.ia : t.ib) + r : -1;
+      } else {
+        const int j = t.h0 + (k - nO);
+        g = f.hlist[j];
+        id = f.inplace ? f.hid[j] : -1;
+      }
+      const double *sp = id >= 0 ? f.hsig_in + (long)id * kShStride : d.sig + g;
+      const long ss = id >= 0 ? 1 : d.ldsig;
       evp1d::StressOut o;
-      stress_eval<SPLIT>(d, g, o);
+      stress_eval_from<SPLIT>(d, g, sp, ss, o);
       p15 = make_double2(o.str[0], o.str[4]);
       p27 = make_double2(o.str[1], o.str[6]);
       p36 = make_double2(o.str[2], o.str[5]);
       p48 = make_double2(o.str[3], o.str[7]);
       if (k < nO) {
+        double *wp = id >= 0 ? f.hsig_out + (long)id * kShStride : f.sig_out + g;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-          f.sig_out[c * d.ldsig + g] = o.sp[c];
-          f.sig_out[(4 + c) * d.ldsig + g] = o.sm[c];
-          f.sig_out[(8 + c) * d.ldsig + g] = o.s12[c];
+          wp[c * ss] = o.sp[c];
+          wp[(4 + c) * ss] = o.sm[c];
+          wp[(8 + c) * ss] = o.s12[c];
         }
         if (last || SPLIT) {   // str1..8 are observable after the loop
           d.s15[g] = p15;
@@ -323,6 +360,22 @@ __global__ __launch_bounds__(B) void k_fused(DevView d, FusedView f, int last) {
 // str1..8 of the constant entries, gathered once per run; *cnz (cleared by
 // k_clear_flag first) becomes 1 if any of them has a bit set
 __global__ void k_clear_flag(int *flag) { *flag = 0; }
+// the shared points' stresses into the copy before the first subcycle, and
+// back after the last
+__global__ void k_scatter_shared(double *sig, long ldsig, const double *hsig, const int32_t *shared, long nsh) {
+  const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nsh) return;
+  const long g = shared[j];
+#pragma unroll
+  for (int c = 0; c < 12; ++c) sig[c * ldsig + g] = hsig[j * kShStride + c];
+}
+__global__ void k_fill_shared(double *hsig, const double *sig, long ldsig, const int32_t *shared, long nsh) {
+  const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nsh) return;
+  const long g = shared[j];
+#pragma unroll
+  for (int c = 0; c < 12; ++c) hsig[j * kShStride + c] = sig[c * ldsig + g];
+}
 __global__ void k_fill_const(DevView d, const int32_t *klist, double2 *cstr, long nk, int *cnz) {
   const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nk) return;
@@ -697,11 +750,15 @@ struct evp1d_handle {
   long nk_total = 0;
   evp1d::TileDesc *tiles_dev = nullptr;
   int32_t *hlist_dev = nullptr, *klist_dev = nullptr;
+  int32_t *hid_dev = nullptr, *shared_dev = nullptr;
+  long nsh = 0;                    // shared points (stress points some other tile evaluates as halo)
+  double *hsig_dev = nullptr;      // their stresses, two buffers of nsh x kShStride
   uint2 *uloc_dev = nullptr;
   double2 *cstr_dev = nullptr;
   int *cnz_dev = nullptr;          // constant entries not all +0.0
   double2 *vel_alt = nullptr;      // second velocity buffer (navel)
-  double *sig_alt = nullptr;       // second stress buffer (12 x ldsig)
+  double *sig_alt = nullptr;       // second stress buffer (12 x ldsig), two-buffer fused runs
+  int fused_inplace = 0;           // fused runs update stresses in place
   void *traj = nullptr;            // gradient trajectory storage, kept between calls
   size_t traj_bytes = 0;
   // 2D grid of the caller's arrays (device-side pack/unpack)
@@ -745,11 +802,15 @@ static void drop_graph(evp1d_handle *h) {
 }
 
 static void free_tiling(evp1d_handle *h) {
-  void *bufs[5] = {h->tiles_dev, h->hlist_dev, h->klist_dev, h->uloc_dev, h->cstr_dev};
+  void *bufs[8] = {h->tiles_dev, h->hlist_dev, h->klist_dev, h->uloc_dev,
+                   h->cstr_dev,  h->hid_dev,   h->shared_dev, h->hsig_dev};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   h->tiles_dev = nullptr;
   h->hlist_dev = h->klist_dev = nullptr;
+  h->hid_dev = h->shared_dev = nullptr;
+  h->hsig_dev = nullptr;
+  h->nsh = 0;
   h->uloc_dev = nullptr;
   h->cstr_dev = nullptr;
   h->fused_ok = false;
@@ -763,12 +824,12 @@ static void release(evp1d_handle *h) {
   drop_graph(h);
   free_tiling(h);
   if (h->vel_alt) (void)hipFree(h->vel_alt);
+  if (h->sig_alt) (void)hipFree(h->sig_alt);
   if (h->cnz_dev) (void)hipFree(h->cnz_dev);
   if (h->traj) (void)hipFree(h->traj);
   if (h->pos2d_dev) (void)hipFree(h->pos2d_dev);
   for (double *g : h->g2d)
     if (g) (void)hipFree(g);
-  if (h->sig_alt) (void)hipFree(h->sig_alt);
   for (void *p : h->allocs) (void)hipFree(p);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -887,6 +948,10 @@ static int install_tiling(evp1d_handle *h, const evp1d::TilePlan &plan) {
   RC_TRY(upload_table(h, &h->tiles_dev, plan.tiles.data(), plan.tiles.size()));
   RC_TRY(upload_table(h, &h->hlist_dev, plan.list.data(), plan.list.size()));
   RC_TRY(upload_table(h, &h->klist_dev, plan.klist.data(), plan.klist.size()));
+  RC_TRY(upload_table(h, &h->hid_dev, plan.hid.data(), plan.hid.size()));
+  RC_TRY(upload_table(h, &h->shared_dev, plan.shared.data(), plan.shared.size()));
+  h->nsh = (long)plan.shared.size();
+  HIP_TRY(hipMalloc((void **)&h->hsig_dev, std::max<size_t>((size_t)2 * kShStride * h->nsh, 1) * sizeof(double)));
   static_assert(sizeof(uint2) == 4 * sizeof(uint16_t), "uloc packing");
   RC_TRY(upload_table(h, &h->uloc_dev, reinterpret_cast<const uint2 *>(plan.uloc.data()), plan.uloc.size() / 4));
   h->nk_total = (long)plan.klist.size();
@@ -1083,31 +1148,71 @@ static int ensure_fused_buffers(evp1d_handle *h) {
                                   kFusedMaxEntries * 4 * (int)sizeof(double2)));
   }
   if (!h->vel_alt) HIP_TRY(hipMalloc((void **)&h->vel_alt, (size_t)h->navel * sizeof(double2)));
-  if (!h->sig_alt) HIP_TRY(hipMalloc((void **)&h->sig_alt, (size_t)12 * h->ldsig * sizeof(double)));
+  if (!h->fused_inplace && !h->sig_alt)
+    HIP_TRY(hipMalloc((void **)&h->sig_alt, (size_t)12 * h->ldsig * sizeof(double)));
   return EVP1D_OK;
 }
 
 
-// nsub fused subcycles.  The state starts in the handle's buffers (X), is
-// copied to the second set (Y), and the subcycles alternate so that the last
-// one writes X: subcycle k writes X when nsub - k is even.
+// Before the first fused subcycle of a run: the constant entries' str1..8
+// and, for an in-place run, the shared points' stresses (into copy 0).
+// Fills the run's view; fused_step points it at subcycle k.
+static int fused_prologue(evp1d_handle *h, const DevView &d, FusedView &f) {
+  const bool ip = h->fused_inplace != 0;
+  hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, h->stream, h->cnz_dev);
+  HIP_TRY(hipGetLastError());
+  if (h->nk_total > 0) {
+    hipLaunchKernelGGL(k_fill_const, dim3((unsigned)((h->nk_total + 255) / 256)), dim3(256), 0, h->stream, d,
+                       (const int32_t *)h->klist_dev, h->cstr_dev, h->nk_total, h->cnz_dev);
+    HIP_TRY(hipGetLastError());
+  }
+  if (ip && h->nsh > 0) {
+    hipLaunchKernelGGL(k_fill_shared, dim3((unsigned)((h->nsh + 255) / 256)), dim3(256), 0, h->stream, h->hsig_dev,
+                       (const double *)d.sig, d.ldsig, (const int32_t *)h->shared_dev, h->nsh);
+    HIP_TRY(hipGetLastError());
+  }
+  f = FusedView{h->tiles_dev, h->hlist_dev, h->hid_dev, h->uloc_dev, h->cstr_dev, h->cnz_dev, h->nk_total,
+                ip ? 1 : 0,   h->nsh,       nullptr,    nullptr,    nullptr,     nullptr,    h->fused_cap};
+  return EVP1D_OK;
+}
+
+// After nsub in-place subcycles: the shared points' stresses (copy nsub % 2)
+// back into d.sig.
+static int fused_epilogue(evp1d_handle *h, const DevView &d, int nsub) {
+  if (!h->fused_inplace || h->nsh == 0) return EVP1D_OK;
+  hipLaunchKernelGGL(k_scatter_shared, dim3((unsigned)((h->nsh + 255) / 256)), dim3(256), 0, h->stream, d.sig,
+                     d.ldsig, (const double *)(h->hsig_dev + (size_t)(nsub % 2) * kShStride * h->nsh),
+                     (const int32_t *)h->shared_dev, h->nsh);
+  HIP_TRY(hipGetLastError());
+  return EVP1D_OK;
+}
+
+// subcycle k (1-based) of an in-place run reads shared copy (k - 1) % 2 and
+// writes copy k % 2
+static void fused_step(evp1d_handle *h, FusedView &f, int k) {
+  const size_t n = (size_t)kShStride * h->nsh;
+  f.hsig_in = h->hsig_dev + ((k - 1) % 2) * n;
+  f.hsig_out = h->hsig_dev + (k % 2) * n;
+}
+
+// nsub fused subcycles.  The state starts in the handle's buffers (X).
+// Velocities, and stresses of a two-buffer run, are copied to the second set
+// (Y) and the subcycles alternate so that the last one writes X: subcycle k
+// writes X when nsub - k is even.  Stresses of an in-place run stay in X
+// (the shared points' through their copy, put back at the end).
 static int issue_fused(evp1d_handle *h, int nsub) {
   const DevView X = h->v;
   DevView Y = h->v;
   Y.vel = Y.vel_out = h->vel_alt;
   Y.velc = h->vel_alt;
-  Y.sig = h->sig_alt;
   HIP_TRY(hipMemcpyAsync(Y.vel, X.vel, (size_t)h->navel * sizeof(double2), hipMemcpyDeviceToDevice, h->stream));
-  HIP_TRY(hipMemcpyAsync(Y.sig, X.sig, (size_t)12 * h->ldsig * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-  hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, h->stream, h->cnz_dev);
-  HIP_TRY(hipGetLastError());
-  if (h->nk_total > 0) {
-    hipLaunchKernelGGL(k_fill_const, dim3((unsigned)((h->nk_total + 255) / 256)), dim3(256), 0, h->stream, X,
-                       (const int32_t *)h->klist_dev, h->cstr_dev, h->nk_total, h->cnz_dev);
-    HIP_TRY(hipGetLastError());
+  if (!h->fused_inplace) {
+    Y.sig = h->sig_alt;
+    HIP_TRY(hipMemcpyAsync(Y.sig, X.sig, (size_t)12 * h->ldsig * sizeof(double), hipMemcpyDeviceToDevice,
+                           h->stream));
   }
-  FusedView f{h->tiles_dev, h->hlist_dev, h->uloc_dev, h->cstr_dev, h->cnz_dev, h->nk_total, nullptr, nullptr,
-              h->fused_cap};
+  FusedView f;
+  RC_TRY(fused_prologue(h, X, f));
   const size_t lds = (size_t)4 * h->fused_cap * sizeof(double2);
   for (int k = 1; k <= nsub; ++k) {
     const bool to_x = ((nsub - k) % 2) == 0;
@@ -1115,6 +1220,7 @@ static int issue_fused(evp1d_handle *h, int nsub) {
     DevView out = to_x ? X : Y;
     f.vel_out = out.vel;
     f.sig_out = out.sig;
+    fused_step(h, f, k);
     if (h->fused_ntiles > 0) {
       hipLaunchKernelGGL(kFusedFns[fused_block_index(h->tile_block)][h->tile_xcd ? 1 : 0], dim3(h->fused_ntiles),
                          dim3(h->tile_block), lds, h->stream, in, f, k == nsub ? 1 : 0);
@@ -1125,7 +1231,7 @@ static int issue_fused(evp1d_handle *h, int nsub) {
       HIP_TRY(hipGetLastError());
     }
   }
-  return EVP1D_OK;
+  return fused_epilogue(h, X, nsub);
 }
 
 static int run_graph(evp1d_handle *h, int nsub, bool fused) {
@@ -1645,28 +1751,66 @@ int evp1d_get_tiling(evp1d_handle *h, int32_t *tile_w, int32_t *tile_h, int32_t 
   return EVP1D_OK;
 }
 
+extern "C++" {
+// Device copies of the evolving state (stresses, velocities, str1..8,
+// diagnostics) in the current slot order, for timing runs that must leave
+// the state as it was.
+struct SavedState {
+  struct Buf { void *dev; void *bak; size_t bytes; };
+  std::vector<Buf> bufs;
+  int save(evp1d_handle *h, const char *who) {
+    auto one = [&](void *p, size_t bytes) -> int {
+      if (bytes == 0) return EVP1D_OK;
+      void *b = nullptr;
+      const hipError_t e = hipMalloc(&b, bytes);
+      if (e != hipSuccess) return fail(EVP1D_ENOMEM, "%s: hipMalloc(%zu): %s", who, bytes, hipGetErrorString(e));
+      bufs.push_back(Buf{p, b, bytes});
+      HIP_TRY(hipMemcpyAsync(b, p, bytes, hipMemcpyDeviceToDevice, h->stream));
+      return EVP1D_OK;
+    };
+    const size_t nV = (size_t)h->navel * sizeof(double2), nA = (size_t)h->na * sizeof(double);
+    RC_TRY(one(h->v.sig, (size_t)12 * h->ldsig * sizeof(double)));
+    void *pairs[5] = {h->v.vel, h->v.s15, h->v.s27, h->v.s36, h->v.s48};
+    for (void *q : pairs) RC_TRY(one(q, nV));
+    void *diag[3] = {h->v.strintx, h->v.strinty, h->v.cb};
+    for (void *q : diag) RC_TRY(one(q, nA));
+    return EVP1D_OK;
+  }
+  // copies back (keeping rc's first error), waits, frees the copies
+  int restore(evp1d_handle *h, int rc, const char *who) {
+    for (const Buf &b : bufs)
+      if (hipMemcpyAsync(b.dev, b.bak, b.bytes, hipMemcpyDeviceToDevice, h->stream) != hipSuccess && !rc)
+        rc = fail(EVP1D_EHIP, "%s: restore failed", who);
+    if (hipStreamSynchronize(h->stream) != hipSuccess && !rc) rc = fail(EVP1D_EHIP, "%s: sync failed", who);
+    for (const Buf &b : bufs) (void)hipFree(b.bak);
+    bufs.clear();
+    return rc;
+  }
+};
+
+// average duration in microseconds of `reps` launches of what `issue` puts
+// on the handle's stream (after one untimed call)
+template <class F>
+static int time_launches(evp1d_handle *h, int reps, F issue, float *us) {
+  RC_TRY(issue());
+  HIP_TRY(hipEventRecord(h->ev0, h->stream));
+  for (int r = 0; r < reps; ++r) RC_TRY(issue());
+  HIP_TRY(hipEventRecord(h->ev1, h->stream));
+  HIP_TRY(hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  *us = ms * 1000.f / (float)reps;
+  return EVP1D_OK;
+}
+}  // extern "C++"
+
 int evp1d_autotune(evp1d_handle *h, int32_t reps, float *stress_us, float *stepu_us) {
   if (!h) return fail(EVP1D_EINVAL, "evp1d_autotune: NULL handle");
   if (!ready(h)) return fail(EVP1D_ESTATE, "evp1d_autotune: neighbour lists not set");
   if (reps < 1) return fail(EVP1D_EINVAL, "evp1d_autotune: reps=%d < 1", reps);
   RC_TRY(ensure(h));
-  struct Saved { void *dev; void *bak; size_t bytes; };
-  std::vector<Saved> saved;
-  auto save = [&](void *p, size_t bytes) -> int {
-    if (bytes == 0) return EVP1D_OK;
-    void *b = nullptr;
-    const hipError_t e = hipMalloc(&b, bytes);
-    if (e != hipSuccess) return fail(EVP1D_ENOMEM, "evp1d_autotune: hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
-    saved.push_back(Saved{p, b, bytes});
-    HIP_TRY(hipMemcpyAsync(b, p, bytes, hipMemcpyDeviceToDevice, h->stream));
-    return EVP1D_OK;
-  };
-  const size_t nV = (size_t)h->navel * sizeof(double2), nA = (size_t)h->na * sizeof(double);
-  int rc = save(h->v.sig, (size_t)12 * h->ldsig * sizeof(double));
-  void *pairs[5] = {h->v.vel, h->v.s15, h->v.s27, h->v.s36, h->v.s48};
-  for (void *q : pairs) if (!rc) rc = save(q, nV);
-  void *diag[3] = {h->v.strintx, h->v.strinty, h->v.cb};
-  for (void *q : diag) if (!rc) rc = save(q, nA);
+  SavedState st;
+  int rc = st.save(h, "evp1d_autotune");
   float best_s = 1e30f, best_u = 1e30f;
   const LaunchCfg keep_s = h->cfg_stress, keep_u = h->cfg_stepu;
   LaunchCfg win_s = keep_s, win_u = keep_u;
@@ -1674,25 +1818,14 @@ int evp1d_autotune(evp1d_handle *h, int32_t reps, float *stress_us, float *stepu
     for (int bi = 0; bi < kNumBlocks && !rc; ++bi) {
       for (int x = 0; x < 2 && !rc; ++x) {
         (which == 0 ? h->cfg_stress : h->cfg_stepu) = LaunchCfg{bi, x};
-        rc = which == 0 ? launch_stress(h) : launch_stepu(h, true);    // warm
-        if (!rc && hipEventRecord(h->ev0, h->stream) != hipSuccess) rc = fail(EVP1D_EHIP, "hipEventRecord");
-        for (int r = 0; r < reps && !rc; ++r) rc = which == 0 ? launch_stress(h) : launch_stepu(h, true);
-        float ms = 0.f;
-        if (!rc && (hipEventRecord(h->ev1, h->stream) != hipSuccess || hipEventSynchronize(h->ev1) != hipSuccess ||
-                    hipEventElapsedTime(&ms, h->ev0, h->ev1) != hipSuccess))
-          rc = fail(EVP1D_EHIP, "evp1d_autotune: event timing failed");
-        const float us = ms * 1000.f / (float)reps;
+        float us = 0.f;
+        rc = time_launches(h, reps, [&] { return which == 0 ? launch_stress(h) : launch_stepu(h, true); }, &us);
         if (!rc && which == 0 && us < best_s) { best_s = us; win_s = h->cfg_stress; }
         if (!rc && which == 1 && us < best_u) { best_u = us; win_u = h->cfg_stepu; }
       }
     }
   }
-  for (const Saved &sv : saved) {
-    if (hipMemcpyAsync(sv.dev, sv.bak, sv.bytes, hipMemcpyDeviceToDevice, h->stream) != hipSuccess && !rc)
-      rc = fail(EVP1D_EHIP, "evp1d_autotune: restore failed");
-  }
-  if (hipStreamSynchronize(h->stream) != hipSuccess && !rc) rc = fail(EVP1D_EHIP, "evp1d_autotune: sync failed");
-  for (const Saved &sv : saved) (void)hipFree(sv.bak);
+  rc = st.restore(h, rc, "evp1d_autotune");
   h->cfg_stress = rc ? keep_s : win_s;
   h->cfg_stepu = rc ? keep_u : win_u;
   h->graph_dirty = true;
@@ -1701,6 +1834,74 @@ int evp1d_autotune(evp1d_handle *h, int32_t reps, float *stress_us, float *stepu
     if (stepu_us) *stepu_us = best_u;
   }
   return rc;
+}
+
+int evp1d_set_fused_inplace(evp1d_handle *h, int32_t inplace) {
+  if (!h) return fail(EVP1D_EINVAL, "evp1d_set_fused_inplace: NULL handle");
+  if (inplace != 0 && inplace != 1) return fail(EVP1D_EINVAL, "evp1d_set_fused_inplace: inplace must be 0 or 1");
+  if (inplace != h->fused_inplace) {
+    HIP_TRY(hipStreamSynchronize(h->stream));   // no queued run may still use the other form's buffers
+    h->fused_inplace = inplace;
+    h->graph_dirty = true;
+  }
+  return EVP1D_OK;
+}
+
+int evp1d_get_fused_inplace(evp1d_handle *h, int32_t *inplace) {
+  if (!h) return fail(EVP1D_EINVAL, "evp1d_get_fused_inplace: NULL handle");
+  if (inplace) *inplace = h->fused_inplace;
+  return EVP1D_OK;
+}
+
+int evp1d_autotune_fused(evp1d_handle *h, int32_t nsub, float *us) {
+  if (!h) return fail(EVP1D_EINVAL, "evp1d_autotune_fused: NULL handle");
+  if (!ready(h)) return fail(EVP1D_ESTATE, "evp1d_autotune_fused: neighbour lists not set");
+  if (nsub < 1) return fail(EVP1D_EINVAL, "evp1d_autotune_fused: nsub=%d < 1", nsub);
+  RC_TRY(ensure(h));
+  if (!h->fused_ok)
+    return fail(EVP1D_ESTATE, "evp1d_autotune_fused: fused form unavailable (%s)", h->fused_why.c_str());
+  struct Cand { int w, h, block; };
+  static const Cand cands[4] = {{64, 16, 512}, {32, 16, 512}, {32, 8, 256}, {16, 8, 128}};
+  const bool keep_auto = h->tiling_auto;
+  const int keep_w = h->tile_w, keep_h = h->tile_h, keep_b = h->tile_block, keep_x = h->tile_xcd,
+            keep_ip = h->fused_inplace;
+  float best = 1e30f;
+  int win = -1, win_ip = 0;
+  int rc = EVP1D_OK;
+  for (int c = 0; c < 4 && !rc; ++c) {
+    rc = evp1d_set_tiling(h, cands[c].w, cands[c].h, cands[c].block, 1);
+    if (!rc) rc = ensure(h);
+    if (rc || !h->fused_ok) continue;   // tile too large for LDS on these lists
+    SavedState st;
+    rc = st.save(h, "evp1d_autotune_fused");
+    for (int ip = 0; ip < 2 && !rc; ++ip) {
+      h->fused_inplace = ip;
+      rc = ensure_fused_buffers(h);
+      float t = 0.f;
+      if (!rc) rc = time_launches(h, 1, [&] { return issue_fused(h, nsub); }, &t);
+      t /= (float)nsub;
+      if (!rc && t < best) { best = t; win = c; win_ip = ip; }
+    }
+    rc = st.restore(h, rc, "evp1d_autotune_fused");
+  }
+  h->graph_dirty = true;
+  if (rc || win < 0) {
+    h->fused_inplace = keep_ip;
+    if (keep_auto) {
+      h->tiling_auto = true;
+      h->layout_dirty = true;
+      h->tile_cache_ok = false;
+    } else {
+      (void)evp1d_set_tiling(h, keep_w, keep_h, keep_b, keep_x);
+    }
+    return rc ? rc : fail(EVP1D_ESTATE, "evp1d_autotune_fused: no tiling fits these lists");
+  }
+  h->fused_inplace = win_ip;
+  RC_TRY(evp1d_set_tiling(h, cands[win].w, cands[win].h, cands[win].block, 1));
+  RC_TRY(ensure(h));
+  RC_TRY(ensure_fused_buffers(h));
+  if (us) *us = best;
+  return EVP1D_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -1916,31 +2117,32 @@ return EVP1D_OK;
       return dk;
     };
     if (h->fused_ok) {
-      // fused forward sweep; the stresses ping-pong between the handle's two
-      // buffers and end in the handle's own (subcycle k writes it when N - k
+      // fused forward sweep; two-buffer stresses ping-pong between the
+      // handle's buffers and end in its own (subcycle k writes it when N - k
       // is even)
       RC_TRY(ensure_fused_buffers(h));
-      HIP_TRY(hipMemcpyAsync(h->sig_alt, W.sig, (size_t)12 * h->ldsig * sizeof(double), hipMemcpyDeviceToDevice,
-                             h->stream));
-      hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, h->stream, h->cnz_dev);
-      if (h->nk_total > 0)
-        hipLaunchKernelGGL(k_fill_const, dim3((unsigned)((h->nk_total + 255) / 256)), dim3(256), 0, h->stream, W,
-                           (const int32_t *)h->klist_dev, h->cstr_dev, h->nk_total, h->cnz_dev);
-      HIP_TRY(hipGetLastError());
-      FusedView f{h->tiles_dev, h->hlist_dev, h->uloc_dev, h->cstr_dev, h->cnz_dev, h->nk_total, nullptr, nullptr,
-                  h->fused_cap};
+      if (!h->fused_inplace)
+        HIP_TRY(hipMemcpyAsync(h->sig_alt, W.sig, (size_t)12 * h->ldsig * sizeof(double), hipMemcpyDeviceToDevice,
+                               h->stream));
+      FusedView f;
+      RC_TRY(fused_prologue(h, W, f));
       const size_t lds = (size_t)4 * h->fused_cap * sizeof(double2);
       for (int k = 1; k <= N; ++k) {
         DevView dk = view(k);
-        const bool to_w = ((N - k) % 2) == 0;
-        dk.sig = to_w ? h->sig_alt : W.sig;
-        f.sig_out = to_w ? W.sig : h->sig_alt;
+        f.sig_out = W.sig;
+        if (!h->fused_inplace) {
+          const bool to_w = ((N - k) % 2) == 0;
+          dk.sig = to_w ? h->sig_alt : W.sig;
+          f.sig_out = to_w ? W.sig : h->sig_alt;
+        }
         f.vel_out = dk.vel_out;
+        fused_step(h, f, k);
         if (h->fused_ntiles > 0)
           hipLaunchKernelGGL(kFusedSplitFns[fused_block_index(h->tile_block)], dim3(h->fused_ntiles),
                              dim3(h->tile_block), lds, h->stream, dk, f, k == N ? 1 : 0);
         HIP_TRY(hipGetLastError());
       }
+      RC_TRY(fused_epilogue(h, W, N));
     } else {
       for (int k = 1; k <= N; ++k) {
         const DevView dk = view(k);

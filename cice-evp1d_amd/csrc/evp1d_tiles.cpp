@@ -180,12 +180,26 @@ TilePlan plan_tiles(int na, int navel, const int32_t *const idx[6], const uint8_
   const int nt = ids.ntiles;
   const std::vector<int32_t> &tid = ids.tid;
 
-  // counting sort by (class, tile), caller order inside
-  std::vector<int32_t> start((size_t)4 * nt + 1, 0);
-  for (int q = 0; q < na; ++q) ++start[(size_t)point_class(skip[q]) * nt + tid[(size_t)q] + 1];
+  // Shared points: active T points that a stepu point of another tile reads
+  // (nw, sse, sw), i.e. that appear in some tile's halo.
+  std::vector<uint8_t> shf((size_t)na, 0);
+  for (int q = 0; q < na; ++q) {
+    const int c = point_class(skip[q]);
+    if (c != 1 && c != 2) continue;
+    for (int n : {3, 5, 4}) {
+      const int32_t r = idx[n][q];
+      if (r >= 0 && r < na && point_class(skip[r]) <= 1 && tid[(size_t)r] != tid[(size_t)q]) shf[(size_t)r] = 1;
+    }
+  }
+  // counting sort by (class, tile, shared first), caller order inside
+  auto key = [&](int q) {
+    return ((size_t)point_class(skip[q]) * nt + tid[(size_t)q]) * 2 + (shf[(size_t)q] ? 0 : 1);
+  };
+  std::vector<int32_t> start((size_t)8 * nt + 1, 0);
+  for (int q = 0; q < na; ++q) ++start[key(q) + 1];
   for (size_t b = 1; b < start.size(); ++b) start[b] += start[b - 1];
   std::vector<int32_t> fill(start.begin(), start.end() - 1);
-  for (int q = 0; q < na; ++q) p.perm[(size_t)fill[(size_t)point_class(skip[q]) * nt + tid[(size_t)q]]++] = q;
+  for (int q = 0; q < na; ++q) p.perm[(size_t)fill[key(q)]++] = q;
   std::vector<int32_t> inv((size_t)na);
   for (int s = 0; s < na; ++s) inv[(size_t)p.perm[(size_t)s]] = s;
   auto slot = [&](int32_t q) { return q < na ? inv[(size_t)q] : q; };
@@ -196,12 +210,15 @@ TilePlan plan_tiles(int na, int navel, const int32_t *const idx[6], const uint8_
   std::vector<int32_t> mark((size_t)navel, -1), lidx((size_t)navel, -1), halo, konst, rr;
   for (int t = 0; t < nt; ++t) {
     TileDesc d{};
-    d.a0 = start[(size_t)0 * nt + t];
-    d.a1 = start[(size_t)0 * nt + t + 1];
-    d.b0 = start[(size_t)1 * nt + t];
-    d.b1 = start[(size_t)1 * nt + t + 1];
-    d.c0 = start[(size_t)2 * nt + t];
-    d.c1 = start[(size_t)2 * nt + t + 1];
+    auto at = [&](int c, int half) { return start[((size_t)c * nt + t) * 2 + half]; };
+    d.a0 = at(0, 0);
+    d.a1 = at(0, 2);
+    d.sa = at(0, 1) - d.a0;
+    d.b0 = at(1, 0);
+    d.b1 = at(1, 2);
+    d.sb = at(1, 1) - d.b0;
+    d.c0 = at(2, 0);
+    d.c1 = at(2, 2);
     const int nA = d.a1 - d.a0, nB = d.b1 - d.b0, nO = nA + nB;
     if (nO + (d.c1 - d.c0) == 0) continue;
     for (int s = d.a0; s < d.a1; ++s) { mark[(size_t)s] = t; lidx[(size_t)s] = s - d.a0; }
@@ -251,6 +268,18 @@ TilePlan plan_tiles(int na, int navel, const int32_t *const idx[6], const uint8_
     p.klist.insert(p.klist.end(), konst.begin(), konst.end());
     p.tiles.push_back(d);
   }
+  // shared ids of the halo points
+  p.shared = p.list;
+  std::sort(p.shared.begin(), p.shared.end());
+  p.shared.erase(std::unique(p.shared.begin(), p.shared.end()), p.shared.end());
+  p.sid.assign((size_t)t_end, -1);
+  for (size_t k = 0; k < p.shared.size(); ++k) p.sid[(size_t)p.shared[k]] = (int32_t)k;
+  p.hid.resize(p.list.size());
+  for (size_t k = 0; k < p.list.size(); ++k) p.hid[k] = p.sid[(size_t)p.list[k]];
+  for (TileDesc &d : p.tiles) {
+    d.ia = d.sa > 0 ? p.sid[(size_t)d.a0] : 0;
+    d.ib = d.sb > 0 ? p.sid[(size_t)d.b0] : 0;
+  }
   p.ok = true;
   return p;
 }
@@ -277,6 +306,7 @@ std::string check_plan(const TilePlan &p, int na, int navel, const int32_t *cons
   auto slot = [&](int32_t q) { return q < na ? inv[(size_t)q] : q; };
   std::vector<char> owned((size_t)na, 0), stepped((size_t)na, 0);
   std::vector<int32_t> emap;
+  if ((int)p.sid.size() != t_end || p.hid.size() != p.list.size()) return "shared-id tables have the wrong size";
   for (size_t t = 0; t < p.tiles.size(); ++t) {
     const TileDesc &d = p.tiles[t];
     if (!(0 <= d.a0 && d.a0 <= d.a1 && d.a1 <= p.cnt[0])) return "stress-only range outside its class";
@@ -286,6 +316,12 @@ std::string check_plan(const TilePlan &p, int na, int navel, const int32_t *cons
     if (d.k0 < 0 || d.nk < 0 || (size_t)d.k0 + d.nk > p.klist.size()) return "constant list range";
     const int nA = d.a1 - d.a0, nO = nA + (d.b1 - d.b0), ne = nO + d.nh + d.nk;
     if (ne > p.max_entries) return "tile exceeds max_entries";
+    // owned shared points first, with consecutive shared ids
+    if (d.sa < 0 || d.sa > nA || d.sb < 0 || d.sb > d.b1 - d.b0) return "shared prefix longer than its range";
+    for (int s = d.a0; s < d.a1; ++s)
+      if (p.sid[(size_t)s] != (s - d.a0 < d.sa ? d.ia + (s - d.a0) : -1)) return "owned slot with a wrong shared id";
+    for (int s = d.b0; s < d.b1; ++s)
+      if (p.sid[(size_t)s] != (s - d.b0 < d.sb ? d.ib + (s - d.b0) : -1)) return "owned slot with a wrong shared id";
     emap.assign((size_t)ne, -1);
     for (int s = d.a0; s < d.a1; ++s) { if (owned[(size_t)s]++) return "T slot owned twice"; emap[(size_t)(s - d.a0)] = s; }
     for (int s = d.b0; s < d.b1; ++s) { if (owned[(size_t)s]++) return "T slot owned twice"; emap[(size_t)(nA + s - d.b0)] = s; }
@@ -314,6 +350,19 @@ std::string check_plan(const TilePlan &p, int na, int navel, const int32_t *cons
       if (!check_u(s)) { snprintf(buf, sizeof buf, "stepu slot %d: bad LDS entry", s); return buf; }
     for (int s = d.c0; s < d.c1; ++s)
       if (!check_u(s)) { snprintf(buf, sizeof buf, "stepu slot %d: bad LDS entry", s); return buf; }
+  }
+  // shared ids: every halo entry's id names its slot, ids are distinct
+  for (size_t k = 0; k < p.shared.size(); ++k) {
+    const int32_t g = p.shared[k];
+    if (g < 0 || g >= t_end || p.sid[(size_t)g] != (int32_t)k) return "shared id table inconsistent";
+  }
+  for (size_t k = 0; k < p.list.size(); ++k) {
+    const int32_t id = p.hid[k];
+    if (id < 0 || id >= (int32_t)p.shared.size() || p.shared[(size_t)id] != p.list[k]) return "halo entry with a wrong shared id";
+  }
+  for (int s = 0; s < t_end; ++s) {
+    const int32_t id = p.sid[(size_t)s];
+    if (id >= (int32_t)p.shared.size() || (id >= 0 && p.shared[(size_t)id] != s)) return "stress slot with a wrong shared id";
   }
   for (int s = 0; s < t_end; ++s)
     if (!owned[(size_t)s]) return "active T slot owned by no tile";
@@ -367,8 +416,8 @@ extern "C" int evp1d_tile_plan(int32_t na, int32_t navel, const int32_t *ee, con
   if (tiles && tiles_cap >= nt)
     for (int t = 0; t < nt; ++t) {
       const evp1d::TileDesc &d = p.tiles[(size_t)t];
-      const int32_t v[10] = {d.a0, d.a1, d.b0, d.b1, d.c0, d.c1, d.h0, d.nh, d.k0, d.nk};
-      for (int k = 0; k < 10; ++k) tiles[10 * t + k] = v[k];
+      const int32_t v[14] = {d.a0, d.a1, d.b0, d.b1, d.c0, d.c1, d.h0, d.nh, d.k0, d.nk, d.sa, d.ia, d.sb, d.ib};
+      for (int k = 0; k < 14; ++k) tiles[14 * t + k] = v[k];
     }
   if (list && list_cap >= (int32_t)p.list.size())
     for (size_t k = 0; k < p.list.size(); ++k) list[k] = p.list[k];

@@ -10,9 +10,10 @@
 // tile) but written only by their owner, so every output is produced once.
 //
 // Slot order.  Points are ordered by class (stress only | stress and stepu |
-// stepu only | neither), then by tile, then by caller position, so a tile's
-// owned T points and its U points are each two contiguous slot ranges and
-// the class ranges that the unfused kernels iterate stay contiguous.  Tiles
+// stepu only | neither), then by tile, then shared points (those some other
+// tile evaluates as halo) before the rest, then by caller position, so a
+// tile's owned T points and its U points are each two contiguous slot ranges
+// and the class ranges that the unfused kernels iterate stay contiguous.  Tiles
 // are W x H boxes of grid coordinates recovered from the neighbour lists:
 // ee=(i-1,j) ne=(i-1,j-1) se=(i,j-1) nw=(i+1,j) sw=(i+1,j+1) sse=(i,j+1)
 // (ice_dyn_evp1d.F90:766-772).  Connected components get disjoint frames.
@@ -31,6 +32,8 @@ struct TileDesc {
   int32_t c0, c1;   // stepu-only slots
   int32_t h0, nh;   // halo stress slots: TilePlan::list[h0, h0 + nh)
   int32_t k0, nk;   // constant slots (str1..8 never written): TilePlan::klist[k0, k0 + nk)
+  int32_t sa, ia;   // [a0, a0 + sa) are shared, with shared ids ia, ia + 1, ...
+  int32_t sb, ib;   // [b0, b0 + sb) are shared, with shared ids ib, ib + 1, ...
 };
 
 struct TilePlan {
@@ -43,6 +46,12 @@ struct TilePlan {
   std::vector<int32_t> list;          // halo stress slots, per tile
   std::vector<int32_t> klist;         // constant slots, per tile
   std::vector<uint16_t> uloc;         // 4 per stepu slot (slot - cnt[0]): own, nw, sse, sw entries
+  // Stresses of shared points (halo of some tile) live, during a fused run,
+  // in a double-buffered copy indexed by a shared id: every reader sees the
+  // previous subcycle's values while their owner writes the next ones.
+  std::vector<int32_t> shared;        // shared id -> slot (sorted slots of all halo entries)
+  std::vector<int32_t> sid;           // stress slot -> shared id, or -1
+  std::vector<int32_t> hid;           // per halo entry (parallel to list): shared id
   int max_entries = 0;
 };
 

@@ -56,6 +56,7 @@ EXPORTS = (
     "evp1d_autotune", "evp1d_set_tuning", "evp1d_get_tuning",
     "evp1d_adjoint_run", "evp1d_loss", "evp1d_tangent_run",
     "evp1d_set_tiling", "evp1d_get_tiling", "evp1d_tile_plan",
+    "evp1d_set_fused_inplace", "evp1d_get_fused_inplace", "evp1d_autotune_fused",
     "evp1d_set_grid", "evp1d_put_2d", "evp1d_get_2d", "evp1d_host_alloc", "evp1d_host_free",
 )
 
@@ -146,6 +147,9 @@ def load(fma: bool = False, path: str | os.PathLike | None = None) -> ctypes.CDL
         "evp1d_host_alloc": [ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)],
         "evp1d_host_free": [ctypes.c_void_p],
         "evp1d_get_tiling": [_H, _IP, _IP, _IP, _IP, _IP],
+        "evp1d_set_fused_inplace": [_H, i32],
+        "evp1d_get_fused_inplace": [_H, _IP],
+        "evp1d_autotune_fused": [_H, i32, ctypes.POINTER(ctypes.c_float)],
         "evp1d_tile_plan": [i32, i32] + [_IP] * 8 + [i32, i32, i32] + [_IP] * 5 + [i32, _IP, i32, _IP, _IP, i32,
                                                                                 _IP, _IP],
     }
@@ -217,7 +221,7 @@ def tile_plan(idx: dict, skipT, skipU, navel: int, tile_w: int = 32, tile_h: int
     """Host-only fused-subcycle tiling of a 1D list (evp1d_tile_plan).
 
     idx: the six 1-based neighbour lists.  Returns perm (slot -> 1-based
-    point), counts, tiles (n x 10: a0,a1,b0,b1,c0,c1,h0,nh,k0,nk), list
+    point), counts, tiles (n x 14: a0,a1,b0,b1,c0,c1,h0,nh,k0,nk,sa,ia,sb,ib), list
     (halo slots), klist (constant slots), uloc (stepu slot x 4: own, nw, sse,
     sw LDS entries) and max_entries; ntiles 0 when the lists are no grid."""
     L = load(fma)
@@ -231,7 +235,7 @@ def tile_plan(idx: dict, skipT, skipU, navel: int, tile_w: int = 32, tile_h: int
     args = [na, navel, *[_ip(a) for a in lists], _ip(skipT), _ip(skipU), tile_w, tile_h, max_entries, _ip(perm),
             _ip(counts), ctypes.byref(nt), ctypes.byref(mx)]
     _check(L, L.evp1d_tile_plan(*args, None, 0, None, 0, ctypes.byref(nl), None, 0, ctypes.byref(nk), None))
-    tiles = np.zeros((max(nt.value, 1), 10), np.int32)
+    tiles = np.zeros((max(nt.value, 1), 14), np.int32)
     hl = np.zeros(max(nl.value, 1), np.int32)
     kl = np.zeros(max(nk.value, 1), np.int32)
     uloc = np.zeros((max(int(counts[1] + counts[2]), 1), 4), np.int32)
@@ -376,6 +380,21 @@ class Engine:
         a = [ctypes.c_int32() for _ in range(5)]
         _check(self.L, self.L.evp1d_get_tiling(self.h, *[ctypes.byref(x) for x in a]))
         return dict(zip(("tile_w", "tile_h", "block", "xcd", "ntiles"), (x.value for x in a)))
+
+    def set_fused_inplace(self, inplace: bool) -> None:
+        _check(self.L, self.L.evp1d_set_fused_inplace(self.h, 1 if inplace else 0))
+
+    def get_fused_inplace(self) -> bool:
+        v = ctypes.c_int32()
+        _check(self.L, self.L.evp1d_get_fused_inplace(self.h, ctypes.byref(v)))
+        return bool(v.value)
+
+    def autotune_fused(self, nsub: int = 100) -> dict:
+        """Time the fused tile ladder x stress storage (evp1d_autotune_fused),
+        keep the fastest; returns it with its time per subcycle."""
+        us = ctypes.c_float()
+        _check(self.L, self.L.evp1d_autotune_fused(self.h, nsub, ctypes.byref(us)))
+        return dict(self.get_tiling(), inplace=self.get_fused_inplace(), us=us.value)
 
     # lifetime ------------------------------------------------------------
     def close(self) -> None:

@@ -53,7 +53,8 @@ module evp1d_capi
             evp1d_member_info, evp1d_set_field_member, evp1d_get_field_member,          &
             evp1d_set_index_member, evp1d_set_skip_member, evp1d_autotune,              &
             evp1d_set_tuning, evp1d_get_tuning, evp1d_set_tiling, evp1d_get_tiling,       &
-            evp1d_set_grid, evp1d_put_2d, evp1d_get_2d, evp1d_host_alloc, evp1d_host_free
+            evp1d_set_grid, evp1d_put_2d, evp1d_get_2d, evp1d_host_alloc, evp1d_host_free, &
+            evp1d_set_fused_inplace, evp1d_get_fused_inplace, evp1d_autotune_fused
 
   interface
     integer(c_int) function evp1d_set_evp_parameters(ndte, dt, p) bind(C, name='evp1d_set_evp_parameters')
@@ -281,6 +282,22 @@ module evp1d_capi
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: h
       integer(c_int32_t), intent(out) :: tw, th, block, xcd, ntiles
+    end function
+    integer(c_int) function evp1d_set_fused_inplace(h, inplace) bind(C, name='evp1d_set_fused_inplace')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: inplace
+    end function
+    integer(c_int) function evp1d_get_fused_inplace(h, inplace) bind(C, name='evp1d_get_fused_inplace')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(out) :: inplace
+    end function
+    integer(c_int) function evp1d_autotune_fused(h, nsub, us) bind(C, name='evp1d_autotune_fused')
+      import :: c_int, c_int32_t, c_ptr, c_float
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nsub
+      real(c_float), intent(out) :: us
     end function
   end interface
 

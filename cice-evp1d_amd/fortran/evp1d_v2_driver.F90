@@ -17,7 +17,9 @@
 ! must be 1 (the reference's replicate() is a timing-only device whose copies
 ! do not line up, src/vars.F90:1801-1838).
 !
-! Usage:  evp1d_v2_driver [graph|persistent|launch|fused]
+! Usage:  evp1d_v2_driver [graph|persistent|launch|fused|fused-inplace|tuned]
+! (fused-inplace: fused form with stresses updated in place; tuned: fused
+! form with the tile and stress storage chosen by evp1d_autotune_fused)
 !===============================================================================
 program evp1d_v2_driver
   use, intrinsic :: iso_c_binding
@@ -37,7 +39,9 @@ program evp1d_v2_driver
   integer :: lun, ios, k, iw, nbb, i
   integer(8) :: t0, t1, rate
   character(len=32) :: arg
-  integer(c_int32_t) :: mode
+  integer(c_int32_t) :: mode, inplace
+  logical :: tune
+  real(c_float) :: tune_us
   ! fields of input_double_1d_v2.bin after strength/uvel/vvel, in file order
   integer(c_int32_t), parameter :: mid(28) = [F_DXT, F_DYT, F_UAREAR, F_CDN_OCN, F_AIU, F_UOCN, &
       F_VOCN, F_WATERX, F_WATERY, F_FORCEX, F_FORCEY, F_UMASSDTI, F_FM, F_STRINTX, F_STRINTY,     &
@@ -57,7 +61,7 @@ program evp1d_v2_driver
   if (na < 0 .or. navel < na) error stop 'ERROR: kernel.nml needs 0 <= na <= navel'
   write(*,'(a,i0,a,i0,a,i0)') ' na = ', na, '  navel = ', navel, '  ndte = ', ndte
 
-  mode = MODE_AUTO
+  mode = MODE_AUTO; inplace = 0; tune = .false.
   if (command_argument_count() >= 1) then
     call get_command_argument(1, arg)
     select case (trim(arg))
@@ -65,7 +69,9 @@ program evp1d_v2_driver
     case ('persistent'); mode = MODE_PERSISTENT
     case ('launch');     mode = MODE_LAUNCH
     case ('fused');      mode = MODE_FUSED
-    case default;        error stop 'usage: evp1d_v2_driver [graph|persistent|launch|fused]'
+    case ('fused-inplace'); mode = MODE_FUSED; inplace = 1
+    case ('tuned');      mode = MODE_FUSED; tune = .true.
+    case default;        error stop 'usage: evp1d_v2_driver [graph|persistent|launch|fused|fused-inplace|tuned]'
     end select
   endif
 
@@ -111,6 +117,11 @@ program evp1d_v2_driver
   do k = 1, 4
     call evp1d_check(evp1d_set_field(h, hid(k), ht(:,k), int(na,c_int32_t)), 'set_field')
   enddo
+  call evp1d_check(evp1d_set_fused_inplace(h, inplace), 'set_fused_inplace')
+  if (tune) then
+    call evp1d_check(evp1d_autotune_fused(h, int(min(ndte, 100),c_int32_t), tune_us), 'autotune_fused')
+    write(*,'(a,f10.2,a)') ' autotune_fused: ', tune_us, ' us per subcycle'
+  endif
   call evp1d_check(evp1d_synchronize(h), 'synchronize')
 
   call system_clock(t0, rate)

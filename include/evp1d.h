@@ -229,15 +229,33 @@ int evp1d_get_tuning(const evp1d_handle *h, int32_t *stress_block, int32_t *stre
 int evp1d_set_tiling(evp1d_handle *h, int32_t tile_w, int32_t tile_h, int32_t block, int32_t xcd);
 int evp1d_get_tiling(evp1d_handle *h, int32_t *tile_w, int32_t *tile_h, int32_t *block, int32_t *xcd,
                      int32_t *ntiles);
+/* Stress storage of fused runs.  inplace = 0 (default): stresses ping-pong
+ * between two buffers like the velocities.  inplace = 1: each point's
+ * stresses are updated in place by the workgroup that owns it; the points
+ * that neighbouring tiles also evaluate go through a small double-buffered
+ * copy during the run.  Which is faster depends on the domain size and the
+ * cache hierarchy (in place moves the same bytes with a smaller footprint);
+ * results are identical.                                                    */
+int evp1d_set_fused_inplace(evp1d_handle *h, int32_t inplace);
+int evp1d_get_fused_inplace(evp1d_handle *h, int32_t *inplace);
+/* Times one fused run of nsub subcycles (after an untimed one) for each tile
+ * of 64x16/512, 32x16/512, 32x8/256, 16x8/128 (XCD-aware order) that fits
+ * the lists, each with both stress storages, on the handle's own data (the
+ * state is saved and restored), keeps the fastest (as evp1d_set_tiling and
+ * evp1d_set_fused_inplace would) and reports its time per subcycle in
+ * microseconds.                                                             */
+int evp1d_autotune_fused(evp1d_handle *h, int32_t nsub, float *us);
 
 /* Host-only tiling of a 1D list (no device use; for inspection and tests).
  * ee..sse: na each, 1-based slots as for evp1d_set_index; skipT, skipU: na
  * flags.  Outputs, each optional: perm[na] (slot -> 1-based point), counts[4]
  * (points that are stress only, stress and stepu, stepu only, neither),
  * *ntiles, *max_entries (largest LDS table), and, when the capacities
- * suffice, tiles[10 * ntiles] (a0,a1,b0,b1,c0,c1,h0,nh,k0,nk: 0-based slot
- * ranges of owned stress-only, owned stress-and-stepu and stepu-only points,
- * then offset and length in list and in klist), list[*nlist] (halo slots:
+ * suffice, tiles[14 * ntiles] (a0,a1,b0,b1,c0,c1,h0,nh,k0,nk,sa,ia,sb,ib:
+ * 0-based slot ranges of owned stress-only, owned stress-and-stepu and
+ * stepu-only points, offset and length in list and in klist, then the owned
+ * points other tiles read as halo: [a0, a0+sa) and [b0, b0+sb), numbered
+ * from ia and ib in the run's table of such points), list[*nlist] (halo slots:
  * stress evaluated by the tile, written by their owner), klist[*nklist]
  * (constant slots: str1..8 read, never written) and uloc[4 * (counts[1] +
  * counts[2])] (LDS entries of own, nw, sse, sw for each stepu slot in slot

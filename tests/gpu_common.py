@@ -31,10 +31,18 @@ def golden_cases():
     return sorted(p.stem for p in GOLD.glob("*.npz"))
 
 
+def set_mode(eng: evp1d.Engine, mode: str) -> None:
+    """mode: a loop form, or "fused-inplace" (fused, stresses updated in place)."""
+    if mode == "fused-inplace":
+        eng.set_fused_inplace(True)
+        mode = "fused"
+    eng.set_mode(mode)
+
+
 def make_engine(st: O.State1D, ndte: int, mode: str = "auto", fma: bool = False) -> evp1d.Engine:
     p = evp1d.set_evp_parameters(ndte, fma=fma, **st.scalars)
     eng = evp1d.Engine(st.na, st.navel, p, fma=fma)
-    eng.set_mode(mode)
+    set_mode(eng, mode)
     eng.load_state(st.f, st.idx, st.skipT, st.skipU)
     return eng
 

@@ -121,6 +121,25 @@ def test_gradient_independent_of_snapshot_count(problem):
         assert g["loss"] == out[0]["loss"]
 
 
+def test_gradient_same_for_both_stress_storages(problem):
+    """The stored trajectory's fused forward sweep with stresses in place
+    gives the same gradient, loss and end state as with two buffers."""
+    st, ut, vt = problem
+    lib, p = oracle_params(st, NDTE)
+    want = st.copy()
+    lib.run(p, want, NDTE)
+    out = []
+    for mode in ("fused", "fused-inplace"):
+        for nsnap in (NDTE, 4):
+            with make_engine(st, NDTE, mode) as eng:
+                out.append(eng.adjoint_run(NDTE, ut, vt, nsnap=nsnap))
+                assert_state_bitexact(fetch_state(eng, st), want)
+    for g in out[1:]:
+        for k in ("strength", "cdn_ocn", "forcex", "forcey"):
+            assert np.array_equal(g[k], out[0][k]), k
+        assert g["loss"] == out[0]["loss"]
+
+
 def test_gradient_is_zero_off_ice(problem):
     """No stress where skipT, no stepu where skipU: those inputs are unused."""
     st, ut, vt = problem

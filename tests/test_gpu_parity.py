@@ -15,10 +15,10 @@ import pytest
 import evp1d
 from oracle import oracle as O
 from gpu_common import (GOLD, assert_state_bitexact, fetch_state, golden_cases, load_golden,
-                        make_engine, oracle_params, synthetic_state)
+                        make_engine, oracle_params, set_mode, synthetic_state)
 
 pytestmark = pytest.mark.gpu
-MODES = ("launch", "graph", "persistent", "fused")
+MODES = ("launch", "graph", "persistent", "fused", "fused-inplace")
 
 
 def _log(msg):
@@ -218,7 +218,7 @@ def test_batch_members_match_independent_oracle_runs(gpu):
     params = evp1d.set_evp_parameters(ndte)
     for mode in MODES:
         with evp1d.Engine.batch([(st.na, st.navel) for st in states], params) as eng:
-            eng.set_mode(mode)
+            set_mode(eng, mode)
             for k, st in enumerate(states):
                 eng.load_member(k, st.f, st.idx, st.skipT, st.skipU)
             info = eng.member_info(3)
@@ -251,17 +251,19 @@ def test_autotune_keeps_results_and_state(gpu):
             assert_state_bitexact(fetch_state(eng, st), want)
 
 
+@pytest.mark.parametrize("inplace", [False, True])
 @pytest.mark.parametrize("tile", [(32, 16, 256, 1), (1, 1, 256, 0), (5, 7, 512, 1), (64, 8, 512, 0), (200, 2, 256, 1)])
 @pytest.mark.parametrize("nsub", [1, 2, 7])
-def test_fused_tilings_match_oracle(gpu, tile, nsub):
+def test_fused_tilings_match_oracle(gpu, tile, nsub, inplace):
     """Any tile shape and XCD order, odd and even subcycle counts (the state
-    ping-pongs between two buffers and must end in the handle's own)."""
+    ping-pongs between two buffers and must end in the handle's own), both
+    stress storages (in place, the shared points' copy alternates too)."""
     st = synthetic_state(70, 50, seed=61)
     lib, p = oracle_params(st, nsub)
     want = st.copy()
     lib.run(p, want, nsub)
-    _log(f"fused tile={tile} nsub={nsub}")
-    with make_engine(st, nsub, "fused") as eng:
+    _log(f"fused tile={tile} nsub={nsub} inplace={inplace}")
+    with make_engine(st, nsub, "fused-inplace" if inplace else "fused") as eng:
         eng.set_tiling(*tile)
         assert eng.get_tiling()["ntiles"] > 0
         eng.run(nsub)
@@ -271,9 +273,10 @@ def test_fused_tilings_match_oracle(gpu, tile, nsub):
         assert_state_bitexact(fetch_state(eng, st), want)
 
 
-def test_fused_sees_fields_set_between_runs(gpu):
-    """str1..8 at points stress never writes (read by stepu from memory) and
-    ghost velocities may be changed by the caller between runs."""
+@pytest.mark.parametrize("mode", ["fused", "fused-inplace"])
+def test_fused_sees_fields_set_between_runs(gpu, mode):
+    """str1..8 at points stress never writes (read by stepu from memory),
+    stresses and ghost velocities may be changed by the caller between runs."""
     st = synthetic_state(50, 40, seed=62)
     lib, p = oracle_params(st, 12)
     want = st.copy()
@@ -283,12 +286,46 @@ def test_fused_sees_fields_set_between_runs(gpu):
     for k in ("str2", "str3", "str7"):
         want.f[k][stale] = rng.standard_normal(stale.sum())
     want.f["uvel"][st.na:] = rng.standard_normal(st.navel - st.na) * 0.1
-    with make_engine(st, 12, "fused") as eng:
+    want.f["stressp_1"] = want.f["stressp_1"] * 0.5
+    with make_engine(st, 12, mode) as eng:
         eng.run(5)
-        for k in ("str2", "str3", "str7", "uvel"):
+        for k in ("str2", "str3", "str7", "uvel", "stressp_1"):
             eng.set_field(k, want.f[k])
         eng.run(7)
         lib.run(p, want, 7)
+        assert_state_bitexact(fetch_state(eng, st), want)
+
+
+def test_fused_stress_storage_switch_between_runs(gpu):
+    """Changing the stress storage between runs keeps the state."""
+    st = synthetic_state(60, 45, seed=65)
+    lib, p = oracle_params(st, 15)
+    want = st.copy()
+    lib.run(p, want, 15)
+    with make_engine(st, 15, "fused") as eng:
+        eng.run(5)
+        eng.set_fused_inplace(True)
+        assert eng.get_fused_inplace()
+        eng.run(7)
+        eng.set_fused_inplace(False)
+        eng.run(3)
+        assert_state_bitexact(fetch_state(eng, st), want)
+
+
+def test_autotune_fused_keeps_results_and_state(gpu):
+    """The fused tuner times every candidate on the handle's own state and
+    restores it; the run after tuning continues from where it was."""
+    st = synthetic_state(120, 90, seed=92)
+    lib, p = oracle_params(st, 12)
+    want = st.copy()
+    lib.run(p, want, 12)
+    with make_engine(st, 12, "fused") as eng:
+        eng.run(5)
+        t = eng.autotune_fused(nsub=4)
+        _log(f"autotune_fused: {t}")
+        assert t["ntiles"] > 0 and t["us"] > 0 and (t["tile_w"], t["tile_h"], t["block"]) in \
+            [(64, 16, 512), (32, 16, 512), (32, 8, 256), (16, 8, 128)]
+        eng.run(7)
         assert_state_bitexact(fetch_state(eng, st), want)
 
 

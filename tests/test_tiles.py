@@ -7,7 +7,10 @@ kernels' ranges stay valid; (2) every active T point is owned (stress stored)
 by exactly one tile and every active U point is stepped by exactly one tile;
 (3) each stepu read goes to an LDS entry that holds the right slot: its own,
 nw, sse and sw neighbour (bench_v2b.F90:526-531), evaluated by the tile when
-that neighbour is an active T point, read from memory otherwise.
+that neighbour is an active T point, read from memory otherwise; (4) the
+owned points that other tiles evaluate as halo ("shared") lead their tile's
+owned ranges and carry consecutive ids in slot order, since during a run
+their stresses live in a table indexed by those ids.
 """
 import numpy as np
 import pytest
@@ -52,7 +55,13 @@ def check_plan(plan, ix, skipT, skipU, tile_w, tile_h):
     owned = np.zeros(na, np.int32)
     stepped = np.zeros(na, np.int32)
     nw, sse, sw = slot(ix["nw"]), slot(ix["sse"]), slot(ix["sw"])
-    for a0, a1, b0, b1, c0, c1, h0, nh, k0, nk in plan["tiles"]:
+    shared = np.unique(plan["list"])                       # id -> slot
+    sid = np.full(na, -1, np.int64)
+    sid[shared] = np.arange(len(shared))
+    for a0, a1, b0, b1, c0, c1, h0, nh, k0, nk, sa, ia, sb, ib in plan["tiles"]:
+        for lo, hi, ns, i0 in ((a0, a1, sa, ia), (b0, b1, sb, ib)):
+            want = np.where(np.arange(lo, hi) < lo + ns, i0 + np.arange(hi - lo), -1)
+            assert np.array_equal(sid[lo:hi], want), "shared points do not lead their range"
         assert 0 <= a0 <= a1 <= cnt[0] and cnt[0] <= b0 <= b1 <= t_end and t_end <= c0 <= c1 <= u_end
         halo = plan["list"][h0:h0 + nh]
         kon = plan["klist"][k0:k0 + nk]
@@ -147,6 +156,20 @@ def test_checker_catches_a_planted_mistake():
     bad = dict(plan)
     bad["uloc"] = plan["uloc"].copy()
     bad["uloc"][:, [1, 2]] = bad["uloc"][:, [2, 1]]
+    with pytest.raises(AssertionError):
+        check_plan(bad, ix, skipT, skipU, 8, 8)
+
+
+def test_checker_catches_a_wrong_shared_prefix():
+    """A tile whose shared prefix is one point short must fail."""
+    ix, skipT, skipU = _lists(40, 30, 12, 0.9, pT=1.0)
+    plan = evp1d.tile_plan(ix, skipT, skipU, ix["navel"], 8, 8)
+    check_plan(plan, ix, skipT, skipU, 8, 8)
+    bad = dict(plan)
+    bad["tiles"] = plan["tiles"].copy()
+    t = int(np.argmax(bad["tiles"][:, 12]))
+    assert bad["tiles"][t, 12] > 0
+    bad["tiles"][t, 12] -= 1
     with pytest.raises(AssertionError):
         check_plan(bad, ix, skipT, skipU, 8, 8)
 
