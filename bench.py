@@ -208,10 +208,16 @@ def time_steps(eng, ndte: int, steps: int, warmup: int) -> float:
     return max(wall, ev_ms / 1e3)
 
 
-def adjoint_measure(s: dict, ndte: int, t_fwd_step: float, nsnap: int, ndir: int, ndot: int) -> dict:
+def adjoint_measure(s: dict, ndte: int, t_fwd_step: float, nsnap: int, ndir: int, ndot: int,
+                    fused_cfg: dict | None = None) -> dict:
     """Gradient of L = 1/2 sum_{skipU=0} |u_N - u*|^2 w.r.t. strength,
-    cdn_ocn, forcex, forcey; u* from a run with strength x 1.2."""
+    cdn_ocn, forcex, forcey; u* from a run with strength x 1.2.  fused_cfg:
+    the forward run's tiling and stress storage, used by the trajectory's
+    forward sweep too."""
     eng = new_engine(s, ndte)
+    if fused_cfg:
+        eng.set_tiling(fused_cfg["tile_w"], fused_cfg["tile_h"], fused_cfg["block"], fused_cfg["xcd"])
+        eng.set_fused_inplace(fused_cfg["inplace"])
     reset_state(eng, s, {"strength": 1.2 * s["fields"]["strength"]})
     eng.run(ndte)
     ut = eng.get_field("uvel")[:s["na"]]
@@ -360,6 +366,11 @@ def main() -> int:
                     help="only measure an N-member ensemble, batched vs sequential")
     ap.add_argument("--autotune", type=int, default=0, metavar="REPS",
                     help="run evp1d_autotune with REPS launches per configuration before timing")
+    ap.add_argument("--tune-fused", type=int, default=100, metavar="NSUB",
+                    help="before timing the fused form, run evp1d_autotune_fused with NSUB subcycles per "
+                         "candidate (tile ladder x stress storage); 0: keep the defaults or --tiling/--inplace")
+    ap.add_argument("--inplace", type=int, default=None, choices=(0, 1),
+                    help="fused stress storage: 1 in place, 0 two buffers (implies --tune-fused 0)")
     ap.add_argument("--tuning", default=None, metavar="SB,SX,UB,UX",
                     help="fixed launch tuning: stress block, stress xcd order, stepu block, stepu xcd order")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -392,12 +403,20 @@ def main() -> int:
         eng.set_tuning(*[int(x) for x in args.tuning.split(",")])
     if args.tiling:
         eng.set_tiling(*[int(x) for x in args.tiling.split(",")])
+    if args.inplace is not None:
+        eng.set_fused_inplace(args.inplace)
     if args.autotune:
         log(f"[bench] autotune: {eng.autotune(args.autotune)}")
-    tuning = eng.get_tuning()
     tiling = eng.get_tiling()
     # the form evp1d_run takes: auto is the fused form whenever the lists tile (and ndte >= 4)
     fused = args.mode == "fused" or (args.mode == "auto" and tiling["ntiles"] > 0 and ndte >= 4)
+    tuned_fused = None
+    if fused and args.tune_fused > 0 and not args.tiling and args.inplace is None:
+        t0 = time.perf_counter()
+        tuned_fused = eng.autotune_fused(args.tune_fused)
+        log(f"[bench] autotune_fused ({time.perf_counter() - t0:.1f}s, setup): {tuned_fused}")
+    tuning = eng.get_tuning()
+    tiling = dict(eng.get_tiling(), inplace=int(eng.get_fused_inplace()))
     log(f"[bench] mode {args.mode} ({'fused' if fused else 'unfused'}), tiling {tiling}, tuning {tuning}")
     secs = time_steps(eng, ndte, args.steps, args.warmup)
     value = na * ndte * args.steps / secs
@@ -438,7 +457,7 @@ def main() -> int:
     if not args.no_adjoint and not args.fma:
         try:
             adjoint = adjoint_measure(s, ndte, secs / args.steps, args.adjoint_snapshots,
-                                      args.fd_directions, args.dot_directions)
+                                      args.fd_directions, args.dot_directions, tiling if fused else None)
         except Exception as e:   # reported, never fatal for the forward figure
             log(f"[bench] adjoint measurement failed: {e!r}")
             adjoint = {"error": repr(e)[:200]}
@@ -461,6 +480,9 @@ def main() -> int:
         "config": {"workload": f"{args.workload} {w['nx']}x{w['ny']} ndte={ndte}", "nx": w["nx"],
                    "ny": w["ny"], "seed": w["seed"], "na": na, "navel": s["navel"], "nT": nT, "nU": nU,
                    "ndte": ndte, "mode": args.mode, "form": "fused" if fused else "unfused", "tiling": tiling,
+                   "tiling_chosen_by": ("evp1d_autotune_fused (setup, untimed)" if tuned_fused else
+                                        "--tiling/--inplace" if (args.tiling or args.inplace is not None)
+                                        else "library default"),
                    "build": "fma" if args.fma else "exact", "tuning": tuning,
                    "us_per_subcycle": round(step_us, 3), "loop_GBps_algorithmic": round(whole, 1),
                    "loop_frac_of_peak": round(whole / HBM_PEAK_GBS, 4)},
