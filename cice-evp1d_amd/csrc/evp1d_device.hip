@@ -444,7 +444,20 @@ struct AdjView {
   double2 *g;             // [navel] uarear * adjoint of (cc1, cc2); zero off the stepu range
   double2 *vc;            // [4][na] stress-adjoint velocity contributions: cc, ee, se, ne
   double *gstr, *gcw, *gfx, *gfy;   // [na] gradient accumulators
+  const int *cnz;         // nonzero when str1..8 at some slot >= t_end (never written) has a bit set
 };
+
+// *cnz (zero on entry) becomes 1 if str1..8 at any slot in [t_end, navel)
+// has a bit set; those slots' values never change during a run
+__global__ void k_const_flag(DevView d, int *cnz) {
+  const long j = d.t_end + (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= d.navel) return;
+  const double2 v[4] = {d.s15[j], d.s27[j], d.s36[j], d.s48[j]};
+  unsigned long long bits = 0;
+  for (int c = 0; c < 4; ++c)
+    bits |= (unsigned long long)__double_as_longlong(v[c].x) | (unsigned long long)__double_as_longlong(v[c].y);
+  if (bits) *cnz = 1;   // every writer stores the same value
+}
 
 // str1..8 of subcycle k from the state before it (stresses are not stored)
 __global__ __launch_bounds__(256) void k_str_recompute(DevView d) {
@@ -485,10 +498,15 @@ __global__ __launch_bounds__(256) void k_stepu_adj(DevView d, AdjView a) {
   if (jsw < d.t_end) { const double2 t = a.vc[3 * na + jsw]; ub.x += t.x; ub.y += t.y; }
   if (js < d.t_end) { const double2 t = a.vc[2 * na + js]; ub.x += t.x; ub.y += t.y; }
   const double2 v = L::vel(d, iw);
-  const double2 s15 = L::pair(d.s15, d.s15c, d, iw);
-  const double2 s27 = L::pair(d.s27, d.s27c, d, jw);
-  const double2 s36 = L::pair(d.s36, d.s36c, d, js);
-  const double2 s48 = L::pair(d.s48, d.s48c, d, jsw);
+  // str1..8 of slots stress never writes are +0.0 unless *cnz: no loads then
+  const bool cz = *a.cnz == 0;
+  auto str = [&](const double2 *p, const double2 *c, int j) {
+    return (cz && j >= d.t_end) ? make_double2(0.0, 0.0) : L::pair(p, c, d, j);
+  };
+  const double2 s15 = str(d.s15, d.s15c, iw);
+  const double2 s27 = str(d.s27, d.s27c, jw);
+  const double2 s36 = str(d.s36, d.s36c, js);
+  const double2 s48 = str(d.s48, d.s48c, jsw);
   evp1d::StepuAdj o;
   evp1d::stepu_adjoint(d.s, v.x, v.y, d.cw[iw], d.aiu[iw], d.uocn[iw], d.vocn[iw], d.waterx[iw],
                        d.watery[iw], d.forcex[iw], d.forcey[iw], d.umassdti[iw], d.fm[iw], d.uarear[iw],
@@ -2242,6 +2260,14 @@ int evp1d_adjoint_run(evp1d_handle *h, int32_t nsub, int32_t nsnap, const double
   RC_TRY(R.alloc((void **)&R.ut, nA));
   RC_TRY(R.alloc((void **)&R.vt, nA));
   RC_TRY(R.alloc((void **)&R.terms, nA));
+  int *cnz = nullptr;
+  RC_TRY(R.alloc((void **)&cnz, sizeof(int)));   // zeroed
+  R.a.cnz = cnz;
+  if (h->navel > h->v.t_end) {
+    hipLaunchKernelGGL(k_const_flag, dim3((unsigned)((h->navel - h->v.t_end + 255) / 256)), dim3(256), 0, h->stream,
+                       h->v, cnz);
+    HIP_TRY(hipGetLastError());
+  }
   RC_TRY(upload_caller_order(h, R.ut, u_target));
   RC_TRY(upload_caller_order(h, R.vt, v_target));
   if (nsnap <= 0) {
