@@ -93,24 +93,19 @@ __host__ __device__ inline void visc_replpress(const Scalars &s, R strength, R D
 }
 
 // stress for one T point (bench_v2b.F90:129-310, strain_rates 321-397).
-// sig_in: the 12 stresses at the start of the subcycle in the order
-// stressp_1..4, stressm_1..4, stress12_1..4.
+// Stress at one T point from its metric terms (the 2D v0 kernel's inputs
+// cxp, cyp, cxm, cym, DminTarea, dxhy, dyhx, bench_v0.F90:62-331; the 1D
+// kernel computes them first, stress_point below).  sig_in: the 12 stresses
+// at the start of the subcycle in the order stressp_1..4, stressm_1..4,
+// stress12_1..4.  Velocities: cc = (i,j), ee = (i-1,j), se = (i,j-1),
+// ne = (i-1,j-1).
 template <class R>
-__host__ __device__ inline void stress_point(const Scalars &s,
+__host__ __device__ inline void stress_core(const Scalars &s,
     R uvel_cc, R vvel_cc, R uvel_ee, R vvel_ee,
     R uvel_se, R vvel_se, R uvel_ne, R vvel_ne,
     R dxT, R dyT, R strength,
-    R hte, R htn, R htem1, R htnm1,
+    R cxp, R cyp, R cxm, R cym, R DminTarea, R dxhy, R dyhx,
     const R *sig_in, StressOutT<R> &o) {
-  // metric terms (bench_v2b.F90:139-147)
-  const R cxp = c1p5 * htn - p5 * htnm1;
-  const R cyp = c1p5 * hte - p5 * htem1;
-  const R cxm = -(c1p5 * htnm1 - p5 * htn);
-  const R cym = -(c1p5 * htem1 - p5 * hte);
-  const R tmparea = dxT * dyT;
-  const R DminTarea = s.deltaminEVP * tmparea;
-  const R dxhy = p5 * (hte - htem1);
-  const R dyhx = p5 * (htn - htnm1);
 
   // strain rates (bench_v2b.F90:363-390)
   const R divune = cyp * uvel_cc - dyT * uvel_ee + cxp * vvel_cc - dxT * vvel_se;
@@ -223,6 +218,27 @@ __host__ __device__ inline void stress_point(const Scalars &s,
   strm_tmp = p25 * dxT * (p333 * ssigmw + p166 * ssigme);
   o.str[6] = -strp_tmp + strm_tmp + str12ns - dyhx * (csigpnw + csigmnw) + dxhy * csig12nw;
   o.str[7] = strp_tmp - strm_tmp + str12sn - dyhx * (csigpsw + csigmsw) + dxhy * csig12sw;
+}
+
+// Stress at one point of the 1D list (bench_v2b.F90:125-311): the metric
+// terms from HTE, HTN and their i-1 / j-1 neighbours (139-147), then the core.
+template <class R>
+__host__ __device__ inline void stress_point(const Scalars &s,
+    R uvel_cc, R vvel_cc, R uvel_ee, R vvel_ee,
+    R uvel_se, R vvel_se, R uvel_ne, R vvel_ne,
+    R dxT, R dyT, R strength,
+    R hte, R htn, R htem1, R htnm1,
+    const R *sig_in, StressOutT<R> &o) {
+  const R cxp = c1p5 * htn - p5 * htnm1;
+  const R cyp = c1p5 * hte - p5 * htem1;
+  const R cxm = -(c1p5 * htnm1 - p5 * htn);
+  const R cym = -(c1p5 * htem1 - p5 * hte);
+  const R tmparea = dxT * dyT;
+  const R DminTarea = s.deltaminEVP * tmparea;
+  const R dxhy = p5 * (hte - htem1);
+  const R dyhx = p5 * (htn - htnm1);
+  stress_core(s, uvel_cc, vvel_cc, uvel_ee, vvel_ee, uvel_se, vvel_se, uvel_ne, vvel_ne, dxT, dyT, strength,
+              cxp, cyp, cxm, cym, DminTarea, dxhy, dyhx, sig_in, o);
 }
 
 template <class R>

@@ -58,6 +58,10 @@ EXPORTS = (
     "evp1d_set_tiling", "evp1d_get_tiling", "evp1d_tile_plan",
     "evp1d_set_fused_inplace", "evp1d_get_fused_inplace", "evp1d_autotune_fused",
     "evp1d_set_grid", "evp1d_put_2d", "evp1d_get_2d", "evp1d_host_alloc", "evp1d_host_free",
+    "evp2d_create", "evp2d_destroy", "evp2d_set_params", "evp2d_set_cells", "evp2d_set_halo",
+    "evp2d_set_field", "evp2d_get_field", "evp2d_run", "evp2d_stress", "evp2d_stepu", "evp2d_halo",
+    "evp2d_run_individual", "evp2d_synchronize", "evp2d_timer_start", "evp2d_timer_stop",
+    "evp2d_get_cells",
 )
 
 
@@ -150,6 +154,19 @@ def load(fma: bool = False, path: str | os.PathLike | None = None) -> ctypes.CDL
         "evp1d_set_fused_inplace": [_H, i32],
         "evp1d_get_fused_inplace": [_H, _IP],
         "evp1d_autotune_fused": [_H, i32, ctypes.POINTER(ctypes.c_float)],
+        "evp2d_create": [i32, i32, i32, ctypes.POINTER(Params), ctypes.POINTER(_H)],
+        "evp2d_destroy": [_H],
+        "evp2d_set_params": [_H, ctypes.POINTER(Params)],
+        "evp2d_set_cells": [_H, i32, i32, _IP, _IP, i32, _IP, _IP],
+        "evp2d_set_halo": [_H, i32, _IP, _IP, _IP, _IP, i32, _IP, _IP],
+        "evp2d_set_field": [_H, i32, _DP, ctypes.c_int64],
+        "evp2d_get_field": [_H, i32, _DP, ctypes.c_int64],
+        "evp2d_run": [_H, i32], "evp2d_stress": [_H], "evp2d_stepu": [_H], "evp2d_halo": [_H],
+        "evp2d_run_individual": [_H, i32, i32],
+        "evp2d_synchronize": [_H],
+        "evp2d_timer_start": [_H],
+        "evp2d_timer_stop": [_H, ctypes.POINTER(ctypes.c_float)],
+        "evp2d_get_cells": [_H, _IP, _IP],
         "evp1d_tile_plan": [i32, i32] + [_IP] * 8 + [i32, i32, i32] + [_IP] * 5 + [i32, _IP, i32, _IP, _IP, i32,
                                                                                 _IP, _IP],
     }
@@ -490,3 +507,122 @@ class Engine:
         ms = ctypes.c_float()
         _check(self.L, self.L.evp1d_timer_stop(self.h, ctypes.byref(ms)))
         return ms.value
+
+
+# ---------------------------------------------------------------------------
+# the v0 2D block path (evp2d_*)
+# ---------------------------------------------------------------------------
+# include/evp1d.h enum evp2d_field: input_double_2d.bin's order (src/vars.F90:
+# 299-305), then uvel_init, vvel_init, taubx, tauby and the str1..8 scratch
+FIELDS2D = (
+    "strength", "uvel", "vvel", "dxt", "dyt", "dxhy", "dyhx", "cxp", "cyp", "cxm", "cym",
+    "DminTarea", "uarear", "cdn_ocn", "aiX", "uocn", "vocn", "waterx", "watery", "forcex", "forcey",
+    "umassdti", "fm", "strintx", "strinty", "Tbu",
+    "stressp_1", "stressp_2", "stressp_3", "stressp_4", "stressm_1", "stressm_2", "stressm_3", "stressm_4",
+    "stress12_1", "stress12_2", "stress12_3", "stress12_4",
+    "uvel_init", "vvel_init", "taubx", "tauby",
+    "str1", "str2", "str3", "str4", "str5", "str6", "str7", "str8",
+)
+FIELD2D_ID = {n: k for k, n in enumerate(FIELDS2D)}
+
+
+class Engine2D:
+    """One evp2d handle: the reference's v0 arrays (nx_block, ny_block,
+    max_blocks) on the device (src/evp_v0.F90, src/bench_v0.F90)."""
+
+    def __init__(self, nx_block: int, ny_block: int, max_blocks: int, params: Params, fma: bool = False):
+        self.L = load(fma)
+        self.shape = (nx_block, ny_block, max_blocks)
+        self.n = nx_block * ny_block * max_blocks
+        self.h = _H()
+        _check(self.L, self.L.evp2d_create(nx_block, ny_block, max_blocks, ctypes.byref(params),
+                                           ctypes.byref(self.h)))
+
+    def set_params(self, params: Params) -> None:
+        _check(self.L, self.L.evp2d_set_params(self.h, ctypes.byref(params)))
+
+    def set_cells(self, iblk: int, ti, tj, ui, uj) -> None:
+        """1-based block iblk's T cells (ti, tj) and U cells (ui, uj)."""
+        ti, tj, ui, uj = (np.ascontiguousarray(a, dtype=np.int32) for a in (ti, tj, ui, uj))
+        _check(self.L, self.L.evp2d_set_cells(self.h, iblk, ti.size, _ip(ti), _ip(tj), ui.size, _ip(ui), _ip(uj)))
+
+    def set_halo(self, ilo, jlo, ihi, jhi, src, dst) -> None:
+        """src, dst: (3, ncopies) arrays of (i, j, block), 1-based."""
+        lo = [np.ascontiguousarray(a, dtype=np.int32) for a in (ilo, jlo, ihi, jhi)]
+        src = np.asfortranarray(src, dtype=np.int32)
+        dst = np.asfortranarray(dst, dtype=np.int32)
+        s = np.ascontiguousarray(src.T)   # (ncopies, 3): i, j, block per copy
+        d = np.ascontiguousarray(dst.T)
+        _check(self.L, self.L.evp2d_set_halo(self.h, lo[0].size, *[_ip(a) for a in lo], s.shape[0], _ip(s), _ip(d)))
+
+    def set_field(self, name: str, a) -> None:
+        a = np.asfortranarray(a, dtype=np.float64).reshape(-1, order="F")
+        if a.size != self.n:
+            raise ValueError(f"{name}: {a.size} values, need {self.n}")
+        a = np.ascontiguousarray(a)
+        _check(self.L, self.L.evp2d_set_field(self.h, FIELD2D_ID[name], _dp(a), a.size))
+
+    def get_field(self, name: str) -> np.ndarray:
+        out = np.zeros(self.n)
+        _check(self.L, self.L.evp2d_get_field(self.h, FIELD2D_ID[name], _dp(out), out.size))
+        return out.reshape(self.shape, order="F")
+
+    def load_state(self, st) -> None:
+        """Fields, cell lists and halo tables of an oracle.State2D-like object
+        (attributes f, icellt, icellu, indxti.., halo)."""
+        for n in FIELDS2D[:42]:
+            self.set_field(n, st.f[n])
+        for b in range(self.shape[2]):
+            nt, nu = int(st.icellt[b]), int(st.icellu[b])
+            self.set_cells(b + 1, st.indxti[:nt, b], st.indxtj[:nt, b], st.indxui[:nu, b], st.indxuj[:nu, b])
+        if st.halo is not None:
+            h = st.halo
+            self.set_halo(h["ilo"], h["jlo"], h["ihi"], h["jhi"], h["src"], h["dst"])
+
+    def run(self, nsub: int) -> None:
+        _check(self.L, self.L.evp2d_run(self.h, nsub))
+
+    def stress(self) -> None:
+        _check(self.L, self.L.evp2d_stress(self.h))
+
+    def stepu(self) -> None:
+        _check(self.L, self.L.evp2d_stepu(self.h))
+
+    def halo(self) -> None:
+        _check(self.L, self.L.evp2d_halo(self.h))
+
+    def run_individual(self, which: str, nsub: int) -> None:
+        _check(self.L, self.L.evp2d_run_individual(self.h, {"stress": 0, "stepu": 1}[which], nsub))
+
+    def synchronize(self) -> None:
+        _check(self.L, self.L.evp2d_synchronize(self.h))
+
+    def timer_start(self) -> None:
+        _check(self.L, self.L.evp2d_timer_start(self.h))
+
+    def timer_stop(self) -> float:
+        ms = ctypes.c_float()
+        _check(self.L, self.L.evp2d_timer_stop(self.h, ctypes.byref(ms)))
+        return ms.value
+
+    def cells(self) -> tuple[int, int]:
+        a, b = ctypes.c_int32(), ctypes.c_int32()
+        _check(self.L, self.L.evp2d_get_cells(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.L.evp2d_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -41,7 +41,7 @@ FLAVOURS = {
     "libevp1d.so": ["-ffp-contract=off"],
     "libevp1d_fma.so": ["-ffp-contract=fast", "-DEVP1D_FMA=1"],
 }
-HIP_SOURCES = ["evp1d_device.hip", "evp1d_index.cpp", "evp1d_tiles.cpp"]
+HIP_SOURCES = ["evp1d_device.hip", "evp2d_device.hip", "evp1d_index.cpp", "evp1d_tiles.cpp"]
 
 FORTRAN_MODULES = ["evp1d_capi.F90", "cice_standins.F90", "ice_dyn_evp1d.F90"]
 FORTRAN_PROGRAMS = {

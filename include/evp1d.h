@@ -320,6 +320,74 @@ int evp1d_index_build(int32_t nx, int32_t ny, int32_t nghost, const int32_t *tma
                       int32_t *indxtij, int32_t *ee, int32_t *ne, int32_t *se,
                       int32_t *nw, int32_t *sw, int32_t *sse);
 
+/* ---- the v0 2D block path --------------------------------------------------
+ * The reference's v0 benchmark keeps CICE's own layout: every field is an
+ * (nx_block, ny_block, max_blocks) array in Fortran order, and each block has
+ * a list of T cells for stress and of U cells for stepu (src/evp_v0.F90:93-136,
+ * src/bench_v0.F90:62-82 stress, 469-486 stepu).  An evp2d handle holds those
+ * arrays on the device and runs the same subcycles; results are bit-identical
+ * to the reference v0 (and, on the same grid, to the 1D path).
+ *
+ * Fields, in the order of input_double_2d.bin (src/vars.F90:299-305) and then
+ * the arrays the v0 driver keeps besides; str1..8 are stress's scratch (the
+ * reference clears them in every stress call, bench_v0.F90:135), so they can
+ * be read but not set.                                                      */
+enum evp2d_field {
+  EVP2D_STRENGTH = 0, EVP2D_UVEL, EVP2D_VVEL, EVP2D_DXT, EVP2D_DYT, EVP2D_DXHY, EVP2D_DYHX,
+  EVP2D_CXP, EVP2D_CYP, EVP2D_CXM, EVP2D_CYM, EVP2D_DMINTAREA, EVP2D_UAREAR, EVP2D_CDN_OCN,
+  EVP2D_AIX, EVP2D_UOCN, EVP2D_VOCN, EVP2D_WATERX, EVP2D_WATERY, EVP2D_FORCEX, EVP2D_FORCEY,
+  EVP2D_UMASSDTI, EVP2D_FM, EVP2D_STRINTX, EVP2D_STRINTY, EVP2D_TBU,
+  EVP2D_STRESSP_1, EVP2D_STRESSP_2, EVP2D_STRESSP_3, EVP2D_STRESSP_4,
+  EVP2D_STRESSM_1, EVP2D_STRESSM_2, EVP2D_STRESSM_3, EVP2D_STRESSM_4,
+  EVP2D_STRESS12_1, EVP2D_STRESS12_2, EVP2D_STRESS12_3, EVP2D_STRESS12_4,
+  EVP2D_UVEL_INIT, EVP2D_VVEL_INIT, EVP2D_TAUBX, EVP2D_TAUBY,
+  EVP2D_STR1, EVP2D_STR2, EVP2D_STR3, EVP2D_STR4, EVP2D_STR5, EVP2D_STR6, EVP2D_STR7, EVP2D_STR8,
+  EVP2D_NFIELDS
+};
+
+typedef struct evp2d_handle evp2d_handle;
+
+/* alloc_2d_v0 (src/vars.F90:222-278): arrays of nx_block x ny_block x
+ * max_blocks on the device, all zero; params as for evp1d_create.           */
+int evp2d_create(int32_t nx_block, int32_t ny_block, int32_t max_blocks, const evp1d_params *p,
+                 evp2d_handle **out);
+int evp2d_destroy(evp2d_handle *h);
+int evp2d_set_params(evp2d_handle *h, const evp1d_params *p);
+/* The icellt/indxti/indxtj and icellu/indxui/indxuj arguments of stress and
+ * stepu for block iblk (1-based; bench_v0.F90:62-76, 469-482), 1-based cell
+ * coordinates.  T cells need 2 <= i <= nx_block, 2 <= j <= ny_block (stress
+ * reads i-1, j-1), U cells i < nx_block, j < ny_block (stepu reads i+1,
+ * j+1); a cell listed twice is rejected (EVP1D_EINDEX).                      */
+int evp2d_set_cells(evp2d_handle *h, int32_t iblk, int32_t icellt, const int32_t *indxti,
+                    const int32_t *indxtj, int32_t icellu, const int32_t *indxui, const int32_t *indxuj);
+/* readin_halo's tables (src/vars.F90:80-99): per local block the interior
+ * bounds ilo, jlo, ihi, jhi, and ncopies copies src[3k..3k+2] = (i, j, block)
+ * -> dst[3k..3k+2]; source block 0 means fill with 0.  Applied to uvel and
+ * vvel after every subcycle when max_blocks > 1 (ice_haloUpdate, 100-143):
+ * the halo rows and columns of each block are set to 0, then the copies run.
+ * A copy whose source cell is written by another copy (whose result would
+ * depend on the list order) is rejected (EVP1D_EINVAL).                     */
+int evp2d_set_halo(evp2d_handle *h, int32_t nblocks, const int32_t *ilo, const int32_t *jlo,
+                   const int32_t *ihi, const int32_t *jhi, int32_t ncopies, const int32_t *src,
+                   const int32_t *dst);
+/* Whole arrays, n = nx_block * ny_block * max_blocks values, Fortran order. */
+int evp2d_set_field(evp2d_handle *h, int32_t field, const double *a, int64_t n);
+int evp2d_get_field(evp2d_handle *h, int32_t field, double *a, int64_t n);
+/* nsub subcycles: stress and stepu of every block, then the halo update when
+ * max_blocks > 1 and tables are set (src/evp_v0.F90:93-136); a captured
+ * hipGraph, asynchronous.                                                   */
+int evp2d_run(evp2d_handle *h, int32_t nsub);
+int evp2d_stress(evp2d_handle *h);
+int evp2d_stepu(evp2d_handle *h);
+int evp2d_halo(evp2d_handle *h);
+/* the lindividual loops of evp_v0.F90:157-204: which = 0 stress, 1 stepu   */
+int evp2d_run_individual(evp2d_handle *h, int32_t which, int32_t nsub);
+int evp2d_synchronize(evp2d_handle *h);
+int evp2d_timer_start(evp2d_handle *h);
+int evp2d_timer_stop(evp2d_handle *h, float *ms);
+/* total T and U cells over all blocks                                       */
+int evp2d_get_cells(evp2d_handle *h, int32_t *nt, int32_t *nu);
+
 /* Copy the message of the last failure on this thread into buf (always NUL
  * terminated); returns the code of that failure (0 if none).                */
 int evp1d_last_error(char *buf, int32_t n);

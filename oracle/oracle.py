@@ -384,3 +384,266 @@ def run_reference(d: Path, binary: str = "evp_v2b_f0", threads: int = 1, timeout
         env.update(env_extra)
     return subprocess.run([str(exe)], cwd=str(d), env=env, capture_output=True, text=True,
                           timeout=timeout, check=True)
+
+
+# ---------------------------------------------------------------------------
+# v0 2D block path (src/bench_v0.F90, src/evp_v0.F90, src/vars.F90)
+# ---------------------------------------------------------------------------
+# Arrays of input_double_2d.bin in readin_2d's order (vars.F90:299-305), then
+# the fields the driver derives or writes itself.
+V0_FILE_FIELDS = (
+    "strength", "uvel", "vvel", "dxt", "dyt", "dxhy", "dyhx", "cxp", "cyp", "cxm", "cym",
+    "DminTarea", "uarear", "cdn_ocn", "aiX", "uocn", "vocn", "waterx", "watery", "forcex", "forcey",
+    "umassdti", "fm", "strintx", "strinty", "Tbu") + STRESS_FIELDS
+V0_FIELDS = V0_FILE_FIELDS + ("uvel_init", "vvel_init", "taubx", "tauby")
+
+
+class State2D:
+    """The v0 arrays: each field (nx_block, ny_block, nblocks) in Fortran
+    order; cell lists (nx_block*ny_block, nblocks), 1-based, zero padded;
+    optional halo description (readin_halo, vars.F90:80-99)."""
+
+    def __init__(self, nxb: int, nyb: int, nblocks: int):
+        self.nxb, self.nyb, self.nblocks = nxb, nyb, nblocks
+        shape = (nxb, nyb, nblocks)
+        self.f = {n: np.zeros(shape, order="F") for n in V0_FIELDS}
+        m = (nxb * nyb, nblocks)
+        self.indxti, self.indxtj, self.indxui, self.indxuj = (np.zeros(m, np.int32, order="F")
+                                                              for _ in range(4))
+        self.icellt = np.zeros(nblocks, np.int32)
+        self.icellu = np.zeros(nblocks, np.int32)
+        self.navel = 0
+        self.scalars = dict(capping=1.0, e_factor=0.25, epp2i=0.25)
+        self.halo = None   # dict(ilo, jlo, ihi, jhi, src (3, n), dst (3, n))
+
+    def copy(self) -> "State2D":
+        s = State2D(self.nxb, self.nyb, self.nblocks)
+        s.f = {k: v.copy(order="F") for k, v in self.f.items()}
+        for n in ("indxti", "indxtj", "indxui", "indxuj", "icellt", "icellu"):
+            setattr(s, n, getattr(self, n).copy(order="F"))
+        s.navel, s.scalars = self.navel, dict(self.scalars)
+        s.halo = None if self.halo is None else {k: v.copy() for k, v in self.halo.items()}
+        return s
+
+
+def _metrics_v0(F: dict, deltaminEVP: float) -> dict:
+    """dxhy, dyhx, cxp, cyp, cxm, cym, DminTarea on the 2D grid with the same
+    expressions and association as the v2 kernel computes them per point
+    from HTE, HTN and their i-1 / j-1 neighbours (bench_v2b.F90:139-147);
+    zero on the first row and column, which have no such neighbours."""
+    hte, htn = F["HTE"], F["HTN"]
+    htem1 = np.zeros_like(hte)
+    htem1[1:, :] = hte[:-1, :]
+    htnm1 = np.zeros_like(htn)
+    htnm1[:, 1:] = htn[:, :-1]
+    out = dict(
+        cxp=1.5 * htn - 0.5 * htnm1, cyp=1.5 * hte - 0.5 * htem1,
+        cxm=-(1.5 * htnm1 - 0.5 * htn), cym=-(1.5 * htem1 - 0.5 * hte),
+        dxhy=0.5 * (hte - htem1), dyhx=0.5 * (htn - htnm1),
+        DminTarea=deltaminEVP * (F["dxT"] * F["dyT"]))
+    for v in out.values():
+        v[0, :] = 0.0
+        v[:, 0] = 0.0
+    return out
+
+
+_V0_FROM_GRID = dict(strength="strength", uvel="uvel", vvel="vvel", dxt="dxT", dyt="dyT", uarear="uarear",
+                     cdn_ocn="cdn_ocn", aiX="aiu", uocn="uocn", vocn="vocn", waterx="waterxU",
+                     watery="wateryU", forcex="forcexU", forcey="forceyU", umassdti="umassdti", fm="fmU",
+                     strintx="strintxU", strinty="strintyU", Tbu="Tbu")
+
+
+def pack_v0(grid, nblocks: int = 1, deltaminEVP: float = 1e-8) -> State2D:
+    """v0 arrays for a synth.Grid2D (nghost = 1).
+
+    nblocks = 1: one block, the whole grid, lists over the interior in CICE
+    order (j outer, i inner) of icetmask / iceumask cells.  nblocks > 1: the
+    interior is split in x into nearly equal column bands; block b holds its
+    band plus a one-cell halo, right-padded to the widest band.  Its T list
+    spans its band and its east halo column (stepu at the band's east edge
+    reads str there), its U list its band; the halo copy list fills each
+    interior-facing halo column from the neighbour band after every subcycle
+    and the outer ones from nothing (fill)."""
+    if grid.nghost != 1:
+        raise ValueError("pack_v0 needs nghost = 1")
+    F = dict(grid.fields)
+    F.update(_metrics_v0(grid.fields, deltaminEVP))
+    NX, NY, nx = grid.NX, grid.NY, grid.nx
+    edges = np.linspace(0, nx, nblocks + 1).round().astype(int)   # interior columns [e_b, e_b+1)
+    widths = np.diff(edges)
+    if widths.min() < 1:
+        raise ValueError("more blocks than interior columns")
+    nxb = int(widths.max()) + 2
+    st = State2D(nxb, NY, nblocks)
+    for b in range(nblocks):
+        g0 = edges[b]                       # global 0-based column of block column 0 (the west halo)
+        w = int(widths[b])
+        cols = np.arange(g0, g0 + w + 2)    # west halo .. east halo, global 0-based
+        for n in V0_FILE_FIELDS:
+            src = F[_V0_FROM_GRID.get(n, n)] if n not in STRESS_FIELDS else F[n]
+            st.f[n][:w + 2, :, b] = src[cols, :]
+        ihi = w + 1                          # 1-based interior [2, ihi]
+        ti, tj, ui, uj = [], [], [], []
+        t_hi = ihi + 1 if nblocks > 1 and b < nblocks - 1 else ihi
+        for j in range(2, NY):
+            for i in range(2, t_hi + 1):
+                if grid.icetmask[cols[i - 1], j - 1]:
+                    ti.append(i)
+                    tj.append(j)
+            for i in range(2, ihi + 1):
+                if grid.iceumask[cols[i - 1], j - 1]:
+                    ui.append(i)
+                    uj.append(j)
+        st.icellt[b], st.icellu[b] = len(ti), len(ui)
+        st.indxti[:len(ti), b], st.indxtj[:len(tj), b] = ti, tj
+        st.indxui[:len(ui), b], st.indxuj[:len(uj), b] = ui, uj
+    st.f["uvel_init"][:] = st.f["uvel"]
+    st.f["vvel_init"][:] = st.f["vvel"]
+    st.navel = int(np.count_nonzero(grid.tmask))
+    if nblocks > 1:
+        ilo = np.full(nblocks, 2, np.int32)
+        jlo = np.full(nblocks, 2, np.int32)
+        ihi = (widths + 1).astype(np.int32)
+        jhi = np.full(nblocks, NY - 1, np.int32)
+        src, dst = [], []
+        for b in range(nblocks):
+            for j in range(2, NY):
+                # west halo column 1 <- east interior column of block b-1
+                src.append((int(ihi[b - 1]), j, b) if b > 0 else (0, 0, 0))
+                dst.append((1, j, b + 1))
+                # east halo column ihi+1 <- west interior column of block b+1
+                src.append((2, j, b + 2) if b < nblocks - 1 else (0, 0, 0))
+                dst.append((int(ihi[b]) + 1, j, b + 1))
+        st.halo = dict(ilo=ilo, jlo=jlo, ihi=ihi, jhi=jhi,
+                       src=np.asfortranarray(np.array(src, np.int32).T),
+                       dst=np.asfortranarray(np.array(dst, np.int32).T))
+    return st
+
+
+def v0_to_global(st: State2D, grid, name: str) -> np.ndarray:
+    """The interior of every block of one field, put back on the (NX, NY) grid
+    (rim from block data where the single-block layout has it)."""
+    if st.nblocks == 1:
+        return st.f[name][:, :, 0].copy(order="F")
+    edges = np.linspace(0, grid.nx, st.nblocks + 1).round().astype(int)
+    out = np.zeros((grid.NX, grid.NY), order="F")
+    for b in range(st.nblocks):
+        w = int(edges[b + 1] - edges[b])
+        out[edges[b] + 1:edges[b] + 1 + w, :] = st.f[name][1:w + 1, :, b]
+    return out
+
+
+class OracleV0:
+    """ctypes wrapper of the v0 restatement (evp2d_oracle.c) in liboracle.so."""
+
+    def __init__(self):
+        self.o = OracleLib()
+        self.lib = self.o.lib
+
+    def params(self, ndte: int, **override) -> Params:
+        return self.o.params(ndte, **override)
+
+    def run(self, p: Params, st: State2D, nsub: int) -> None:
+        for n in V0_FIELDS:
+            a = st.f[n]
+            assert a.dtype == np.float64 and a.flags.f_contiguous, n
+        fptr = (_DP * len(V0_FIELDS))(*[_dp(st.f[n]) for n in V0_FIELDS])
+        scratch = np.zeros(8 * st.nxb * st.nyb)
+        lists = [np.asfortranarray(getattr(st, n), dtype=np.int32) for n in ("indxti", "indxtj", "indxui", "indxuj")]
+        if st.halo is None:
+            nhb, lo = 0, [np.zeros(1, np.int32)] * 4
+            nc, src, dst = 0, np.zeros(3, np.int32), np.zeros(3, np.int32)
+        else:
+            h = st.halo
+            nhb = len(h["ilo"])
+            lo = [np.ascontiguousarray(h[k], dtype=np.int32) for k in ("ilo", "jlo", "ihi", "jhi")]
+            nc = h["src"].shape[1]
+            src = np.asfortranarray(h["src"], dtype=np.int32)
+            dst = np.asfortranarray(h["dst"], dtype=np.int32)
+        self.lib.oracle_v0_run(ctypes.byref(p), st.nxb, st.nyb, st.nblocks, nsub, fptr,
+                               _ip(np.ascontiguousarray(st.icellt)), _ip(np.ascontiguousarray(st.icellu)),
+                               *[_ip(a) for a in lists], _dp(scratch), nhb, *[_ip(a) for a in lo], nc,
+                               _ip(src), _ip(dst))
+
+
+def params_v0(lib_or_params, st: State2D) -> Params:
+    """set_evp_parameters then readin_2d's three file scalars (vars.F90:305)."""
+    p = lib_or_params
+    for k in ("capping", "e_factor", "epp2i"):
+        setattr(p, k, st.scalars[k])
+    return p
+
+
+def write_inputs_v0(d: Path, st: State2D, ndte: int) -> None:
+    """input_double_2d.bin, input_integer_2d.bin (readin_2d, vars.F90:295-316),
+    with max_blocks > 1 the two halo files of readin_halo (vars.F90:80-99),
+    and kernel.nml (vars.F90:19-69)."""
+    d = Path(d)
+    d.mkdir(parents=True, exist_ok=True)
+    with open(d / "input_double_2d.bin", "wb") as fh:
+        for n in V0_FILE_FIELDS:
+            fh.write(np.asfortranarray(st.f[n]).tobytes(order="F"))
+        s = st.scalars
+        fh.write(np.array([s["capping"], s["e_factor"], s["epp2i"]]).tobytes())
+    with open(d / "input_integer_2d.bin", "wb") as fh:
+        for n in ("indxti", "indxtj", "indxui", "indxuj"):
+            fh.write(np.asfortranarray(getattr(st, n), dtype="<i4").tobytes(order="F"))
+        fh.write(st.icellt.astype("<i4").tobytes())
+        fh.write(st.icellu.astype("<i4").tobytes())
+        fh.write(np.array([st.navel], "<i4").tobytes())
+    if st.nblocks > 1:
+        h = st.halo
+        with open(d / "base_output_integer_blockindices_ref_v0_2d.bin", "wb") as fh:
+            fh.write(np.array([len(h["ilo"])], "<i4").tobytes())
+            for k in ("ilo", "jlo", "ihi", "jhi"):
+                fh.write(np.asarray(h[k], "<i4").tobytes())
+        with open(d / "base_output_integer_halo_ref_v0_2d.bin", "wb") as fh:
+            fh.write(np.array([h["src"].shape[1]], "<i4").tobytes())
+            fh.write(np.asfortranarray(h["src"], dtype="<i4").tobytes(order="F"))
+            fh.write(np.asfortranarray(h["dst"], dtype="<i4").tobytes(order="F"))
+    (d / "kernel.nml").write_text(
+        "&kernel_nml\n"
+        f" ndte = {ndte}\n testscale = 1\n binoutput = .true.\n lindividual = .false.\n"
+        f" nx_block = {st.nxb}\n ny_block = {st.nyb}\n max_blocks = {st.nblocks}\n nghost = 1\n"
+        f" na = {int(st.icellt[0])}\n nb = {int(st.icellu[0])}\n navel = {st.navel}\n/\n")
+
+
+def read_inputs_v0(d: Path, nxb: int, nyb: int, nblocks: int) -> State2D:
+    """Inverse of write_inputs_v0 (halo files included when nblocks > 1)."""
+    d = Path(d)
+    st = State2D(nxb, nyb, nblocks)
+    m = nxb * nyb * nblocks
+    raw = np.fromfile(d / "input_double_2d.bin", dtype="<f8")
+    if raw.size != len(V0_FILE_FIELDS) * m + 3:
+        raise ValueError(f"input_double_2d.bin has {raw.size} doubles, expected {len(V0_FILE_FIELDS) * m + 3}")
+    for k, n in enumerate(V0_FILE_FIELDS):
+        st.f[n][:] = raw[k * m:(k + 1) * m].reshape((nxb, nyb, nblocks), order="F")
+    st.scalars = dict(zip(("capping", "e_factor", "epp2i"), raw[-3:].tolist()))
+    st.f["uvel_init"][:] = st.f["uvel"]
+    st.f["vvel_init"][:] = st.f["vvel"]
+    ints = np.fromfile(d / "input_integer_2d.bin", dtype="<i4")
+    L = nxb * nyb * nblocks
+    for k, n in enumerate(("indxti", "indxtj", "indxui", "indxuj")):
+        setattr(st, n, np.asfortranarray(ints[k * L:(k + 1) * L].reshape((nxb * nyb, nblocks), order="F")))
+    p = 4 * L
+    st.icellt = ints[p:p + nblocks].copy()
+    st.icellu = ints[p + nblocks:p + 2 * nblocks].copy()
+    st.navel = int(ints[p + 2 * nblocks])
+    if nblocks > 1:
+        bi = np.fromfile(d / "base_output_integer_blockindices_ref_v0_2d.bin", dtype="<i4")
+        nb = int(bi[0])
+        lo = bi[1:].reshape(4, nb)
+        hh = np.fromfile(d / "base_output_integer_halo_ref_v0_2d.bin", dtype="<i4")
+        nc = int(hh[0])
+        st.halo = dict(ilo=lo[0].copy(), jlo=lo[1].copy(), ihi=lo[2].copy(), jhi=lo[3].copy(),
+                       src=np.asfortranarray(hh[1:1 + 3 * nc].reshape((3, nc), order="F")),
+                       dst=np.asfortranarray(hh[1 + 3 * nc:1 + 6 * nc].reshape((3, nc), order="F")))
+    return st
+
+
+def output_bytes_v0(st: State2D) -> tuple[bytes, bytes]:
+    """Contents of output_stress_v0_2d.bin and output_stepu_v0_2d.bin
+    (writeout_2d, vars.F90:317-334): whole arrays, all blocks."""
+    stress = b"".join(np.asfortranarray(st.f[n]).tobytes(order="F") for n in STRESS_FIELDS)
+    stepu = b"".join(np.asfortranarray(st.f[n]).tobytes(order="F") for n in ("strintx", "strinty", "uvel", "vvel"))
+    return stress, stepu

@@ -28,7 +28,27 @@ def load_golden(name: str):
 
 
 def golden_cases():
-    return sorted(p.stem for p in GOLD.glob("*.npz"))
+    """The 1D (v2b) golden cases."""
+    return sorted(p.stem for p in GOLD.glob("*.npz") if not p.stem.startswith("v0_"))
+
+
+def golden_cases_v0():
+    """The 2D v0 golden cases (tests/golden/make_golden_v0.py)."""
+    return sorted(p.stem for p in GOLD.glob("v0_*.npz"))
+
+
+def load_golden_v0(case: str):
+    """(State2D, ndte, output_stress bytes, output_stepu bytes) of a v0 case:
+    the reference's own input files, read back with the oracle's reader."""
+    import tempfile
+    z = np.load(GOLD / f"{case}.npz")
+    with tempfile.TemporaryDirectory() as d:
+        for k in z.files:
+            if k.startswith("in_"):
+                name = k[3:][::-1].replace("_", ".", 1)[::-1]   # in_input_double_2d_bin -> input_double_2d.bin
+                (Path(d) / name).write_bytes(z[k].tobytes())
+        st = O.read_inputs_v0(Path(d), int(z["nxb"]), int(z["nyb"]), int(z["nblocks"]))
+    return st, int(z["ndte"]), z["out_stress"].tobytes(), z["out_stepu"].tobytes()
 
 
 def set_mode(eng: evp1d.Engine, mode: str) -> None:

@@ -18,7 +18,7 @@ REPO = Path(__file__).resolve().parent.parent
 def _declared_symbols():
     text = (REPO / "include" / "evp1d.h").read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\bint\s+(evp1d_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\bint\s+(evp[12]d_\w+)\s*\(", text)))
 
 
 @pytest.mark.parametrize("fma", [False, True])
