@@ -47,6 +47,7 @@ FORTRAN_MODULES = ["evp1d_capi.F90", "cice_standins.F90", "ice_dyn_evp1d.F90"]
 FORTRAN_PROGRAMS = {
     "evp1d_v2_driver": "evp1d_v2_driver.F90",
     "evp1d_api_driver": "evp1d_api_driver.F90",
+    "evp1d_v0_driver": "evp1d_v0_driver.F90",
 }
 
 

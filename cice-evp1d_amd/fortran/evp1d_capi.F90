@@ -42,6 +42,15 @@ module evp1d_capi
   integer(c_int32_t), parameter, public :: S_SKIPT = 0, S_SKIPU = 1
   integer(c_int32_t), parameter, public :: MODE_LAUNCH = 0, MODE_GRAPH = 1, &
                                            MODE_PERSISTENT = 2, MODE_AUTO = 3, MODE_FUSED = 4
+  ! fields of the v0 2D block path (enum evp2d_field): input_double_2d.bin's
+  ! order (src/vars.F90:299-305), then the driver's own arrays
+  integer(c_int32_t), parameter, public :: &
+    V_STRENGTH = 0, V_UVEL = 1, V_VVEL = 2, V_DXT = 3, V_DYT = 4, V_DXHY = 5, V_DYHX = 6, &
+    V_CXP = 7, V_CYP = 8, V_CXM = 9, V_CYM = 10, V_DMINTAREA = 11, V_UAREAR = 12, &
+    V_CDN_OCN = 13, V_AIX = 14, V_UOCN = 15, V_VOCN = 16, V_WATERX = 17, V_WATERY = 18, &
+    V_FORCEX = 19, V_FORCEY = 20, V_UMASSDTI = 21, V_FM = 22, V_STRINTX = 23, &
+    V_STRINTY = 24, V_TBU = 25, V_STRESSP_1 = 26, V_STRESS12_4 = 37, V_UVEL_INIT = 38, &
+    V_VVEL_INIT = 39, V_TAUBX = 40, V_TAUBY = 41, V_STR1 = 42, V_STR8 = 49
 
   public :: evp1d_set_evp_parameters, evp1d_create, evp1d_destroy, evp1d_set_params,   &
             evp1d_get_params, evp1d_get_sizes, evp1d_set_field, evp1d_get_field,        &
@@ -54,7 +63,11 @@ module evp1d_capi
             evp1d_set_index_member, evp1d_set_skip_member, evp1d_autotune,              &
             evp1d_set_tuning, evp1d_get_tuning, evp1d_set_tiling, evp1d_get_tiling,       &
             evp1d_set_grid, evp1d_put_2d, evp1d_get_2d, evp1d_host_alloc, evp1d_host_free, &
-            evp1d_set_fused_inplace, evp1d_get_fused_inplace, evp1d_autotune_fused
+            evp1d_set_fused_inplace, evp1d_get_fused_inplace, evp1d_autotune_fused,    &
+            evp2d_create, evp2d_destroy, evp2d_set_params, evp2d_set_cells, evp2d_set_halo, &
+            evp2d_set_field, evp2d_get_field, evp2d_run, evp2d_stress, evp2d_stepu,      &
+            evp2d_halo, evp2d_run_individual, evp2d_synchronize, evp2d_timer_start,    &
+            evp2d_timer_stop, evp2d_get_cells
 
   interface
     integer(c_int) function evp1d_set_evp_parameters(ndte, dt, p) bind(C, name='evp1d_set_evp_parameters')
@@ -292,6 +305,90 @@ module evp1d_capi
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: h
       integer(c_int32_t), intent(out) :: inplace
+    end function
+    ! ---- the v0 2D block path -------------------------------------------
+    integer(c_int) function evp2d_create(nxb, nyb, nblocks, p, h) bind(C, name='evp2d_create')
+      import :: c_int, c_int32_t, c_ptr, evp1d_params_t
+      integer(c_int32_t), value :: nxb, nyb, nblocks
+      type(evp1d_params_t), intent(in) :: p
+      type(c_ptr), intent(out) :: h
+    end function
+    integer(c_int) function evp2d_destroy(h) bind(C, name='evp2d_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function evp2d_set_params(h, p) bind(C, name='evp2d_set_params')
+      import :: c_int, c_ptr, evp1d_params_t
+      type(c_ptr), value :: h
+      type(evp1d_params_t), intent(in) :: p
+    end function
+    integer(c_int) function evp2d_set_cells(h, iblk, icellt, indxti, indxtj, icellu, indxui, indxuj) &
+        bind(C, name='evp2d_set_cells')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: iblk, icellt, icellu
+      integer(c_int32_t), intent(in) :: indxti(*), indxtj(*), indxui(*), indxuj(*)
+    end function
+    integer(c_int) function evp2d_set_halo(h, nblocks, ilo, jlo, ihi, jhi, ncopies, src, dst) &
+        bind(C, name='evp2d_set_halo')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nblocks, ncopies
+      integer(c_int32_t), intent(in) :: ilo(*), jlo(*), ihi(*), jhi(*), src(*), dst(*)
+    end function
+    integer(c_int) function evp2d_set_field(h, field, a, n) bind(C, name='evp2d_set_field')
+      import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: field
+      real(c_double), intent(in) :: a(*)
+      integer(c_int64_t), value :: n
+    end function
+    integer(c_int) function evp2d_get_field(h, field, a, n) bind(C, name='evp2d_get_field')
+      import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: field
+      real(c_double), intent(out) :: a(*)
+      integer(c_int64_t), value :: n
+    end function
+    integer(c_int) function evp2d_run(h, nsub) bind(C, name='evp2d_run')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nsub
+    end function
+    integer(c_int) function evp2d_stress(h) bind(C, name='evp2d_stress')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function evp2d_stepu(h) bind(C, name='evp2d_stepu')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function evp2d_halo(h) bind(C, name='evp2d_halo')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function evp2d_run_individual(h, which, nsub) bind(C, name='evp2d_run_individual')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: which, nsub
+    end function
+    integer(c_int) function evp2d_synchronize(h) bind(C, name='evp2d_synchronize')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function evp2d_timer_start(h) bind(C, name='evp2d_timer_start')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function evp2d_timer_stop(h, ms) bind(C, name='evp2d_timer_stop')
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: h
+      real(c_float), intent(out) :: ms
+    end function
+    integer(c_int) function evp2d_get_cells(h, nt, nu) bind(C, name='evp2d_get_cells')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(out) :: nt, nu
     end function
     integer(c_int) function evp1d_autotune_fused(h, nsub, us) bind(C, name='evp1d_autotune_fused')
       import :: c_int, c_int32_t, c_ptr, c_float

@@ -135,3 +135,44 @@ def test_api_driver_matches_reference_dataflow(gpu, nx, ny, seed, ndte, nrun, de
     for k, n in enumerate(DYN):
         a, b = got[k], want[n]
         assert np.array_equal(a.view(np.uint64), np.asfortranarray(b).view(np.uint64)), n
+
+
+@pytest.mark.parametrize("case", [c for c in __import__("gpu_common").golden_cases_v0()])
+def test_v0_driver_reproduces_reference_files(gpu, case):
+    """evp1d_v0_driver on the reference's own v0 input files: both 2D output
+    files byte-identical to the reference v0's; with lconvert21d the 1D files
+    hold block 1's cells of the same arrays (convert_2d_1d, vars.F90:534-584)."""
+    from gpu_common import load_golden_v0
+    st, ndte, want_stress, want_stepu = load_golden_v0(case)
+    with tempfile.TemporaryDirectory() as d:
+        O.write_inputs_v0(d, st, ndte)
+        nml = (Path(d) / "kernel.nml").read_text().replace("/\n", " lconvert21d = .true.\n/\n")
+        (Path(d) / "kernel.nml").write_text(nml)
+        r = _run(FBIN / "evp1d_v0_driver", d)
+        assert "Benchmark took:" in r.stdout
+        got_stress = (Path(d) / "output_stress_v0_2d.bin").read_bytes()
+        got_stepu = (Path(d) / "output_stepu_v0_2d.bin").read_bytes()
+        s1 = (Path(d) / "output_stress_v0_1d.bin").read_bytes()
+        u1 = (Path(d) / "output_stepu_v0_1d.bin").read_bytes()
+    assert O.md5(got_stress) == O.md5(want_stress)
+    assert O.md5(got_stepu) == O.md5(want_stepu)
+    n = st.nxb * st.nyb * st.nblocks
+    sa = np.frombuffer(want_stress, "<f8").reshape(12, n)
+    ua = np.frombuffer(want_stepu, "<f8").reshape(4, n)
+    nt, nu = int(st.icellt[0]), int(st.icellu[0])
+    tc = (st.indxti[:nt, 0] - 1) + (st.indxtj[:nt, 0] - 1) * st.nxb
+    uc = (st.indxui[:nu, 0] - 1) + (st.indxuj[:nu, 0] - 1) * st.nxb
+    assert s1 == sa[:, tc].tobytes()
+    assert u1 == ua[:, uc].tobytes()
+
+
+def test_v0_driver_evp_c0_md5(gpu):
+    from evp1d import synth
+    rec = json.loads((GOLD / "golden.json").read_text())["evp_c0_v0"]
+    g = synth.make_grid(rec["nx"], rec["ny"], rec["seed"])
+    st = O.pack_v0(g, 1)
+    with tempfile.TemporaryDirectory() as d:
+        O.write_inputs_v0(d, st, rec["ndte"])
+        _run(FBIN / "evp1d_v0_driver", d)
+        assert O.md5((Path(d) / "output_stress_v0_2d.bin").read_bytes()) == rec["output_stress_md5"]
+        assert O.md5((Path(d) / "output_stepu_v0_2d.bin").read_bytes()) == rec["output_stepu_md5"]
