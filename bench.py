@@ -343,6 +343,72 @@ def ensemble_measure(nmem: int, steps: int, autotune: int) -> dict:
     return out
 
 
+def v0_measure(workload: str, steps: int, warmup: int, kernel_reps: int) -> dict:
+    """The v0 2D block path (evp2d_*) on the workload's grid as one block:
+    CICE's (nx_block, ny_block) arrays, per-block cell lists.  Algorithmic
+    bytes per subcycle in the reference's data model: stress 360 B per T cell
+    (indxti/indxtj 8, own uvel/vvel 16, the ten metric/strength arrays 80,
+    12 stresses in 96 and out 96, str 64), stepu 248 B per U cell
+    (indxui/indxuj 8, uvel/vvel 16, 14 forcing/metric arrays 112, str 64,
+    uvel/vvel 16, strintx/y 16, taubx/y 16); the reference's clearing of the
+    whole str scratch in every stress call is not counted."""
+    import evp1d
+    from evp1d import layout, synth
+    w = dict(synth.WORKLOADS[workload])
+    ndte = w.pop("ndte")
+    g = synth.make_grid(**w)
+    blk = layout.Blocks2D(g)
+    nT, nU = int(blk.icellt[0]), int(blk.icellu[0])
+    eng = evp1d.Engine2D(blk.nxb, blk.nyb, 1, evp1d.set_evp_parameters(ndte))
+    eng.load_state(blk)
+    log(f"[bench] v0 path: {blk.nxb}x{blk.nyb} one block, icellt={nT} icellu={nU}")
+    for k in range(warmup):
+        eng.run(ndte)
+        eng.synchronize()
+    eng.synchronize()
+    t0 = time.perf_counter()
+    eng.timer_start()
+    for _ in range(steps):
+        eng.run(ndte)
+    ev_ms = eng.timer_stop()
+    wall = time.perf_counter() - t0
+    secs = max(wall, ev_ms / 1e3)
+    B = {"stress": 360 * nT, "stepu": 248 * nU}
+    kernels = {}
+    for which in ("stress", "stepu"):
+        eng.synchronize()
+        eng.timer_start()
+        eng.run_individual(which, kernel_reps)
+        us = eng.timer_stop() * 1e3 / kernel_reps
+        kernels[which] = {"avg_us": round(us, 3), "bytes": B[which], "GBps": round(B[which] / (us * 1e-6) / 1e9, 1)}
+        log(f"[bench] v0 {which}: {us:.2f} us/launch, {kernels[which]['GBps']} GB/s algorithmic")
+    eng.close()
+    name = max(kernels, key=lambda k: kernels[k]["avg_us"])
+    dom_us = kernels[name]["avg_us"]
+    achieved = B[name] / (dom_us * 1e-6) / 1e9
+    traffic = None
+    tf = REPO / "profiles" / f"traffic_{workload}.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get("k2_" + name)
+    step_us = secs / (steps * ndte) * 1e6
+    sub_bytes = B["stress"] + B["stepu"]
+    return {
+        "metric": "ice-pts*subcycles/s + % HBM roofline (bit-exact FP64), v0 2D block path",
+        "value": nT * ndte * steps / secs, "unit": "ice-T-cells*subcycles/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": secs / steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{workload} {w['nx']}x{w['ny']} ndte={ndte}, v0 2D block path (one block)",
+                   "nx_block": blk.nxb, "ny_block": blk.nyb, "icellt": nT, "icellu": nU, "ndte": ndte,
+                   "us_per_subcycle": round(step_us, 3),
+                   "loop_GBps_algorithmic": round(sub_bytes / (step_us * 1e-6) / 1e9, 1)},
+        "kernels": kernels,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k2_" + name,
+                     "bytes_per_launch": B[name]},
+        "cpu_baseline": None,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -360,6 +426,8 @@ def main() -> int:
     ap.add_argument("--adjoint-snapshots", type=int, default=32)
     ap.add_argument("--fd-directions", type=int, default=3)
     ap.add_argument("--dot-directions", type=int, default=2)
+    ap.add_argument("--path", default="1d", choices=("1d", "v0"),
+                    help="v0: time the v0 2D block path (evp2d_*) on the workload instead of the 1D engine")
     ap.add_argument("--api", action="store_true",
                     help="only time dyn_evp1d_run through the Fortran driver API, device vs host 2D<->1D copies")
     ap.add_argument("--ensemble", type=int, default=0, metavar="N",
@@ -383,6 +451,9 @@ def main() -> int:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design "
                                    "note (DESIGN.md), not a measured path", "n_gpus": args.gpus}))
         return 2
+    if args.path == "v0":
+        print(json.dumps(v0_measure(args.workload, args.steps, args.warmup, args.kernel_reps)), flush=True)
+        return 0
     if args.api:
         out = api_measure(args.workload, 3)
         out.update({"metric": "dyn_evp1d_run wall time, 2D fields in and out (Fortran CICE driver API)",

@@ -483,17 +483,15 @@ def pack_v0(grid, nblocks: int = 1, deltaminEVP: float = 1e-8) -> State2D:
             src = F[_V0_FROM_GRID.get(n, n)] if n not in STRESS_FIELDS else F[n]
             st.f[n][:w + 2, :, b] = src[cols, :]
         ihi = w + 1                          # 1-based interior [2, ihi]
-        ti, tj, ui, uj = [], [], [], []
         t_hi = ihi + 1 if nblocks > 1 and b < nblocks - 1 else ihi
-        for j in range(2, NY):
-            for i in range(2, t_hi + 1):
-                if grid.icetmask[cols[i - 1], j - 1]:
-                    ti.append(i)
-                    tj.append(j)
-            for i in range(2, ihi + 1):
-                if grid.iceumask[cols[i - 1], j - 1]:
-                    ui.append(i)
-                    uj.append(j)
+
+        def cells(mask, i_hi):
+            # CICE order: j outer, i inner, over i in [2, i_hi], j in [2, NY-1]
+            sub = mask[cols[1:i_hi], 1:NY - 1]           # block columns 2..i_hi, rows 2..NY-1
+            jj, ii = np.nonzero(sub.T)                   # row-major in (j, i)
+            return (ii + 2).astype(np.int32), (jj + 2).astype(np.int32)
+        ti, tj = cells(grid.icetmask, t_hi)
+        ui, uj = cells(grid.iceumask, ihi)
         st.icellt[b], st.icellu[b] = len(ti), len(ui)
         st.indxti[:len(ti), b], st.indxtj[:len(tj), b] = ti, tj
         st.indxui[:len(ui), b], st.indxuj[:len(uj), b] = ui, uj

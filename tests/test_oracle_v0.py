@@ -168,3 +168,16 @@ def test_block_check_catches_a_t_list_without_the_east_halo_column():
     V.run(_params(V, one, 15), one, 15)
     V.run(_params(V, many, 15), many, 15)
     assert _interior_diff(g, one, many, ("uvel", "vvel")) != []
+
+
+def test_package_block_builder_equals_the_oracles():
+    """evp1d.layout.Blocks2D (what bench.py feeds the 2D engine) builds the
+    same arrays and lists as oracle.pack_v0."""
+    from evp1d import layout
+    g = synth.make_grid(37, 29, 61)
+    a = layout.Blocks2D(g)
+    b = O.pack_v0(g, 1)
+    for n in O.V0_FIELDS:
+        assert np.array_equal(a.f[n].view(np.uint64), b.f[n].view(np.uint64)), n
+    for n in ("indxti", "indxtj", "indxui", "indxuj", "icellt", "icellu"):
+        assert np.array_equal(getattr(a, n), getattr(b, n)), n
