@@ -4,8 +4,11 @@
 #include "evp1d_tiles.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
+#include <string>
+#include <thread>
 
 namespace evp1d {
 
@@ -167,13 +170,103 @@ TilePlan plan_tiles(int na, int navel, const int32_t *const idx[6], const uint8_
   return plan_tiles(na, navel, idx, skip, tile_ids(na, navel, idx, tile_w, tile_h), max_entries);
 }
 
+namespace {
+
+// Host threads for the planner and its check: the machine's, at most 16.
+int plan_threads() {
+  const unsigned hw = std::thread::hardware_concurrency();
+  return (int)std::max(1u, std::min(hw ? hw : 1u, 16u));
+}
+
+// f(lo, hi, w) over [0, n) in contiguous chunks, one per worker w, in order.
+template <class F>
+void par_chunks(int64_t n, int64_t grain, F f) {
+  const int T = (int)std::max<int64_t>(1, std::min<int64_t>(plan_threads(), n / std::max<int64_t>(grain, 1)));
+  if (T <= 1) {
+    f((int64_t)0, n, 0);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve((size_t)T);
+  for (int w = 0; w < T; ++w) th.emplace_back([&, w] { f(n * w / T, n * (w + 1) / T, w); });
+  for (auto &t : th) t.join();
+}
+
+// f(i, w) for every i in [0, n), items handed out dynamically (uneven work)
+template <class F>
+void par_items(int n, F f) {
+  const int T = std::max(1, std::min(plan_threads(), n / 4));
+  if (T <= 1) {
+    for (int i = 0; i < n; ++i) f(i, 0);
+    return;
+  }
+  std::atomic<int> next{0};
+  std::vector<std::thread> th;
+  th.reserve((size_t)T);
+  for (int w = 0; w < T; ++w)
+    th.emplace_back([&, w] {
+      for (int i = next.fetch_add(8); i < n; i = next.fetch_add(8))
+        for (int j = i; j < std::min(n, i + 8); ++j) f(j, w);
+    });
+  for (auto &t : th) t.join();
+}
+
+// Small open-addressing map slot -> local entry number, cleared per tile;
+// it doubles when half full, so any tile size works.
+struct SlotMap {
+  std::vector<int32_t> key, val, used;
+  explicit SlotMap(int cap_pow2 = 1 << 13) : key((size_t)cap_pow2, -1), val((size_t)cap_pow2, 0) {}
+  void grow() {
+    std::vector<int32_t> k2, v2;
+    for (int32_t h : used) { k2.push_back(key[(size_t)h]); v2.push_back(val[(size_t)h]); }
+    key.assign(key.size() * 2, -1);
+    val.assign(val.size(), 0);
+    used.clear();
+    bool a;
+    for (size_t i = 0; i < k2.size(); ++i) find_or_add(k2[i], v2[i], a);
+  }
+  int find_or_add(int32_t k, int32_t v_new, bool &added) {
+    if (2 * (used.size() + 1) > key.size()) grow();
+    const size_t m = key.size() - 1;
+    size_t h = ((uint32_t)k * 2654435761u) & m;
+    while (key[h] != -1 && key[h] != k) h = (h + 1) & m;
+    added = key[h] == -1;
+    if (added) {
+      key[h] = k;
+      val[h] = v_new;
+      used.push_back((int32_t)h);
+    }
+    return val[h];
+  }
+  int get(int32_t k) const {
+    const size_t m = key.size() - 1;
+    size_t h = ((uint32_t)k * 2654435761u) & m;
+    while (key[h] != k) h = (h + 1) & m;
+    return val[h];
+  }
+  void clear() {
+    for (int32_t h : used) key[(size_t)h] = -1;
+    used.clear();
+  }
+};
+
+struct TileWork {
+  std::vector<int32_t> halo, konst;
+  int entries = 0;
+};
+
+}  // namespace
+
+// The plan is a pure function of the lists, the flags and the tile ids; the
+// work is split over host threads, and every part is computed in an order
+// that does not depend on the number of threads, so the plan does not either.
 TilePlan plan_tiles(int na, int navel, const int32_t *const idx[6], const uint8_t *skip, const TileIds &ids,
                     int max_entries) {
   TilePlan p;
   p.tile_w = ids.tile_w;
   p.tile_h = ids.tile_h;
-  class_order(na, skip, p);
-  if (ids.ntiles == 0 || (int)ids.tid.size() != na) {
+  if (ids.ntiles == 0 || (int)ids.tid.size() != na || !idx) {
+    class_order(na, skip, p);
     p.why = ids.why.empty() ? "no tiles" : ids.why;
     return p;
   }
@@ -181,36 +274,80 @@ TilePlan plan_tiles(int na, int navel, const int32_t *const idx[6], const uint8_
   const std::vector<int32_t> &tid = ids.tid;
 
   // Shared points: active T points that a stepu point of another tile reads
-  // (nw, sse, sw), i.e. that appear in some tile's halo.
+  // (nw, sse, sw), i.e. that appear in some tile's halo.  Every writer stores 1.
   std::vector<uint8_t> shf((size_t)na, 0);
-  for (int q = 0; q < na; ++q) {
-    const int c = point_class(skip[q]);
-    if (c != 1 && c != 2) continue;
-    for (int n : {3, 5, 4}) {
-      const int32_t r = idx[n][q];
-      if (r >= 0 && r < na && point_class(skip[r]) <= 1 && tid[(size_t)r] != tid[(size_t)q]) shf[(size_t)r] = 1;
+  par_chunks(na, 1 << 15, [&](int64_t lo, int64_t hi, int) {
+    for (int64_t q = lo; q < hi; ++q) {
+      const int c = point_class(skip[q]);
+      if (c != 1 && c != 2) continue;
+      for (int n : {3, 5, 4}) {
+        const int32_t r = idx[n][q];
+        if (r >= 0 && r < na && point_class(skip[r]) <= 1 && tid[(size_t)r] != tid[(size_t)q])
+          __atomic_store_n(&shf[(size_t)r], (uint8_t)1, __ATOMIC_RELAXED);
+      }
     }
-  }
-  // counting sort by (class, tile, shared first), caller order inside
-  auto key = [&](int q) {
+  });
+
+  // counting sort by (class, tile, shared first), caller order inside:
+  // per-chunk histograms, then each chunk scatters its points in order
+  const size_t nkeys = (size_t)8 * nt;
+  auto key = [&](int64_t q) {
     return ((size_t)point_class(skip[q]) * nt + tid[(size_t)q]) * 2 + (shf[(size_t)q] ? 0 : 1);
   };
-  std::vector<int32_t> start((size_t)8 * nt + 1, 0);
-  for (int q = 0; q < na; ++q) ++start[key(q) + 1];
-  for (size_t b = 1; b < start.size(); ++b) start[b] += start[b - 1];
-  std::vector<int32_t> fill(start.begin(), start.end() - 1);
-  for (int q = 0; q < na; ++q) p.perm[(size_t)fill[key(q)]++] = q;
+  const int T = std::max(1, std::min(plan_threads(), na / (1 << 15)));
+  std::vector<std::vector<int32_t>> hist((size_t)T, std::vector<int32_t>(nkeys, 0));
+  auto chunk_lo = [&](int w) { return (int64_t)na * w / T; };
+  {
+    std::vector<std::thread> th;
+    for (int w = 0; w < T; ++w)
+      th.emplace_back([&, w] {
+        for (int64_t q = chunk_lo(w); q < chunk_lo(w + 1); ++q) ++hist[(size_t)w][key(q)];
+      });
+    for (auto &t : th) t.join();
+  }
+  std::vector<int32_t> start(nkeys + 1, 0);
+  {
+    int64_t run = 0;
+    for (size_t k = 0; k < nkeys; ++k) {
+      start[k] = (int32_t)run;
+      for (int w = 0; w < T; ++w) {
+        const int32_t c = hist[(size_t)w][k];
+        hist[(size_t)w][k] = (int32_t)run;   // this chunk's first position for key k
+        run += c;
+      }
+    }
+    start[nkeys] = (int32_t)run;
+  }
+  p.perm.assign((size_t)na, 0);
+  {
+    std::vector<std::thread> th;
+    for (int w = 0; w < T; ++w)
+      th.emplace_back([&, w] {
+        std::vector<int32_t> &pos = hist[(size_t)w];
+        for (int64_t q = chunk_lo(w); q < chunk_lo(w + 1); ++q) p.perm[(size_t)pos[key(q)]++] = (int32_t)q;
+      });
+    for (auto &t : th) t.join();
+  }
+  for (int c = 0; c < 4; ++c) p.cnt[c] = start[(size_t)(c + 1) * nt * 2] - start[(size_t)c * nt * 2];
   std::vector<int32_t> inv((size_t)na);
-  for (int s = 0; s < na; ++s) inv[(size_t)p.perm[(size_t)s]] = s;
+  par_chunks(na, 1 << 16, [&](int64_t lo, int64_t hi, int) {
+    for (int64_t s = lo; s < hi; ++s) inv[(size_t)p.perm[(size_t)s]] = (int32_t)s;
+  });
   auto slot = [&](int32_t q) { return q < na ? inv[(size_t)q] : q; };
 
   const int t_end = p.cnt[0] + p.cnt[1], u_begin = p.cnt[0];
   const int nu = p.cnt[1] + p.cnt[2];
   p.uloc.assign((size_t)4 * nu, 0);
-  std::vector<int32_t> mark((size_t)navel, -1), lidx((size_t)navel, -1), halo, konst, rr;
-  for (int t = 0; t < nt; ++t) {
-    TileDesc d{};
+  std::vector<TileDesc> desc((size_t)nt);
+  std::vector<TileWork> work((size_t)nt);
+  std::vector<int32_t> lidx((size_t)t_end, 0);   // owned slot -> entry in its tile's table
+  std::vector<SlotMap> maps;
+  maps.reserve((size_t)plan_threads());
+  for (int w = 0; w < plan_threads(); ++w) maps.emplace_back();
+  par_items(nt, [&](int t, int w) {
     auto at = [&](int c, int half) { return start[((size_t)c * nt + t) * 2 + half]; };
+    TileDesc &d = desc[(size_t)t];
+    d = TileDesc{};
     d.a0 = at(0, 0);
     d.a1 = at(0, 2);
     d.sa = at(0, 1) - d.a0;
@@ -219,63 +356,91 @@ TilePlan plan_tiles(int na, int navel, const int32_t *const idx[6], const uint8_
     d.sb = at(1, 1) - d.b0;
     d.c0 = at(2, 0);
     d.c1 = at(2, 2);
-    const int nA = d.a1 - d.a0, nB = d.b1 - d.b0, nO = nA + nB;
-    if (nO + (d.c1 - d.c0) == 0) continue;
-    for (int s = d.a0; s < d.a1; ++s) { mark[(size_t)s] = t; lidx[(size_t)s] = s - d.a0; }
-    for (int s = d.b0; s < d.b1; ++s) { mark[(size_t)s] = t; lidx[(size_t)s] = nA + (s - d.b0); }
-    halo.clear();
-    konst.clear();
-    auto visit = [&](int32_t r) {
-      if (mark[(size_t)r] == t) return;
-      mark[(size_t)r] = t;
-      (r < t_end ? halo : konst).push_back(r);
-    };
-    auto refs = [&](int s, int32_t out[4]) {
+    const int nA = d.a1 - d.a0, nB = d.b1 - d.b0, nO = nA + nB, nUt = nB + (d.c1 - d.c0);
+    if (nO + (d.c1 - d.c0) == 0) return;
+    for (int s = d.a0; s < d.a1; ++s) lidx[(size_t)s] = s - d.a0;
+    for (int s = d.b0; s < d.b1; ++s) lidx[(size_t)s] = nA + (s - d.b0);
+    auto owned = [&](int32_t r) { return (r >= d.a0 && r < d.a1) || (r >= d.b0 && r < d.b1); };
+    // the four slots each stepu point reads (own, nw: str2/str7, sse: str3/str6,
+    // sw: str4/str8), B then C points; non-owned ones get halo (active T) or
+    // constant entries in first-visit order
+    SlotMap &m = maps[(size_t)w];
+    TileWork &tw = work[(size_t)t];
+    auto refs = [&](int k, int32_t out[4]) {
+      const int s = k < nB ? d.b0 + k : d.c0 + (k - nB);
       const int32_t q = p.perm[(size_t)s];
       out[0] = s;
-      out[1] = slot(idx[3][q]);   // nw: str2, str7
-      out[2] = slot(idx[5][q]);   // sse: str3, str6
-      out[3] = slot(idx[4][q]);   // sw: str4, str8
+      out[1] = slot(idx[3][q]);
+      out[2] = slot(idx[5][q]);
+      out[3] = slot(idx[4][q]);
+      return s;
     };
-    // the four slots each stepu point reads, B then C points
-    const int nBt = d.b1 - d.b0, nUt = nBt + (d.c1 - d.c0);
-    rr.resize((size_t)4 * nUt);
     for (int k = 0; k < nUt; ++k) {
-      refs(k < nBt ? d.b0 + k : d.c0 + (k - nBt), &rr[(size_t)4 * k]);
-      for (int c = 0; c < 4; ++c) visit(rr[(size_t)4 * k + c]);
+      int32_t r[4];
+      refs(k, r);
+      for (int c = 0; c < 4; ++c) {
+        if (owned(r[c])) continue;
+        bool added = false;
+        std::vector<int32_t> &lst = r[c] < t_end ? tw.halo : tw.konst;
+        m.find_or_add(r[c], (int32_t)lst.size() | (r[c] < t_end ? 0 : (int32_t)0x40000000), added);
+        if (added) lst.push_back(r[c]);
+      }
     }
-    const int nH = (int)halo.size(), nK = (int)konst.size();
-    for (int k = 0; k < nH; ++k) lidx[(size_t)halo[(size_t)k]] = nO + k;
-    for (int k = 0; k < nK; ++k) lidx[(size_t)konst[(size_t)k]] = nO + nH + k;
-    const int entries = nO + nH + nK;
-    if (entries > max_entries || entries > 65535) {
+    const int nH = (int)tw.halo.size(), nK = (int)tw.konst.size();
+    tw.entries = nO + nH + nK;
+    if (tw.entries <= max_entries && tw.entries <= 65535) {
+      for (int k = 0; k < nUt; ++k) {
+        int32_t r[4];
+        const int s = refs(k, r);
+        for (int c = 0; c < 4; ++c) {
+          int e;
+          if (owned(r[c])) {
+            e = lidx[(size_t)r[c]];
+          } else {
+            const int v = m.get(r[c]);
+            e = (v & 0x40000000) ? nO + nH + (v & 0x3fffffff) : nO + v;
+          }
+          p.uloc[(size_t)4 * (s - u_begin) + c] = (uint16_t)e;
+        }
+      }
+    }
+    m.clear();
+  });
+  // tiles with work, in tile order; the first that overflows LDS fails the plan
+  for (int t = 0; t < nt; ++t) {
+    TileDesc d = desc[(size_t)t];
+    const TileWork &tw = work[(size_t)t];
+    if ((d.a1 - d.a0) + (d.b1 - d.b0) + (d.c1 - d.c0) == 0) continue;
+    if (tw.entries > max_entries || tw.entries > 65535) {
       char buf[160];
-      snprintf(buf, sizeof buf, "tile %d needs %d LDS entries (limit %d)", t, entries, max_entries);
+      snprintf(buf, sizeof buf, "tile %d needs %d LDS entries (limit %d)", t, tw.entries, max_entries);
       p.why = buf;
       p.tiles.clear();
+      p.list.clear();
+      p.klist.clear();
       return p;
     }
-    p.max_entries = std::max(p.max_entries, entries);
-    for (int k = 0; k < nUt; ++k) {
-      const int s = k < nBt ? d.b0 + k : d.c0 + (k - nBt);
-      for (int c = 0; c < 4; ++c) p.uloc[(size_t)4 * (s - u_begin) + c] = (uint16_t)lidx[(size_t)rr[(size_t)4 * k + c]];
-    }
+    p.max_entries = std::max(p.max_entries, tw.entries);
     d.h0 = (int32_t)p.list.size();
-    d.nh = nH;
+    d.nh = (int32_t)tw.halo.size();
     d.k0 = (int32_t)p.klist.size();
-    d.nk = nK;
-    p.list.insert(p.list.end(), halo.begin(), halo.end());
-    p.klist.insert(p.klist.end(), konst.begin(), konst.end());
+    d.nk = (int32_t)tw.konst.size();
+    p.list.insert(p.list.end(), tw.halo.begin(), tw.halo.end());
+    p.klist.insert(p.klist.end(), tw.konst.begin(), tw.konst.end());
     p.tiles.push_back(d);
   }
-  // shared ids of the halo points
-  p.shared = p.list;
-  std::sort(p.shared.begin(), p.shared.end());
-  p.shared.erase(std::unique(p.shared.begin(), p.shared.end()), p.shared.end());
+  // shared ids: the flagged stress slots in slot order (exactly the slots of
+  // the halo lists; the check re-derives that), per-entry and per-slot maps
   p.sid.assign((size_t)t_end, -1);
-  for (size_t k = 0; k < p.shared.size(); ++k) p.sid[(size_t)p.shared[k]] = (int32_t)k;
+  for (int s = 0; s < t_end; ++s)
+    if (shf[(size_t)p.perm[(size_t)s]]) {
+      p.sid[(size_t)s] = (int32_t)p.shared.size();
+      p.shared.push_back(s);
+    }
   p.hid.resize(p.list.size());
-  for (size_t k = 0; k < p.list.size(); ++k) p.hid[k] = p.sid[(size_t)p.list[k]];
+  par_chunks((int64_t)p.list.size(), 1 << 15, [&](int64_t lo, int64_t hi, int) {
+    for (int64_t k = lo; k < hi; ++k) p.hid[(size_t)k] = p.sid[(size_t)p.list[(size_t)k]];
+  });
   for (TileDesc &d : p.tiles) {
     d.ia = d.sa > 0 ? p.sid[(size_t)d.a0] : 0;
     d.ib = d.sb > 0 ? p.sid[(size_t)d.b0] : 0;
@@ -284,90 +449,143 @@ TilePlan plan_tiles(int na, int navel, const int32_t *const idx[6], const uint8_
   return p;
 }
 
+namespace {
+// every non-empty range [lo, hi) of `r`, sorted, must tile [from, to) exactly
+bool ranges_partition(std::vector<std::pair<int32_t, int32_t>> r, int32_t from, int32_t to) {
+  std::sort(r.begin(), r.end());
+  int32_t at = from;
+  for (const auto &x : r) {
+    if (x.first == x.second) continue;
+    if (x.first != at) return false;
+    at = x.second;
+  }
+  return at == to;
+}
+}  // namespace
+
 std::string check_plan(const TilePlan &p, int na, int navel, const int32_t *const idx[6], const uint8_t *skip) {
-  char buf[200];
   if ((int)p.perm.size() != na) return "perm has the wrong length";
   std::vector<int32_t> inv((size_t)na, -1);
-  for (int s = 0; s < na; ++s) {
-    const int32_t q = p.perm[(size_t)s];
-    if (q < 0 || q >= na || inv[(size_t)q] >= 0) return "perm is not a permutation";
-    inv[(size_t)q] = s;
+  std::atomic<bool> bad_perm{false};
+  par_chunks(na, 1 << 16, [&](int64_t lo, int64_t hi, int) {
+    for (int64_t s = lo; s < hi; ++s) {
+      const int32_t q = p.perm[(size_t)s];
+      int32_t expect = -1;
+      if (q < 0 || q >= na ||
+          !__atomic_compare_exchange_n(&inv[(size_t)q], &expect, (int32_t)s, false, __ATOMIC_RELAXED,
+                                       __ATOMIC_RELAXED)) {
+        bad_perm = true;
+        return;
+      }
+    }
+  });
+  if (bad_perm) return "perm is not a permutation";
+  {
+    int lo = 0;
+    for (int c = 0; c < 4; ++c) {
+      if (p.cnt[c] < 0 || lo + p.cnt[c] > na) return "class sizes do not add up to na";
+      lo += p.cnt[c];
+    }
+    if (lo != na) return "class sizes do not add up to na";
   }
-  int lo = 0;
-  for (int c = 0; c < 4; ++c) {
-    for (int s = lo; s < lo + p.cnt[c]; ++s)
-      if (point_class(skip[p.perm[(size_t)s]]) != c) return "slot outside its class range";
-    lo += p.cnt[c];
-  }
-  if (lo != na) return "class sizes do not add up to na";
+  std::atomic<bool> bad_class{false};
+  par_chunks(na, 1 << 16, [&](int64_t lo, int64_t hi, int) {
+    int c = 0, edge = p.cnt[0];
+    for (int64_t s = lo; s < hi; ++s) {
+      while (s >= edge && c < 3) edge += p.cnt[++c];
+      if (point_class(skip[p.perm[(size_t)s]]) != c) {
+        bad_class = true;
+        return;
+      }
+    }
+  });
+  if (bad_class) return "slot outside its class range";
   if (!p.ok) return "";
   const int t_end = p.cnt[0] + p.cnt[1], u_begin = p.cnt[0], u_end = p.cnt[0] + p.cnt[1] + p.cnt[2];
   if ((int)p.uloc.size() != 4 * (u_end - u_begin)) return "uloc has the wrong length";
-  auto slot = [&](int32_t q) { return q < na ? inv[(size_t)q] : q; };
-  std::vector<char> owned((size_t)na, 0), stepped((size_t)na, 0);
-  std::vector<int32_t> emap;
   if ((int)p.sid.size() != t_end || p.hid.size() != p.list.size()) return "shared-id tables have the wrong size";
-  for (size_t t = 0; t < p.tiles.size(); ++t) {
-    const TileDesc &d = p.tiles[t];
+  auto slot = [&](int32_t q) { return q < na ? inv[(size_t)q] : q; };
+  // ownership: the owned stress ranges of all tiles partition the stress
+  // slots, and the stepu ranges partition the stepu slots
+  std::vector<std::pair<int32_t, int32_t>> ra, rb, rc;
+  for (const TileDesc &d : p.tiles) {
     if (!(0 <= d.a0 && d.a0 <= d.a1 && d.a1 <= p.cnt[0])) return "stress-only range outside its class";
     if (!(p.cnt[0] <= d.b0 && d.b0 <= d.b1 && d.b1 <= t_end)) return "stress-and-stepu range outside its class";
     if (!(t_end <= d.c0 && d.c0 <= d.c1 && d.c1 <= u_end)) return "stepu-only range outside its class";
-    if (d.h0 < 0 || d.nh < 0 || (size_t)d.h0 + d.nh > p.list.size()) return "halo list range";
-    if (d.k0 < 0 || d.nk < 0 || (size_t)d.k0 + d.nk > p.klist.size()) return "constant list range";
+    ra.emplace_back(d.a0, d.a1);
+    rb.emplace_back(d.b0, d.b1);
+    rc.emplace_back(d.c0, d.c1);
+  }
+  if (!ranges_partition(ra, 0, p.cnt[0]) || !ranges_partition(rb, p.cnt[0], t_end))
+    return "active T slots not owned exactly once";
+  if (!ranges_partition(rc, t_end, u_end)) return "active U slots not stepped exactly once";
+  // every tile's table: bounds, entries and the LDS entry of every stepu read
+  std::vector<std::string> err((size_t)p.tiles.size());
+  std::vector<std::vector<int32_t>> emaps((size_t)plan_threads());
+  par_items((int)p.tiles.size(), [&](int t, int w) {
+    const TileDesc &d = p.tiles[(size_t)t];
+    std::string &e = err[(size_t)t];
+    if (d.h0 < 0 || d.nh < 0 || (size_t)d.h0 + d.nh > p.list.size()) { e = "halo list range"; return; }
+    if (d.k0 < 0 || d.nk < 0 || (size_t)d.k0 + d.nk > p.klist.size()) { e = "constant list range"; return; }
     const int nA = d.a1 - d.a0, nO = nA + (d.b1 - d.b0), ne = nO + d.nh + d.nk;
-    if (ne > p.max_entries) return "tile exceeds max_entries";
+    if (ne > p.max_entries) { e = "tile exceeds max_entries"; return; }
     // owned shared points first, with consecutive shared ids
-    if (d.sa < 0 || d.sa > nA || d.sb < 0 || d.sb > d.b1 - d.b0) return "shared prefix longer than its range";
+    if (d.sa < 0 || d.sa > nA || d.sb < 0 || d.sb > d.b1 - d.b0) { e = "shared prefix longer than its range"; return; }
     for (int s = d.a0; s < d.a1; ++s)
-      if (p.sid[(size_t)s] != (s - d.a0 < d.sa ? d.ia + (s - d.a0) : -1)) return "owned slot with a wrong shared id";
+      if (p.sid[(size_t)s] != (s - d.a0 < d.sa ? d.ia + (s - d.a0) : -1)) { e = "owned slot with a wrong shared id"; return; }
     for (int s = d.b0; s < d.b1; ++s)
-      if (p.sid[(size_t)s] != (s - d.b0 < d.sb ? d.ib + (s - d.b0) : -1)) return "owned slot with a wrong shared id";
+      if (p.sid[(size_t)s] != (s - d.b0 < d.sb ? d.ib + (s - d.b0) : -1)) { e = "owned slot with a wrong shared id"; return; }
+    std::vector<int32_t> &emap = emaps[(size_t)w];
     emap.assign((size_t)ne, -1);
-    for (int s = d.a0; s < d.a1; ++s) { if (owned[(size_t)s]++) return "T slot owned twice"; emap[(size_t)(s - d.a0)] = s; }
-    for (int s = d.b0; s < d.b1; ++s) { if (owned[(size_t)s]++) return "T slot owned twice"; emap[(size_t)(nA + s - d.b0)] = s; }
+    for (int s = d.a0; s < d.a1; ++s) emap[(size_t)(s - d.a0)] = s;
+    for (int s = d.b0; s < d.b1; ++s) emap[(size_t)(nA + s - d.b0)] = s;
     for (int k = 0; k < d.nh; ++k) {
       const int32_t g = p.list[(size_t)d.h0 + k];
-      if (g < 0 || g >= t_end) return "halo entry is not an active T slot";
-      if ((g >= d.a0 && g < d.a1) || (g >= d.b0 && g < d.b1)) return "halo entry owned by its own tile";
+      if (g < 0 || g >= t_end) { e = "halo entry is not an active T slot"; return; }
+      if ((g >= d.a0 && g < d.a1) || (g >= d.b0 && g < d.b1)) { e = "halo entry owned by its own tile"; return; }
       emap[(size_t)(nO + k)] = g;
     }
     for (int k = 0; k < d.nk; ++k) {
       const int32_t g = p.klist[(size_t)d.k0 + k];
-      if (g < t_end || g >= navel) return "constant entry is an active T slot or out of range";
+      if (g < t_end || g >= navel) { e = "constant entry is an active T slot or out of range"; return; }
       emap[(size_t)(nO + d.nh + k)] = g;
     }
     auto check_u = [&](int s) -> bool {
-      if (stepped[(size_t)s]++) return false;
       const int32_t q = p.perm[(size_t)s];
       const int32_t want[4] = {s, slot(idx[3][q]), slot(idx[5][q]), slot(idx[4][q])};
       for (int k = 0; k < 4; ++k) {
-        const int e = p.uloc[(size_t)4 * (s - u_begin) + k];
-        if (e >= ne || emap[(size_t)e] != want[k]) return false;
+        const int en = p.uloc[(size_t)4 * (s - u_begin) + k];
+        if (en >= ne || emap[(size_t)en] != want[k]) return false;
       }
       return true;
     };
     for (int s = d.b0; s < d.b1; ++s)
-      if (!check_u(s)) { snprintf(buf, sizeof buf, "stepu slot %d: bad LDS entry", s); return buf; }
+      if (!check_u(s)) { e = "stepu slot " + std::to_string(s) + ": bad LDS entry"; return; }
     for (int s = d.c0; s < d.c1; ++s)
-      if (!check_u(s)) { snprintf(buf, sizeof buf, "stepu slot %d: bad LDS entry", s); return buf; }
-  }
+      if (!check_u(s)) { e = "stepu slot " + std::to_string(s) + ": bad LDS entry"; return; }
+  });
+  for (const std::string &e : err)
+    if (!e.empty()) return e;
   // shared ids: every halo entry's id names its slot, ids are distinct
   for (size_t k = 0; k < p.shared.size(); ++k) {
     const int32_t g = p.shared[k];
     if (g < 0 || g >= t_end || p.sid[(size_t)g] != (int32_t)k) return "shared id table inconsistent";
   }
-  for (size_t k = 0; k < p.list.size(); ++k) {
-    const int32_t id = p.hid[k];
-    if (id < 0 || id >= (int32_t)p.shared.size() || p.shared[(size_t)id] != p.list[k]) return "halo entry with a wrong shared id";
-  }
+  std::atomic<bool> bad_hid{false};
+  par_chunks((int64_t)p.list.size(), 1 << 15, [&](int64_t lo, int64_t hi, int) {
+    for (int64_t k = lo; k < hi; ++k) {
+      const int32_t id = p.hid[(size_t)k];
+      if (id < 0 || id >= (int32_t)p.shared.size() || p.shared[(size_t)id] != p.list[(size_t)k]) {
+        bad_hid = true;
+        return;
+      }
+    }
+  });
+  if (bad_hid) return "halo entry with a wrong shared id";
   for (int s = 0; s < t_end; ++s) {
     const int32_t id = p.sid[(size_t)s];
     if (id >= (int32_t)p.shared.size() || (id >= 0 && p.shared[(size_t)id] != s)) return "stress slot with a wrong shared id";
   }
-  for (int s = 0; s < t_end; ++s)
-    if (!owned[(size_t)s]) return "active T slot owned by no tile";
-  for (int s = u_begin; s < u_end; ++s)
-    if (!stepped[(size_t)s]) return "active U slot in no tile";
   return "";
 }
 

@@ -12,6 +12,8 @@ owned points that other tiles evaluate as halo ("shared") lead their tile's
 owned ranges and carry consecutive ids in slot order, since during a run
 their stresses live in a table indexed by those ids.
 """
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -183,3 +185,33 @@ def test_plan_on_the_bench_grid_has_small_halo():
     check_plan(plan, ix, s["skipT"], s["skipU"], 32, 16)
     nT = plan["counts"][0] + plan["counts"][1]
     assert plan["tiles"][:, 7].sum() < 0.2 * nT
+
+
+def test_host_check_catches_corrupted_plans(tmp_path):
+    """Build tests/cpp/plan_check_corruptions.cpp against the planner source
+    and run it on a synthetic grid: every corruption of a valid plan (LDS
+    entries, ownership ranges, shared ids, permutation, constant entries)
+    must be reported by check_plan."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    repo = Path(__file__).resolve().parent.parent
+    csrc = repo / "cice-evp1d_amd" / "csrc"
+    exe = tmp_path / "corrupt"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-pthread", f"-I{csrc}", str(repo / "tests" / "cpp" / "plan_check_corruptions.cpp"),
+                    str(csrc / "evp1d_tiles.cpp"), "-o", str(exe)], check=True, capture_output=True)
+    g = synth.make_grid(300, 200, 71)
+    ix = evp1d.build_index(g.tmask, g.nghost)
+    from evp1d import layout
+    s = layout.pack(g, ix)
+    na, navel = s["na"], s["navel"]
+    skip = (s["skipT"] != 0).astype(np.uint8) | ((s["skipU"] != 0).astype(np.uint8) << 1)
+    with open(tmp_path / "in.bin", "wb") as fh:
+        fh.write(np.array([na, navel], np.int32).tobytes())
+        for n in evp1d.NBR:
+            fh.write((s["idx"][n].astype(np.int32) - 1).tobytes())
+        fh.write(skip.tobytes())
+    r = subprocess.run([str(exe), str(tmp_path / "in.bin")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "NOT CAUGHT" not in r.stdout and "good: ''" in r.stdout
