@@ -185,7 +185,8 @@ struct evp2d_handle {
   // cell lists per block (linear cells), concatenated on the device
   std::vector<std::vector<int32_t>> tl, ul;
   int32_t *tcell_dev = nullptr, *ucell_dev = nullptr;
-  bool cells_dirty = true;
+  bool cells_dirty = true;   // device tables out of date
+  bool lists_dirty = true;   // the T/U lists changed: str must read as zero again
   // halo (used when nblocks > 1 and set)
   bool halo_on = false;
   std::vector<int32_t> fill_h, dst_h, src_h;
@@ -257,8 +258,10 @@ int ensure2(evp2d_handle *h) {
   RC2(up(&h->dst_dev, h->dst_h));
   RC2(up(&h->src_dev, h->src_h));
   // str outside the lists must read as zero (the reference clears it)
-  for (double2 *s : h->str) HIP2(hipMemsetAsync(s, 0, (size_t)h->n * sizeof(double2), h->stream));
+  if (h->lists_dirty)
+    for (double2 *s : h->str) HIP2(hipMemsetAsync(s, 0, (size_t)h->n * sizeof(double2), h->stream));
   HIP2(hipStreamSynchronize(h->stream));
+  h->lists_dirty = false;
   h->v.tcell = h->tcell_dev;
   h->v.ucell = h->ucell_dev;
   h->v.nt = (int)t.size();
@@ -422,6 +425,7 @@ int evp2d_set_cells(evp2d_handle *h, int32_t iblk, int32_t icellt, const int32_t
   h->tl[(size_t)(iblk - 1)] = std::move(t);
   h->ul[(size_t)(iblk - 1)] = std::move(u);
   h->cells_dirty = true;
+  h->lists_dirty = true;
   return EVP1D_OK;
 }
 
