@@ -25,7 +25,10 @@
 ! (namelist evp1d_nml, default .true.) the 2D <-> 1D copies of the dynamic
 ! fields (convert_2d_1d_dyn / convert_1d_2d_dyn, ice_dyn_evp1d.F90:818-947)
 ! also run on the device: whole 2D arrays cross PCIe and the library gathers
-! and scatters them; .false. keeps the host loops.
+! and scatters them; .false. keeps the host loops.  With lautotune (default
+! .false.) the first run first times the fused form's tiles and stress
+! storages on the first step's own fields (evp1d_autotune_fused, state
+! restored) and keeps the fastest; results do not depend on it.
 !===============================================================================
 module ice_dyn_evp1d
   use, intrinsic :: iso_c_binding
@@ -39,7 +42,9 @@ module ice_dyn_evp1d
 
   type(c_ptr), save :: handle = c_null_ptr
   logical(kind=log_kind), save :: ldebug = .false., linfo = .true., ldevpack = .true.
+  logical(kind=log_kind), save :: lautotune = .false., tuned = .false.
   integer(kind=int_kind), save :: ts = 0, iu06 = -1, buf1d = 0
+  logical(kind=log_kind), save :: log_open = .false.   ! newunit numbers are negative
   integer(kind=int_kind), save :: nx, ny, nActive, navel, nghost, nx_block, ny_block, max_block
   integer(kind=int_kind), allocatable, save :: indxti(:), indxtj(:)
   real(kind=dbl_kind), allocatable, save :: buf(:)
@@ -84,7 +89,7 @@ contains
   !=============================================================================
   subroutine read_nml()
     integer(kind=int_kind) :: lun, ios
-    namelist /evp1d_nml/ ldebug, linfo, buf1d, ldevpack
+    namelist /evp1d_nml/ ldebug, linfo, buf1d, ldevpack, lautotune
     open(newunit=lun, file='evp_1d.nml', status='old', iostat=ios)
     if (ios == 0) then
       read(lun, nml=evp1d_nml, iostat=ios)
@@ -110,10 +115,13 @@ contains
     integer(c_int32_t) :: na32, navel32
     integer(kind=int_kind) :: k, ios
 
+    if (log_open) close(iu06)
     open(newunit=iu06, file='evp1d.log', status='replace', iostat=ios)
     if (ios /= 0) error stop 'Unable to open evp1d.log'
+    log_open = .true.
     write(iu06,'(a)') 'Initializing evp1d (MI355X engine)'
     call read_nml()
+    tuned = .false.
     if (c_associated(handle)) call dyn_evp1d_finalize_handle()
     ts = 0
     nx_block = cnx_block
@@ -238,6 +246,7 @@ contains
     real(kind=dbl_kind), allocatable :: G(:,:)
     logical(kind=log_kind), allocatable :: GT(:,:), GU(:,:)
     type(evp1d_params_t) :: p
+    real(c_float) :: tune_us
     integer(kind=int_kind) :: iw
 
     if (.not. c_associated(handle)) error stop 'dyn_evp1d_run: dyn_evp1d_init was not called'
@@ -317,6 +326,11 @@ contains
     ! the subcycles (ice_dyn_evp1d.F90:188-207), device resident
     call fill_params(p)
     call evp1d_check(evp1d_set_params(handle, p), 'evp1d_set_params')
+    if (lautotune .and. .not. tuned) then
+      call evp1d_check(evp1d_autotune_fused(handle, int(min(ndte, 100),c_int32_t), tune_us), 'autotune_fused')
+      if (linfo .and. log_open) write(iu06, '(a,f10.2,a)') ' evp1d: autotune_fused ', tune_us, ' us/subcycle'
+      tuned = .true.
+    endif
     call evp1d_check(evp1d_run(handle, int(ndte,c_int32_t)), 'evp1d_run')
 
     if (ldevpack) then
@@ -448,10 +462,11 @@ contains
     if (allocated(indxti)) deallocate(indxti, indxtj, buf, ibuf)
     call free_gall()
     write(*,'(a21)') 'Time to die for evp1d'
-    if (iu06 >= 0) then
+    if (log_open) then
       write(iu06,'(a21)') 'Time to die for evp1d'
       close(iu06)
       iu06 = -1
+      log_open = .false.
     endif
   end subroutine dyn_evp1d_finalize
 

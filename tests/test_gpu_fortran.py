@@ -105,9 +105,9 @@ def _emulate_api(g, ndte, nrun):
     return G
 
 
-@pytest.mark.parametrize("devpack", [True, False])
+@pytest.mark.parametrize("devpack,autotune", [(True, False), (False, False), (True, True)])
 @pytest.mark.parametrize("nx,ny,seed,ndte,nrun", [(40, 30, 71, 15, 2), (90, 64, 72, 40, 3)])
-def test_api_driver_matches_reference_dataflow(gpu, nx, ny, seed, ndte, nrun, devpack):
+def test_api_driver_matches_reference_dataflow(gpu, nx, ny, seed, ndte, nrun, devpack, autotune):
     from evp1d import synth
     g = synth.make_grid(nx, ny, seed)
     # a nonzero stress start, so the unpack of every stress matters
@@ -125,11 +125,15 @@ def test_api_driver_matches_reference_dataflow(gpu, nx, ny, seed, ndte, nrun, de
             for n in DYN:
                 fh.write(g.fields[n].tobytes(order="F"))
         # 2D <-> 1D copies on the device (default) or in host loops
-        (Path(d) / "evp_1d.nml").write_text(f"&evp1d_nml\n  ldevpack = {'.true.' if devpack else '.false.'}\n/\n")
+        # with lautotune the first run tunes the fused form on its own fields
+        (Path(d) / "evp_1d.nml").write_text(f"&evp1d_nml\n  ldevpack = {'.true.' if devpack else '.false.'}\n"
+                                            f"  lautotune = {'.true.' if autotune else '.false.'}\n/\n")
         _run(FBIN / "evp1d_api_driver", d)
         raw = np.fromfile(Path(d) / "api_out.bin", dtype="<f8")
         log = (Path(d) / "evp1d.log").read_text()
     assert "nActive" in log
+    assert ("autotune_fused" in log) == autotune
+    assert "Time to die for evp1d" in log      # dyn_evp1d_finalize closes the log
     shape = g.tmask.shape
     got = raw.reshape((29,) + shape[::-1]).transpose(0, 2, 1)
     for k, n in enumerate(DYN):
