@@ -178,6 +178,26 @@ int plan_threads() {
   return (int)std::max(1u, std::min(hw ? hw : 1u, 16u));
 }
 
+// Run body(w) for w = 0..T-1, on T-1 new threads and the caller; when a
+// thread cannot be started (a process thread limit) its share runs on the
+// caller, so the work is always done and no exception leaves the library.
+template <class B>
+void run_workers(int T, B body) {
+  std::vector<std::thread> th;
+  std::vector<int> left;
+  th.reserve((size_t)T);
+  for (int w = 1; w < T; ++w) {
+    try {
+      th.emplace_back([&body, w] { body(w); });
+    } catch (...) {
+      left.push_back(w);
+    }
+  }
+  body(0);
+  for (int w : left) body(w);
+  for (auto &t : th) t.join();
+}
+
 // f(lo, hi, w) over [0, n) in contiguous chunks, one per worker w, in order.
 template <class F>
 void par_chunks(int64_t n, int64_t grain, F f) {
@@ -186,10 +206,7 @@ void par_chunks(int64_t n, int64_t grain, F f) {
     f((int64_t)0, n, 0);
     return;
   }
-  std::vector<std::thread> th;
-  th.reserve((size_t)T);
-  for (int w = 0; w < T; ++w) th.emplace_back([&, w] { f(n * w / T, n * (w + 1) / T, w); });
-  for (auto &t : th) t.join();
+  run_workers(T, [&](int w) { f(n * w / T, n * (w + 1) / T, w); });
 }
 
 // f(i, w) for every i in [0, n), items handed out dynamically (uneven work)
@@ -201,14 +218,10 @@ void par_items(int n, F f) {
     return;
   }
   std::atomic<int> next{0};
-  std::vector<std::thread> th;
-  th.reserve((size_t)T);
-  for (int w = 0; w < T; ++w)
-    th.emplace_back([&, w] {
-      for (int i = next.fetch_add(8); i < n; i = next.fetch_add(8))
-        for (int j = i; j < std::min(n, i + 8); ++j) f(j, w);
-    });
-  for (auto &t : th) t.join();
+  run_workers(T, [&](int w) {
+    for (int i = next.fetch_add(8); i < n; i = next.fetch_add(8))
+      for (int j = i; j < std::min(n, i + 8); ++j) f(j, w);
+  });
 }
 
 // Small open-addressing map slot -> local entry number, cleared per tile;
@@ -297,14 +310,9 @@ TilePlan plan_tiles(int na, int navel, const int32_t *const idx[6], const uint8_
   const int T = std::max(1, std::min(plan_threads(), na / (1 << 15)));
   std::vector<std::vector<int32_t>> hist((size_t)T, std::vector<int32_t>(nkeys, 0));
   auto chunk_lo = [&](int w) { return (int64_t)na * w / T; };
-  {
-    std::vector<std::thread> th;
-    for (int w = 0; w < T; ++w)
-      th.emplace_back([&, w] {
-        for (int64_t q = chunk_lo(w); q < chunk_lo(w + 1); ++q) ++hist[(size_t)w][key(q)];
-      });
-    for (auto &t : th) t.join();
-  }
+  run_workers(T, [&](int w) {
+    for (int64_t q = chunk_lo(w); q < chunk_lo(w + 1); ++q) ++hist[(size_t)w][key(q)];
+  });
   std::vector<int32_t> start(nkeys + 1, 0);
   {
     int64_t run = 0;
@@ -319,15 +327,10 @@ TilePlan plan_tiles(int na, int navel, const int32_t *const idx[6], const uint8_
     start[nkeys] = (int32_t)run;
   }
   p.perm.assign((size_t)na, 0);
-  {
-    std::vector<std::thread> th;
-    for (int w = 0; w < T; ++w)
-      th.emplace_back([&, w] {
-        std::vector<int32_t> &pos = hist[(size_t)w];
-        for (int64_t q = chunk_lo(w); q < chunk_lo(w + 1); ++q) p.perm[(size_t)pos[key(q)]++] = (int32_t)q;
-      });
-    for (auto &t : th) t.join();
-  }
+  run_workers(T, [&](int w) {
+    std::vector<int32_t> &pos = hist[(size_t)w];
+    for (int64_t q = chunk_lo(w); q < chunk_lo(w + 1); ++q) p.perm[(size_t)pos[key(q)]++] = (int32_t)q;
+  });
   for (int c = 0; c < 4; ++c) p.cnt[c] = start[(size_t)(c + 1) * nt * 2] - start[(size_t)c * nt * 2];
   std::vector<int32_t> inv((size_t)na);
   par_chunks(na, 1 << 16, [&](int64_t lo, int64_t hi, int) {
