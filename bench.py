@@ -343,7 +343,7 @@ def ensemble_measure(nmem: int, steps: int, autotune: int) -> dict:
     return out
 
 
-def v0_measure(workload: str, steps: int, warmup: int, kernel_reps: int) -> dict:
+def v0_measure(workload: str, steps: int, warmup: int, kernel_reps: int, form: str = "fused") -> dict:
     """The v0 2D block path (evp2d_*) on the workload's grid as one block:
     CICE's (nx_block, ny_block) arrays, per-block cell lists.  Algorithmic
     bytes per subcycle in the reference's data model: stress 360 B per T cell
@@ -351,7 +351,8 @@ def v0_measure(workload: str, steps: int, warmup: int, kernel_reps: int) -> dict
     12 stresses in 96 and out 96, str 64), stepu 248 B per U cell
     (indxui/indxuj 8, uvel/vvel 16, 14 forcing/metric arrays 112, str 64,
     uvel/vvel 16, strintx/y 16, taubx/y 16); the reference's clearing of the
-    whole str scratch in every stress call is not counted."""
+    whole str scratch in every stress call is not counted.  form: "fused"
+    (one tile kernel per subcycle, the default) or "plain" (--mode graph)."""
     import evp1d
     from evp1d import layout, synth
     w = dict(synth.WORKLOADS[workload])
@@ -360,6 +361,7 @@ def v0_measure(workload: str, steps: int, warmup: int, kernel_reps: int) -> dict
     blk = layout.Blocks2D(g)
     nT, nU = int(blk.icellt[0]), int(blk.icellu[0])
     eng = evp1d.Engine2D(blk.nxb, blk.nyb, 1, evp1d.set_evp_parameters(ndte))
+    eng.set_mode(form)
     eng.load_state(blk)
     log(f"[bench] v0 path: {blk.nxb}x{blk.nyb} one block, icellt={nT} icellu={nU}")
     for k in range(warmup):
@@ -374,16 +376,18 @@ def v0_measure(workload: str, steps: int, warmup: int, kernel_reps: int) -> dict
     wall = time.perf_counter() - t0
     secs = max(wall, ev_ms / 1e3)
     B = {"stress": 360 * nT, "stepu": 248 * nU}
+    B["fused"] = B["stress"] + B["stepu"]   # one whole subcycle
     kernels = {}
-    for which in ("stress", "stepu"):
+    for which in ("stress", "stepu", "fused"):
         eng.synchronize()
         eng.timer_start()
         eng.run_individual(which, kernel_reps)
         us = eng.timer_stop() * 1e3 / kernel_reps
         kernels[which] = {"avg_us": round(us, 3), "bytes": B[which], "GBps": round(B[which] / (us * 1e-6) / 1e9, 1)}
         log(f"[bench] v0 {which}: {us:.2f} us/launch, {kernels[which]['GBps']} GB/s algorithmic")
+    mode = eng.get_mode()
     eng.close()
-    name = max(kernels, key=lambda k: kernels[k]["avg_us"])
+    name = "fused" if mode == "fused" else max(("stress", "stepu"), key=lambda k: kernels[k]["avg_us"])
     dom_us = kernels[name]["avg_us"]
     achieved = B[name] / (dom_us * 1e-6) / 1e9
     traffic = None
@@ -398,6 +402,7 @@ def v0_measure(workload: str, steps: int, warmup: int, kernel_reps: int) -> dict
         "warmup": warmup, "ms_per_step": secs / steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{workload} {w['nx']}x{w['ny']} ndte={ndte}, v0 2D block path (one block)",
+                   "form": mode,
                    "nx_block": blk.nxb, "ny_block": blk.nyb, "icellt": nT, "icellu": nU, "ndte": ndte,
                    "us_per_subcycle": round(step_us, 3),
                    "loop_GBps_algorithmic": round(sub_bytes / (step_us * 1e-6) / 1e9, 1)},
@@ -452,7 +457,8 @@ def main() -> int:
                                    "note (DESIGN.md), not a measured path", "n_gpus": args.gpus}))
         return 2
     if args.path == "v0":
-        print(json.dumps(v0_measure(args.workload, args.steps, args.warmup, args.kernel_reps)), flush=True)
+        form = "plain" if args.mode == "graph" else "fused"
+        print(json.dumps(v0_measure(args.workload, args.steps, args.warmup, args.kernel_reps, form)), flush=True)
         return 0
     if args.api:
         out = api_measure(args.workload, 3)

@@ -145,6 +145,100 @@ __global__ __launch_bounds__(kB2) void k2_stepu(View2 v) {
   v.tauby[c] = -(o.vvel * o.Cb);
 }
 
+// Fused subcycle: stress and stepu of one tile of kTW x kTH cells per
+// workgroup.  The tile's T cells, plus the T cells of the column east of it
+// and the row north of it that its U cells read (evaluated again, not
+// stored), put their str1..8 into LDS tables laid out like the cells,
+// (kTW+1) x (kTH+1) entries, zero where no T cell is (the reference's
+// cleared str); after a barrier stepu reads its four entries at (i,j),
+// (i+1,j), (i,j+1), (i+1,j+1) by position.  Velocities and stresses are read
+// from the view and written to f.vel_out / f.sig_out, the other buffer, so no
+// tile reads a half-updated subcycle.  str1..8, strintx/y and taubx/y are
+// observable after the run only: stored in the last subcycle (`last`).
+#ifndef EVP2D_TILE_W
+#define EVP2D_TILE_W 32
+#endif
+#ifndef EVP2D_TILE_H
+#define EVP2D_TILE_H 8
+#endif
+constexpr int kTW = EVP2D_TILE_W, kTH = EVP2D_TILE_H, kCap2 = (kTW + 1) * (kTH + 1);
+constexpr int kB2F = 256;
+struct Tile2 {
+  int32_t t0, nto, nth;   // T entries [t0, t0 + nto) owned, then nth halo entries
+  int32_t u0, nu;         // U entries [u0, u0 + nu)
+};
+struct Fused2 {
+  const Tile2 *tiles;
+  const int32_t *tcell, *ucell;   // linear cells
+  const uint16_t *tlds, *ulds;    // their LDS entries
+  double2 *vel_out;
+  double *sig_out;
+};
+
+__global__ __launch_bounds__(kB2F) void k2_fused(View2 v, Fused2 f, int last) {
+  __shared__ double2 lds[4 * kCap2];
+  const Tile2 t = f.tiles[blockIdx.x];
+  for (int k = threadIdx.x; k < 4 * kCap2; k += kB2F) lds[k] = make_double2(0.0, 0.0);
+  __syncthreads();
+  const int nT = t.nto + t.nth;
+  for (int k = threadIdx.x; k < nT; k += kB2F) {
+    const long c = f.tcell[t.t0 + k];
+    const int e = f.tlds[t.t0 + k];
+    const double2 cc = v.vel[c];
+    const double2 ve = v.vel[c - 1];
+    const double2 vs = v.vel[c - v.nxb];
+    const double2 vn = v.vel[c - v.nxb - 1];
+    double sig[12];
+#pragma unroll
+    for (int q = 0; q < 12; ++q) sig[q] = v.sig[q * v.n + c];
+    evp1d::StressOut o;
+    evp1d::stress_core(v.s, cc.x, cc.y, ve.x, ve.y, vs.x, vs.y, vn.x, vn.y, v.dxt[c], v.dyt[c], v.strength[c],
+                       v.cxp[c], v.cyp[c], v.cxm[c], v.cym[c], v.dmin[c], v.dxhy[c], v.dyhx[c], sig, o);
+    const double2 p15 = make_double2(o.str[0], o.str[4]), p27 = make_double2(o.str[1], o.str[6]);
+    const double2 p36 = make_double2(o.str[2], o.str[5]), p48 = make_double2(o.str[3], o.str[7]);
+    if (k < t.nto) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        f.sig_out[q * v.n + c] = o.sp[q];
+        f.sig_out[(4 + q) * v.n + c] = o.sm[q];
+        f.sig_out[(8 + q) * v.n + c] = o.s12[q];
+      }
+      if (last) {
+        v.s15[c] = p15;
+        v.s27[c] = p27;
+        v.s36[c] = p36;
+        v.s48[c] = p48;
+      }
+    }
+    lds[e] = p15;
+    lds[kCap2 + e] = p27;
+    lds[2 * kCap2 + e] = p36;
+    lds[3 * kCap2 + e] = p48;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < t.nu; k += kB2F) {
+    const long c = f.ucell[t.u0 + k];
+    const int e = f.ulds[t.u0 + k];
+    const double2 a = lds[e];                        // (i, j): str1, str5
+    const double2 b = lds[kCap2 + e + 1];            // (i+1, j): str2, str7
+    const double2 g = lds[2 * kCap2 + e + kTW + 1];  // (i, j+1): str3, str6
+    const double2 h = lds[3 * kCap2 + e + kTW + 2];  // (i+1, j+1): str4, str8
+    const double2 vv = v.vel[c];
+    const double2 vi = v.vinit[c];
+    evp1d::StepuOut o;
+    evp1d::stepu_point(v.s, vv.x, vv.y, v.cw[c], v.aix[c], v.uocn[c], v.vocn[c], v.waterx[c], v.watery[c],
+                       v.forcex[c], v.forcey[c], v.umassdti[c], v.fm[c], v.uarear[c], vi.x, vi.y, v.tbu[c],
+                       a.x, b.x, g.x, h.x, a.y, g.y, b.y, h.y, o);
+    f.vel_out[c] = make_double2(o.uvel, o.vvel);
+    if (last) {
+      v.strintx[c] = o.strintx;
+      v.strinty[c] = o.strinty;
+      v.taubx[c] = -(o.uvel * o.Cb);
+      v.tauby[c] = -(o.vvel * o.Cb);
+    }
+  }
+}
+
 // halo update of uvel and vvel (vars.F90:113-142): the fill, then the copies
 __global__ void k2_fill(double2 *vel, const int32_t *cells, int n) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -191,6 +285,14 @@ struct evp2d_handle {
   bool halo_on = false;
   std::vector<int32_t> fill_h, dst_h, src_h;
   int32_t *fill_dev = nullptr, *dst_dev = nullptr, *src_dev = nullptr;
+  // fused form (default): tile tables and the second velocity / stress buffers
+  int mode = EVP2D_MODE_FUSED;
+  Tile2 *tiles_dev = nullptr;
+  int32_t *ftcell_dev = nullptr, *fucell_dev = nullptr;
+  uint16_t *ftlds_dev = nullptr, *fulds_dev = nullptr;
+  int ntiles = 0;
+  double2 *vel_alt = nullptr;
+  double *sig_alt = nullptr;
   // captured subcycle loop
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -209,10 +311,15 @@ void drop_graph2(evp2d_handle *h) {
 }
 
 void free_lists2(evp2d_handle *h) {
-  int32_t *bufs[5] = {h->tcell_dev, h->ucell_dev, h->fill_dev, h->dst_dev, h->src_dev};
-  for (int32_t *b : bufs)
+  void *bufs[10] = {h->tcell_dev, h->ucell_dev, h->fill_dev, h->dst_dev, h->src_dev,
+                    h->tiles_dev, h->ftcell_dev, h->fucell_dev, h->ftlds_dev, h->fulds_dev};
+  for (void *b : bufs)
     if (b) (void)hipFree(b);
   h->tcell_dev = h->ucell_dev = h->fill_dev = h->dst_dev = h->src_dev = nullptr;
+  h->tiles_dev = nullptr;
+  h->ftcell_dev = h->fucell_dev = nullptr;
+  h->ftlds_dev = h->fulds_dev = nullptr;
+  h->ntiles = 0;
 }
 
 void release2(evp2d_handle *h) {
@@ -220,6 +327,8 @@ void release2(evp2d_handle *h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   drop_graph2(h);
   free_lists2(h);
+  if (h->vel_alt) (void)hipFree(h->vel_alt);
+  if (h->sig_alt) (void)hipFree(h->sig_alt);
   for (void *a : h->allocs) (void)hipFree(a);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -232,6 +341,98 @@ int alloc2(evp2d_handle *h, void **ptr, size_t bytes) {
   if (e != hipSuccess) return fail2(EVP1D_ENOMEM, "evp2d_create: hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
   h->allocs.push_back(*ptr);
   HIP2(hipMemsetAsync(*ptr, 0, std::max<size_t>(bytes, 256), h->stream));
+  return EVP1D_OK;
+}
+
+// The fused form's tables from the per-block lists: tile (tx, ty) of block
+// b covers cells i in [tx*kTW + 1, tx*kTW + kTW], j in [ty*kTH + 1, ty*kTH +
+// kTH].  A T cell is owned by its tile, and is a halo entry of the tile west
+// of it when it is on its tile's first column, of the tile south of it when
+// on the first row, and of the south-west tile when both; LDS entry of cell
+// (i, j) in tile (tx, ty): (j-1 - ty*kTH) * (kTW+1) + (i-1 - tx*kTW).
+int build_tiles2(evp2d_handle *h) {
+  const int ntx = (h->nxb + kTW - 1) / kTW, nty = (h->nyb + kTH - 1) / kTH;
+  const long nt_all = (long)h->nblocks * ntx * nty;
+  std::vector<std::vector<int32_t>> own((size_t)nt_all), halo((size_t)nt_all), uc((size_t)nt_all);
+  std::vector<std::vector<uint16_t>> own_e((size_t)nt_all), halo_e((size_t)nt_all), uc_e((size_t)nt_all);
+  auto tile = [&](int b, int tx, int ty) { return ((long)b * nty + ty) * ntx + tx; };
+  auto entry = [&](int i, int j, int tx, int ty) {
+    const int di = i - 1 - tx * kTW, dj = j - 1 - ty * kTH;
+    return (uint16_t)(dj * (kTW + 1) + di);
+  };
+  for (int b = 0; b < h->nblocks; ++b) {
+    const long base = (long)b * h->plane;
+    for (int32_t c : h->tl[(size_t)b]) {
+      const long l = c - base;
+      const int i = (int)(l % h->nxb) + 1, j = (int)(l / h->nxb) + 1;
+      const int tx = (i - 1) / kTW, ty = (j - 1) / kTH;
+      own[(size_t)tile(b, tx, ty)].push_back(c);
+      own_e[(size_t)tile(b, tx, ty)].push_back(entry(i, j, tx, ty));
+      const bool col0 = (i - 1) % kTW == 0 && tx > 0, row0 = (j - 1) % kTH == 0 && ty > 0;
+      if (col0) { halo[(size_t)tile(b, tx - 1, ty)].push_back(c); halo_e[(size_t)tile(b, tx - 1, ty)].push_back(entry(i, j, tx - 1, ty)); }
+      if (row0) { halo[(size_t)tile(b, tx, ty - 1)].push_back(c); halo_e[(size_t)tile(b, tx, ty - 1)].push_back(entry(i, j, tx, ty - 1)); }
+      if (col0 && row0) {
+        halo[(size_t)tile(b, tx - 1, ty - 1)].push_back(c);
+        halo_e[(size_t)tile(b, tx - 1, ty - 1)].push_back(entry(i, j, tx - 1, ty - 1));
+      }
+    }
+    for (int32_t c : h->ul[(size_t)b]) {
+      const long l = c - base;
+      const int i = (int)(l % h->nxb) + 1, j = (int)(l / h->nxb) + 1;
+      const int tx = (i - 1) / kTW, ty = (j - 1) / kTH;
+      uc[(size_t)tile(b, tx, ty)].push_back(c);
+      uc_e[(size_t)tile(b, tx, ty)].push_back(entry(i, j, tx, ty));
+    }
+  }
+  std::vector<Tile2> tiles;
+  std::vector<int32_t> tcell, ucell;
+  std::vector<uint16_t> tlds, ulds;
+  for (long t = 0; t < nt_all; ++t) {
+    const size_t k = (size_t)t;
+    if (own[k].empty() && uc[k].empty()) continue;
+    Tile2 d;
+    d.t0 = (int32_t)tcell.size();
+    d.nto = (int32_t)own[k].size();
+    d.nth = (int32_t)halo[k].size();
+    d.u0 = (int32_t)ucell.size();
+    d.nu = (int32_t)uc[k].size();
+    tcell.insert(tcell.end(), own[k].begin(), own[k].end());
+    tcell.insert(tcell.end(), halo[k].begin(), halo[k].end());
+    tlds.insert(tlds.end(), own_e[k].begin(), own_e[k].end());
+    tlds.insert(tlds.end(), halo_e[k].begin(), halo_e[k].end());
+    ucell.insert(ucell.end(), uc[k].begin(), uc[k].end());
+    ulds.insert(ulds.end(), uc_e[k].begin(), uc_e[k].end());
+    tiles.push_back(d);
+  }
+  // every entry inside its tile's table, every stepu read too (the kernel
+  // trusts these): the U entry e reads e+1, e+kTW+1 and e+kTW+2
+  for (uint16_t e : tlds)
+    if (e >= kCap2) return fail2(EVP1D_EINDEX, "evp2d: T entry %d outside the tile table", (int)e);
+  for (const Tile2 &d : tiles)
+    for (int k = 0; k < d.nu; ++k) {
+      const int e = ulds[(size_t)d.u0 + k];
+      if (e % (kTW + 1) >= kTW || e / (kTW + 1) >= kTH)
+        return fail2(EVP1D_EINDEX, "evp2d: U entry %d outside its tile", e);
+    }
+  auto up32 = [&](int32_t **dst, const std::vector<int32_t> &src) -> int {
+    HIP2(hipMalloc((void **)dst, std::max<size_t>(src.size(), 1) * sizeof(int32_t)));
+    if (!src.empty()) HIP2(hipMemcpyAsync(*dst, src.data(), src.size() * 4, hipMemcpyHostToDevice, h->stream));
+    return EVP1D_OK;
+  };
+  auto up16 = [&](uint16_t **dst, const std::vector<uint16_t> &src) -> int {
+    HIP2(hipMalloc((void **)dst, std::max<size_t>(src.size(), 1) * sizeof(uint16_t)));
+    if (!src.empty()) HIP2(hipMemcpyAsync(*dst, src.data(), src.size() * 2, hipMemcpyHostToDevice, h->stream));
+    return EVP1D_OK;
+  };
+  HIP2(hipMalloc((void **)&h->tiles_dev, std::max<size_t>(tiles.size(), 1) * sizeof(Tile2)));
+  if (!tiles.empty())
+    HIP2(hipMemcpyAsync(h->tiles_dev, tiles.data(), tiles.size() * sizeof(Tile2), hipMemcpyHostToDevice, h->stream));
+  RC2(up32(&h->ftcell_dev, tcell));
+  RC2(up32(&h->fucell_dev, ucell));
+  RC2(up16(&h->ftlds_dev, tlds));
+  RC2(up16(&h->fulds_dev, ulds));
+  HIP2(hipStreamSynchronize(h->stream));   // the host vectors go out of scope
+  h->ntiles = (int)tiles.size();
   return EVP1D_OK;
 }
 
@@ -257,6 +458,7 @@ int ensure2(evp2d_handle *h) {
   RC2(up(&h->fill_dev, h->fill_h));
   RC2(up(&h->dst_dev, h->dst_h));
   RC2(up(&h->src_dev, h->src_h));
+  RC2(build_tiles2(h));
   // str outside the lists must read as zero (the reference clears it)
   if (h->lists_dirty)
     for (double2 *s : h->str) HIP2(hipMemsetAsync(s, 0, (size_t)h->n * sizeof(double2), h->stream));
@@ -295,6 +497,39 @@ int issue2(evp2d_handle *h, int nsub) {
     RC2(launch_stress2(h));
     RC2(launch_stepu2(h));
     RC2(launch_halo2(h));
+  }
+  return EVP1D_OK;
+}
+
+// halo update on the buffer a subcycle wrote (out)
+int launch_halo2_on(evp2d_handle *h, double2 *vel) {
+  if (!h->halo_on || h->nblocks <= 1) return EVP1D_OK;
+  const int nf = (int)h->fill_h.size(), nc = (int)h->dst_h.size();
+  if (nf > 0) hipLaunchKernelGGL(k2_fill, dim3(blocks2(nf)), dim3(kB2), 0, h->stream, vel, h->fill_dev, nf);
+  if (nc > 0) hipLaunchKernelGGL(k2_copy, dim3(blocks2(nc)), dim3(kB2), 0, h->stream, vel, h->dst_dev, h->src_dev, nc);
+  HIP2(hipGetLastError());
+  return EVP1D_OK;
+}
+
+// nsub fused subcycles.  The state starts in the handle's buffers (X), is
+// copied to the second set (Y), and the subcycles alternate so that the last
+// one writes X: subcycle k writes X when nsub - k is even.
+int issue_fused2(evp2d_handle *h, int nsub) {
+  const View2 X = h->v;
+  View2 Y = h->v;
+  Y.vel = h->vel_alt;
+  Y.sig = h->sig_alt;
+  HIP2(hipMemcpyAsync(Y.vel, X.vel, (size_t)h->n * sizeof(double2), hipMemcpyDeviceToDevice, h->stream));
+  HIP2(hipMemcpyAsync(Y.sig, X.sig, (size_t)12 * h->n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  for (int k = 1; k <= nsub; ++k) {
+    const bool to_x = ((nsub - k) % 2) == 0;
+    const View2 &in = to_x ? Y : X;
+    const View2 &out = to_x ? X : Y;
+    Fused2 f{h->tiles_dev, h->ftcell_dev, h->fucell_dev, h->ftlds_dev, h->fulds_dev, out.vel, out.sig};
+    if (h->ntiles > 0)
+      hipLaunchKernelGGL(k2_fused, dim3(h->ntiles), dim3(kB2F), 0, h->stream, in, f, k == nsub ? 1 : 0);
+    HIP2(hipGetLastError());
+    RC2(launch_halo2_on(h, out.vel));
   }
   return EVP1D_OK;
 }
@@ -551,8 +786,11 @@ int evp2d_run(evp2d_handle *h, int32_t nsub) {
   RC2(ensure2(h));
   if (h->graph_dirty || h->graph_nsub != nsub || !h->exec) {
     drop_graph2(h);
+    const bool fused = h->mode == EVP2D_MODE_FUSED;
+    if (fused && !h->vel_alt) HIP2(hipMalloc((void **)&h->vel_alt, (size_t)h->n * sizeof(double2)));
+    if (fused && !h->sig_alt) HIP2(hipMalloc((void **)&h->sig_alt, (size_t)12 * h->n * sizeof(double)));
     HIP2(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = issue2(h, nsub);
+    const int rc = fused ? issue_fused2(h, nsub) : issue2(h, nsub);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(h->stream, &g);
     if (rc) {
@@ -566,6 +804,22 @@ int evp2d_run(evp2d_handle *h, int32_t nsub) {
     h->graph_dirty = false;
   }
   HIP2(hipGraphLaunch(h->exec, h->stream));
+  return EVP1D_OK;
+}
+
+int evp2d_set_mode(evp2d_handle *h, int32_t mode) {
+  if (!h) return fail2(EVP1D_EINVAL, "evp2d_set_mode: NULL handle");
+  if (mode != EVP2D_MODE_PLAIN && mode != EVP2D_MODE_FUSED) return fail2(EVP1D_EINVAL, "evp2d_set_mode: mode %d", mode);
+  if (mode != h->mode) {
+    h->mode = mode;
+    h->graph_dirty = true;
+  }
+  return EVP1D_OK;
+}
+
+int evp2d_get_mode(evp2d_handle *h, int32_t *mode) {
+  if (!h || !mode) return fail2(EVP1D_EINVAL, "evp2d_get_mode: NULL argument");
+  *mode = h->mode;
   return EVP1D_OK;
 }
 
@@ -587,8 +841,14 @@ int evp2d_halo(evp2d_handle *h) {
 
 int evp2d_run_individual(evp2d_handle *h, int32_t which, int32_t nsub) {
   if (!h) return fail2(EVP1D_EINVAL, "evp2d_run_individual: NULL handle");
-  if (which != 0 && which != 1) return fail2(EVP1D_EINVAL, "evp2d_run_individual: which=%d", which);
+  if (which < 0 || which > 2) return fail2(EVP1D_EINVAL, "evp2d_run_individual: which=%d", which);
   RC2(ensure2(h));
+  if (which == 2) {   // fused subcycles launched one by one (no graph)
+    if (nsub <= 0) return EVP1D_OK;
+    if (!h->vel_alt) HIP2(hipMalloc((void **)&h->vel_alt, (size_t)h->n * sizeof(double2)));
+    if (!h->sig_alt) HIP2(hipMalloc((void **)&h->sig_alt, (size_t)12 * h->n * sizeof(double)));
+    return issue_fused2(h, nsub);
+  }
   for (int k = 0; k < nsub; ++k) RC2(which == 0 ? launch_stress2(h) : launch_stepu2(h));
   return EVP1D_OK;
 }

@@ -61,7 +61,7 @@ EXPORTS = (
     "evp2d_create", "evp2d_destroy", "evp2d_set_params", "evp2d_set_cells", "evp2d_set_halo",
     "evp2d_set_field", "evp2d_get_field", "evp2d_run", "evp2d_stress", "evp2d_stepu", "evp2d_halo",
     "evp2d_run_individual", "evp2d_synchronize", "evp2d_timer_start", "evp2d_timer_stop",
-    "evp2d_get_cells",
+    "evp2d_get_cells", "evp2d_set_mode", "evp2d_get_mode",
 )
 
 
@@ -167,6 +167,8 @@ def load(fma: bool = False, path: str | os.PathLike | None = None) -> ctypes.CDL
         "evp2d_timer_start": [_H],
         "evp2d_timer_stop": [_H, ctypes.POINTER(ctypes.c_float)],
         "evp2d_get_cells": [_H, _IP, _IP],
+        "evp2d_set_mode": [_H, i32],
+        "evp2d_get_mode": [_H, _IP],
         "evp1d_tile_plan": [i32, i32] + [_IP] * 8 + [i32, i32, i32] + [_IP] * 5 + [i32, _IP, i32, _IP, _IP, i32,
                                                                                 _IP, _IP],
     }
@@ -582,6 +584,15 @@ class Engine2D:
     def run(self, nsub: int) -> None:
         _check(self.L, self.L.evp2d_run(self.h, nsub))
 
+    def set_mode(self, mode: str) -> None:
+        """"fused" (default): one tile kernel per subcycle; "plain": stress then stepu."""
+        _check(self.L, self.L.evp2d_set_mode(self.h, {"plain": 0, "fused": 1}[mode]))
+
+    def get_mode(self) -> str:
+        m = ctypes.c_int32()
+        _check(self.L, self.L.evp2d_get_mode(self.h, ctypes.byref(m)))
+        return ("plain", "fused")[m.value]
+
     def stress(self) -> None:
         _check(self.L, self.L.evp2d_stress(self.h))
 
@@ -592,7 +603,7 @@ class Engine2D:
         _check(self.L, self.L.evp2d_halo(self.h))
 
     def run_individual(self, which: str, nsub: int) -> None:
-        _check(self.L, self.L.evp2d_run_individual(self.h, {"stress": 0, "stepu": 1}[which], nsub))
+        _check(self.L, self.L.evp2d_run_individual(self.h, {"stress": 0, "stepu": 1, "fused": 2}[which], nsub))
 
     def synchronize(self) -> None:
         _check(self.L, self.L.evp2d_synchronize(self.h))

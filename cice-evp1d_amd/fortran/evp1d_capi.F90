@@ -51,6 +51,7 @@ module evp1d_capi
     V_FORCEX = 19, V_FORCEY = 20, V_UMASSDTI = 21, V_FM = 22, V_STRINTX = 23, &
     V_STRINTY = 24, V_TBU = 25, V_STRESSP_1 = 26, V_STRESS12_4 = 37, V_UVEL_INIT = 38, &
     V_VVEL_INIT = 39, V_TAUBX = 40, V_TAUBY = 41, V_STR1 = 42, V_STR8 = 49
+  integer(c_int32_t), parameter, public :: MODE2D_PLAIN = 0, MODE2D_FUSED = 1
 
   public :: evp1d_set_evp_parameters, evp1d_create, evp1d_destroy, evp1d_set_params,   &
             evp1d_get_params, evp1d_get_sizes, evp1d_set_field, evp1d_get_field,        &
@@ -67,7 +68,7 @@ module evp1d_capi
             evp2d_create, evp2d_destroy, evp2d_set_params, evp2d_set_cells, evp2d_set_halo, &
             evp2d_set_field, evp2d_get_field, evp2d_run, evp2d_stress, evp2d_stepu,      &
             evp2d_halo, evp2d_run_individual, evp2d_synchronize, evp2d_timer_start,    &
-            evp2d_timer_stop, evp2d_get_cells
+            evp2d_timer_stop, evp2d_get_cells, evp2d_set_mode, evp2d_get_mode
 
   interface
     integer(c_int) function evp1d_set_evp_parameters(ndte, dt, p) bind(C, name='evp1d_set_evp_parameters')
@@ -384,6 +385,16 @@ module evp1d_capi
       import :: c_int, c_ptr, c_float
       type(c_ptr), value :: h
       real(c_float), intent(out) :: ms
+    end function
+    integer(c_int) function evp2d_set_mode(h, mode) bind(C, name='evp2d_set_mode')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: mode
+    end function
+    integer(c_int) function evp2d_get_mode(h, mode) bind(C, name='evp2d_get_mode')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(out) :: mode
     end function
     integer(c_int) function evp2d_get_cells(h, nt, nu) bind(C, name='evp2d_get_cells')
       import :: c_int, c_int32_t, c_ptr

@@ -18,7 +18,7 @@
 ! src/vars.F90:298-314) and testscale must be 1 (resize_2d is a timing-only
 ! device).
 !
-! Usage:  evp1d_v0_driver
+! Usage:  evp1d_v0_driver [fused|plain]   (the subcycle form, default fused)
 !===============================================================================
 program evp1d_v0_driver
   use, intrinsic :: iso_c_binding
@@ -39,8 +39,19 @@ program evp1d_v0_driver
   type(c_ptr) :: h
   integer :: lun, ios, k, iblk, ij, i, j, ncell
   integer(8) :: t0, t1, rate, n
+  character(len=32) :: arg
+  integer(c_int32_t) :: mode
   integer, parameter :: nfile = 38   ! arrays of input_double_2d.bin
 
+  mode = MODE2D_FUSED
+  if (command_argument_count() >= 1) then
+    call get_command_argument(1, arg)
+    select case (trim(arg))
+    case ('fused'); mode = MODE2D_FUSED
+    case ('plain'); mode = MODE2D_PLAIN
+    case default;   error stop 'usage: evp1d_v0_driver [fused|plain]'
+    end select
+  endif
   ndte = 1; inpfname = 'pathtoinputfile'; nb = -1; na = -1; navel = -1
   nx_block = -1; ny_block = -1; testscale = 1; lconvert21d = .false.
   lindividual = .false.; binoutput = .true.; max_blocks = 1; nghost = 1
@@ -78,6 +89,7 @@ program evp1d_v0_driver
   p%capping = scal(1); p%e_factor = scal(2); p%epp2i = scal(3)
   call evp1d_check(evp2d_create(int(nx_block,c_int32_t), int(ny_block,c_int32_t), &
                                 int(max_blocks,c_int32_t), p, h), 'evp2d_create')
+  call evp1d_check(evp2d_set_mode(h, mode), 'evp2d_set_mode')
   do k = 1, nfile
     call evp1d_check(evp2d_set_field(h, int(k-1,c_int32_t), f(:,k), n), 'evp2d_set_field')
   enddo

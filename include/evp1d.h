@@ -377,10 +377,20 @@ int evp2d_get_field(evp2d_handle *h, int32_t field, double *a, int64_t n);
  * max_blocks > 1 and tables are set (src/evp_v0.F90:93-136); a captured
  * hipGraph, asynchronous.                                                   */
 int evp2d_run(evp2d_handle *h, int32_t nsub);
+/* How evp2d_run runs a subcycle: EVP2D_MODE_FUSED (default) runs stress and
+ * stepu of one 32 x 8 tile of cells per workgroup with str1..8 in LDS (the
+ * T cells of the column east and the row north of the tile evaluated again),
+ * velocities and stresses alternating between two buffers; EVP2D_MODE_PLAIN
+ * runs the two kernels one after the other, as the reference does.  Results
+ * are identical.                                                            */
+enum evp2d_mode { EVP2D_MODE_PLAIN = 0, EVP2D_MODE_FUSED = 1 };
+int evp2d_set_mode(evp2d_handle *h, int32_t mode);
+int evp2d_get_mode(evp2d_handle *h, int32_t *mode);
 int evp2d_stress(evp2d_handle *h);
 int evp2d_stepu(evp2d_handle *h);
 int evp2d_halo(evp2d_handle *h);
-/* the lindividual loops of evp_v0.F90:157-204: which = 0 stress, 1 stepu   */
+/* the lindividual loops of evp_v0.F90:157-204: which = 0 stress, 1 stepu;
+ * which = 2 runs nsub fused subcycles launched one by one (timing)          */
 int evp2d_run_individual(evp2d_handle *h, int32_t which, int32_t nsub);
 int evp2d_synchronize(evp2d_handle *h);
 int evp2d_timer_start(evp2d_handle *h);

@@ -141,8 +141,9 @@ def test_api_driver_matches_reference_dataflow(gpu, nx, ny, seed, ndte, nrun, de
         assert np.array_equal(a.view(np.uint64), np.asfortranarray(b).view(np.uint64)), n
 
 
+@pytest.mark.parametrize("mode", ["fused", "plain"])
 @pytest.mark.parametrize("case", [c for c in __import__("gpu_common").golden_cases_v0()])
-def test_v0_driver_reproduces_reference_files(gpu, case):
+def test_v0_driver_reproduces_reference_files(gpu, case, mode):
     """evp1d_v0_driver on the reference's own v0 input files: both 2D output
     files byte-identical to the reference v0's; with lconvert21d the 1D files
     hold block 1's cells of the same arrays (convert_2d_1d, vars.F90:534-584)."""
@@ -152,7 +153,7 @@ def test_v0_driver_reproduces_reference_files(gpu, case):
         O.write_inputs_v0(d, st, ndte)
         nml = (Path(d) / "kernel.nml").read_text().replace("/\n", " lconvert21d = .true.\n/\n")
         (Path(d) / "kernel.nml").write_text(nml)
-        r = _run(FBIN / "evp1d_v0_driver", d)
+        r = _run(FBIN / "evp1d_v0_driver", d, mode)
         assert "Benchmark took:" in r.stdout
         got_stress = (Path(d) / "output_stress_v0_2d.bin").read_bytes()
         got_stepu = (Path(d) / "output_stepu_v0_2d.bin").read_bytes()

@@ -30,8 +30,12 @@ def _params(st, ndte):
     return p
 
 
-def _engine(st, ndte):
+MODES2D = ("fused", "plain")
+
+
+def _engine(st, ndte, mode="fused"):
     eng = evp1d.Engine2D(st.nxb, st.nyb, st.nblocks, _params(st, ndte))
+    eng.set_mode(mode)
     eng.load_state(st)
     return eng
 
@@ -47,11 +51,12 @@ def _bits_equal(a, b):
     return np.array_equal(np.asarray(a).view(np.uint64), np.asarray(b).view(np.uint64))
 
 
+@pytest.mark.parametrize("mode", MODES2D)
 @pytest.mark.parametrize("case", golden_cases_v0())
-def test_v0_golden_outputs_bitexact(gpu, case):
-    _log(f"v0 golden {case}")
+def test_v0_golden_outputs_bitexact(gpu, case, mode):
+    _log(f"v0 golden {case} {mode}")
     st, ndte, want_a, want_b = load_golden_v0(case)
-    with _engine(st, ndte) as eng:
+    with _engine(st, ndte, mode) as eng:
         eng.run(ndte)
         got = _fetch(eng, st)
     a, b = O.output_bytes_v0(got)
@@ -59,11 +64,12 @@ def test_v0_golden_outputs_bitexact(gpu, case):
     assert O.md5(b) == O.md5(want_b)
 
 
-def test_v0_c0_md5(gpu):
+@pytest.mark.parametrize("mode", MODES2D)
+def test_v0_c0_md5(gpu, mode):
     rec = json.loads((GOLD / "golden.json").read_text())["evp_c0_v0"]
     g = synth.make_grid(rec["nx"], rec["ny"], rec["seed"])
     st = O.pack_v0(g, 1)
-    with _engine(st, rec["ndte"]) as eng:
+    with _engine(st, rec["ndte"], mode) as eng:
         eng.run(rec["ndte"])
         got = _fetch(eng, st)
     a, b = O.output_bytes_v0(got)
@@ -71,8 +77,9 @@ def test_v0_c0_md5(gpu):
     assert O.md5(b) == rec["output_stepu_md5"]
 
 
-@pytest.mark.parametrize("nblocks,ndte", [(1, 7), (3, 12), (4, 1)])
-def test_v0_every_field_matches_the_oracle(gpu, nblocks, ndte):
+@pytest.mark.parametrize("mode", MODES2D)
+@pytest.mark.parametrize("nblocks,ndte", [(1, 7), (3, 12), (4, 1), (2, 2)])
+def test_v0_every_field_matches_the_oracle(gpu, nblocks, ndte, mode):
     """All arrays after the run, taubx/tauby included; with one block also
     the str1..8 scratch (with several the reference keeps only one block of
     it, the engine one per block)."""
@@ -82,8 +89,8 @@ def test_v0_every_field_matches_the_oracle(gpu, nblocks, ndte):
     V = O.OracleV0()
     p = V.params(ndte)
     V.run(p, want, ndte)
-    _log(f"v0 fields nblocks={nblocks} ndte={ndte}")
-    with _engine(st, ndte) as eng:
+    _log(f"v0 fields nblocks={nblocks} ndte={ndte} {mode}")
+    with _engine(st, ndte, mode) as eng:
         eng.run(ndte)
         for n in O.V0_FIELDS:
             assert _bits_equal(eng.get_field(n), want.f[n]), n
@@ -112,14 +119,15 @@ def test_v0_every_field_matches_the_oracle(gpu, nblocks, ndte):
                 assert _bits_equal(got, planes[k]), f"str{k + 1}"
 
 
+@pytest.mark.parametrize("mode", MODES2D)
 @pytest.mark.parametrize("nx,ny,seed,ndte", [(64, 48, 51, 30), (120, 40, 52, 15)])
-def test_v0_engine_equals_1d_engine(gpu, nx, ny, seed, ndte):
+def test_v0_engine_equals_1d_engine(gpu, nx, ny, seed, ndte, mode):
     """The same grid through both device paths gives the same bits."""
     g = synth.make_grid(nx, ny, seed)
     st = O.pack_v0(g, 1)
     ix = evp1d.build_index(g.tmask, g.nghost)
     s1 = O.pack_dyn(g, ix)
-    with _engine(st, ndte) as e2:
+    with _engine(st, ndte, mode) as e2:
         e2.run(ndte)
         got2 = _fetch(e2, st)
     with make_engine(s1, ndte) as e1:
@@ -134,15 +142,16 @@ def test_v0_engine_equals_1d_engine(gpu, nx, ny, seed, ndte):
         assert _bits_equal(got2.f[n2][:, :, 0][i, j][keep], got1.f[n1][:s1.na][keep]), n2
 
 
-def test_v0_single_calls_and_restart(gpu):
+@pytest.mark.parametrize("mode", MODES2D)
+def test_v0_single_calls_and_restart(gpu, mode):
     """stress/stepu/halo one by one equal run; a run continues from the end
-    state; lindividual-style loops leave the run's results alone."""
+    state, in either form, odd and even subcycle counts."""
     g = synth.make_grid(40, 30, 53)
     st = O.pack_v0(g, 3)
     want = st.copy()
     V = O.OracleV0()
     V.run(V.params(9), want, 9)
-    with _engine(st, 9) as eng:
+    with _engine(st, 9, mode) as eng:
         for _ in range(4):
             eng.stress()
             eng.stepu()
@@ -202,3 +211,22 @@ def test_v0_bad_inputs_rejected(gpu):
         V.run(V.params(3), want, 3)
         eng.run(3)
         assert _bits_equal(eng.get_field("uvel"), want.f["uvel"])
+
+
+def test_v0_fused_one_by_one_and_mode_switch(gpu):
+    """Fused subcycles launched one by one (the bench's kernel timing) and a
+    switch of form between runs keep the results."""
+    g = synth.make_grid(70, 45, 56)
+    st = O.pack_v0(g, 2)
+    want = st.copy()
+    V = O.OracleV0()
+    V.run(V.params(11), want, 11)
+    with _engine(st, 11, "fused") as eng:
+        eng.run_individual("fused", 4)
+        eng.set_mode("plain")
+        assert eng.get_mode() == "plain"
+        eng.run(3)
+        eng.set_mode("fused")
+        eng.run(4)
+        for n in O.V0_FIELDS:
+            assert _bits_equal(eng.get_field(n), want.f[n]), n
