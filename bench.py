@@ -524,7 +524,13 @@ def main() -> int:
     traffic = None
     tf = REPO / "profiles" / f"traffic_{args.workload}.json"
     if tf.exists():
-        traffic = json.loads(tf.read_text()).get(dom["name"])
+        # per-launch HBM bytes from rocprofv3 FETCH_SIZE/WRITE_SIZE (profiles/); the fused
+        # kernel's depend on the stress storage the run used
+        tab = json.loads(tf.read_text())
+        key = dom["name"]
+        if key == "k_fused":
+            key += "@inplace" if tiling.get("inplace") else "@twobuf"
+        traffic = tab.get(key)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom["name"],
                 "bytes_per_launch": dom["bytes"]}
