@@ -164,3 +164,35 @@ def test_repeated_gradient_calls_on_one_handle(problem):
             for k in ("strength", "cdn_ocn", "forcex", "forcey"):
                 assert np.array_equal(g[k], ref[k]), (nsnap, k)
             assert g["loss"] == ref["loss"]
+
+
+@pytest.mark.parametrize("nsnap", [NDTE, 5])
+def test_batch_gradient_equals_member_gradients(gpu, nsnap):
+    """A batched handle's gradient: each member's part is bit-identical to the
+    member's own single-domain gradient (members are independent), and the
+    loss is the sum of the members' losses."""
+    sizes = [(30, 22, 121), (16, 12, 122), (40, 18, 123)]
+    states = [synthetic_state(nx, ny, seed) for nx, ny, seed in sizes]
+    targets, singles = [], []
+    for st in states:
+        tgt = st.copy()
+        tgt.f["strength"] *= 1.2
+        with make_engine(tgt, NDTE) as e:
+            e.run(NDTE)
+            targets.append((e.get_field("uvel")[:st.na], e.get_field("vvel")[:st.na]))
+        with make_engine(st, NDTE) as e:
+            singles.append(e.adjoint_run(NDTE, *targets[-1], nsnap=nsnap))
+    params = evp1d.set_evp_parameters(NDTE, **states[0].scalars)
+    with evp1d.Engine.batch([(st.na, st.navel) for st in states], params) as eng:
+        for k, st in enumerate(states):
+            eng.load_member(k, st.f, st.idx, st.skipT, st.skipU)
+        ut = np.concatenate([t[0] for t in targets])
+        vt = np.concatenate([t[1] for t in targets])
+        g = eng.adjoint_run(NDTE, ut, vt, nsnap=nsnap)
+    off = 0
+    for k, st in enumerate(states):
+        for key in ("strength", "cdn_ocn", "forcex", "forcey"):
+            assert np.array_equal(g[key][off:off + st.na], singles[k][key]), (k, key)
+        off += st.na
+    want = sum(s["loss"] for s in singles)
+    assert abs(g["loss"] - want) <= 1e-12 * want

@@ -230,3 +230,26 @@ def test_v0_fused_one_by_one_and_mode_switch(gpu):
         eng.run(4)
         for n in O.V0_FIELDS:
             assert _bits_equal(eng.get_field(n), want.f[n]), n
+
+
+@pytest.mark.parametrize("mode", MODES2D)
+def test_v0_fma_build_within_1e12(gpu, mode):
+    """libevp1d_fma.so's 2D path (contraction on): max|d|/max|ref| <= 1e-12
+    per array against the exact oracle."""
+    g = synth.make_grid(60, 40, 57)
+    st = O.pack_v0(g, 2)
+    want = st.copy()
+    V = O.OracleV0()
+    V.run(V.params(25), want, 25)
+    eng = evp1d.Engine2D(st.nxb, st.nyb, st.nblocks, _params(st, 25), fma=True)
+    with eng:
+        eng.set_mode(mode)
+        eng.load_state(st)
+        eng.run(25)
+        worst = 0.0
+        for n in O.V0_FIELDS:
+            a, b = eng.get_field(n), want.f[n]
+            den = np.max(np.abs(b))
+            if den > 0:
+                worst = max(worst, float(np.max(np.abs(a - b)) / den))
+    assert worst <= 1e-12, worst
