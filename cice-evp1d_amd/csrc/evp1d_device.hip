@@ -357,6 +357,190 @@ __global__ __launch_bounds__(B) void k_fused(DevView d, FusedView f, int last) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// two subcycles per workgroup ("pair" form)
+// ---------------------------------------------------------------------------
+// Level 1 runs stress on the tile's T1 entries (owned, halo, T1-only) from the
+// input state d.vel / d.sig, then stepu on its U1 entries into an LDS
+// velocity table; level 2 runs stress on the owned and halo entries from
+// those velocities and the level-1 stresses, then stepu on the owned U
+// points, and writes the state two subcycles on to vel_out / sig_out (two
+// buffers each).  Every point computes exactly what the one-subcycle kernels
+// compute, so results are bit-identical; neighbour tiles evaluate the border
+// rings again instead of exchanging them, which halves the loads and stores
+// of the state and the constant fields per subcycle.  Level-2 stress entry k
+// is thread k (the plan guarantees at most B of them), so its level-1
+// stresses stay in that thread's registers.  The 20 constant fields come
+// from a copy packed in pairs at the start of the run (k_pack_const), one
+// 16-byte load per pair.
+struct PairView {
+  const evp1d::PairTile *tiles;
+  const int32_t *hlist, *xlist, *elist;
+  const uint2 *uloc;        // per U1 entry: own | nw << 16, sse | sw << 16 (str table entries)
+  const uint2 *vloc;        // per T2 entry: cc | ee << 16, ne | se << 16 (velocity table entries)
+  const double2 *cstr;      // [4][nk]: s15, s27, s36, s48 of the constant entries
+  const int *cnz;
+  long nk;
+  double2 *vel_out;
+  double *sig_out;
+  int cap;                  // entries of the str table (the velocity table follows it)
+  const double2 *pc;        // [kPairConst][lp]: the constant fields in pairs (k_pack_const)
+  long lp;
+};
+
+// The constant fields in pairs, one double2 array each: (dxT, dyT), (hte,
+// htn), (htem1, htnm1), (cw, aiu), (uocn, vocn), (waterx, watery), (forcex,
+// forcey), (umassdti, fm), (uarear, tbu), (uvel_init, vvel_init).
+constexpr int kPairConst = 10;
+__global__ void k_pack_const(DevView d, double2 *pc, long lp) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.na) return;
+  pc[i] = make_double2(d.dxT[i], d.dyT[i]);
+  pc[lp + i] = make_double2(d.hte[i], d.htn[i]);
+  pc[2 * lp + i] = make_double2(d.htem1[i], d.htnm1[i]);
+  pc[3 * lp + i] = make_double2(d.cw[i], d.aiu[i]);
+  pc[4 * lp + i] = make_double2(d.uocn[i], d.vocn[i]);
+  pc[5 * lp + i] = make_double2(d.waterx[i], d.watery[i]);
+  pc[6 * lp + i] = make_double2(d.forcex[i], d.forcey[i]);
+  pc[7 * lp + i] = make_double2(d.umassdti[i], d.fm[i]);
+  pc[8 * lp + i] = make_double2(d.uarear[i], d.tbu[i]);
+  pc[9 * lp + i] = make_double2(d.uvel_init[i], d.vvel_init[i]);
+}
+
+// At most 128 VGPRs (4 waves per SIMD): workgroups of 256, 512 or 1024
+// threads then fill 16 waves per CU.
+template <int B, bool X>
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_pair(DevView d, PairView f, int last) {
+  extern __shared__ double2 lds[];
+  const int cap = f.cap;
+  double2 *vt = lds + 4 * cap;
+  const evp1d::PairTile pt = f.tiles[tile_of(blockIdx.x, gridDim.x, X)];
+  const evp1d::TileDesc &t = pt.t;
+  const int nA = t.a1 - t.a0, nB = t.b1 - t.b0, nO = nA + nB, nS = nO + t.nh, nT = nS + pt.nx, nE = nT + pt.nq;
+  const int nUo = nB + (t.c1 - t.c0), nU = nUo + pt.ne;
+  const bool cz = *f.cnz == 0;
+  auto tslot = [&](int k) {
+    return k < nA ? t.a0 + k : k < nO ? t.b0 + (k - nA) : k < nS ? f.hlist[t.h0 + (k - nO)] : f.xlist[pt.x0 + (k - nS)];
+  };
+  auto uslot = [&](int k) { return k < nB ? t.b0 + k : k < nUo ? t.c0 + (k - nB) : f.elist[pt.e0 + (k - nUo)]; };
+  auto put = [&](int k, const evp1d::StressOut &o) {
+    lds[k] = make_double2(o.str[0], o.str[4]);
+    lds[cap + k] = make_double2(o.str[1], o.str[6]);
+    lds[2 * cap + k] = make_double2(o.str[2], o.str[5]);
+    lds[3 * cap + k] = make_double2(o.str[3], o.str[7]);
+  };
+  const double2 *pc = f.pc;
+  const long lp = f.lp;
+  auto stepu = [&](int k, int q, double2 v, evp1d::StepuOut &o) {
+    const uint2 w = f.uloc[pt.u0 + k];
+    const double2 a = lds[w.x & 0xffffu];
+    const double2 b = lds[cap + (w.x >> 16)];
+    const double2 c = lds[2 * cap + (w.y & 0xffffu)];
+    const double2 e = lds[3 * cap + (w.y >> 16)];
+    const double2 f0 = pc[3 * lp + q], f1 = pc[4 * lp + q], f2 = pc[5 * lp + q], f3 = pc[6 * lp + q];
+    const double2 f4 = pc[7 * lp + q], f5 = pc[8 * lp + q], f6 = pc[9 * lp + q];
+    evp1d::stepu_point(d.s, v.x, v.y, f0.x, f0.y, f1.x, f1.y, f2.x, f2.y, f3.x, f3.y, f4.x, f4.y, f5.x, f6.x, f6.y,
+                       f5.y, a.x, b.x, c.x, e.x, a.y, c.y, b.y, e.y, o);
+  };
+  // stress at slot g from the velocities cc, ee, ne, se and the stresses sig
+  auto stress = [&](int g, double2 cc, double2 ve, double2 vn, double2 vs, const double *sig, evp1d::StressOut &o) {
+    const double2 m0 = pc[g], m1 = pc[lp + g], m2 = pc[2 * lp + g];
+    evp1d::stress_point(d.s, cc.x, cc.y, ve.x, ve.y, vs.x, vs.y, vn.x, vn.y, m0.x, m0.y, d.strength[g], m1.x, m1.y,
+                        m2.x, m2.y, sig, o);
+  };
+  const int me = threadIdx.x;
+  double sg[12];   // level-1 stresses of entry `me` when me < nS
+  // level 1, stress (or constants) of entry k
+  auto level1 = [&](int k) {
+    if (k < nT) {
+      const int g = tslot(k);
+      double sig[12];
+#pragma unroll
+      for (int c = 0; c < 12; ++c) sig[c] = d.sig[c * d.ldsig + g];
+      evp1d::StressOut o;
+      stress(g, d.vel[g], d.vel[d.ee[g]], d.vel[d.ne[g]], d.vel[d.se[g]], sig, o);
+      if (k == me) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          sg[c] = o.sp[c];
+          sg[4 + c] = o.sm[c];
+          sg[8 + c] = o.s12[c];
+        }
+      }
+      put(k, o);
+    } else {
+      double2 p15, p27, p36, p48;
+      if (cz) {
+        p15 = p27 = p36 = p48 = make_double2(0.0, 0.0);
+      } else {
+        const long j = pt.q0 + (k - nT);
+        p15 = f.cstr[j];
+        p27 = f.cstr[f.nk + j];
+        p36 = f.cstr[2 * f.nk + j];
+        p48 = f.cstr[3 * f.nk + j];
+      }
+      lds[k] = p15;
+      lds[cap + k] = p27;
+      lds[2 * cap + k] = p36;
+      lds[3 * cap + k] = p48;
+    }
+  };
+  // entries me + B, me + 2B, ... first and entry me (kept for level 2) last,
+  // so no other evaluation overlaps the lifetime of its stresses
+  for (int k = me + B; k < nE; k += B) level1(k);
+  if (me < nE) level1(me);
+  __syncthreads();
+  // level 1, stepu: every U1 entry into the velocity table
+  for (int k = me; k < nU; k += B) {
+    const int q = uslot(k);
+    evp1d::StepuOut o;
+    stepu(k, q, d.vel[q], o);
+    vt[k] = make_double2(o.uvel, o.vvel);
+  }
+  __syncthreads();
+  // level 2, stress: owned and halo entries, one per thread
+  if (me < nS) {
+    const int g = tslot(me);
+    const uint2 w = f.vloc[pt.v0 + me];
+    const unsigned e0 = w.x & 0xffffu, e1 = w.x >> 16, e2 = w.y & 0xffffu, e3 = w.y >> 16;
+    constexpr unsigned kNo = evp1d::PairPlan::kNoEntry;
+    const double2 cc = e0 == kNo ? d.vel[g] : vt[e0];
+    const double2 ve = e1 == kNo ? d.vel[d.ee[g]] : vt[e1];
+    const double2 vn = e2 == kNo ? d.vel[d.ne[g]] : vt[e2];
+    const double2 vs = e3 == kNo ? d.vel[d.se[g]] : vt[e3];
+    evp1d::StressOut o;
+    stress(g, cc, ve, vn, vs, sg, o);
+    if (me < nO) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        f.sig_out[c * d.ldsig + g] = o.sp[c];
+        f.sig_out[(4 + c) * d.ldsig + g] = o.sm[c];
+        f.sig_out[(8 + c) * d.ldsig + g] = o.s12[c];
+      }
+      if (last) {
+        d.s15[g] = make_double2(o.str[0], o.str[4]);
+        d.s27[g] = make_double2(o.str[1], o.str[6]);
+        d.s36[g] = make_double2(o.str[2], o.str[5]);
+        d.s48[g] = make_double2(o.str[3], o.str[7]);
+      }
+    }
+    put(me, o);
+  }
+  __syncthreads();
+  // level 2, stepu: the owned U points
+  for (int k = me; k < nUo; k += B) {
+    const int q = uslot(k);
+    evp1d::StepuOut o;
+    stepu(k, q, vt[k], o);
+    f.vel_out[q] = make_double2(o.uvel, o.vvel);
+    if (last) {
+      d.strintx[q] = o.strintx;
+      d.strinty[q] = o.strinty;
+      d.cb[q] = o.Cb;
+    }
+  }
+}
+
 // str1..8 of the constant entries, gathered once per run; *cnz (cleared by
 // k_clear_flag first) becomes 1 if any of them has a bit set
 __global__ void k_clear_flag(int *flag) { *flag = 0; }
@@ -391,6 +575,10 @@ __global__ void k_fill_const(DevView d, const int32_t *klist, double2 *cstr, lon
 
 constexpr int kFusedMaxEntries = 2560;   // 160 KiB of LDS per workgroup (gfx950)
 constexpr int kFusedMinSub = 4;          // evp1d_run in AUTO mode: fused from this many subcycles
+// pair form: block sizes (one level-2 stress per thread) and LDS limit
+constexpr int kNumPairBlocks = 3;
+constexpr int kPairBlocks[kNumPairBlocks] = {256, 512, 1024};
+constexpr int kPairMaxLds = 160 * 1024;
 
 // launch configurations the autotuner chooses from
 constexpr int kNumBlocks = 5;
@@ -777,6 +965,18 @@ struct evp1d_handle {
   double2 *vel_alt = nullptr;      // second velocity buffer (navel)
   double *sig_alt = nullptr;       // second stress buffer (12 x ldsig), two-buffer fused runs
   int fused_inplace = 0;           // fused runs update stresses in place
+  // pair form (two subcycles per fused launch): its tables, planned on the
+  // current tiling when fused_depth is 2
+  int fused_depth = 1;
+  bool pair_ok = false;
+  std::string pair_why = "pair form not selected";
+  int pair_block = 0, pair_cap = 0, pair_capv = 0;
+  long pk_total = 0;
+  evp1d::PairTile *ptiles_dev = nullptr;
+  int32_t *pxlist_dev = nullptr, *pelist_dev = nullptr, *pklist_dev = nullptr;
+  uint2 *puloc_dev = nullptr, *pvloc_dev = nullptr;
+  double2 *pcstr_dev = nullptr;
+  double2 *pconst = nullptr;       // constant fields in pairs, [kPairConst][na], packed per run
   void *traj = nullptr;            // gradient trajectory storage, kept between calls
   size_t traj_bytes = 0;
   // 2D grid of the caller's arrays (device-side pack/unpack)
@@ -819,11 +1019,26 @@ static void drop_graph(evp1d_handle *h) {
   h->graph_nsub = -1;
 }
 
+static void free_pairs(evp1d_handle *h) {
+  void *bufs[7] = {h->ptiles_dev, h->pxlist_dev, h->pelist_dev, h->pklist_dev,
+                   h->puloc_dev,  h->pvloc_dev,  h->pcstr_dev};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  h->ptiles_dev = nullptr;
+  h->pxlist_dev = h->pelist_dev = h->pklist_dev = nullptr;
+  h->puloc_dev = h->pvloc_dev = nullptr;
+  h->pcstr_dev = nullptr;
+  h->pair_ok = false;
+  h->pair_block = h->pair_cap = h->pair_capv = 0;
+  h->pk_total = 0;
+}
+
 static void free_tiling(evp1d_handle *h) {
   void *bufs[8] = {h->tiles_dev, h->hlist_dev, h->klist_dev, h->uloc_dev,
                    h->cstr_dev,  h->hid_dev,   h->shared_dev, h->hsig_dev};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
+  free_pairs(h);
   h->tiles_dev = nullptr;
   h->hlist_dev = h->klist_dev = nullptr;
   h->hid_dev = h->shared_dev = nullptr;
@@ -843,6 +1058,7 @@ static void release(evp1d_handle *h) {
   free_tiling(h);
   if (h->vel_alt) (void)hipFree(h->vel_alt);
   if (h->sig_alt) (void)hipFree(h->sig_alt);
+  if (h->pconst) (void)hipFree(h->pconst);
   if (h->cnz_dev) (void)hipFree(h->cnz_dev);
   if (h->traj) (void)hipFree(h->traj);
   if (h->pos2d_dev) (void)hipFree(h->pos2d_dev);
@@ -981,6 +1197,45 @@ static int install_tiling(evp1d_handle *h, const evp1d::TilePlan &plan) {
   return EVP1D_OK;
 }
 
+// The pair form's tables on the tiling just installed (when selected).
+static int install_pairs(evp1d_handle *h, const evp1d::TilePlan &plan, const int32_t *const idx[6]) {
+  free_pairs(h);
+  if (h->fused_depth != 2) {
+    h->pair_why = "pair form not selected";
+    return EVP1D_OK;
+  }
+  const evp1d::PairPlan q = evp1d::plan_pairs(plan, h->na, h->navel, idx, h->skip_host.data(), kPairMaxLds / 64,
+                                              kPairMaxLds / 16, kPairBlocks[kNumPairBlocks - 1]);
+  if (!q.ok) {
+    h->pair_why = q.why;
+    return EVP1D_OK;
+  }
+  const std::string bad = evp1d::check_pairs(q, plan, h->na, h->navel, idx, h->skip_host.data());
+  if (!bad.empty()) return fail(EVP1D_EINDEX, "pair plan: %s", bad.c_str());
+  const size_t lds = (size_t)(4 * std::max(q.cap_s, 1) + std::max(q.cap_v, 1)) * sizeof(double2);
+  if (lds > (size_t)kPairMaxLds) {
+    h->pair_why = "tables need " + std::to_string(lds) + " bytes of LDS";
+    return EVP1D_OK;
+  }
+  int b = 0;
+  while (kPairBlocks[b] < q.max_t2) ++b;   // max_t2 <= the largest block (plan limit)
+  RC_TRY(upload_table(h, &h->ptiles_dev, q.tiles.data(), q.tiles.size()));
+  RC_TRY(upload_table(h, &h->pxlist_dev, q.xlist.data(), q.xlist.size()));
+  RC_TRY(upload_table(h, &h->pelist_dev, q.elist.data(), q.elist.size()));
+  RC_TRY(upload_table(h, &h->pklist_dev, q.klist.data(), q.klist.size()));
+  RC_TRY(upload_table(h, &h->puloc_dev, reinterpret_cast<const uint2 *>(q.uloc.data()), q.uloc.size() / 4));
+  RC_TRY(upload_table(h, &h->pvloc_dev, reinterpret_cast<const uint2 *>(q.vloc.data()), q.vloc.size() / 4));
+  h->pk_total = (long)q.klist.size();
+  HIP_TRY(hipMalloc((void **)&h->pcstr_dev, std::max<size_t>((size_t)4 * h->pk_total, 1) * sizeof(double2)));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->pair_block = kPairBlocks[b];
+  h->pair_cap = std::max(q.cap_s, 1);
+  h->pair_capv = std::max(q.cap_v, 1);
+  h->pair_ok = true;
+  h->pair_why.clear();
+  return EVP1D_OK;
+}
+
 // Rebuild the slot order (class, then fused tile, then caller position) from
 // the skip flags and the neighbour lists, and move every resident array into
 // it.  Without a complete set of lists the order is class, caller position.
@@ -1051,6 +1306,7 @@ static int relayout(evp1d_handle *h) {
     h->perm_dev_dirty = true;
   }
   RC_TRY(install_tiling(h, plan));
+  RC_TRY(install_pairs(h, plan, all ? idx : nullptr));
   h->v.t_end = plan.cnt[0] + plan.cnt[1];
   h->v.u_begin = plan.cnt[0];
   h->v.u_end = plan.cnt[0] + plan.cnt[1] + plan.cnt[2];
@@ -1147,6 +1403,19 @@ static const FusedFn kFusedFns[4][2] = {{k_fused<64, false>, k_fused<64, true>},
                                         {k_fused<512, false>, k_fused<512, true>}};
 static const FusedFn kFusedSplitFns[4] = {k_fused<64, true, true>, k_fused<128, true, true>,
                                          k_fused<256, true, true>, k_fused<512, true, true>};
+typedef void (*PairFn)(DevView, PairView, int);
+static const PairFn kPairFns[kNumPairBlocks][2] = {{k_pair<256, false>, k_pair<256, true>},
+                                                   {k_pair<512, false>, k_pair<512, true>},
+                                                   {k_pair<1024, false>, k_pair<1024, true>}};
+static int pair_block_index(int b) {
+  for (int i = 0; i < kNumPairBlocks; ++i)
+    if (kPairBlocks[i] == b) return i;
+  return -1;
+}
+// the pair form runs when selected, planned, and no ghost copy-list needs
+// to run between the two subcycles of a launch
+static bool pair_form(const evp1d_handle *h) { return h->fused_depth == 2 && h->pair_ok && h->v.nhalo == 0; }
+
 static int fused_block_index(int b) {
   for (int i = 0; i < 4; ++i)
     if (kFusedBlocks[i] == b) return i;
@@ -1164,10 +1433,15 @@ static int ensure_fused_buffers(evp1d_handle *h) {
     for (FusedFn fn : kFusedSplitFns)
       HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   kFusedMaxEntries * 4 * (int)sizeof(double2)));
+    for (auto &row : kPairFns)
+      for (PairFn fn : row)
+        HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, kPairMaxLds));
   }
   if (!h->vel_alt) HIP_TRY(hipMalloc((void **)&h->vel_alt, (size_t)h->navel * sizeof(double2)));
-  if (!h->fused_inplace && !h->sig_alt)
+  if ((!h->fused_inplace || pair_form(h)) && !h->sig_alt)
     HIP_TRY(hipMalloc((void **)&h->sig_alt, (size_t)12 * h->ldsig * sizeof(double)));
+  if (pair_form(h) && !h->pconst)
+    HIP_TRY(hipMalloc((void **)&h->pconst, (size_t)kPairConst * std::max(h->na, 1) * sizeof(double2)));
   return EVP1D_OK;
 }
 
@@ -1175,8 +1449,7 @@ static int ensure_fused_buffers(evp1d_handle *h) {
 // Before the first fused subcycle of a run: the constant entries' str1..8
 // and, for an in-place run, the shared points' stresses (into copy 0).
 // Fills the run's view; fused_step points it at subcycle k.
-static int fused_prologue(evp1d_handle *h, const DevView &d, FusedView &f) {
-  const bool ip = h->fused_inplace != 0;
+static int fused_prologue(evp1d_handle *h, const DevView &d, FusedView &f, bool ip) {
   hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, h->stream, h->cnz_dev);
   HIP_TRY(hipGetLastError());
   if (h->nk_total > 0) {
@@ -1196,8 +1469,8 @@ static int fused_prologue(evp1d_handle *h, const DevView &d, FusedView &f) {
 
 // After nsub in-place subcycles: the shared points' stresses (copy nsub % 2)
 // back into d.sig.
-static int fused_epilogue(evp1d_handle *h, const DevView &d, int nsub) {
-  if (!h->fused_inplace || h->nsh == 0) return EVP1D_OK;
+static int fused_epilogue(evp1d_handle *h, const DevView &d, int nsub, bool ip) {
+  if (!ip || h->nsh == 0) return EVP1D_OK;
   hipLaunchKernelGGL(k_scatter_shared, dim3((unsigned)((h->nsh + 255) / 256)), dim3(256), 0, h->stream, d.sig,
                      d.ldsig, (const double *)(h->hsig_dev + (size_t)(nsub % 2) * kShStride * h->nsh),
                      (const int32_t *)h->shared_dev, h->nsh);
@@ -1215,33 +1488,61 @@ static void fused_step(evp1d_handle *h, FusedView &f, int k) {
 
 // nsub fused subcycles.  The state starts in the handle's buffers (X).
 // Velocities, and stresses of a two-buffer run, are copied to the second set
-// (Y) and the subcycles alternate so that the last one writes X: subcycle k
-// writes X when nsub - k is even.  Stresses of an in-place run stay in X
-// (the shared points' through their copy, put back at the end).
+// (Y) and the launches alternate so that the last one writes X: launch j of
+// L writes X when L - j is even.  Stresses of an in-place run stay in X
+// (the shared points' through their copy, put back at the end).  The pair
+// form (two buffers) runs nsub / 2 two-subcycle launches, after one
+// one-subcycle launch when nsub is odd.
 static int issue_fused(evp1d_handle *h, int nsub) {
+  const bool pair = pair_form(h);
+  const bool ip = h->fused_inplace && !pair;
   const DevView X = h->v;
   DevView Y = h->v;
   Y.vel = Y.vel_out = h->vel_alt;
   Y.velc = h->vel_alt;
   HIP_TRY(hipMemcpyAsync(Y.vel, X.vel, (size_t)h->navel * sizeof(double2), hipMemcpyDeviceToDevice, h->stream));
-  if (!h->fused_inplace) {
+  if (!ip) {
     Y.sig = h->sig_alt;
     HIP_TRY(hipMemcpyAsync(Y.sig, X.sig, (size_t)12 * h->ldsig * sizeof(double), hipMemcpyDeviceToDevice,
                            h->stream));
   }
   FusedView f;
-  RC_TRY(fused_prologue(h, X, f));
+  RC_TRY(fused_prologue(h, X, f, ip));
+  PairView pv{h->ptiles_dev, h->hlist_dev, h->pxlist_dev, h->pelist_dev, h->puloc_dev, h->pvloc_dev,
+              h->pcstr_dev,  h->cnz_dev,   h->pk_total,   nullptr,       nullptr,      h->pair_cap,
+              h->pconst,     (long)h->na};
+  if (pair && h->na > 0) {
+    hipLaunchKernelGGL(k_pack_const, dim3(blocks_for(h->na)), dim3(kBlock), 0, h->stream, X, h->pconst, (long)h->na);
+    HIP_TRY(hipGetLastError());
+  }
+  if (pair && h->pk_total > 0) {
+    hipLaunchKernelGGL(k_fill_const, dim3((unsigned)((h->pk_total + 255) / 256)), dim3(256), 0, h->stream, X,
+                       (const int32_t *)h->pklist_dev, h->pcstr_dev, h->pk_total, h->cnz_dev);
+    HIP_TRY(hipGetLastError());
+  }
   const size_t lds = (size_t)4 * h->fused_cap * sizeof(double2);
-  for (int k = 1; k <= nsub; ++k) {
-    const bool to_x = ((nsub - k) % 2) == 0;
+  const size_t lds2 = (size_t)(4 * h->pair_cap + h->pair_capv) * sizeof(double2);
+  const int L = pair ? nsub / 2 + nsub % 2 : nsub;
+  for (int j = 1; j <= L; ++j) {
+    const bool to_x = ((L - j) % 2) == 0;
     DevView in = to_x ? Y : X;
     DevView out = to_x ? X : Y;
+    if (pair && (j > 1 || nsub % 2 == 0)) {
+      pv.vel_out = out.vel;
+      pv.sig_out = out.sig;
+      if (h->fused_ntiles > 0) {
+        hipLaunchKernelGGL(kPairFns[pair_block_index(h->pair_block)][h->tile_xcd ? 1 : 0], dim3(h->fused_ntiles),
+                           dim3(h->pair_block), lds2, h->stream, in, pv, j == L ? 1 : 0);
+        HIP_TRY(hipGetLastError());
+      }
+      continue;
+    }
     f.vel_out = out.vel;
     f.sig_out = out.sig;
-    fused_step(h, f, k);
+    fused_step(h, f, j);
     if (h->fused_ntiles > 0) {
       hipLaunchKernelGGL(kFusedFns[fused_block_index(h->tile_block)][h->tile_xcd ? 1 : 0], dim3(h->fused_ntiles),
-                         dim3(h->tile_block), lds, h->stream, in, f, k == nsub ? 1 : 0);
+                         dim3(h->tile_block), lds, h->stream, in, f, j == L ? 1 : 0);
       HIP_TRY(hipGetLastError());
     }
     if (h->v.nhalo > 0) {
@@ -1249,7 +1550,7 @@ static int issue_fused(evp1d_handle *h, int nsub) {
       HIP_TRY(hipGetLastError());
     }
   }
-  return fused_epilogue(h, X, nsub);
+  return fused_epilogue(h, X, nsub, ip);
 }
 
 static int run_graph(evp1d_handle *h, int nsub, bool fused) {
@@ -1871,6 +2172,27 @@ int evp1d_get_fused_inplace(evp1d_handle *h, int32_t *inplace) {
   return EVP1D_OK;
 }
 
+int evp1d_set_fused_depth(evp1d_handle *h, int32_t depth) {
+  if (!h) return fail(EVP1D_EINVAL, "evp1d_set_fused_depth: NULL handle");
+  if (depth != 1 && depth != 2) return fail(EVP1D_EINVAL, "evp1d_set_fused_depth: depth must be 1 or 2");
+  if (depth != h->fused_depth) {
+    HIP_TRY(hipStreamSynchronize(h->stream));   // no queued run may still use the old tables
+    h->fused_depth = depth;
+    h->layout_dirty = true;                     // plan (or drop) the pair tables
+    h->graph_dirty = true;
+  }
+  return EVP1D_OK;
+}
+
+int evp1d_get_fused_depth(evp1d_handle *h, int32_t *depth, int32_t *active, int32_t *block) {
+  if (!h) return fail(EVP1D_EINVAL, "evp1d_get_fused_depth: NULL handle");
+  if (ready(h)) RC_TRY(ensure(h));
+  if (depth) *depth = h->fused_depth;
+  if (active) *active = pair_form(h) ? 1 : 0;
+  if (block) *block = pair_form(h) ? h->pair_block : 0;
+  return EVP1D_OK;
+}
+
 int evp1d_autotune_fused(evp1d_handle *h, int32_t nsub, float *us) {
   if (!h) return fail(EVP1D_EINVAL, "evp1d_autotune_fused: NULL handle");
   if (!ready(h)) return fail(EVP1D_ESTATE, "evp1d_autotune_fused: neighbour lists not set");
@@ -1878,21 +2200,25 @@ int evp1d_autotune_fused(evp1d_handle *h, int32_t nsub, float *us) {
   RC_TRY(ensure(h));
   if (!h->fused_ok)
     return fail(EVP1D_ESTATE, "evp1d_autotune_fused: fused form unavailable (%s)", h->fused_why.c_str());
-  struct Cand { int w, h, block; };
-  static const Cand cands[4] = {{64, 16, 512}, {32, 16, 512}, {32, 8, 256}, {16, 8, 128}};
+  // depth 1 with both stress storages, then the pair form (two buffers)
+  struct Cand { int w, h, block, depth; };
+  static const Cand cands[8] = {{64, 16, 512, 1}, {32, 16, 512, 1}, {32, 8, 256, 1}, {16, 8, 128, 1},
+                                {31, 15, 512, 2}, {62, 15, 512, 2}, {15, 15, 256, 2}, {62, 7, 512, 2}};
   const bool keep_auto = h->tiling_auto;
   const int keep_w = h->tile_w, keep_h = h->tile_h, keep_b = h->tile_block, keep_x = h->tile_xcd,
-            keep_ip = h->fused_inplace;
+            keep_ip = h->fused_inplace, keep_d = h->fused_depth;
   float best = 1e30f;
   int win = -1, win_ip = 0;
   int rc = EVP1D_OK;
-  for (int c = 0; c < 4 && !rc; ++c) {
-    rc = evp1d_set_tiling(h, cands[c].w, cands[c].h, cands[c].block, 1);
+  for (int c = 0; c < 8 && !rc; ++c) {
+    rc = evp1d_set_fused_depth(h, cands[c].depth);
+    if (!rc) rc = evp1d_set_tiling(h, cands[c].w, cands[c].h, cands[c].block, 1);
     if (!rc) rc = ensure(h);
-    if (rc || !h->fused_ok) continue;   // tile too large for LDS on these lists
+    if (rc || !h->fused_ok) continue;                       // tile too large for LDS on these lists
+    if (cands[c].depth == 2 && !pair_form(h)) continue;     // pair tables do not fit
     SavedState st;
     rc = st.save(h, "evp1d_autotune_fused");
-    for (int ip = 0; ip < 2 && !rc; ++ip) {
+    for (int ip = 0; ip < (cands[c].depth == 1 ? 2 : 1) && !rc; ++ip) {
       h->fused_inplace = ip;
       rc = ensure_fused_buffers(h);
       float t = 0.f;
@@ -1905,6 +2231,7 @@ int evp1d_autotune_fused(evp1d_handle *h, int32_t nsub, float *us) {
   h->graph_dirty = true;
   if (rc || win < 0) {
     h->fused_inplace = keep_ip;
+    (void)evp1d_set_fused_depth(h, keep_d);
     if (keep_auto) {
       h->tiling_auto = true;
       h->layout_dirty = true;
@@ -1915,6 +2242,7 @@ int evp1d_autotune_fused(evp1d_handle *h, int32_t nsub, float *us) {
     return rc ? rc : fail(EVP1D_ESTATE, "evp1d_autotune_fused: no tiling fits these lists");
   }
   h->fused_inplace = win_ip;
+  RC_TRY(evp1d_set_fused_depth(h, cands[win].depth));
   RC_TRY(evp1d_set_tiling(h, cands[win].w, cands[win].h, cands[win].block, 1));
   RC_TRY(ensure(h));
   RC_TRY(ensure_fused_buffers(h));
@@ -2143,7 +2471,7 @@ return EVP1D_OK;
         HIP_TRY(hipMemcpyAsync(h->sig_alt, W.sig, (size_t)12 * h->ldsig * sizeof(double), hipMemcpyDeviceToDevice,
                                h->stream));
       FusedView f;
-      RC_TRY(fused_prologue(h, W, f));
+      RC_TRY(fused_prologue(h, W, f, h->fused_inplace != 0));
       const size_t lds = (size_t)4 * h->fused_cap * sizeof(double2);
       for (int k = 1; k <= N; ++k) {
         DevView dk = view(k);
@@ -2160,7 +2488,7 @@ return EVP1D_OK;
                              dim3(h->tile_block), lds, h->stream, dk, f, k == N ? 1 : 0);
         HIP_TRY(hipGetLastError());
       }
-      RC_TRY(fused_epilogue(h, W, N));
+      RC_TRY(fused_epilogue(h, W, N, h->fused_inplace != 0));
     } else {
       for (int k = 1; k <= N; ++k) {
         const DevView dk = view(k);

@@ -592,6 +592,251 @@ std::string check_plan(const TilePlan &p, int na, int navel, const int32_t *cons
   return "";
 }
 
+// ---------------------------------------------------------------------------
+// two subcycles per workgroup
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int32_t kConstTag = 0x40000000;
+
+std::vector<int32_t> inverse_perm(const TilePlan &p, int na) {
+  std::vector<int32_t> inv((size_t)na);
+  par_chunks(na, 1 << 16, [&](int64_t lo, int64_t hi, int) {
+    for (int64_t s = lo; s < hi; ++s) inv[(size_t)p.perm[(size_t)s]] = (int32_t)s;
+  });
+  return inv;
+}
+
+struct PairWork {
+  std::vector<int32_t> x, k, e;
+  std::vector<uint16_t> uloc, vloc;
+  int ns = 0, ne_s = 0, nu = 0;
+  std::string err;
+};
+}  // namespace
+
+PairPlan plan_pairs(const TilePlan &p, int na, int navel, const int32_t *const idx[6], const uint8_t *skip,
+                    int cap_s, int cap_v, int max_t2) {
+  (void)navel;
+  (void)skip;
+  PairPlan r;
+  if (!p.ok || !idx) {
+    r.why = p.ok ? "neighbour lists not set" : "no tiling (" + p.why + ")";
+    return r;
+  }
+  const int t_end = p.cnt[0] + p.cnt[1], u_begin = p.cnt[0], u_end = t_end + p.cnt[2];
+  const std::vector<int32_t> inv = inverse_perm(p, na);
+  auto slot = [&](int32_t q) { return q < na ? inv[(size_t)q] : q; };
+  const int nt = (int)p.tiles.size();
+  std::vector<PairWork> work((size_t)nt);
+  std::vector<SlotMap> smaps, vmaps;
+  for (int w = 0; w < plan_threads(); ++w) {
+    smaps.emplace_back();
+    vmaps.emplace_back();
+  }
+  par_items(nt, [&](int t, int w) {
+    const TileDesc &d = p.tiles[(size_t)t];
+    PairWork &wk = work[(size_t)t];
+    SlotMap &sm = smaps[(size_t)w], &vm = vmaps[(size_t)w];
+    const int nA = d.a1 - d.a0, nB = d.b1 - d.b0, nO = nA + nB, nS = nO + d.nh, nUo = nB + (d.c1 - d.c0);
+    auto tslot = [&](int k) { return k < nA ? d.a0 + k : k < nO ? d.b0 + (k - nA) : p.list[(size_t)d.h0 + (k - nO)]; };
+    bool added;
+    for (int k = 0; k < nS; ++k) sm.find_or_add(tslot(k), k, added);
+    for (int k = 0; k < nUo; ++k) vm.find_or_add(k < nB ? d.b0 + k : d.c0 + (k - nB), k, added);
+    // level-2 stresses gather cc, ee, ne, se: active U points come from the
+    // velocity table (U1-only entries in first-visit order), others from memory
+    wk.vloc.resize((size_t)4 * nS);
+    for (int k = 0; k < nS; ++k) {
+      const int32_t g = tslot(k), q = p.perm[(size_t)g];
+      const int32_t src[4] = {g, slot(idx[0][q]), slot(idx[1][q]), slot(idx[2][q])};
+      for (int c = 0; c < 4; ++c) {
+        uint16_t en = PairPlan::kNoEntry;
+        if (src[c] >= u_begin && src[c] < u_end) {
+          const int v = vm.find_or_add(src[c], nUo + (int)wk.e.size(), added);
+          if (added) wk.e.push_back(src[c]);
+          en = (uint16_t)std::min(v, 0xfffe);
+        }
+        wk.vloc[(size_t)4 * k + c] = en;
+      }
+    }
+    // level-1 stepu points gather own, nw, sse, sw: active T points outside
+    // T2 become T1-only entries, the rest constant entries
+    const int nU = nUo + (int)wk.e.size();
+    std::vector<int32_t> tag((size_t)4 * nU);
+    for (int k = 0; k < nU; ++k) {
+      const int32_t s = k < nB ? d.b0 + k : k < nUo ? d.c0 + (k - nB) : wk.e[(size_t)(k - nUo)];
+      const int32_t q = p.perm[(size_t)s];
+      const int32_t src[4] = {s, slot(idx[3][q]), slot(idx[5][q]), slot(idx[4][q])};
+      for (int c = 0; c < 4; ++c) {
+        int v;
+        if (src[c] < t_end) {
+          v = sm.find_or_add(src[c], nS + (int)wk.x.size(), added);
+          if (added) wk.x.push_back(src[c]);
+          if (k < nUo && v >= nS && wk.err.empty()) wk.err = "owned stepu point reads a T point outside its halo";
+        } else {
+          v = sm.find_or_add(src[c], kConstTag | (int)wk.k.size(), added);
+          if (added) wk.k.push_back(src[c]);
+        }
+        tag[(size_t)4 * k + c] = v;
+      }
+    }
+    const int nT = nS + (int)wk.x.size(), nE = nT + (int)wk.k.size();
+    wk.ns = nS;
+    wk.ne_s = nE;
+    wk.nu = nU;
+    if (wk.err.empty() && nS > max_t2)
+      wk.err = "tile " + std::to_string(t) + " has " + std::to_string(nS) + " level-2 stress points (limit " +
+               std::to_string(max_t2) + ")";
+    if (wk.err.empty() && (nE > cap_s || nE > 0xfffe))
+      wk.err = "tile " + std::to_string(t) + " needs " + std::to_string(nE) + " str entries (limit " +
+               std::to_string(cap_s) + ")";
+    if (wk.err.empty() && (nU > cap_v || nU > 0xfffe))
+      wk.err = "tile " + std::to_string(t) + " needs " + std::to_string(nU) + " velocity entries (limit " +
+               std::to_string(cap_v) + ")";
+    if (wk.err.empty()) {
+      wk.uloc.resize(tag.size());
+      for (size_t j = 0; j < tag.size(); ++j)
+        wk.uloc[j] = (uint16_t)((tag[j] & kConstTag) ? nT + (tag[j] & (kConstTag - 1)) : tag[j]);
+    }
+    sm.clear();
+    vm.clear();
+  });
+  for (int t = 0; t < nt; ++t) {
+    const PairWork &wk = work[(size_t)t];
+    if (!wk.err.empty()) {
+      r.why = wk.err;
+      return r;
+    }
+  }
+  r.tiles.resize((size_t)nt);
+  size_t nx = 0, nk = 0, ne = 0, nu = 0, nv = 0;
+  for (int t = 0; t < nt; ++t) {
+    const PairWork &wk = work[(size_t)t];
+    PairTile &pt = r.tiles[(size_t)t];
+    pt.t = p.tiles[(size_t)t];
+    pt.x0 = (int32_t)nx;
+    pt.nx = (int32_t)wk.x.size();
+    pt.q0 = (int32_t)nk;
+    pt.nq = (int32_t)wk.k.size();
+    pt.e0 = (int32_t)ne;
+    pt.ne = (int32_t)wk.e.size();
+    pt.u0 = (int32_t)nu;
+    pt.v0 = (int32_t)nv;
+    nx += wk.x.size();
+    nk += wk.k.size();
+    ne += wk.e.size();
+    nu += wk.uloc.size() / 4;
+    nv += wk.vloc.size() / 4;
+    r.cap_s = std::max(r.cap_s, wk.ne_s);
+    r.cap_v = std::max(r.cap_v, wk.nu);
+    r.max_t2 = std::max(r.max_t2, wk.ns);
+  }
+  if (nu > 0x7fffffff || nv > 0x7fffffff) {
+    r.why = "tables too large";
+    r.tiles.clear();
+    return r;
+  }
+  r.xlist.resize(nx);
+  r.klist.resize(nk);
+  r.elist.resize(ne);
+  r.uloc.resize(4 * nu);
+  r.vloc.resize(4 * nv);
+  par_items(nt, [&](int t, int) {
+    const PairWork &wk = work[(size_t)t];
+    const PairTile &pt = r.tiles[(size_t)t];
+    std::copy(wk.x.begin(), wk.x.end(), r.xlist.begin() + pt.x0);
+    std::copy(wk.k.begin(), wk.k.end(), r.klist.begin() + pt.q0);
+    std::copy(wk.e.begin(), wk.e.end(), r.elist.begin() + pt.e0);
+    std::copy(wk.uloc.begin(), wk.uloc.end(), r.uloc.begin() + (size_t)4 * pt.u0);
+    std::copy(wk.vloc.begin(), wk.vloc.end(), r.vloc.begin() + (size_t)4 * pt.v0);
+  });
+  r.ok = true;
+  return r;
+}
+
+std::string check_pairs(const PairPlan &q, const TilePlan &p, int na, int navel, const int32_t *const idx[6],
+                        const uint8_t *skip) {
+  (void)skip;
+  if (!q.ok) return "";
+  if (!p.ok || !idx) return "pair plan without a tiling";
+  if (q.tiles.size() != p.tiles.size()) return "pair plan and tiling differ in tiles";
+  const int t_end = p.cnt[0] + p.cnt[1], u_begin = p.cnt[0], u_end = t_end + p.cnt[2];
+  const std::vector<int32_t> inv = inverse_perm(p, na);
+  auto slot = [&](int32_t r) { return r < na ? inv[(size_t)r] : r; };
+  std::vector<std::string> err(q.tiles.size());
+  std::vector<std::vector<int32_t>> smaps((size_t)plan_threads()), vmaps((size_t)plan_threads());
+  par_items((int)q.tiles.size(), [&](int t, int w) {
+    const PairTile &pt = q.tiles[(size_t)t];
+    const TileDesc &d = pt.t, &o = p.tiles[(size_t)t];
+    std::string &e = err[(size_t)t];
+    if (d.a0 != o.a0 || d.a1 != o.a1 || d.b0 != o.b0 || d.b1 != o.b1 || d.c0 != o.c0 || d.c1 != o.c1 ||
+        d.h0 != o.h0 || d.nh != o.nh) {
+      e = "pair tile differs from its tile";
+      return;
+    }
+    const int nA = d.a1 - d.a0, nB = d.b1 - d.b0, nO = nA + nB, nS = nO + d.nh, nUo = nB + (d.c1 - d.c0);
+    const int nT = nS + pt.nx, nE = nT + pt.nq, nU = nUo + pt.ne;
+    if (pt.nx < 0 || pt.x0 < 0 || (size_t)pt.x0 + pt.nx > q.xlist.size()) { e = "T1-only list range"; return; }
+    if (pt.nq < 0 || pt.q0 < 0 || (size_t)pt.q0 + pt.nq > q.klist.size()) { e = "constant list range"; return; }
+    if (pt.ne < 0 || pt.e0 < 0 || (size_t)pt.e0 + pt.ne > q.elist.size()) { e = "U1-only list range"; return; }
+    if (pt.u0 < 0 || (size_t)4 * ((size_t)pt.u0 + nU) > q.uloc.size()) { e = "uloc range"; return; }
+    if (pt.v0 < 0 || (size_t)4 * ((size_t)pt.v0 + nS) > q.vloc.size()) { e = "vloc range"; return; }
+    if (nS > q.max_t2 || nE > q.cap_s || nU > q.cap_v) { e = "tile exceeds the plan's limits"; return; }
+    std::vector<int32_t> &sm = smaps[(size_t)w], &vm = vmaps[(size_t)w];
+    sm.assign((size_t)nE, -1);
+    vm.assign((size_t)nU, -1);
+    for (int k = 0; k < nA; ++k) sm[(size_t)k] = d.a0 + k;
+    for (int k = 0; k < nB; ++k) sm[(size_t)(nA + k)] = d.b0 + k;
+    for (int k = 0; k < d.nh; ++k) sm[(size_t)(nO + k)] = p.list[(size_t)d.h0 + k];
+    for (int k = 0; k < pt.nx; ++k) {
+      const int32_t g = q.xlist[(size_t)pt.x0 + k];
+      if (g < 0 || g >= t_end) { e = "T1-only entry is not an active T slot"; return; }
+      sm[(size_t)(nS + k)] = g;
+    }
+    for (int k = 0; k < pt.nq; ++k) {
+      const int32_t g = q.klist[(size_t)pt.q0 + k];
+      if (g < t_end || g >= navel) { e = "constant entry is an active T slot or out of range"; return; }
+      sm[(size_t)(nT + k)] = g;
+    }
+    for (int k = 0; k < nB; ++k) vm[(size_t)k] = d.b0 + k;
+    for (int k = 0; k < d.c1 - d.c0; ++k) vm[(size_t)(nB + k)] = d.c0 + k;
+    for (int k = 0; k < pt.ne; ++k) {
+      const int32_t g = q.elist[(size_t)pt.e0 + k];
+      if (g < u_begin || g >= u_end) { e = "U1-only entry is not an active U slot"; return; }
+      vm[(size_t)(nUo + k)] = g;
+    }
+    // every level-1 stepu read names its source's entry; the owned points'
+    // (read again at level 2) never name a T1-only entry
+    for (int k = 0; k < nU; ++k) {
+      const int32_t s = vm[(size_t)k], r0 = p.perm[(size_t)s];
+      const int32_t want[4] = {s, slot(idx[3][r0]), slot(idx[5][r0]), slot(idx[4][r0])};
+      for (int c = 0; c < 4; ++c) {
+        const int en = q.uloc[(size_t)4 * (pt.u0 + k) + c];
+        if (en >= nE || sm[(size_t)en] != want[c] || (k < nUo && en >= nS && en < nT)) {
+          e = "stepu entry " + std::to_string(k) + " of tile " + std::to_string(t) + ": bad str entry";
+          return;
+        }
+      }
+    }
+    // every level-2 stress read names the level-1 velocity of an active U
+    // point, and only those
+    for (int k = 0; k < nS; ++k) {
+      const int32_t g = sm[(size_t)k], r0 = p.perm[(size_t)g];
+      const int32_t want[4] = {g, slot(idx[0][r0]), slot(idx[1][r0]), slot(idx[2][r0])};
+      for (int c = 0; c < 4; ++c) {
+        const int en = q.vloc[(size_t)4 * (pt.v0 + k) + c];
+        const bool active = want[c] >= u_begin && want[c] < u_end;
+        if (en == PairPlan::kNoEntry ? active : (en >= nU || vm[(size_t)en] != want[c])) {
+          e = "stress entry " + std::to_string(k) + " of tile " + std::to_string(t) + ": bad velocity entry";
+          return;
+        }
+      }
+    }
+  });
+  for (const std::string &e : err)
+    if (!e.empty()) return e;
+  return "";
+}
+
 }  // namespace evp1d
 
 #include "../../include/evp1d.h"

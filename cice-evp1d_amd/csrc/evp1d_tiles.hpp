@@ -91,4 +91,45 @@ TilePlan plan_tiles(int na, int navel, const int32_t *const idx[6], const uint8_
 // is consistent, else the first violation.
 std::string check_plan(const TilePlan &p, int na, int navel, const int32_t *const idx[6], const uint8_t *skip);
 
+// Two subcycles per workgroup (the "pair" form), on the slot order and
+// tiles of a TilePlan.  Level 2 is the one-subcycle tile: stress on the
+// owned and halo T points ("T2"), stepu on the owned U points.  Level 1
+// produces what level 2 reads: stepu on every active U point whose velocity
+// a T2 stress gathers (cc, ee, ne, se) plus the owned U points ("U1"), and
+// stress on every active T point whose str1..8 a U1 stepu gathers ("T1",
+// T2 first).  Level 1 reads the input state only, level 2 reads level 1's
+// results from LDS, so the tile writes the state two subcycles on.
+//
+// Entries of a tile's str table: [owned T | halo T | T1-only | constants];
+// of its velocity table: [owned U (B then C) | U1-only].
+struct PairTile {
+  TileDesc t;               // owned ranges and halo list (h0, nh) of the TilePlan tile
+  int32_t x0, nx;           // T1-only stress slots: PairPlan::xlist[x0, x0 + nx)
+  int32_t q0, nq;           // constant slots (str1..8 never written): PairPlan::klist[q0, q0 + nq)
+  int32_t e0, ne;           // U1-only stepu slots: PairPlan::elist[e0, e0 + ne)
+  int32_t u0, v0;           // first uloc entry (per U1 entry), first vloc entry (per T2 entry)
+};
+
+struct PairPlan {
+  bool ok = false;
+  std::string why;
+  std::vector<PairTile> tiles;        // parallel to TilePlan::tiles
+  std::vector<int32_t> xlist, klist, elist;
+  // 4 per U1 entry: str table entries of own, nw, sse, sw
+  std::vector<uint16_t> uloc;
+  // 4 per T2 entry: velocity table entries of cc, ee, ne, se; kNoEntry when
+  // that velocity never changes (not an active U point: read from memory)
+  std::vector<uint16_t> vloc;
+  int cap_s = 0, cap_v = 0;           // largest str / velocity table
+  int max_t2 = 0;                     // largest T2 (level-2 stresses, one per thread)
+  static constexpr uint16_t kNoEntry = 0xffff;
+};
+
+// cap_s, cap_v, max_t2: limits on the tables and on T2 (plan fails beyond).
+PairPlan plan_pairs(const TilePlan &p, int na, int navel, const int32_t *const idx[6], const uint8_t *skip,
+                    int cap_s, int cap_v, int max_t2);
+// Re-derives every table of a pair plan from the lists; "" when consistent.
+std::string check_pairs(const PairPlan &q, const TilePlan &p, int na, int navel, const int32_t *const idx[6],
+                        const uint8_t *skip);
+
 }  // namespace evp1d

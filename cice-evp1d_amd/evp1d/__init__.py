@@ -57,6 +57,7 @@ EXPORTS = (
     "evp1d_adjoint_run", "evp1d_loss", "evp1d_tangent_run",
     "evp1d_set_tiling", "evp1d_get_tiling", "evp1d_tile_plan",
     "evp1d_set_fused_inplace", "evp1d_get_fused_inplace", "evp1d_autotune_fused",
+    "evp1d_set_fused_depth", "evp1d_get_fused_depth",
     "evp1d_set_grid", "evp1d_put_2d", "evp1d_get_2d", "evp1d_host_alloc", "evp1d_host_free",
     "evp2d_create", "evp2d_destroy", "evp2d_set_params", "evp2d_set_cells", "evp2d_set_halo",
     "evp2d_set_field", "evp2d_get_field", "evp2d_run", "evp2d_stress", "evp2d_stepu", "evp2d_halo",
@@ -154,6 +155,8 @@ def load(fma: bool = False, path: str | os.PathLike | None = None) -> ctypes.CDL
         "evp1d_set_fused_inplace": [_H, i32],
         "evp1d_get_fused_inplace": [_H, _IP],
         "evp1d_autotune_fused": [_H, i32, ctypes.POINTER(ctypes.c_float)],
+        "evp1d_set_fused_depth": [_H, i32],
+        "evp1d_get_fused_depth": [_H, _IP, _IP, _IP],
         "evp2d_create": [i32, i32, i32, ctypes.POINTER(Params), ctypes.POINTER(_H)],
         "evp2d_destroy": [_H],
         "evp2d_set_params": [_H, ctypes.POINTER(Params)],
@@ -408,12 +411,23 @@ class Engine:
         _check(self.L, self.L.evp1d_get_fused_inplace(self.h, ctypes.byref(v)))
         return bool(v.value)
 
+    def set_fused_depth(self, depth: int) -> None:
+        """Subcycles per fused launch: 1, or 2 (the pair form; evp1d_set_fused_depth)."""
+        _check(self.L, self.L.evp1d_set_fused_depth(self.h, int(depth)))
+
+    def get_fused_depth(self) -> dict:
+        """{"depth": setting, "active": runs use the pair form, "block": its workgroup size}."""
+        d, a, b = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _check(self.L, self.L.evp1d_get_fused_depth(self.h, ctypes.byref(d), ctypes.byref(a), ctypes.byref(b)))
+        return {"depth": d.value, "active": bool(a.value), "block": b.value}
+
     def autotune_fused(self, nsub: int = 100) -> dict:
         """Time the fused tile ladder x stress storage (evp1d_autotune_fused),
         keep the fastest; returns it with its time per subcycle."""
         us = ctypes.c_float()
         _check(self.L, self.L.evp1d_autotune_fused(self.h, nsub, ctypes.byref(us)))
-        return dict(self.get_tiling(), inplace=self.get_fused_inplace(), us=us.value)
+        return dict(self.get_tiling(), inplace=self.get_fused_inplace(), depth=self.get_fused_depth()["depth"],
+                    us=us.value)
 
     # lifetime ------------------------------------------------------------
     def close(self) -> None:

@@ -238,12 +238,26 @@ int evp1d_get_tiling(evp1d_handle *h, int32_t *tile_w, int32_t *tile_h, int32_t 
  * results are identical.                                                    */
 int evp1d_set_fused_inplace(evp1d_handle *h, int32_t inplace);
 int evp1d_get_fused_inplace(evp1d_handle *h, int32_t *inplace);
+/* Subcycles per fused launch.  depth = 1 (default): one.  depth = 2 (the
+ * pair form): each workgroup runs two subcycles of its tile, the first over
+ * a two-cell border ring that the neighbouring tiles also evaluate, so state
+ * and constant fields cross HBM once per two subcycles; an odd nsub starts
+ * with one one-subcycle launch.  The pair form uses two stress buffers, needs
+ * (tile_w+1)(tile_h+1) <= 1024 on the lists and its tables within 160 KiB of
+ * LDS, and picks its own workgroup size (one thread per level-2 stress
+ * point); where it does not fit, or a ghost copy-list is set
+ * (evp1d_set_halo), fused runs use depth 1.  evp1d_get_fused_depth reports
+ * the setting and, in *active, whether runs use the pair form (0/1) and, in
+ * *block, its workgroup size.  Results are identical.                       */
+int evp1d_set_fused_depth(evp1d_handle *h, int32_t depth);
+int evp1d_get_fused_depth(evp1d_handle *h, int32_t *depth, int32_t *active, int32_t *block);
 /* Times one fused run of nsub subcycles (after an untimed one) for each tile
  * of 64x16/512, 32x16/512, 32x8/256, 16x8/128 (XCD-aware order) that fits
- * the lists, each with both stress storages, on the handle's own data (the
- * state is saved and restored), keeps the fastest (as evp1d_set_tiling and
- * evp1d_set_fused_inplace would) and reports its time per subcycle in
- * microseconds.                                                             */
+ * the lists, each with both stress storages, and the pair form on tiles of
+ * 31x15, 62x15, 15x15 and 62x7, on the handle's own data (the state is saved
+ * and restored), keeps the fastest (as evp1d_set_tiling,
+ * evp1d_set_fused_inplace and evp1d_set_fused_depth would) and reports its
+ * time per subcycle in microseconds.                                        */
 int evp1d_autotune_fused(evp1d_handle *h, int32_t nsub, float *us);
 
 /* Host-only tiling of a 1D list (no device use; for inspection and tests).

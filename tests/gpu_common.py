@@ -52,9 +52,14 @@ def load_golden_v0(case: str):
 
 
 def set_mode(eng: evp1d.Engine, mode: str) -> None:
-    """mode: a loop form, or "fused-inplace" (fused, stresses updated in place)."""
+    """mode: a loop form, "fused-inplace" (fused, stresses updated in place)
+    or "fused-pair" (fused, two subcycles per launch on 32x16 tiles)."""
     if mode == "fused-inplace":
         eng.set_fused_inplace(True)
+        mode = "fused"
+    elif mode == "fused-pair":
+        eng.set_fused_depth(2)
+        eng.set_tiling(32, 16, 512, 1)
         mode = "fused"
     eng.set_mode(mode)
 

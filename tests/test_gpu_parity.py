@@ -18,7 +18,7 @@ from gpu_common import (GOLD, assert_state_bitexact, fetch_state, golden_cases, 
                         make_engine, oracle_params, set_mode, synthetic_state)
 
 pytestmark = pytest.mark.gpu
-MODES = ("launch", "graph", "persistent", "fused", "fused-inplace")
+MODES = ("launch", "graph", "persistent", "fused", "fused-inplace", "fused-pair")
 
 
 def _log(msg):
@@ -95,6 +95,8 @@ def test_halo_copy_list(gpu):
     for mode in MODES:
         with make_engine(st, 9, mode) as eng:
             eng.set_halo(dst, src)
+            if mode == "fused-pair":   # no copy-list between the two subcycles of a launch
+                assert not eng.get_fused_depth()["active"]
             eng.run(9)
             got = fetch_state(eng, st)
         assert_state_bitexact(got, want)
@@ -323,9 +325,58 @@ def test_autotune_fused_keeps_results_and_state(gpu):
         eng.run(5)
         t = eng.autotune_fused(nsub=4)
         _log(f"autotune_fused: {t}")
-        assert t["ntiles"] > 0 and t["us"] > 0 and (t["tile_w"], t["tile_h"], t["block"]) in \
-            [(64, 16, 512), (32, 16, 512), (32, 8, 256), (16, 8, 128)]
+        assert t["ntiles"] > 0 and t["us"] > 0 and (t["tile_w"], t["tile_h"], t["block"], t["depth"]) in \
+            [(64, 16, 512, 1), (32, 16, 512, 1), (32, 8, 256, 1), (16, 8, 128, 1),
+             (32, 16, 512, 2), (64, 8, 512, 2), (16, 16, 256, 2), (32, 8, 256, 2)]
         eng.run(7)
+        assert_state_bitexact(fetch_state(eng, st), want)
+
+
+@pytest.mark.parametrize("tile", [(32, 16), (16, 16), (64, 8), (5, 7), (1, 1), (200, 2)])
+@pytest.mark.parametrize("nsub", [1, 2, 3, 8])
+def test_pair_form_matches_oracle(gpu, tile, nsub):
+    """Two subcycles per launch (each tile evaluates a two-cell border ring
+    again), odd counts starting with one one-subcycle launch, any tile shape:
+    every output bit as the oracle, over two runs."""
+    st = synthetic_state(70, 50, seed=66)
+    lib, p = oracle_params(st, nsub)
+    want = st.copy()
+    lib.run(p, want, nsub)
+    _log(f"pair tile={tile} nsub={nsub}")
+    with make_engine(st, nsub, "fused") as eng:
+        eng.set_fused_depth(2)
+        eng.set_tiling(tile[0], tile[1], 256, 1)
+        d = eng.get_fused_depth()
+        assert d["active"] and d["block"] >= 64, d
+        eng.run(nsub)
+        assert_state_bitexact(fetch_state(eng, st), want)
+        eng.run(nsub)
+        lib.run(p, want, nsub)
+        assert_state_bitexact(fetch_state(eng, st), want)
+
+
+def test_pair_form_switches_and_falls_back(gpu):
+    """Depth changes between runs keep the state; a tile whose level-2 set
+    exceeds 1024 points (64x16 on an open domain) and a ghost copy-list both
+    leave runs on the one-subcycle form, with the same results."""
+    st = synthetic_state(150, 60, seed=67)
+    lib, p = oracle_params(st, 20)
+    want = st.copy()
+    lib.run(p, want, 20)
+    with make_engine(st, 20, "fused-pair") as eng:
+        assert eng.get_fused_depth()["active"]
+        eng.run(5)
+        eng.set_fused_depth(1)
+        assert not eng.get_fused_depth()["active"]
+        eng.run(4)
+        eng.set_fused_depth(2)
+        eng.set_tiling(64, 16, 512, 1)
+        d = eng.get_fused_depth()
+        _log(f"64x16 pair: {d}")
+        eng.run(6)
+        eng.set_tiling(16, 16, 256, 0)
+        assert eng.get_fused_depth()["active"]
+        eng.run(5)
         assert_state_bitexact(fetch_state(eng, st), want)
 
 

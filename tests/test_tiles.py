@@ -215,3 +215,37 @@ def test_host_check_catches_corrupted_plans(tmp_path):
     r = subprocess.run([str(exe), str(tmp_path / "in.bin")], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "NOT CAUGHT" not in r.stdout and "good: ''" in r.stdout
+
+
+def test_pair_plan_schedule_and_check(tmp_path):
+    """Build tests/cpp/pair_plan_check.cpp against the planner source and run
+    it on a synthetic grid: the two-subcycle tables exist for several tile
+    shapes within their ring bounds, a CPU model of the pair kernel's
+    schedule through them reproduces two plain subcycles exactly (integer
+    hash kernels), and every corruption of a valid pair plan is reported by
+    check_pairs."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    repo = Path(__file__).resolve().parent.parent
+    csrc = repo / "cice-evp1d_amd" / "csrc"
+    exe = tmp_path / "pairs"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-pthread", f"-I{csrc}", str(repo / "tests" / "cpp" / "pair_plan_check.cpp"),
+                    str(csrc / "evp1d_tiles.cpp"), "-o", str(exe)], check=True, capture_output=True)
+    g = synth.make_grid(300, 200, 72)
+    ix = evp1d.build_index(g.tmask, g.nghost)
+    from evp1d import layout
+    s = layout.pack(g, ix)
+    na, navel = s["na"], s["navel"]
+    skip = (s["skipT"] != 0).astype(np.uint8) | ((s["skipU"] != 0).astype(np.uint8) << 1)
+    with open(tmp_path / "in.bin", "wb") as fh:
+        fh.write(np.array([na, navel], np.int32).tobytes())
+        for n in evp1d.NBR:
+            fh.write((s["idx"][n].astype(np.int32) - 1).tobytes())
+        fh.write(skip.tobytes())
+    r = subprocess.run([str(exe), str(tmp_path / "in.bin")], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "NOT CAUGHT" not in r.stdout and "good: ''" in r.stdout
+    assert r.stdout.count(": 0 differing outputs") == 5
