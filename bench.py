@@ -9,8 +9,9 @@ Prints ONE JSON line:
 
   value      ice-points x subcycles per second (na * ndte * steps / time),
              exact (bit-identical) FP64 build
-  roofline   dominant kernel (k_fused: one whole subcycle): algorithmic bytes
-             per launch / average launch duration (HIP events on the
+  roofline   dominant kernel (k_pair: two whole subcycles per launch, or
+             k_fused: one, whichever form the tuner picked): algorithmic
+             bytes per launch / average launch duration (HIP events on the
              library's stream) vs 8 TB/s; traffic from rocprofv3 counters
   adjoint    time of one gradient evaluation (hand-written adjoint kernels;
              whole trajectory stored, and binomial checkpointing with 32
@@ -218,12 +219,28 @@ def adjoint_measure(s: dict, ndte: int, t_fwd_step: float, nsnap: int, ndir: int
     if fused_cfg:
         eng.set_tiling(fused_cfg["tile_w"], fused_cfg["tile_h"], fused_cfg["block"], fused_cfg["xcd"])
         eng.set_fused_inplace(fused_cfg["inplace"])
+        eng.set_fused_depth(fused_cfg.get("depth", 1))
     reset_state(eng, s, {"strength": 1.2 * s["fields"]["strength"]})
     eng.run(ndte)
     ut = eng.get_field("uvel")[:s["na"]]
     vt = eng.get_field("vvel")[:s["na"]]
     reset_state(eng, s, {"strength": s["fields"]["strength"]})
     eng.run(ndte)                      # warm the forward graph and kernels
+    # the gradient's forward sweep runs one subcycle per launch (it stores
+    # every subcycle): the one-subcycle fused forward on the same tiling,
+    # for the ratio against the same kernels
+    t_fwd1 = None
+    if fused_cfg and fused_cfg.get("depth", 1) == 2:
+        eng.set_fused_depth(1)
+        reset_state(eng, s)
+        eng.run(ndte)
+        eng.synchronize()
+        t0 = time.perf_counter()
+        eng.run(ndte)
+        eng.synchronize()
+        t_fwd1 = time.perf_counter() - t0
+        eng.set_fused_depth(2)
+        eng.run(ndte)
     timings = {}
     grads = {}
     for ns in sorted({min(nsnap, ndte), ndte}):
@@ -276,8 +293,12 @@ def adjoint_measure(s: dict, ndte: int, t_fwd_step: float, nsnap: int, ndir: int
     eng.close()
     log(f"[bench] adjoint checks: FD max rel err {fd_err:.2e} ({ndir} directions, eps 1e-6 relative), "
         f"tangent-linear dot product max rel err {dot_err:.2e} ({ndot} directions)")
-    return {"t_adj_over_t_fwd": round(t_adj / t_fwd_step, 3), "t_adj_s": round(t_adj, 4),
-            "t_fwd_s": round(t_fwd_step, 4), "stored_states": ndte, "loss": g["loss"],
+    out = {"t_adj_over_t_fwd": round(t_adj / t_fwd_step, 3), "t_adj_s": round(t_adj, 4),
+           "t_fwd_s": round(t_fwd_step, 4)}
+    if t_fwd1:
+        out.update({"t_fwd_one_subcycle_per_launch_s": round(t_fwd1, 4),
+                    "t_adj_over_t_fwd_one_subcycle_per_launch": round(t_adj / t_fwd1, 3)})
+    return {**out, "stored_states": ndte, "loss": g["loss"],
             "checkpointed": {"stored_states": min(nsnap, ndte),
                              "t_adj_over_t_fwd": round(timings[min(nsnap, ndte)] / t_fwd_step, 3)},
             "gradients_identical": same,
@@ -444,6 +465,8 @@ def main() -> int:
                          "candidate (tile ladder x stress storage); 0: keep the defaults or --tiling/--inplace")
     ap.add_argument("--inplace", type=int, default=None, choices=(0, 1),
                     help="fused stress storage: 1 in place, 0 two buffers (implies --tune-fused 0)")
+    ap.add_argument("--depth", type=int, default=None, choices=(1, 2),
+                    help="subcycles per fused launch: 2 the pair form (implies --tune-fused 0)")
     ap.add_argument("--tuning", default=None, metavar="SB,SX,UB,UX",
                     help="fixed launch tuning: stress block, stress xcd order, stepu block, stepu xcd order")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -482,18 +505,24 @@ def main() -> int:
         eng.set_tiling(*[int(x) for x in args.tiling.split(",")])
     if args.inplace is not None:
         eng.set_fused_inplace(args.inplace)
+    if args.depth is not None:
+        eng.set_fused_depth(args.depth)
     if args.autotune:
         log(f"[bench] autotune: {eng.autotune(args.autotune)}")
     tiling = eng.get_tiling()
     # the form evp1d_run takes: auto is the fused form whenever the lists tile (and ndte >= 4)
     fused = args.mode == "fused" or (args.mode == "auto" and tiling["ntiles"] > 0 and ndte >= 4)
     tuned_fused = None
-    if fused and args.tune_fused > 0 and not args.tiling and args.inplace is None:
+    if fused and args.tune_fused > 0 and not args.tiling and args.inplace is None and args.depth is None:
         t0 = time.perf_counter()
         tuned_fused = eng.autotune_fused(args.tune_fused)
         log(f"[bench] autotune_fused ({time.perf_counter() - t0:.1f}s, setup): {tuned_fused}")
     tuning = eng.get_tuning()
     tiling = dict(eng.get_tiling(), inplace=int(eng.get_fused_inplace()))
+    dinfo = eng.get_fused_depth()
+    tiling["depth"] = 2 if dinfo["active"] else 1
+    if dinfo["active"]:
+        tiling.update(inplace=0, pair_block=dinfo["block"])   # the pair form keeps two stress buffers
     log(f"[bench] mode {args.mode} ({'fused' if fused else 'unfused'}), tiling {tiling}, tuning {tuning}")
     secs = time_steps(eng, ndte, args.steps, args.warmup)
     value = na * ndte * args.steps / secs
@@ -515,6 +544,10 @@ def main() -> int:
     step_us = secs / (args.steps * ndte) * 1e6
     if args.mode == "persistent":
         dom = {"name": "k_persistent", "bytes": B["subcycle"] * ndte, "avg_us": secs / args.steps * 1e6}
+    elif fused and tiling["depth"] == 2:
+        # one k_pair launch = two subcycles (run_individual counts subcycles)
+        dom = {"name": "k_pair", "bytes": 2 * B["subcycle"], "avg_us": 2 * kernels["fused"]["avg_us"],
+               "subcycles_per_launch": 2}
     elif fused:
         dom = {"name": "k_fused", "bytes": B["fused"], "avg_us": kernels["fused"]["avg_us"]}
     else:
@@ -533,7 +566,7 @@ def main() -> int:
         traffic = tab.get(key)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom["name"],
-                "bytes_per_launch": dom["bytes"]}
+                "bytes_per_launch": dom["bytes"], "subcycles_per_launch": dom.get("subcycles_per_launch", 1)}
     whole = B["subcycle"] / (step_us * 1e-6) / 1e9
 
     adjoint = None
@@ -564,7 +597,8 @@ def main() -> int:
                    "ny": w["ny"], "seed": w["seed"], "na": na, "navel": s["navel"], "nT": nT, "nU": nU,
                    "ndte": ndte, "mode": args.mode, "form": "fused" if fused else "unfused", "tiling": tiling,
                    "tiling_chosen_by": ("evp1d_autotune_fused (setup, untimed)" if tuned_fused else
-                                        "--tiling/--inplace" if (args.tiling or args.inplace is not None)
+                                        "--tiling/--inplace/--depth" if (args.tiling or args.inplace is not None
+                                                                         or args.depth is not None)
                                         else "library default"),
                    "build": "fma" if args.fma else "exact", "tuning": tuning,
                    "us_per_subcycle": round(step_us, 3), "loop_GBps_algorithmic": round(whole, 1),

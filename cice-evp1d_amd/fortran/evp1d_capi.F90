@@ -65,6 +65,7 @@ module evp1d_capi
             evp1d_set_tuning, evp1d_get_tuning, evp1d_set_tiling, evp1d_get_tiling,       &
             evp1d_set_grid, evp1d_put_2d, evp1d_get_2d, evp1d_host_alloc, evp1d_host_free, &
             evp1d_set_fused_inplace, evp1d_get_fused_inplace, evp1d_autotune_fused,    &
+            evp1d_set_fused_depth, evp1d_get_fused_depth,                                &
             evp2d_create, evp2d_destroy, evp2d_set_params, evp2d_set_cells, evp2d_set_halo, &
             evp2d_set_field, evp2d_get_field, evp2d_run, evp2d_stress, evp2d_stepu,      &
             evp2d_halo, evp2d_run_individual, evp2d_synchronize, evp2d_timer_start,    &
@@ -306,6 +307,16 @@ module evp1d_capi
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: h
       integer(c_int32_t), intent(out) :: inplace
+    end function
+    integer(c_int) function evp1d_set_fused_depth(h, depth) bind(C, name='evp1d_set_fused_depth')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: depth
+    end function
+    integer(c_int) function evp1d_get_fused_depth(h, depth, active, block) bind(C, name='evp1d_get_fused_depth')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(out) :: depth, active, block
     end function
     ! ---- the v0 2D block path -------------------------------------------
     integer(c_int) function evp2d_create(nxb, nyb, nblocks, p, h) bind(C, name='evp2d_create')

@@ -34,7 +34,7 @@ def _run(exe, cwd, *args, timeout=120):
     return r
 
 
-@pytest.mark.parametrize("mode", ["graph", "persistent", "launch", "fused", "fused-inplace", "tuned"])
+@pytest.mark.parametrize("mode", ["graph", "persistent", "launch", "fused", "fused-inplace", "fused-pair", "tuned"])
 @pytest.mark.parametrize("case", golden_cases())
 def test_v2_driver_reproduces_reference_files(gpu, case, mode):
     st, ndte, want_stress, want_stepu = load_golden(case)
