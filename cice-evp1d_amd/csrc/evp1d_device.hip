@@ -371,8 +371,10 @@ __global__ __launch_bounds__(B) void k_fused(DevView d, FusedView f, int last) {
 // of the state and the constant fields per subcycle.  Level-2 stress entry k
 // is thread k (the plan guarantees at most B of them), so its level-1
 // stresses stay in that thread's registers.  The 20 constant fields come
-// from a copy packed in pairs at the start of the run (k_pack_const), one
-// 16-byte load per pair.
+// from a copy packed in pairs at the start of the run (k_pack_const), the
+// stresses live during the run in pairs too (k_pack_sig / k_unpack_sig), and
+// ee, ne, se of each stress point in one int4 (k_pack_nbr): one 16-byte load
+// per pair, since the kernel's time follows its load instructions.
 struct PairView {
   const evp1d::PairTile *tiles;
   const int32_t *hlist, *xlist, *elist;
@@ -382,11 +384,37 @@ struct PairView {
   const int *cnz;
   long nk;
   double2 *vel_out;
-  double *sig_out;
   int cap;                  // entries of the str table (the velocity table follows it)
   const double2 *pc;        // [kPairConst][lp]: the constant fields in pairs (k_pack_const)
   long lp;
+  const double2 *sg_in;     // [6][ls]: stresses in pairs (p1 p2, p3 p4, m1 m2, m3 m4, 12_1 12_2, 12_3 12_4)
+  double2 *sg_out;
+  long ls;
+  const int4 *nbr;          // per stress slot: ee, ne, se
 };
+
+// the stresses of the stress slots [0, n) into pairs, and back
+__global__ void k_pack_sig(double2 *p, long ls, const double *sig, long ldsig, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) p[c * ls + i] = make_double2(sig[2 * c * ldsig + i], sig[(2 * c + 1) * ldsig + i]);
+}
+__global__ void k_unpack_sig(double *sig, long ldsig, const double2 *p, long ls, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const double2 v = p[c * ls + i];
+    sig[2 * c * ldsig + i] = v.x;
+    sig[(2 * c + 1) * ldsig + i] = v.y;
+  }
+}
+__global__ void k_pack_nbr(DevView d, int4 *nbr, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  nbr[i] = make_int4(d.ee[i], d.ne[i], d.se[i], 0);
+}
 
 // The constant fields in pairs, one double2 array each: (dxT, dyT), (hte,
 // htn), (htem1, htnm1), (cw, aiu), (uocn, vocn), (waterx, watery), (forcex,
@@ -454,11 +482,16 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
   auto level1 = [&](int k) {
     if (k < nT) {
       const int g = tslot(k);
+      const int4 n = f.nbr[g];
       double sig[12];
 #pragma unroll
-      for (int c = 0; c < 12; ++c) sig[c] = d.sig[c * d.ldsig + g];
+      for (int c = 0; c < 6; ++c) {
+        const double2 v = f.sg_in[c * f.ls + g];
+        sig[2 * c] = v.x;
+        sig[2 * c + 1] = v.y;
+      }
       evp1d::StressOut o;
-      stress(g, d.vel[g], d.vel[d.ee[g]], d.vel[d.ne[g]], d.vel[d.se[g]], sig, o);
+      stress(g, d.vel[g], d.vel[n.x], d.vel[n.y], d.vel[n.z], sig, o);
       if (k == me) {
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
@@ -505,18 +538,18 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
     const unsigned e0 = w.x & 0xffffu, e1 = w.x >> 16, e2 = w.y & 0xffffu, e3 = w.y >> 16;
     constexpr unsigned kNo = evp1d::PairPlan::kNoEntry;
     const double2 cc = e0 == kNo ? d.vel[g] : vt[e0];
-    const double2 ve = e1 == kNo ? d.vel[d.ee[g]] : vt[e1];
-    const double2 vn = e2 == kNo ? d.vel[d.ne[g]] : vt[e2];
-    const double2 vs = e3 == kNo ? d.vel[d.se[g]] : vt[e3];
+    const double2 ve = e1 == kNo ? d.vel[f.nbr[g].x] : vt[e1];
+    const double2 vn = e2 == kNo ? d.vel[f.nbr[g].y] : vt[e2];
+    const double2 vs = e3 == kNo ? d.vel[f.nbr[g].z] : vt[e3];
     evp1d::StressOut o;
     stress(g, cc, ve, vn, vs, sg, o);
     if (me < nO) {
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        f.sig_out[c * d.ldsig + g] = o.sp[c];
-        f.sig_out[(4 + c) * d.ldsig + g] = o.sm[c];
-        f.sig_out[(8 + c) * d.ldsig + g] = o.s12[c];
-      }
+      f.sg_out[g] = make_double2(o.sp[0], o.sp[1]);
+      f.sg_out[f.ls + g] = make_double2(o.sp[2], o.sp[3]);
+      f.sg_out[2 * f.ls + g] = make_double2(o.sm[0], o.sm[1]);
+      f.sg_out[3 * f.ls + g] = make_double2(o.sm[2], o.sm[3]);
+      f.sg_out[4 * f.ls + g] = make_double2(o.s12[0], o.s12[1]);
+      f.sg_out[5 * f.ls + g] = make_double2(o.s12[2], o.s12[3]);
       if (last) {
         d.s15[g] = make_double2(o.str[0], o.str[4]);
         d.s27[g] = make_double2(o.str[1], o.str[6]);
@@ -579,6 +612,8 @@ constexpr int kFusedMinSub = 4;          // evp1d_run in AUTO mode: fused from t
 constexpr int kNumPairBlocks = 3;
 constexpr int kPairBlocks[kNumPairBlocks] = {256, 512, 1024};
 constexpr int kPairMaxLds = 160 * 1024;
+// LDS of a pair launch: str table and velocity table
+static size_t pair_lds(int cap, int capv) { return (size_t)(4 * cap + capv) * sizeof(double2); }
 
 // launch configurations the autotuner chooses from
 constexpr int kNumBlocks = 5;
@@ -977,6 +1012,8 @@ struct evp1d_handle {
   uint2 *puloc_dev = nullptr, *pvloc_dev = nullptr;
   double2 *pcstr_dev = nullptr;
   double2 *pconst = nullptr;       // constant fields in pairs, [kPairConst][na], packed per run
+  double *sig_pair = nullptr;      // stresses in pairs during a pair run (with sig_alt: two buffers)
+  int4 *nbr4 = nullptr;            // ee, ne, se per stress slot, packed per pair run
   void *traj = nullptr;            // gradient trajectory storage, kept between calls
   size_t traj_bytes = 0;
   // 2D grid of the caller's arrays (device-side pack/unpack)
@@ -1059,6 +1096,8 @@ static void release(evp1d_handle *h) {
   if (h->vel_alt) (void)hipFree(h->vel_alt);
   if (h->sig_alt) (void)hipFree(h->sig_alt);
   if (h->pconst) (void)hipFree(h->pconst);
+  if (h->sig_pair) (void)hipFree(h->sig_pair);
+  if (h->nbr4) (void)hipFree(h->nbr4);
   if (h->cnz_dev) (void)hipFree(h->cnz_dev);
   if (h->traj) (void)hipFree(h->traj);
   if (h->pos2d_dev) (void)hipFree(h->pos2d_dev);
@@ -1212,13 +1251,13 @@ static int install_pairs(evp1d_handle *h, const evp1d::TilePlan &plan, const int
   }
   const std::string bad = evp1d::check_pairs(q, plan, h->na, h->navel, idx, h->skip_host.data());
   if (!bad.empty()) return fail(EVP1D_EINDEX, "pair plan: %s", bad.c_str());
-  const size_t lds = (size_t)(4 * std::max(q.cap_s, 1) + std::max(q.cap_v, 1)) * sizeof(double2);
+  int b = 0;
+  while (kPairBlocks[b] < q.max_t2) ++b;   // max_t2 <= the largest block (plan limit)
+  const size_t lds = pair_lds(std::max(q.cap_s, 1), std::max(q.cap_v, 1));
   if (lds > (size_t)kPairMaxLds) {
     h->pair_why = "tables need " + std::to_string(lds) + " bytes of LDS";
     return EVP1D_OK;
   }
-  int b = 0;
-  while (kPairBlocks[b] < q.max_t2) ++b;   // max_t2 <= the largest block (plan limit)
   RC_TRY(upload_table(h, &h->ptiles_dev, q.tiles.data(), q.tiles.size()));
   RC_TRY(upload_table(h, &h->pxlist_dev, q.xlist.data(), q.xlist.size()));
   RC_TRY(upload_table(h, &h->pelist_dev, q.elist.data(), q.elist.size()));
@@ -1442,6 +1481,9 @@ static int ensure_fused_buffers(evp1d_handle *h) {
     HIP_TRY(hipMalloc((void **)&h->sig_alt, (size_t)12 * h->ldsig * sizeof(double)));
   if (pair_form(h) && !h->pconst)
     HIP_TRY(hipMalloc((void **)&h->pconst, (size_t)kPairConst * std::max(h->na, 1) * sizeof(double2)));
+  if (pair_form(h) && !h->sig_pair)
+    HIP_TRY(hipMalloc((void **)&h->sig_pair, (size_t)12 * h->ldsig * sizeof(double)));
+  if (pair_form(h) && !h->nbr4) HIP_TRY(hipMalloc((void **)&h->nbr4, (size_t)std::max(h->na, 1) * sizeof(int4)));
   return EVP1D_OK;
 }
 
@@ -1491,45 +1533,66 @@ static void fused_step(evp1d_handle *h, FusedView &f, int k) {
 // (Y) and the launches alternate so that the last one writes X: launch j of
 // L writes X when L - j is even.  Stresses of an in-place run stay in X
 // (the shared points' through their copy, put back at the end).  The pair
-// form (two buffers) runs nsub / 2 two-subcycle launches, after one
-// one-subcycle launch when nsub is odd.
+// form runs nsub / 2 two-subcycle launches, after one one-subcycle launch
+// when nsub is odd; its stresses ping-pong in pairs between sig_pair and
+// sig_alt (packed before its first launch, unpacked into X after the last).
 static int issue_fused(evp1d_handle *h, int nsub) {
   const bool pair = pair_form(h);
+  const bool odd_first = pair && nsub % 2 == 1;
   const bool ip = h->fused_inplace && !pair;
   const DevView X = h->v;
   DevView Y = h->v;
   Y.vel = Y.vel_out = h->vel_alt;
   Y.velc = h->vel_alt;
   HIP_TRY(hipMemcpyAsync(Y.vel, X.vel, (size_t)h->navel * sizeof(double2), hipMemcpyDeviceToDevice, h->stream));
-  if (!ip) {
+  if (!ip && (!pair || odd_first)) {
     Y.sig = h->sig_alt;
     HIP_TRY(hipMemcpyAsync(Y.sig, X.sig, (size_t)12 * h->ldsig * sizeof(double), hipMemcpyDeviceToDevice,
                            h->stream));
   }
   FusedView f;
   RC_TRY(fused_prologue(h, X, f, ip));
+  const long t_end = h->v.t_end;
   PairView pv{h->ptiles_dev, h->hlist_dev, h->pxlist_dev, h->pelist_dev, h->puloc_dev, h->pvloc_dev,
-              h->pcstr_dev,  h->cnz_dev,   h->pk_total,   nullptr,       nullptr,      h->pair_cap,
-              h->pconst,     (long)h->na};
+              h->pcstr_dev,  h->cnz_dev,   h->pk_total,   nullptr,       h->pair_cap,  h->pconst,
+              (long)h->na,   nullptr,      nullptr,       h->ldsig,      h->nbr4};
   if (pair && h->na > 0) {
     hipLaunchKernelGGL(k_pack_const, dim3(blocks_for(h->na)), dim3(kBlock), 0, h->stream, X, h->pconst, (long)h->na);
     HIP_TRY(hipGetLastError());
   }
+  if (pair && t_end > 0) {
+    hipLaunchKernelGGL(k_pack_nbr, dim3(blocks_for((int)t_end)), dim3(kBlock), 0, h->stream, X, h->nbr4, t_end);
+    HIP_TRY(hipGetLastError());
+  }
+  double2 *P[2] = {reinterpret_cast<double2 *>(h->sig_pair), reinterpret_cast<double2 *>(h->sig_alt)};
+  int pin = 0;                 // P[pin]: the pair launches' current stresses
+  bool packed = false;
+  const double *soa = X.sig;   // the current stresses before packing
   if (pair && h->pk_total > 0) {
     hipLaunchKernelGGL(k_fill_const, dim3((unsigned)((h->pk_total + 255) / 256)), dim3(256), 0, h->stream, X,
                        (const int32_t *)h->pklist_dev, h->pcstr_dev, h->pk_total, h->cnz_dev);
     HIP_TRY(hipGetLastError());
   }
   const size_t lds = (size_t)4 * h->fused_cap * sizeof(double2);
-  const size_t lds2 = (size_t)(4 * h->pair_cap + h->pair_capv) * sizeof(double2);
+  const size_t lds2 = pair_lds(h->pair_cap, h->pair_capv);
   const int L = pair ? nsub / 2 + nsub % 2 : nsub;
   for (int j = 1; j <= L; ++j) {
     const bool to_x = ((L - j) % 2) == 0;
     DevView in = to_x ? Y : X;
     DevView out = to_x ? X : Y;
-    if (pair && (j > 1 || nsub % 2 == 0)) {
+    if (pair && (j > 1 || !odd_first)) {
+      if (!packed) {
+        if (t_end > 0) {
+          hipLaunchKernelGGL(k_pack_sig, dim3(blocks_for((int)t_end)), dim3(kBlock), 0, h->stream, P[0], h->ldsig,
+                             soa, h->ldsig, t_end);
+          HIP_TRY(hipGetLastError());
+        }
+        packed = true;
+      }
       pv.vel_out = out.vel;
-      pv.sig_out = out.sig;
+      pv.sg_in = P[pin];
+      pv.sg_out = P[1 - pin];
+      pin = 1 - pin;
       if (h->fused_ntiles > 0) {
         hipLaunchKernelGGL(kPairFns[pair_block_index(h->pair_block)][h->tile_xcd ? 1 : 0], dim3(h->fused_ntiles),
                            dim3(h->pair_block), lds2, h->stream, in, pv, j == L ? 1 : 0);
@@ -1539,6 +1602,7 @@ static int issue_fused(evp1d_handle *h, int nsub) {
     }
     f.vel_out = out.vel;
     f.sig_out = out.sig;
+    soa = out.sig;
     fused_step(h, f, j);
     if (h->fused_ntiles > 0) {
       hipLaunchKernelGGL(kFusedFns[fused_block_index(h->tile_block)][h->tile_xcd ? 1 : 0], dim3(h->fused_ntiles),
@@ -1549,6 +1613,11 @@ static int issue_fused(evp1d_handle *h, int nsub) {
       hipLaunchKernelGGL(k_halo, dim3(blocks_for(h->v.nhalo)), dim3(kBlock), 0, h->stream, out);
       HIP_TRY(hipGetLastError());
     }
+  }
+  if (packed && t_end > 0) {
+    hipLaunchKernelGGL(k_unpack_sig, dim3(blocks_for((int)t_end)), dim3(kBlock), 0, h->stream, X.sig, h->ldsig,
+                       (const double2 *)P[pin], h->ldsig, t_end);
+    HIP_TRY(hipGetLastError());
   }
   return fused_epilogue(h, X, nsub, ip);
 }
