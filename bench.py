@@ -563,6 +563,8 @@ def main() -> int:
         key = dom["name"]
         if key == "k_fused":
             key += "@inplace" if tiling.get("inplace") else "@twobuf"
+        if args.fma:   # counters were taken on the exact build
+            key += "@fma"
         traffic = tab.get(key)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom["name"],
