@@ -196,3 +196,22 @@ def test_batch_gradient_equals_member_gradients(gpu, nsnap):
         off += st.na
     want = sum(s["loss"] for s in singles)
     assert abs(g["loss"] - want) <= 1e-12 * want
+
+
+def test_gradient_unchanged_by_the_pair_form(problem):
+    """With two subcycles per fused launch selected (the handle's forward
+    runs), the gradient, whose forward sweep stores every subcycle, is the
+    same bit for bit, also after a pair-form run on the handle."""
+    st, ut, vt = problem
+    with make_engine(st, NDTE, "fused") as eng:
+        eng.set_tiling(31, 15, 512, 1)   # the same slot order: the loss sums in the same order
+        ref = eng.adjoint_run(NDTE, ut, vt)
+    with make_engine(st, NDTE, "fused-pair") as eng:
+        assert eng.get_fused_depth()["active"]
+        eng.run(3)
+        for k, v in st.f.items():       # back to the start state
+            eng.set_field(k, v)
+        g = eng.adjoint_run(NDTE, ut, vt)
+    for k in ("strength", "cdn_ocn", "forcex", "forcey"):
+        assert np.array_equal(g[k], ref[k]), k
+    assert g["loss"] == ref["loss"]

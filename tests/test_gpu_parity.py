@@ -275,7 +275,7 @@ def test_fused_tilings_match_oracle(gpu, tile, nsub, inplace):
         assert_state_bitexact(fetch_state(eng, st), want)
 
 
-@pytest.mark.parametrize("mode", ["fused", "fused-inplace"])
+@pytest.mark.parametrize("mode", ["fused", "fused-inplace", "fused-pair"])
 def test_fused_sees_fields_set_between_runs(gpu, mode):
     """str1..8 at points stress never writes (read by stepu from memory),
     stresses and ghost velocities may be changed by the caller between runs."""
@@ -391,7 +391,7 @@ def test_fused_unavailable_without_a_grid(gpu):
             eng.run(3)
 
 
-@pytest.mark.parametrize("mode", ["graph", "fused"])
+@pytest.mark.parametrize("mode", ["graph", "fused", "fused-pair"])
 def test_parameters_change_between_runs(gpu, mode):
     """The captured graph holds the solver scalars: a changed parameter set
     must reach the next run, an unchanged one must not disturb it."""
