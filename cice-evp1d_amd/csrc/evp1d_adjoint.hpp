@@ -25,14 +25,16 @@ struct StepuAdj {
   double cw_b;             // d/d Cw
 };
 
+// The forward result (u, v) = (uvel, vvel) after the update is an input: with
+// w = cc1 + i cc2 and a = cca + i ccb the update is u + i v = conj(a) w / |a|^2,
+// so w = a (u + i v), i.e. cc1 = cca u - ccb v, cc2 = ccb u + cca v.  That
+// gives cc1 and cc2 from the stored velocities alone, without the str1..8
+// gathers they are summed from.
 __host__ __device__ inline void stepu_adjoint(const Scalars &s,
     double uold, double vold, double Cw, double aiX, double uocn, double vocn,
-    double waterx, double watery, double forcex, double forcey, double umassdti,
-    double fm, double uarear, double uvel_init, double vvel_init, double Tbu,
-    double str1, double str2_nw, double str3_sse, double str4_sw,
-    double str5, double str6_sse, double str7_nw, double str8_sw,
-    double u_b, double v_b, StepuAdj &o) {
-  // forward recompute
+    double waterx, double watery, double umassdti, double fm, double uarear, double Tbu,
+    double u, double v, double u_b, double v_b, StepuAdj &o) {
+  // forward recompute of the coefficients
   const double du = uocn - uold, dv = vocn - vold;
   const double R = sqrt(du * du + dv * dv);
   const double A = aiX * s.rhow * Cw;
@@ -44,12 +46,8 @@ __host__ __device__ inline void stepu_adjoint(const Scalars &s,
   const double cca = (s.brlx + s.revp) * umassdti + vrel * s.cosw + Cb;
   const double ccb = fm + sg * vrel * s.sinw;
   const double ab2 = cca * cca + ccb * ccb;
-  const double strintx = uarear * (str1 + str2_nw + str3_sse + str4_sw);
-  const double strinty = uarear * (str5 + str6_sse + str7_nw + str8_sw);
-  const double cc1 = strintx + forcex + vrel * waterx + umassdti * (s.brlx * uold + s.revp * uvel_init);
-  const double cc2 = strinty + forcey + vrel * watery + umassdti * (s.brlx * vold + s.revp * vvel_init);
-  const double u = (cca * cc1 + ccb * cc2) / ab2;
-  const double v = (cca * cc2 - ccb * cc1) / ab2;
+  const double cc1 = cca * u - ccb * v;
+  const double cc2 = ccb * u + cca * v;
   // u = (cca cc1 + ccb cc2)/ab2, v = (cca cc2 - ccb cc1)/ab2, ab2 = cca^2 + ccb^2
   const double cc1_b = (u_b * cca - v_b * ccb) / ab2;
   const double cc2_b = (u_b * ccb + v_b * cca) / ab2;
@@ -80,11 +78,6 @@ __host__ __device__ inline void stepu_adjoint(const Scalars &s,
 // ---------------------------------------------------------------------------
 // stress (bench_v2b.F90:129-310)
 // ---------------------------------------------------------------------------
-struct StressAdj {
-  double sig_b[12];   // adjoint of the stresses at the start of the subcycle
-  double vel_b[8];    // adjoint contributions to u,v at cc, ee, se, ne
-  double strength_b;  // d/d strength
-};
 
 // Reverse of the str1..8 combinations (bench_v2b.F90:217-310), which are
 // linear in the updated stresses: accumulates M^T b into sb[12].
@@ -136,15 +129,20 @@ __host__ __device__ inline void combos_adjoint(double dxT, double dyT, double dx
 }
 
 // Reverse of one subcycle's stress at one T point.  Inputs: the forward
-// inputs of stress_point, the adjoint of the updated stresses (sig_b_new)
-// and of str1..8 (str_b).  Outputs the adjoint of the stresses before the
-// update, the velocity adjoint contributions and d/d strength.
-__host__ __device__ inline void stress_adjoint(const Scalars &s,
+// inputs of stress_point, the adjoint of the updated stresses (sig_b_new[c *
+// ss], c = 0..11) and of str1..8 (str_b).  Writes the adjoint of the
+// stresses before the update to sig_b_out[c * ss] (which may be sig_b_new:
+// every element is read first), the velocity adjoint contributions at cc,
+// ee, se, ne to vel_b[8], and returns d/d strength.  The four corners are
+// reversed one after the other, each from its own strain rates, and feed the
+// velocity accumulators as they go: few values stay live (no per-corner
+// arrays), so the kernel keeps several waves per SIMD without spills.
+__host__ __device__ inline double stress_adjoint(const Scalars &s,
     double uvel_cc, double vvel_cc, double uvel_ee, double vvel_ee,
     double uvel_se, double vvel_se, double uvel_ne, double vvel_ne,
     double dxT, double dyT, double strength,
     double hte, double htn, double htem1, double htnm1,
-    const double *sig_b_new, const double *str_b, StressAdj &o) {
+    const double *sig_b_new, const double *str_b, double *sig_b_out, long ss, double *vel_b) {
   const double cxp = c1p5 * htn - p5 * htnm1;
   const double cyp = c1p5 * hte - p5 * htem1;
   const double cxm = -(c1p5 * htnm1 - p5 * htn);
@@ -153,32 +151,38 @@ __host__ __device__ inline void stress_adjoint(const Scalars &s,
   const double dxhy = p5 * (hte - htem1);
   const double dyhx = p5 * (htn - htnm1);
 
-  // strain rates and Delta per corner: 0 ne, 1 nw, 2 sw, 3 se (stress index 1..4)
-  double dv[4], tn[4], sh[4], De[4];
-  dv[0] = cyp * uvel_cc - dyT * uvel_ee + cxp * vvel_cc - dxT * vvel_se;
-  dv[1] = cym * uvel_ee + dyT * uvel_cc + cxp * vvel_ee - dxT * vvel_ne;
-  dv[2] = cym * uvel_ne + dyT * uvel_se + cxm * vvel_ne + dxT * vvel_ee;
-  dv[3] = cyp * uvel_se - dyT * uvel_ne + cxm * vvel_se + dxT * vvel_cc;
-  tn[0] = -cym * uvel_cc - dyT * uvel_ee + cxm * vvel_cc + dxT * vvel_se;
-  tn[1] = -cyp * uvel_ee + dyT * uvel_cc + cxm * vvel_ee + dxT * vvel_ne;
-  tn[2] = -cyp * uvel_ne + dyT * uvel_se + cxp * vvel_ne - dxT * vvel_ee;
-  tn[3] = -cym * uvel_se - dyT * uvel_ne + cxp * vvel_se - dxT * vvel_cc;
-  sh[0] = -cym * vvel_cc - dyT * vvel_ee - cxm * uvel_cc - dxT * uvel_se;
-  sh[1] = -cyp * vvel_ee + dyT * vvel_cc - cxm * uvel_ee - dxT * uvel_ne;
-  sh[2] = -cyp * vvel_ne + dyT * vvel_se - cxp * uvel_ne + dxT * uvel_ee;
-  sh[3] = -cym * vvel_se - dyT * vvel_ne - cxp * uvel_se + dxT * uvel_cc;
-  for (int c = 0; c < 4; ++c) De[c] = sqrt(dv[c] * dv[c] + s.e_factor * (tn[c] * tn[c] + sh[c] * sh[c]));
-
   // adjoint of the updated stresses including the str1..8 path
   double sb[12];
-  for (int k = 0; k < 12; ++k) sb[k] = sig_b_new[k];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) sb[k] = sig_b_new[k * ss];
   combos_adjoint(dxT, dyT, dxhy, dyhx, str_b, sb);
 
   const double keep = c1 - s.arlx1i * s.revp;
-  double dv_b[4], tn_b[4], sh_b[4], strength_b = 0.0;
+  double strength_b = 0.0;
+  double ucc = 0, vcc = 0, uee = 0, vee = 0, use = 0, vse = 0, une = 0, vne = 0;
+#pragma unroll
   for (int c = 0; c < 4; ++c) {
+    // strain rates and Delta of corner c: 0 ne, 1 nw, 2 sw, 3 se (stress index 1..4)
+    double dv, tn, sh;
+    if (c == 0) {
+      dv = cyp * uvel_cc - dyT * uvel_ee + cxp * vvel_cc - dxT * vvel_se;
+      tn = -cym * uvel_cc - dyT * uvel_ee + cxm * vvel_cc + dxT * vvel_se;
+      sh = -cym * vvel_cc - dyT * vvel_ee - cxm * uvel_cc - dxT * uvel_se;
+    } else if (c == 1) {
+      dv = cym * uvel_ee + dyT * uvel_cc + cxp * vvel_ee - dxT * vvel_ne;
+      tn = -cyp * uvel_ee + dyT * uvel_cc + cxm * vvel_ee + dxT * vvel_ne;
+      sh = -cyp * vvel_ee + dyT * vvel_cc - cxm * uvel_ee - dxT * uvel_ne;
+    } else if (c == 2) {
+      dv = cym * uvel_ne + dyT * uvel_se + cxm * vvel_ne + dxT * vvel_ee;
+      tn = -cyp * uvel_ne + dyT * uvel_se + cxp * vvel_ne - dxT * vvel_ee;
+      sh = -cyp * vvel_ne + dyT * vvel_se - cxp * uvel_ne + dxT * uvel_ee;
+    } else {
+      dv = cyp * uvel_se - dyT * uvel_ne + cxm * vvel_se + dxT * vvel_cc;
+      tn = -cym * uvel_se - dyT * uvel_ne + cxp * vvel_se - dxT * vvel_cc;
+      sh = -cym * vvel_se - dyT * vvel_ne - cxp * uvel_se + dxT * uvel_cc;
+    }
+    const double D = sqrt(dv * dv + s.e_factor * (tn * tn + sh * sh));
     // forward: tmpcalc, zetax2, etax2, rep_prs (visc_replpress)
-    const double D = De[c];
     const double mx = (D > DminTarea) ? D : DminTarea;
     const double dsum = D + DminTarea;
     const double T = s.capping * (strength / mx) + (c1 - s.capping) * (strength / dsum);
@@ -187,15 +191,15 @@ __host__ __device__ inline void stress_adjoint(const Scalars &s,
     // sp = (sp0*keep + arlx1i*(zeta*div - rep))*denom1, rep = (1-K)*T*D
     // sm = (sm0*keep + arlx1i*eta*ten)*denom1, s12 = (s120*keep + arlx1i*p5*eta*shr)*denom1
     const double P = sb[c] * s.denom1, M = sb[4 + c] * s.denom1, Q = sb[8 + c] * s.denom1;
-    o.sig_b[c] = P * keep;
-    o.sig_b[4 + c] = M * keep;
-    o.sig_b[8 + c] = Q * keep;
-    const double zeta_b0 = P * s.arlx1i * dv[c];
+    sig_b_out[c * ss] = P * keep;
+    sig_b_out[(4 + c) * ss] = M * keep;
+    sig_b_out[(8 + c) * ss] = Q * keep;
+    const double zeta_b0 = P * s.arlx1i * dv;
     const double rep_b = -P * s.arlx1i;
-    const double eta_b = M * s.arlx1i * tn[c] + Q * s.arlx1i * p5 * sh[c];
-    dv_b[c] = P * s.arlx1i * zeta;
-    tn_b[c] = M * s.arlx1i * eta;
-    sh_b[c] = Q * s.arlx1i * p5 * eta;
+    const double eta_b = M * s.arlx1i * tn + Q * s.arlx1i * p5 * sh;
+    double dv_b = P * s.arlx1i * zeta;
+    double tn_b = M * s.arlx1i * eta;
+    double sh_b = Q * s.arlx1i * p5 * eta;
     const double zeta_b = zeta_b0 + s.epp2i * eta_b;
     const double T_b = (c1 + s.Ktens) * zeta_b + (c1 - s.Ktens) * D * rep_b;
     double D_b = (c1 - s.Ktens) * T * rep_b;
@@ -204,34 +208,34 @@ __host__ __device__ inline void stress_adjoint(const Scalars &s,
                          (c1 - s.capping) * (-strength / (dsum * dsum));
     D_b += T_b * dT_dD;
     if (D > 0.0) {
-      dv_b[c] += D_b * dv[c] / D;
-      tn_b[c] += D_b * s.e_factor * tn[c] / D;
-      sh_b[c] += D_b * s.e_factor * sh[c] / D;
+      dv_b += D_b * dv / D;
+      tn_b += D_b * s.e_factor * tn / D;
+      sh_b += D_b * s.e_factor * sh / D;
+    }
+    // reverse of this corner's strain rates (linear in the 8 velocities)
+    if (c == 0) {
+      ucc += cyp * dv_b; uee -= dyT * dv_b; vcc += cxp * dv_b; vse -= dxT * dv_b;
+      ucc -= cym * tn_b; uee -= dyT * tn_b; vcc += cxm * tn_b; vse += dxT * tn_b;
+      vcc -= cym * sh_b; vee -= dyT * sh_b; ucc -= cxm * sh_b; use -= dxT * sh_b;
+    } else if (c == 1) {
+      uee += cym * dv_b; ucc += dyT * dv_b; vee += cxp * dv_b; vne -= dxT * dv_b;
+      uee -= cyp * tn_b; ucc += dyT * tn_b; vee += cxm * tn_b; vne += dxT * tn_b;
+      vee -= cyp * sh_b; vcc += dyT * sh_b; uee -= cxm * sh_b; une -= dxT * sh_b;
+    } else if (c == 2) {
+      une += cym * dv_b; use += dyT * dv_b; vne += cxm * dv_b; vee += dxT * dv_b;
+      une -= cyp * tn_b; use += dyT * tn_b; vne += cxp * tn_b; vee -= dxT * tn_b;
+      vne -= cyp * sh_b; vse += dyT * sh_b; une -= cxp * sh_b; uee += dxT * sh_b;
+    } else {
+      use += cyp * dv_b; une -= dyT * dv_b; vse += cxm * dv_b; vcc += dxT * dv_b;
+      use -= cym * tn_b; une -= dyT * tn_b; vse += cxp * tn_b; vcc -= dxT * tn_b;
+      vse -= cym * sh_b; vne -= dyT * sh_b; use -= cxp * sh_b; ucc += dxT * sh_b;
     }
   }
-  o.strength_b = strength_b;
-
-  // reverse of the strain rates (linear in the 8 velocities)
-  double ucc = 0, vcc = 0, uee = 0, vee = 0, use = 0, vse = 0, une = 0, vne = 0;
-  // divergence
-  ucc += cyp * dv_b[0]; uee -= dyT * dv_b[0]; vcc += cxp * dv_b[0]; vse -= dxT * dv_b[0];
-  uee += cym * dv_b[1]; ucc += dyT * dv_b[1]; vee += cxp * dv_b[1]; vne -= dxT * dv_b[1];
-  une += cym * dv_b[2]; use += dyT * dv_b[2]; vne += cxm * dv_b[2]; vee += dxT * dv_b[2];
-  use += cyp * dv_b[3]; une -= dyT * dv_b[3]; vse += cxm * dv_b[3]; vcc += dxT * dv_b[3];
-  // tension
-  ucc -= cym * tn_b[0]; uee -= dyT * tn_b[0]; vcc += cxm * tn_b[0]; vse += dxT * tn_b[0];
-  uee -= cyp * tn_b[1]; ucc += dyT * tn_b[1]; vee += cxm * tn_b[1]; vne += dxT * tn_b[1];
-  une -= cyp * tn_b[2]; use += dyT * tn_b[2]; vne += cxp * tn_b[2]; vee -= dxT * tn_b[2];
-  use -= cym * tn_b[3]; une -= dyT * tn_b[3]; vse += cxp * tn_b[3]; vcc -= dxT * tn_b[3];
-  // shear
-  vcc -= cym * sh_b[0]; vee -= dyT * sh_b[0]; ucc -= cxm * sh_b[0]; use -= dxT * sh_b[0];
-  vee -= cyp * sh_b[1]; vcc += dyT * sh_b[1]; uee -= cxm * sh_b[1]; une -= dxT * sh_b[1];
-  vne -= cyp * sh_b[2]; vse += dyT * sh_b[2]; une -= cxp * sh_b[2]; uee += dxT * sh_b[2];
-  vse -= cym * sh_b[3]; vne -= dyT * sh_b[3]; use -= cxp * sh_b[3]; ucc += dxT * sh_b[3];
-  o.vel_b[0] = ucc; o.vel_b[1] = vcc;
-  o.vel_b[2] = uee; o.vel_b[3] = vee;
-  o.vel_b[4] = use; o.vel_b[5] = vse;
-  o.vel_b[6] = une; o.vel_b[7] = vne;
+  vel_b[0] = ucc; vel_b[1] = vcc;
+  vel_b[2] = uee; vel_b[3] = vee;
+  vel_b[4] = use; vel_b[5] = vse;
+  vel_b[6] = une; vel_b[7] = vne;
+  return strength_b;
 }
 
 }  // namespace evp1d

@@ -96,10 +96,11 @@ struct DevView {
   double *sig;
   double2 *s15, *s27, *s36, *s48;
   // Split addressing (gradient trajectories): vel holds only the stepu slots
-  // [u_begin, u_end) and the str pairs only the stress slots [0, t_end); the
-  // other slots, whose values never change during a run, are read from velc
-  // and s15c..s48c.  vel_out receives stepu's result.  Unsplit kernels use
-  // vel for everything and vel_out == vel.
+  // [u_begin, u_end); the other slots, whose values never change during a
+  // run, are read from velc.  The str pairs may be split the same way over
+  // the stress slots [0, t_end), with s15c..s48c for the rest (the gradient
+  // sweep passes the handle's own arrays as both).  vel_out receives stepu's
+  // result.  Unsplit kernels use vel for everything and vel_out == vel.
   double2 *vel_out;
   const double2 *velc, *s15c, *s27c, *s36c, *s48c;
   const double *dxT, *dyT, *strength, *hte, *htn, *htem1, *htnm1;
@@ -271,9 +272,8 @@ struct FusedView {
 };
 
 // SPLIT: the gradient trajectory's forward sweep.  Velocities are split
-// addressed (see Ld), stepu writes f.vel_out[q - u_begin], and str1..8 of the
-// owned points are stored every subcycle (d.s15..s48 then hold one subcycle's
-// t_end entries of the trajectory).
+// addressed (see Ld) and stepu writes f.vel_out[q - u_begin], one subcycle's
+// entries of the trajectory.
 template <int B, bool X, bool SPLIT = false>
 __global__ __launch_bounds__(B) void k_fused(DevView d, FusedView f, int last) {
   extern __shared__ double2 lds[];
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(B) void k_fused(DevView d, FusedView f, int last) {
           wp[(4 + c) * ss] = o.sm[c];
           wp[(8 + c) * ss] = o.s12[c];
         }
-        if (last || SPLIT) {   // str1..8 are observable after the loop
+        if (last) {   // str1..8 are observable after the loop
           d.s15[g] = p15;
           d.s27[g] = p27;
           d.s36[g] = p36;
@@ -391,6 +391,11 @@ struct PairView {
   double2 *sg_out;
   long ls;
   const int4 *nbr;          // per stress slot: ee, ne, se
+  // gradient trajectory (k_pair<.., TRAJ>): the velocities of the stepu
+  // slots after the launch's first subcycle (tr_mid) and after its second
+  // (tr_end, null when the handle's own buffer is the only copy wanted),
+  // indexed by slot - u_begin
+  double2 *tr_mid, *tr_end;
 };
 
 // the stresses of the stress slots [0, n) into pairs, and back
@@ -436,8 +441,10 @@ __global__ void k_pack_const(DevView d, double2 *pc, long lp) {
 }
 
 // At most 128 VGPRs (4 waves per SIMD): workgroups of 256, 512 or 1024
-// threads then fill 16 waves per CU.
-template <int B, bool X>
+// threads then fill 16 waves per CU.  TRAJ: the gradient's forward sweep,
+// which also stores both subcycles' stepu-slot velocities (f.tr_mid,
+// f.tr_end); the values computed are the same.
+template <int B, bool X, bool TRAJ = false>
 __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_pair(DevView d, PairView f, int last) {
   extern __shared__ double2 lds[];
   const int cap = f.cap;
@@ -529,6 +536,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
     evp1d::StepuOut o;
     stepu(k, q, d.vel[q], o);
     vt[k] = make_double2(o.uvel, o.vvel);
+    if (TRAJ && k < nUo) f.tr_mid[q - d.u_begin] = vt[k];
   }
   __syncthreads();
   // level 2, stress: owned and halo entries, one per thread
@@ -566,6 +574,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
     evp1d::StepuOut o;
     stepu(k, q, vt[k], o);
     f.vel_out[q] = make_double2(o.uvel, o.vvel);
+    if (TRAJ && f.tr_end) f.tr_end[q - d.u_begin] = make_double2(o.uvel, o.vvel);
     if (last) {
       d.strintx[q] = o.strintx;
       d.strinty[q] = o.strinty;
@@ -707,7 +716,10 @@ __global__ __launch_bounds__(256) void k_str_recompute(DevView d) {
 // the contributions of subcycle k+1's stress adjoints that read q: its own
 // (cc), and those of nw(q), sw(q), sse(q), which read q as their ee, ne, se
 // neighbour (the inverse maps).  Only stepu slots need it: nothing else reads
-// the velocity adjoint.
+// the velocity adjoint.  The update's result u_{k+1} is read from the stored
+// trajectory (SPLIT), or recomputed from the str1..8 that k_str_recompute
+// left in the working arrays: stepu_point is the forward kernels' own
+// function, so both give the same bits.
 template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_stepu_adj(DevView d, AdjView a) {
   using L = Ld<SPLIT>;
@@ -721,20 +733,23 @@ __global__ __launch_bounds__(256) void k_stepu_adj(DevView d, AdjView a) {
   if (jsw < d.t_end) { const double2 t = a.vc[3 * na + jsw]; ub.x += t.x; ub.y += t.y; }
   if (js < d.t_end) { const double2 t = a.vc[2 * na + js]; ub.x += t.x; ub.y += t.y; }
   const double2 v = L::vel(d, iw);
-  // str1..8 of slots stress never writes are +0.0 unless *cnz: no loads then
-  const bool cz = *a.cnz == 0;
-  auto str = [&](const double2 *p, const double2 *c, int j) {
-    return (cz && j >= d.t_end) ? make_double2(0.0, 0.0) : L::pair(p, c, d, j);
-  };
-  const double2 s15 = str(d.s15, d.s15c, iw);
-  const double2 s27 = str(d.s27, d.s27c, jw);
-  const double2 s36 = str(d.s36, d.s36c, js);
-  const double2 s48 = str(d.s48, d.s48c, jsw);
+  double2 vn;
+  if (SPLIT) {
+    vn = d.vel_out[iw - d.u_begin];
+  } else {
+    // str1..8 of slots stress never writes are +0.0 unless *cnz: no loads then
+    const bool cz = *a.cnz == 0;
+    auto str = [&](const double2 *p, int j) { return (cz && j >= d.t_end) ? make_double2(0.0, 0.0) : p[j]; };
+    const double2 s15 = str(d.s15, iw), s27 = str(d.s27, jw), s36 = str(d.s36, js), s48 = str(d.s48, jsw);
+    evp1d::StepuOut f;
+    evp1d::stepu_point(d.s, v.x, v.y, d.cw[iw], d.aiu[iw], d.uocn[iw], d.vocn[iw], d.waterx[iw], d.watery[iw],
+                       d.forcex[iw], d.forcey[iw], d.umassdti[iw], d.fm[iw], d.uarear[iw], d.uvel_init[iw],
+                       d.vvel_init[iw], d.tbu[iw], s15.x, s27.x, s36.x, s48.x, s15.y, s36.y, s27.y, s48.y, f);
+    vn = make_double2(f.uvel, f.vvel);
+  }
   evp1d::StepuAdj o;
-  evp1d::stepu_adjoint(d.s, v.x, v.y, d.cw[iw], d.aiu[iw], d.uocn[iw], d.vocn[iw], d.waterx[iw],
-                       d.watery[iw], d.forcex[iw], d.forcey[iw], d.umassdti[iw], d.fm[iw], d.uarear[iw],
-                       d.uvel_init[iw], d.vvel_init[iw], d.tbu[iw], s15.x, s27.x, s36.x, s48.x,
-                       s15.y, s36.y, s27.y, s48.y, ub.x, ub.y, o);
+  evp1d::stepu_adjoint(d.s, v.x, v.y, d.cw[iw], d.aiu[iw], d.uocn[iw], d.vocn[iw], d.waterx[iw], d.watery[iw],
+                       d.umassdti[iw], d.fm[iw], d.uarear[iw], d.tbu[iw], vn.x, vn.y, ub.x, ub.y, o);
   a.ub[iw] = make_double2(o.uold_b, o.vold_b);
   a.g[iw] = make_double2(o.gx, o.gy);
   a.gfx[iw] += o.fx_b;
@@ -754,20 +769,16 @@ __global__ __launch_bounds__(256) void k_stress_adj(DevView d, AdjView a) {
   const double2 cc = L::vel(d, iw), ve = L::vel(d, ie), vn = L::vel(d, in), vs = L::vel(d, is);
   const double2 g0 = a.g[iw], ge = a.g[ie], gs = a.g[is], gn = a.g[in];
   const double str_b[8] = {g0.x, ge.x, gs.x, gn.x, g0.y, gs.y, ge.y, gn.y};
-  double sb[12];
-#pragma unroll
-  for (int c = 0; c < 12; ++c) sb[c] = a.sigb[c * d.ldsig + iw];
-  evp1d::StressAdj o;
-  evp1d::stress_adjoint(d.s, cc.x, cc.y, ve.x, ve.y, vs.x, vs.y, vn.x, vn.y, d.dxT[iw], d.dyT[iw],
-                        d.strength[iw], d.hte[iw], d.htn[iw], d.htem1[iw], d.htnm1[iw], sb, str_b, o);
-#pragma unroll
-  for (int c = 0; c < 12; ++c) a.sigb[c * d.ldsig + iw] = o.sig_b[c];
+  double vb[8];
+  const double gs_b = evp1d::stress_adjoint(d.s, cc.x, cc.y, ve.x, ve.y, vs.x, vs.y, vn.x, vn.y, d.dxT[iw],
+                                            d.dyT[iw], d.strength[iw], d.hte[iw], d.htn[iw], d.htem1[iw],
+                                            d.htnm1[iw], a.sigb + iw, str_b, a.sigb + iw, d.ldsig, vb);
   const long na = d.na;
-  a.vc[iw] = make_double2(o.vel_b[0], o.vel_b[1]);
-  a.vc[na + iw] = make_double2(o.vel_b[2], o.vel_b[3]);
-  a.vc[2 * na + iw] = make_double2(o.vel_b[4], o.vel_b[5]);
-  a.vc[3 * na + iw] = make_double2(o.vel_b[6], o.vel_b[7]);
-  a.gstr[iw] += o.strength_b;
+  a.vc[iw] = make_double2(vb[0], vb[1]);
+  a.vc[na + iw] = make_double2(vb[2], vb[3]);
+  a.vc[2 * na + iw] = make_double2(vb[4], vb[5]);
+  a.vc[3 * na + iw] = make_double2(vb[6], vb[7]);
+  a.gstr[iw] += gs_b;
 }
 
 // ---------------------------------------------------------------------------
@@ -1446,6 +1457,9 @@ typedef void (*PairFn)(DevView, PairView, int);
 static const PairFn kPairFns[kNumPairBlocks][2] = {{k_pair<256, false>, k_pair<256, true>},
                                                    {k_pair<512, false>, k_pair<512, true>},
                                                    {k_pair<1024, false>, k_pair<1024, true>}};
+// the gradient sweep's trajectory-storing form (XCD-aware tile order only)
+static const PairFn kPairTrajFns[kNumPairBlocks] = {k_pair<256, true, true>, k_pair<512, true, true>,
+                                                    k_pair<1024, true, true>};
 static int pair_block_index(int b) {
   for (int i = 0; i < kNumPairBlocks; ++i)
     if (kPairBlocks[i] == b) return i;
@@ -1475,6 +1489,8 @@ static int ensure_fused_buffers(evp1d_handle *h) {
     for (auto &row : kPairFns)
       for (PairFn fn : row)
         HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, kPairMaxLds));
+    for (PairFn fn : kPairTrajFns)
+      HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, kPairMaxLds));
   }
   if (!h->vel_alt) HIP_TRY(hipMalloc((void **)&h->vel_alt, (size_t)h->navel * sizeof(double2)));
   if ((!h->fused_inplace || pair_form(h)) && !h->sig_alt)
@@ -1536,9 +1552,14 @@ static void fused_step(evp1d_handle *h, FusedView &f, int k) {
 // form runs nsub / 2 two-subcycle launches, after one one-subcycle launch
 // when nsub is odd; its stresses ping-pong in pairs between sig_pair and
 // sig_alt (packed before its first launch, unpacked into X after the last).
-static int issue_fused(evp1d_handle *h, int nsub) {
+// traj (pair form only; the gradient's forward sweep): also store the stepu
+// slots' velocities after subcycles 1 .. nsub - 1 at traj + k * nu, k the
+// subcycle, nu = u_end - u_begin (the state after the last stays in X only).
+static int issue_fused(evp1d_handle *h, int nsub, double2 *traj = nullptr) {
   const bool pair = pair_form(h);
+  if (traj && !pair) return fail(EVP1D_ESTATE, "issue_fused: a trajectory needs the pair form");
   const bool odd_first = pair && nsub % 2 == 1;
+  const long nu = (long)h->v.u_end - h->v.u_begin;
   const bool ip = h->fused_inplace && !pair;
   const DevView X = h->v;
   DevView Y = h->v;
@@ -1593,9 +1614,15 @@ static int issue_fused(evp1d_handle *h, int nsub) {
       pv.sg_in = P[pin];
       pv.sg_out = P[1 - pin];
       pin = 1 - pin;
+      PairFn fn = kPairFns[pair_block_index(h->pair_block)][h->tile_xcd ? 1 : 0];
+      if (traj) {
+        const int before = odd_first ? 2 * j - 3 : 2 * j - 2;   // subcycles done before this launch
+        pv.tr_mid = traj + (size_t)(before + 1) * nu;
+        pv.tr_end = before + 2 < nsub ? traj + (size_t)(before + 2) * nu : nullptr;
+        fn = kPairTrajFns[pair_block_index(h->pair_block)];
+      }
       if (h->fused_ntiles > 0) {
-        hipLaunchKernelGGL(kPairFns[pair_block_index(h->pair_block)][h->tile_xcd ? 1 : 0], dim3(h->fused_ntiles),
-                           dim3(h->pair_block), lds2, h->stream, in, pv, j == L ? 1 : 0);
+        hipLaunchKernelGGL(fn, dim3(h->fused_ntiles), dim3(h->pair_block), lds2, h->stream, in, pv, j == L ? 1 : 0);
         HIP_TRY(hipGetLastError());
       }
       continue;
@@ -1613,6 +1640,9 @@ static int issue_fused(evp1d_handle *h, int nsub) {
       hipLaunchKernelGGL(k_halo, dim3(blocks_for(h->v.nhalo)), dim3(kBlock), 0, h->stream, out);
       HIP_TRY(hipGetLastError());
     }
+    if (traj && nsub > 1 && nu > 0)   // the odd first subcycle of a pair-form sweep
+      HIP_TRY(hipMemcpyAsync(traj + nu, out.vel + h->v.u_begin, (size_t)nu * sizeof(double2),
+                             hipMemcpyDeviceToDevice, h->stream));
   }
   if (packed && t_end > 0) {
     hipLaunchKernelGGL(k_unpack_sig, dim3(blocks_for((int)t_end)), dim3(kBlock), 0, h->stream, X.sig, h->ldsig,
@@ -2365,12 +2395,10 @@ struct StateBuf {
 }  // namespace
 
 // device bytes of a stored gradient trajectory of nsub subcycles: the
-// stepu-range velocities of every subcycle and the stress-range str1..8 of
-// all but the last (AdjointRun::run_stored)
+// stepu-range velocities of every subcycle (AdjointRun::run_stored)
 static size_t trajectory_bytes(const evp1d_handle *h, int nsub) {
-  const size_t nu = (size_t)(h->v.u_end - h->v.u_begin), nt = (size_t)h->v.t_end;
-  const size_t ub = (size_t)nsub * nu * sizeof(double2), sb = (size_t)(nsub > 1 ? nsub - 1 : 1) * nt * sizeof(double2);
-  return (ub + 255) / 256 * 256 + 4 * ((sb + 255) / 256 * 256);
+  const size_t nu = (size_t)(h->v.u_end - h->v.u_begin);
+  return ((size_t)nsub * nu * sizeof(double2) + 255) / 256 * 256;
 }
 
 struct AdjointRun {
@@ -2482,18 +2510,19 @@ struct AdjointRun {
 return EVP1D_OK;
   }
   // All nsub states kept (as much HBM as the trajectory needs): the forward
-  // sweep writes each subcycle's velocities and str1..8 out of place into
-  // per-subcycle buffers (split addressing: only the slots a subcycle changes),
-  // the reverse sweep reads them back; no recomputation, no state copies.
+  // sweep writes each subcycle's velocities out of place into per-subcycle
+  // buffers (split addressing: only the slots a subcycle changes), the
+  // reverse sweep reads them back; no recomputation, no state copies.  The
+  // reverse needs no stresses and no str1..8: the stress adjoint reads only
+  // velocities, and the stepu adjoint takes its forward sums from u_k and
+  // u_{k+1} (stepu_adjoint).
   int run_stored() {
     const DevView W = h->v;
     const int N = nsub, nu = W.u_end - W.u_begin, nt = W.t_end;
-    const size_t ub = (size_t)N * nu * sizeof(double2), sb = (size_t)(N > 1 ? N - 1 : 1) * nt * sizeof(double2);
-    // (trajectory_bytes() restates this layout)
     // The trajectory storage stays with the handle (large allocations and
     // frees are slow) until it is destroyed or a checkpointed run releases
     // it.  Every entry is written before it is read.
-    const size_t ua = (ub + 255) / 256 * 256, sa = (sb + 255) / 256 * 256, need = ua + 4 * sa;
+    const size_t need = trajectory_bytes(h, N);
     if (h->traj_bytes < need) {
       HIP_TRY(hipStreamSynchronize(h->stream));
       if (h->traj) HIP_TRY(hipFree(h->traj));
@@ -2511,27 +2540,25 @@ return EVP1D_OK;
       }
       h->traj_bytes = need;
     }
-    char *base = static_cast<char *>(h->traj);
-    double2 *U = reinterpret_cast<double2 *>(base);
-    double2 *S[4];
-    for (int c = 0; c < 4; ++c) S[c] = reinterpret_cast<double2 *>(base + ua + (size_t)c * sa);
+    double2 *U = static_cast<double2 *>(h->traj);
     if (nu > 0)
       HIP_TRY(hipMemcpyAsync(U, W.vel + W.u_begin, (size_t)nu * sizeof(double2), hipMemcpyDeviceToDevice, h->stream));
-    auto view = [&](int k) {   // the split view of subcycle k (x_{k-1} -> x_k)
+    // the split view of subcycle k (x_{k-1} -> x_k): velocities split, str1..8
+    // in the handle's arrays (the unfused sweep's stepu reads what its stress
+    // wrote there; the last subcycle leaves them as a run would)
+    auto view = [&](int k) {
       DevView dk = W;
       dk.vel = U + (size_t)(k - 1) * nu;
       dk.velc = W.vel;
       dk.s15c = W.s15; dk.s27c = W.s27; dk.s36c = W.s36; dk.s48c = W.s48;
-      if (k < N) {
-        const size_t o = (size_t)(k - 1) * nt;
-        dk.s15 = S[0] + o; dk.s27 = S[1] + o; dk.s36 = S[2] + o; dk.s48 = S[3] + o;
-        dk.vel_out = U + (size_t)k * nu;
-      } else {
-        dk.vel_out = W.vel + W.u_begin;
-      }
+      dk.vel_out = k < N ? U + (size_t)k * nu : W.vel + W.u_begin;
       return dk;
     };
-    if (h->fused_ok) {
+    if (pair_form(h)) {
+      // two subcycles per launch, both stored (k_pair<.., TRAJ>)
+      RC_TRY(ensure_fused_buffers(h));
+      RC_TRY(issue_fused(h, N, U));
+    } else if (h->fused_ok) {
       // fused forward sweep; two-buffer stresses ping-pong between the
       // handle's buffers and end in its own (subcycle k writes it when N - k
       // is even)

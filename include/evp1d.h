@@ -292,7 +292,7 @@ int evp1d_tile_plan(int32_t na, int32_t navel, const int32_t *ee, const int32_t 
  * na each; any output may be NULL).  The reverse sweep uses hand-written
  * adjoint stress and stepu kernels and binomial checkpointing with nsnap
  * stored subcycle states (nsnap >= nsub stores them all: no recomputation;
- * the handle keeps that storage, about 80 B per point and subcycle, for later
+ * the handle keeps that storage, 16 B per stepu point and subcycle, for later
  * calls until it is destroyed or a call with 0 < nsnap < nsub releases it;
  * nsnap <= 0 chooses: all of them if they fit in free device memory, else as
  * many stored states as fit).

@@ -198,20 +198,29 @@ def test_batch_gradient_equals_member_gradients(gpu, nsnap):
     assert abs(g["loss"] - want) <= 1e-12 * want
 
 
-def test_gradient_unchanged_by_the_pair_form(problem):
-    """With two subcycles per fused launch selected (the handle's forward
-    runs), the gradient, whose forward sweep stores every subcycle, is the
-    same bit for bit, also after a pair-form run on the handle."""
+@pytest.mark.parametrize("nsub", [NDTE, NDTE - 1, 2, 1])
+def test_gradient_unchanged_by_the_pair_form(problem, nsub):
+    """With two subcycles per fused launch selected, the gradient's forward
+    sweep runs the pair form and stores both subcycles of each launch (after
+    one single subcycle when nsub is odd).  The gradient and the end state
+    are the same bit for bit as with one subcycle per launch, also after a
+    pair-form run on the handle, and with the whole trajectory or 3 stored
+    states."""
     st, ut, vt = problem
+    lib, p = oracle_params(st, NDTE)
+    want = st.copy()
+    lib.run(p, want, nsub)
     with make_engine(st, NDTE, "fused") as eng:
-        eng.set_tiling(31, 15, 512, 1)   # the same slot order: the loss sums in the same order
-        ref = eng.adjoint_run(NDTE, ut, vt)
-    with make_engine(st, NDTE, "fused-pair") as eng:
-        assert eng.get_fused_depth()["active"]
-        eng.run(3)
-        for k, v in st.f.items():       # back to the start state
-            eng.set_field(k, v)
-        g = eng.adjoint_run(NDTE, ut, vt)
-    for k in ("strength", "cdn_ocn", "forcex", "forcey"):
-        assert np.array_equal(g[k], ref[k]), k
-    assert g["loss"] == ref["loss"]
+        eng.set_tiling(32, 16, 512, 1)   # fused-pair's tiling, so the loss sums in the same slot order
+        ref = eng.adjoint_run(nsub, ut, vt)
+    for nsnap in (0, 3):
+        with make_engine(st, NDTE, "fused-pair") as eng:
+            assert eng.get_fused_depth()["active"]
+            eng.run(3)
+            for k, v in st.f.items():       # back to the start state
+                eng.set_field(k, v)
+            g = eng.adjoint_run(nsub, ut, vt, nsnap=nsnap)
+            assert_state_bitexact(fetch_state(eng, st), want)
+        for k in ("strength", "cdn_ocn", "forcex", "forcey"):
+            assert np.array_equal(g[k], ref[k]), (nsnap, k)
+        assert g["loss"] == ref["loss"]
