@@ -307,7 +307,11 @@ def adjoint_measure(s: dict, ndte: int, t_fwd_step: float, nsnap: int, ndir: int
             "grad_cdn_ocn_scalar": float(np.sum(g["cdn_ocn"]))}
 
 
-def ensemble_measure(nmem: int, steps: int, autotune: int) -> dict:
+def ensemble_measure(nmem: int, steps: int, autotune: int, tune_fused: int = 0) -> dict:
+    """64-member ensemble: one batched handle against member after member
+    and lockstep launches.  tune_fused > 0: the batch and every single member
+    first pick their fused form with evp1d_autotune_fused (tile ladder,
+    stress storage, one or two subcycles per launch) on their own data."""
     import evp1d
     from evp1d import synth
     t = time.time()
@@ -325,9 +329,17 @@ def ensemble_measure(nmem: int, steps: int, autotune: int) -> dict:
     for k, s in enumerate(members):
         batch.load_member(k, s["fields"], s["idx"], s["skipT"], s["skipU"])
     tuning = batch.autotune(autotune) if autotune else batch.get_tuning()
+    fused_batch = batch.autotune_fused(tune_fused) if tune_fused > 0 else None
+    if fused_batch:
+        log(f"[bench] ensemble: batch autotune_fused: {fused_batch}")
     t_batch = time_steps(batch, ndte, steps, 1) / steps
     batch.close()
     singles = [new_engine(s, ndte) for s in members]
+    depths = []
+    if tune_fused > 0:
+        for e in singles:
+            depths.append(e.autotune_fused(tune_fused)["depth"])
+        log(f"[bench] ensemble: members tuned, {depths.count(2)} of {nmem} on two subcycles per launch")
     for e in singles:
         e.run(ndte)
     for e in singles:
@@ -358,7 +370,8 @@ def ensemble_measure(nmem: int, steps: int, autotune: int) -> dict:
            "batched_pts_sub_per_s": tot * ndte / t_batch, "sequential_pts_sub_per_s": tot * ndte / t_seq,
            "lockstep_pts_sub_per_s": tot * ndte / t_lock,
            "speedup": round(t_seq / t_batch, 3), "speedup_vs_lockstep": round(t_lock / t_batch, 3),
-           "tuning": tuning}
+           "tuning": tuning, "fused_batch": fused_batch,
+           "members_on_pair_form": depths.count(2) if depths else None}
     log(f"[bench] ensemble: batched {t_batch:.3f}s, member after member {t_seq:.3f}s (x{t_seq / t_batch:.2f}), "
         f"lockstep {t_lock:.3f}s (x{t_lock / t_batch:.2f})")
     return out
@@ -490,7 +503,7 @@ def main() -> int:
         print(json.dumps(out), flush=True)
         return 0
     if args.ensemble:
-        out = ensemble_measure(args.ensemble, args.steps, args.autotune)
+        out = ensemble_measure(args.ensemble, args.steps, args.autotune, args.tune_fused)
         out.update({"metric": "ensemble per-member throughput, batched vs sequential", "n_gpus": 1,
                     "dtype": "f64", "data": "synthetic"})
         print(json.dumps(out), flush=True)
