@@ -1014,6 +1014,9 @@ struct evp1d_handle {
   // pair form (two subcycles per fused launch): its tables, planned on the
   // current tiling when fused_depth is 2
   int fused_depth = 1;
+  // subcycles per checkpointing unit of a gradient with fewer stored states
+  // than subcycles (0: chosen per call)
+  int adj_segment = 0;
   bool pair_ok = false;
   std::string pair_why = "pair form not selected";
   int pair_block = 0, pair_cap = 0, pair_capv = 0;
@@ -1027,6 +1030,8 @@ struct evp1d_handle {
   int4 *nbr4 = nullptr;            // ee, ne, se per stress slot, packed per pair run
   void *traj = nullptr;            // gradient trajectory storage, kept between calls
   size_t traj_bytes = 0;
+  void *adj_ws = nullptr;          // the gradient's per-call buffers, kept between calls
+  size_t adj_ws_bytes = 0;
   // 2D grid of the caller's arrays (device-side pack/unpack)
   int grid_nx = 0, grid_ny = 0;
   std::vector<int32_t> grid_ij;    // caller position < na -> 0-based linear 2D index
@@ -1111,6 +1116,7 @@ static void release(evp1d_handle *h) {
   if (h->nbr4) (void)hipFree(h->nbr4);
   if (h->cnz_dev) (void)hipFree(h->cnz_dev);
   if (h->traj) (void)hipFree(h->traj);
+  if (h->adj_ws) (void)hipFree(h->adj_ws);
   if (h->pos2d_dev) (void)hipFree(h->pos2d_dev);
   for (double *g : h->g2d)
     if (g) (void)hipFree(g);
@@ -2283,6 +2289,19 @@ int evp1d_set_fused_depth(evp1d_handle *h, int32_t depth) {
   return EVP1D_OK;
 }
 
+int evp1d_set_adjoint_segment(evp1d_handle *h, int32_t seg) {
+  if (!h) return fail(EVP1D_EINVAL, "evp1d_set_adjoint_segment: NULL handle");
+  if (seg < 0) return fail(EVP1D_EINVAL, "evp1d_set_adjoint_segment: seg=%d < 0", seg);
+  h->adj_segment = seg;
+  return EVP1D_OK;
+}
+
+int evp1d_get_adjoint_segment(evp1d_handle *h, int32_t *seg) {
+  if (!h) return fail(EVP1D_EINVAL, "evp1d_get_adjoint_segment: NULL handle");
+  if (seg) *seg = h->adj_segment;
+  return EVP1D_OK;
+}
+
 int evp1d_get_fused_depth(evp1d_handle *h, int32_t *depth, int32_t *active, int32_t *block) {
   if (!h) return fail(EVP1D_EINVAL, "evp1d_get_fused_depth: NULL handle");
   if (ready(h)) RC_TRY(ensure(h));
@@ -2395,7 +2414,7 @@ struct StateBuf {
 }  // namespace
 
 // device bytes of a stored gradient trajectory of nsub subcycles: the
-// stepu-range velocities of every subcycle (AdjointRun::run_stored)
+// stepu-range velocities of every subcycle (AdjointRun::run_segment)
 static size_t trajectory_bytes(const evp1d_handle *h, int nsub) {
   const size_t nu = (size_t)(h->v.u_end - h->v.u_begin);
   return ((size_t)nsub * nu * sizeof(double2) + 255) / 256 * 256;
@@ -2412,7 +2431,10 @@ struct AdjointRun {
   double *ut = nullptr, *vt = nullptr, *terms = nullptr;
   std::vector<void *> mem;
   Schedule sched;
-  int working = -1;              // index i of the state x_i held by the working arrays
+  int seg = 1;                   // subcycles per checkpointing unit (the last unit may be shorter)
+  int nunits = 0;                // ceil(nsub / seg)
+  double *diag_save = nullptr;   // [3][na]: strintx, strinty, Cb of x_nsub (checkpointed, seg > 1)
+  int working = -1;              // unit index i of the state held by the working arrays
   double loss = 0.0;
 
   int alloc(void **p, size_t bytes, bool zero = true) {
@@ -2440,13 +2462,49 @@ struct AdjointRun {
     working = index;
     return EVP1D_OK;
   }
+  // first subcycle of unit i (0-based; unit nunits ends at nsub)
+  int unit_start(int i) const { return std::min(i * seg, nsub); }
+  // n units forward on the working arrays: segments as one fused run (the
+  // handle's form), single subcycles as unfused launches (no per-run set-up)
   int advance(int n) {
-    for (int k = 0; k < n; ++k) {
-      RC_TRY(launch_stress(h));
-      RC_TRY(launch_stepu(h, false));
-      RC_TRY(launch_halo(h));
+    const int m = unit_start(working + n) - unit_start(working);
+    if (seg > 1 && m > 0 && h->fused_ok) {
+      RC_TRY(ensure_fused_buffers(h));
+      RC_TRY(issue_fused(h, m));
+    } else {
+      for (int k = 0; k < m; ++k) {
+        RC_TRY(launch_stress(h));
+        RC_TRY(launch_stepu(h, false));
+        RC_TRY(launch_halo(h));
+      }
     }
     working += n;
+    return EVP1D_OK;
+  }
+  // x_nsub of a segmented run: its state, str1..8 and diagnostics, put back
+  // into the handle after the reverse sweep
+  int save_final() {
+    RC_TRY(copy_state(final_state, work(), true));
+    const size_t nV = (size_t)h->navel * sizeof(double2), nA = (size_t)h->na * sizeof(double);
+    double2 *pairs[4] = {h->v.s15, h->v.s27, h->v.s36, h->v.s48};
+    for (int k = 0; k < 4; ++k)
+      HIP_TRY(hipMemcpyAsync(str_save + (size_t)k * h->navel, pairs[k], nV, hipMemcpyDeviceToDevice, h->stream));
+    double *dg[3] = {h->v.strintx, h->v.strinty, h->v.cb};
+    for (int k = 0; k < 3; ++k)
+      HIP_TRY(hipMemcpyAsync(diag_save + (size_t)k * h->na, dg[k], nA, hipMemcpyDeviceToDevice, h->stream));
+    return EVP1D_OK;
+  }
+  int restore_final() {
+    RC_TRY(copy_state(work(), final_state, false));
+    const size_t nV = (size_t)h->navel * sizeof(double2), nA = (size_t)h->na * sizeof(double);
+    double2 *pairs[4] = {h->v.s15, h->v.s27, h->v.s36, h->v.s48};
+    for (int k = 0; k < 4; ++k)
+      HIP_TRY(hipMemcpyAsync(pairs[k], str_save + (size_t)k * h->navel, nV, hipMemcpyDeviceToDevice, h->stream));
+    if (diag_save) {
+      double *dg[3] = {h->v.strintx, h->v.strinty, h->v.cb};
+      for (int k = 0; k < 3; ++k)
+        HIP_TRY(hipMemcpyAsync(dg[k], diag_save + (size_t)k * h->na, nA, hipMemcpyDeviceToDevice, h->stream));
+    }
     return EVP1D_OK;
   }
   int take(int index) {
@@ -2491,8 +2549,16 @@ struct AdjointRun {
     for (double x : t) loss += x;
     return EVP1D_OK;
   }
-  // reverse subcycle k (x_{k-1} -> x_k) with the working arrays holding x_{k-1}
+  // reverse unit k with the working arrays holding its start state: one
+  // subcycle from recomputed str1..8 (seg = 1), or a segment through a
+  // stored velocity trajectory
   int reverse_step(int k) {
+    if (seg > 1) {
+      const int b = unit_start(k - 1), e = unit_start(k);
+      RC_TRY(run_segment(e - b, k == nunits));
+      working = k;   // the segment's forward sweep left its end state
+      return EVP1D_OK;
+    }
     if (k == nsub) RC_TRY(final_step());
     const DevView &d = h->v;
     if (d.t_end > 0) {
@@ -2516,9 +2582,14 @@ return EVP1D_OK;
   // reverse needs no stresses and no str1..8: the stress adjoint reads only
   // velocities, and the stepu adjoint takes its forward sums from u_k and
   // u_{k+1} (stepu_adjoint).
-  int run_stored() {
+  // len subcycles from the working state, reversed: the whole run (len =
+  // nsub, final) or one segment of a segmented checkpointed run.  final: the
+  // segment ends at x_nsub, whose loss seeds the adjoint (and which a
+  // segmented checkpointed run, the one with diag_save, saves: the sweeps of
+  // earlier segments overwrite the handle's arrays).
+  int run_segment(int len, bool final) {
     const DevView W = h->v;
-    const int N = nsub, nu = W.u_end - W.u_begin, nt = W.t_end;
+    const int N = len, nu = W.u_end - W.u_begin, nt = W.t_end;
     // The trajectory storage stays with the handle (large allocations and
     // frees are slow) until it is destroyed or a checkpointed run releases
     // it.  Every entry is written before it is read.
@@ -2594,18 +2665,22 @@ return EVP1D_OK;
         HIP_TRY(hipGetLastError());
       }
     }
-    // the handle now holds x_N (stresses, velocities, str1..8, strint, Cb)
-    if (h->na > 0) {
-      hipLaunchKernelGGL(k_seed, dim3(blocks(h->na)), dim3(256), 0, h->stream, W, a, (const double *)ut,
-                         (const double *)vt, terms);
-      HIP_TRY(hipGetLastError());
+    // the handle now holds the segment's end state (stresses, velocities,
+    // str1..8, strint, Cb)
+    if (final) {
+      if (diag_save) RC_TRY(save_final());   // a segmented checkpointed run
+      if (h->na > 0) {
+        hipLaunchKernelGGL(k_seed, dim3(blocks(h->na)), dim3(256), 0, h->stream, W, a, (const double *)ut,
+                           (const double *)vt, terms);
+        HIP_TRY(hipGetLastError());
+      }
+      std::vector<double> t((size_t)h->na);
+      if (h->na > 0)
+        HIP_TRY(hipMemcpyAsync(t.data(), terms, (size_t)h->na * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+      HIP_TRY(hipStreamSynchronize(h->stream));
+      loss = 0.0;
+      for (double x : t) loss += x;
     }
-    std::vector<double> t((size_t)h->na);
-    if (h->na > 0)
-      HIP_TRY(hipMemcpyAsync(t.data(), terms, (size_t)h->na * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    loss = 0.0;
-    for (double x : t) loss += x;
     for (int k = N; k >= 1; --k) {
       const DevView dk = view(k);
       if (nu > 0) hipLaunchKernelGGL(k_stepu_adj<true>, dim3(blocks(nu)), dim3(256), 0, h->stream, dk, a);
@@ -2673,19 +2748,49 @@ int evp1d_adjoint_run(evp1d_handle *h, int32_t nsub, int32_t nsnap, const double
   R.h = h;
   R.nsub = nsub;
   const size_t nA = (size_t)h->na * sizeof(double), nV = (size_t)h->navel * sizeof(double2);
-  RC_TRY(R.alloc((void **)&R.a.ub, nV));
-  RC_TRY(R.alloc((void **)&R.a.sigb, R.sig_bytes()));
-  RC_TRY(R.alloc((void **)&R.a.g, nV));
-  RC_TRY(R.alloc((void **)&R.a.vc, 4 * (size_t)h->na * sizeof(double2)));
-  RC_TRY(R.alloc((void **)&R.a.gstr, nA));
-  RC_TRY(R.alloc((void **)&R.a.gcw, nA));
-  RC_TRY(R.alloc((void **)&R.a.gfx, nA));
-  RC_TRY(R.alloc((void **)&R.a.gfy, nA));
-  RC_TRY(R.alloc((void **)&R.ut, nA));
-  RC_TRY(R.alloc((void **)&R.vt, nA));
-  RC_TRY(R.alloc((void **)&R.terms, nA));
   int *cnz = nullptr;
-  RC_TRY(R.alloc((void **)&cnz, sizeof(int)));   // zeroed
+  // adjoint state, gradient accumulators, targets and loss terms: carved
+  // from one allocation that the handle keeps (large allocations are slow),
+  // cleared on every call
+  {
+    const size_t parts[12] = {nV, R.sig_bytes(), nV, 4 * (size_t)h->na * sizeof(double2), nA, nA, nA, nA,
+                              nA, nA, nA, sizeof(int)};
+    size_t total = 0;
+    for (size_t b : parts) total += (b + 255) / 256 * 256;
+    if (h->adj_ws_bytes < total) {
+      HIP_TRY(hipStreamSynchronize(h->stream));
+      if (h->adj_ws) HIP_TRY(hipFree(h->adj_ws));
+      h->adj_ws = nullptr;
+      h->adj_ws_bytes = 0;
+      const hipError_t e = hipMalloc(&h->adj_ws, total);
+      if (e != hipSuccess) {
+        h->adj_ws = nullptr;
+        return fail(EVP1D_ENOMEM, "evp1d_adjoint_run: hipMalloc(%zu): %s", total, hipGetErrorString(e));
+      }
+      h->adj_ws_bytes = total;
+    }
+    HIP_TRY(hipMemsetAsync(h->adj_ws, 0, total, h->stream));
+    char *base = static_cast<char *>(h->adj_ws);
+    size_t off = 0;
+    int part = 0;
+    auto carve = [&]() {
+      void *p = base + off;
+      off += (parts[part++] + 255) / 256 * 256;
+      return p;
+    };
+    R.a.ub = static_cast<double2 *>(carve());
+    R.a.sigb = static_cast<double *>(carve());
+    R.a.g = static_cast<double2 *>(carve());
+    R.a.vc = static_cast<double2 *>(carve());
+    R.a.gstr = static_cast<double *>(carve());
+    R.a.gcw = static_cast<double *>(carve());
+    R.a.gfx = static_cast<double *>(carve());
+    R.a.gfy = static_cast<double *>(carve());
+    R.ut = static_cast<double *>(carve());
+    R.vt = static_cast<double *>(carve());
+    R.terms = static_cast<double *>(carve());
+    cnz = static_cast<int *>(carve());
+  }
   R.a.cnz = cnz;
   if (h->navel > h->v.t_end) {
     hipLaunchKernelGGL(k_const_flag, dim3((unsigned)((h->navel - h->v.t_end + 255) / 256)), dim3(256), 0, h->stream,
@@ -2710,9 +2815,10 @@ int evp1d_adjoint_run(evp1d_handle *h, int32_t nsub, int32_t nsnap, const double
       nsnap = (int)std::max<size_t>(1, std::min<size_t>(room / 2, (size_t)nsub - 1));
     }
   }
-  const int S = std::min(nsnap, nsub);       // more slots than steps are never used
   if (nsnap >= nsub) {
-    RC_TRY(R.run_stored());
+    R.seg = nsub;
+    R.nunits = 1;
+    RC_TRY(R.run_segment(nsub, true));
   } else {
     if (h->traj) {   // a checkpointed run gives the stored trajectory's memory back
       HIP_TRY(hipStreamSynchronize(h->stream));
@@ -2720,6 +2826,24 @@ int evp1d_adjoint_run(evp1d_handle *h, int32_t nsub, int32_t nsnap, const double
       h->traj = nullptr;
       h->traj_bytes = 0;
     }
+    // Checkpointing units of seg subcycles: binomial checkpointing over the
+    // units, each unit reversed through a stored velocity trajectory of seg
+    // states (seg = 1: one subcycle from recomputed str1..8).  Automatic:
+    // the shortest unit that gives every unit start a stored state (no
+    // recomputation beyond one forward sweep per unit), if its trajectory
+    // fits beside the stored states, else seg = 1.
+    const int S0 = nsnap;
+    int seg = h->adj_segment > 0 ? std::min<int>(h->adj_segment, nsub) : (nsub + S0 - 1) / S0;
+    if (h->adj_segment == 0 && seg > 1) {
+      size_t free_b = 0, total_b = 0;
+      HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+      const size_t slot = R.sig_bytes() + (size_t)h->na * sizeof(double2);
+      if (trajectory_bytes(h, seg) + (size_t)S0 * slot + ((size_t)1 << 30) > free_b) seg = 1;
+    }
+    R.seg = seg;
+    R.nunits = (nsub + seg - 1) / seg;
+    const int S = std::min(S0, R.nunits);   // more slots than units are never used
+    if (seg > 1) RC_TRY(R.alloc((void **)&R.diag_save, 3 * nA));
     RC_TRY(R.alloc((void **)&R.str_save, 4 * nV));
     RC_TRY(R.alloc((void **)&R.final_state.sig, R.sig_bytes()));
     RC_TRY(R.alloc((void **)&R.final_state.vel, nV));
@@ -2729,16 +2853,13 @@ int evp1d_adjoint_run(evp1d_handle *h, int32_t nsub, int32_t nsnap, const double
       RC_TRY(R.alloc((void **)&R.slots[(size_t)k].sig, R.sig_bytes()));
       RC_TRY(R.alloc((void **)&R.slots[(size_t)k].vel, (size_t)h->na * sizeof(double2)));
     }
-    R.sched.build(nsub, S - 1);
+    R.sched.build(R.nunits, S - 1);
     R.working = 0;
     const int s0 = R.take(0);
     if (s0 < 0) return -s0;
-    RC_TRY(R.rev(0, nsub, S - 1, s0));
+    RC_TRY(R.rev(0, R.nunits, S - 1, s0));
     // leave the handle in the forward final state x_N
-    RC_TRY(R.copy_state(R.work(), R.final_state, false));
-    double2 *pairs[4] = {h->v.s15, h->v.s27, h->v.s36, h->v.s48};
-    for (int k = 0; k < 4; ++k)
-      HIP_TRY(hipMemcpyAsync(pairs[k], R.str_save + (size_t)k * h->navel, nV, hipMemcpyDeviceToDevice, h->stream));
+    RC_TRY(R.restore_final());
   }
   RC_TRY(download_caller_order(h, g_strength, R.a.gstr));
   RC_TRY(download_caller_order(h, g_cdn_ocn, R.a.gcw));

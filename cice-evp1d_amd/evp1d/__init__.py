@@ -54,7 +54,8 @@ EXPORTS = (
     "evp1d_create_batch", "evp1d_get_members", "evp1d_member_info", "evp1d_set_field_member",
     "evp1d_get_field_member", "evp1d_set_index_member", "evp1d_set_skip_member",
     "evp1d_autotune", "evp1d_set_tuning", "evp1d_get_tuning",
-    "evp1d_adjoint_run", "evp1d_loss", "evp1d_tangent_run",
+    "evp1d_adjoint_run", "evp1d_loss", "evp1d_tangent_run", "evp1d_set_adjoint_segment",
+    "evp1d_get_adjoint_segment",
     "evp1d_set_tiling", "evp1d_get_tiling", "evp1d_tile_plan",
     "evp1d_set_fused_inplace", "evp1d_get_fused_inplace", "evp1d_autotune_fused",
     "evp1d_set_fused_depth", "evp1d_get_fused_depth",
@@ -144,6 +145,8 @@ def load(fma: bool = False, path: str | os.PathLike | None = None) -> ctypes.CDL
         "evp1d_get_tuning": [_H, _IP, _IP, _IP, _IP],
         "evp1d_adjoint_run": [_H, i32, i32, _DP, _DP, _DP, _DP, _DP, _DP, _DP],
         "evp1d_loss": [_H, _DP, _DP, _DP],
+        "evp1d_set_adjoint_segment": [_H, i32],
+        "evp1d_get_adjoint_segment": [_H, _IP],
         "evp1d_tangent_run": [_H, i32, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP],
         "evp1d_set_tiling": [_H, i32, i32, i32, i32],
         "evp1d_set_grid": [_H, i32, i32, _IP, i32],
@@ -414,6 +417,16 @@ class Engine:
     def set_fused_depth(self, depth: int) -> None:
         """Subcycles per fused launch: 1, or 2 (the pair form; evp1d_set_fused_depth)."""
         _check(self.L, self.L.evp1d_set_fused_depth(self.h, int(depth)))
+
+    def set_adjoint_segment(self, seg: int) -> None:
+        """Subcycles per checkpointing unit of adjoint_run with fewer stored
+        states than subcycles (evp1d_set_adjoint_segment; 0 = automatic)."""
+        _check(self.L, self.L.evp1d_set_adjoint_segment(self.h, int(seg)))
+
+    def get_adjoint_segment(self) -> int:
+        s = ctypes.c_int32()
+        _check(self.L, self.L.evp1d_get_adjoint_segment(self.h, ctypes.byref(s)))
+        return s.value
 
     def get_fused_depth(self) -> dict:
         """{"depth": setting, "active": runs use the pair form, "block": its workgroup size}."""

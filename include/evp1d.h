@@ -303,6 +303,18 @@ int evp1d_adjoint_run(evp1d_handle *h, int32_t nsub, int32_t nsnap, const double
                       const double *v_target, double *loss, double *g_strength, double *g_cdn_ocn,
                       double *g_forcex, double *g_forcey);
 int evp1d_loss(evp1d_handle *h, const double *u_target, const double *v_target, double *loss);
+/* Checkpointing unit of evp1d_adjoint_run when 0 < nsnap < nsub (no
+ * counterpart in the reference).  The run stores full states (stresses and
+ * velocities) at unit starts by binomial checkpointing over units of seg
+ * subcycles, and reverses each unit through a stored trajectory of its seg
+ * subcycles' velocities (seg x 16 B per stepu point beside the nsnap stored
+ * states).  seg = 1: units of one subcycle, reversed from str1..8 recomputed
+ * from the stored state.  seg = 0 (default): ceil(nsub / nsnap), which stores
+ * every unit start and recomputes each unit once, when that trajectory fits
+ * in free device memory beside the stored states, else 1.  The gradient is
+ * the same bit for bit for every seg and nsnap.                            */
+int evp1d_set_adjoint_segment(evp1d_handle *h, int32_t seg);
+int evp1d_get_adjoint_segment(evp1d_handle *h, int32_t *seg);
 /* Tangent-linear run: nsub subcycles from the current state propagating the
  * perturbation (d_strength, d_cdn_ocn, d_forcex, d_forcey) (caller order, na
  * each, NULL = zero) by forward-mode differentiation of the same point

@@ -109,16 +109,30 @@ def test_central_finite_differences(problem):
     assert worst <= 1e-6
 
 
-def test_gradient_independent_of_snapshot_count(problem):
+@pytest.mark.parametrize("mode", ["graph", "fused-pair"])
+def test_gradient_independent_of_snapshot_count(problem, mode):
+    """From one stored state to all of them, and for every checkpointing
+    unit: one subcycle (binomial checkpointing with recomputed str1..8),
+    automatic (one unit per stored state, each reversed through its stored
+    velocity trajectory), and 5 or 3 subcycles (binomial checkpointing over
+    more units than stored states).  Same gradient and loss bit for bit; the
+    handle ends in the exact forward state."""
     st, ut, vt = problem
+    lib, p = oracle_params(st, NDTE)
+    want = st.copy()
+    lib.run(p, want, NDTE)
     out = []
-    for nsnap in (1, 3, 7, NDTE + 5, 0):     # 0: chosen by the library (all, here)
-        with make_engine(st, NDTE) as eng:
+    cases = [(1, 1), (3, 1), (7, 1), (1, 0), (3, 0), (7, 0), (3, 5), (2, 3), (NDTE + 5, 0), (0, 0)]
+    for nsnap, seg in cases:     # nsnap 0: chosen by the library (all, here)
+        with make_engine(st, NDTE, mode) as eng:
+            eng.set_adjoint_segment(seg)
+            assert eng.get_adjoint_segment() == seg
             out.append(eng.adjoint_run(NDTE, ut, vt, nsnap=nsnap))
-    for g in out[1:]:
+            assert_state_bitexact(fetch_state(eng, st), want)
+    for (nsnap, seg), g in zip(cases[1:], out[1:]):
         for k in ("strength", "cdn_ocn", "forcex", "forcey"):
-            assert np.array_equal(g[k], out[0][k]), k
-        assert g["loss"] == out[0]["loss"]
+            assert np.array_equal(g[k], out[0][k]), (nsnap, seg, k)
+        assert g["loss"] == out[0]["loss"], (nsnap, seg)
 
 
 def test_gradient_same_for_both_stress_storages(problem):
