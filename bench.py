@@ -576,6 +576,8 @@ def main() -> int:
         key = dom["name"]
         if key == "k_fused":
             key += "@inplace" if tiling.get("inplace") else "@twobuf"
+        elif key == "k_pair":   # measured per tile shape (the tuner's choice varies between boxes)
+            key += f"@{tiling['tile_w']}x{tiling['tile_h']}"
         if args.fma:   # counters were taken on the exact build
             key += "@fma"
         traffic = tab.get(key)

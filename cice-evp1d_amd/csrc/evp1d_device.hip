@@ -677,6 +677,9 @@ struct AdjView {
   double2 *vc;            // [4][na] stress-adjoint velocity contributions: cc, ee, se, ne
   double *gstr, *gcw, *gfx, *gfy;   // [na] gradient accumulators
   const int *cnz;         // nonzero when str1..8 at some slot >= t_end (never written) has a bit set
+  const double2 *pc;      // [kPairConst][lp]: the constant fields in pairs (k_pack_const)
+  long lp;
+  const int4 *nbr;        // per stress slot: ee, ne, se (k_pack_nbr)
 };
 
 // *cnz (zero on entry) becomes 1 if str1..8 at any slot in [t_end, navel)
@@ -747,9 +750,12 @@ __global__ __launch_bounds__(256) void k_stepu_adj(DevView d, AdjView a) {
                        d.vvel_init[iw], d.tbu[iw], s15.x, s27.x, s36.x, s48.x, s15.y, s36.y, s27.y, s48.y, f);
     vn = make_double2(f.uvel, f.vvel);
   }
+  const long lp = a.lp;
+  const double2 f0 = a.pc[3 * lp + iw], f1 = a.pc[4 * lp + iw], f2 = a.pc[5 * lp + iw];
+  const double2 f4 = a.pc[7 * lp + iw], f5 = a.pc[8 * lp + iw];
   evp1d::StepuAdj o;
-  evp1d::stepu_adjoint(d.s, v.x, v.y, d.cw[iw], d.aiu[iw], d.uocn[iw], d.vocn[iw], d.waterx[iw], d.watery[iw],
-                       d.umassdti[iw], d.fm[iw], d.uarear[iw], d.tbu[iw], vn.x, vn.y, ub.x, ub.y, o);
+  evp1d::stepu_adjoint(d.s, v.x, v.y, f0.x, f0.y, f1.x, f1.y, f2.x, f2.y, f4.x, f4.y, f5.x, f5.y, vn.x, vn.y,
+                       ub.x, ub.y, o);
   a.ub[iw] = make_double2(o.uold_b, o.vold_b);
   a.g[iw] = make_double2(o.gx, o.gy);
   a.gfx[iw] += o.fx_b;
@@ -765,14 +771,17 @@ __global__ __launch_bounds__(256) void k_stress_adj(DevView d, AdjView a) {
   using L = Ld<SPLIT>;
   const int iw = blockIdx.x * 256 + threadIdx.x;
   if (iw >= d.t_end) return;
-  const int ie = d.ee[iw], in = d.ne[iw], is = d.se[iw];
+  const int4 nb = a.nbr[iw];
+  const int ie = nb.x, in = nb.y, is = nb.z;
   const double2 cc = L::vel(d, iw), ve = L::vel(d, ie), vn = L::vel(d, in), vs = L::vel(d, is);
   const double2 g0 = a.g[iw], ge = a.g[ie], gs = a.g[is], gn = a.g[in];
   const double str_b[8] = {g0.x, ge.x, gs.x, gn.x, g0.y, gs.y, ge.y, gn.y};
+  const long lp = a.lp;
+  const double2 m0 = a.pc[iw], m1 = a.pc[lp + iw], m2 = a.pc[2 * lp + iw];
   double vb[8];
-  const double gs_b = evp1d::stress_adjoint(d.s, cc.x, cc.y, ve.x, ve.y, vs.x, vs.y, vn.x, vn.y, d.dxT[iw],
-                                            d.dyT[iw], d.strength[iw], d.hte[iw], d.htn[iw], d.htem1[iw],
-                                            d.htnm1[iw], a.sigb + iw, str_b, a.sigb + iw, d.ldsig, vb);
+  const double gs_b = evp1d::stress_adjoint(d.s, cc.x, cc.y, ve.x, ve.y, vs.x, vs.y, vn.x, vn.y, m0.x, m0.y,
+                                            d.strength[iw], m1.x, m1.y, m2.x, m2.y, a.sigb + iw, str_b,
+                                            a.sigb + iw, d.ldsig, vb);
   const long na = d.na;
   a.vc[iw] = make_double2(vb[0], vb[1]);
   a.vc[na + iw] = make_double2(vb[2], vb[3]);
@@ -2797,6 +2806,23 @@ int evp1d_adjoint_run(evp1d_handle *h, int32_t nsub, int32_t nsnap, const double
                        h->v, cnz);
     HIP_TRY(hipGetLastError());
   }
+  // constant fields and stress neighbours in 16-byte groups for the
+  // reverse kernels (the same buffers the pair form packs per run)
+  if (!h->pconst) HIP_TRY(hipMalloc((void **)&h->pconst, (size_t)kPairConst * std::max(h->na, 1) * sizeof(double2)));
+  if (!h->nbr4) HIP_TRY(hipMalloc((void **)&h->nbr4, (size_t)std::max(h->na, 1) * sizeof(int4)));
+  if (h->na > 0) {
+    hipLaunchKernelGGL(k_pack_const, dim3(blocks_for(h->na)), dim3(kBlock), 0, h->stream, h->v, h->pconst,
+                       (long)h->na);
+    HIP_TRY(hipGetLastError());
+  }
+  if (h->v.t_end > 0) {
+    hipLaunchKernelGGL(k_pack_nbr, dim3(blocks_for(h->v.t_end)), dim3(kBlock), 0, h->stream, h->v, h->nbr4,
+                       (long)h->v.t_end);
+    HIP_TRY(hipGetLastError());
+  }
+  R.a.pc = h->pconst;
+  R.a.lp = h->na;
+  R.a.nbr = h->nbr4;
   RC_TRY(upload_caller_order(h, R.ut, u_target));
   RC_TRY(upload_caller_order(h, R.vt, v_target));
   if (nsnap <= 0) {
