@@ -66,6 +66,8 @@ module evp1d_capi
             evp1d_set_grid, evp1d_put_2d, evp1d_get_2d, evp1d_host_alloc, evp1d_host_free, &
             evp1d_set_fused_inplace, evp1d_get_fused_inplace, evp1d_autotune_fused,    &
             evp1d_set_fused_depth, evp1d_get_fused_depth,                                &
+            evp1d_adjoint_run, evp1d_loss, evp1d_tangent_run, evp1d_set_adjoint_segment, &
+            evp1d_get_adjoint_segment,                                                   &
             evp2d_create, evp2d_destroy, evp2d_set_params, evp2d_set_cells, evp2d_set_halo, &
             evp2d_set_field, evp2d_get_field, evp2d_run, evp2d_stress, evp2d_stepu,      &
             evp2d_halo, evp2d_run_individual, evp2d_synchronize, evp2d_timer_start,    &
@@ -411,6 +413,42 @@ module evp1d_capi
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: h
       integer(c_int32_t), intent(out) :: nt, nu
+    end function
+    ! gradients (include/evp1d.h): arrays in caller order, na each; every
+    ! output array must be present (the C side also accepts NULL)
+    integer(c_int) function evp1d_adjoint_run(h, nsub, nsnap, u_target, v_target, loss, g_strength, &
+                                              g_cdn_ocn, g_forcex, g_forcey) bind(C, name='evp1d_adjoint_run')
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nsub, nsnap
+      real(c_double), intent(in) :: u_target(*), v_target(*)
+      real(c_double), intent(out) :: loss
+      real(c_double), intent(out) :: g_strength(*), g_cdn_ocn(*), g_forcex(*), g_forcey(*)
+    end function
+    integer(c_int) function evp1d_loss(h, u_target, v_target, loss) bind(C, name='evp1d_loss')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: u_target(*), v_target(*)
+      real(c_double), intent(out) :: loss
+    end function
+    integer(c_int) function evp1d_tangent_run(h, nsub, d_strength, d_cdn_ocn, d_forcex, d_forcey, u_target, &
+                                              v_target, loss, dloss) bind(C, name='evp1d_tangent_run')
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nsub
+      real(c_double), intent(in) :: d_strength(*), d_cdn_ocn(*), d_forcex(*), d_forcey(*)
+      real(c_double), intent(in) :: u_target(*), v_target(*)
+      real(c_double), intent(out) :: loss, dloss
+    end function
+    integer(c_int) function evp1d_set_adjoint_segment(h, seg) bind(C, name='evp1d_set_adjoint_segment')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: seg
+    end function
+    integer(c_int) function evp1d_get_adjoint_segment(h, seg) bind(C, name='evp1d_get_adjoint_segment')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(out) :: seg
     end function
     integer(c_int) function evp1d_autotune_fused(h, nsub, us) bind(C, name='evp1d_autotune_fused')
       import :: c_int, c_int32_t, c_ptr, c_float

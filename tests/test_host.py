@@ -31,6 +31,37 @@ def test_library_exports_every_declared_symbol(fma):
     assert set(declared) == set(evp1d.EXPORTS)
 
 
+def _c_prototypes():
+    """name -> number of parameters of every function include/evp1d.h declares."""
+    text = (REPO / "include" / "evp1d.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = {}
+    for name, args in re.findall(r"\b(evp[12]d_\w+)\s*\(([^;{]*?)\)\s*;", text):
+        a = args.strip()
+        out[name] = 0 if a in ("", "void") else a.count(",") + 1
+    return out
+
+
+def test_fortran_binding_matches_the_c_abi():
+    """Every bind(C) interface of the Fortran C-ABI module names a symbol the
+    header declares and the library exports, with as many arguments as the
+    C prototype."""
+    src = (REPO / "cice-evp1d_amd" / "fortran" / "evp1d_capi.F90").read_text()
+    src = re.sub(r"&\s*\n\s*", " ", src)            # join continuation lines
+    binds = re.findall(r"(?:function|subroutine)\s+\w+\s*\(([^)]*)\)\s*bind\s*\(\s*C\s*,\s*name\s*=\s*'(\w+)'",
+                       src, flags=re.I)
+    assert len(binds) >= 40
+    protos = _c_prototypes()
+    L = ctypes.CDLL(str(evp1d.lib_path(False)))
+    for args, name in binds:
+        assert name in protos, name
+        assert hasattr(L, name), name
+        n = 0 if not args.strip() else args.count(",") + 1
+        assert n == protos[name], (name, n, protos[name])
+    for name in ("evp1d_adjoint_run", "evp1d_tangent_run", "evp1d_set_adjoint_segment", "evp1d_run"):
+        assert name in {b[1] for b in binds}, name
+
+
 def test_build_info_flavours():
     assert evp1d.build_info(False) == {"abi": 1, "fma": 0}
     assert evp1d.build_info(True) == {"abi": 1, "fma": 1}
