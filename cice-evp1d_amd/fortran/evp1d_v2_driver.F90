@@ -41,7 +41,9 @@ program evp1d_v2_driver
   integer(8) :: t0, t1, rate
   character(len=32) :: arg
   integer(c_int32_t) :: mode, inplace, depth
-  logical :: tune
+  logical :: tune, lgrad
+  real(dbl) :: loss
+  real(dbl), allocatable :: g(:,:)
   real(c_float) :: tune_us
   ! fields of input_double_1d_v2.bin after strength/uvel/vvel, in file order
   integer(c_int32_t), parameter :: mid(28) = [F_DXT, F_DYT, F_UAREAR, F_CDN_OCN, F_AIU, F_UOCN, &
@@ -73,8 +75,19 @@ program evp1d_v2_driver
     case ('fused-inplace'); mode = MODE_FUSED; inplace = 1
     case ('fused-pair'); mode = MODE_FUSED; depth = 2
     case ('tuned');      mode = MODE_FUSED; tune = .true.
-    case default;        error stop 'usage: evp1d_v2_driver [graph|persistent|launch|fused|fused-inplace|fused-pair|tuned]'
+    case default;        error stop 'usage: evp1d_v2_driver [graph|persistent|launch|fused|fused-inplace|fused-pair|tuned] [grad]'
     end select
+  endif
+  ! grad: the subcycles run as evp1d_adjoint_run (whole trajectory when it
+  ! fits), which leaves the same state as evp1d_run, and the gradient of
+  ! L = 1/2 sum_{skipU=0} (uvel - uvel_in)^2 + (vvel - vvel_in)^2 with respect
+  ! to strength, cdn_ocn, forcex and forcey goes to output_gradient_v2g_1d.bin
+  ! (L, then the four arrays, na each, in the input's point order)
+  lgrad = .false.
+  if (command_argument_count() >= 2) then
+    call get_command_argument(2, arg)
+    if (trim(arg) /= 'grad') error stop 'usage: evp1d_v2_driver [mode] [grad]'
+    lgrad = .true.
   endif
 
   allocate(f(na,28), uvel(navel), vvel(navel), ht(na,4), idx(na,6), skipU(na), skipT(na))
@@ -131,7 +144,13 @@ program evp1d_v2_driver
   call evp1d_check(evp1d_synchronize(h), 'synchronize')
 
   call system_clock(t0, rate)
-  call evp1d_check(evp1d_run(h, int(ndte,c_int32_t)), 'evp1d_run')
+  if (lgrad) then
+    allocate(g(max(na,1),4))
+    call evp1d_check(evp1d_adjoint_run(h, int(ndte,c_int32_t), 0_c_int32_t, uvel, vvel, loss, g(:,1), g(:,2), &
+                                       g(:,3), g(:,4)), 'evp1d_adjoint_run')
+  else
+    call evp1d_check(evp1d_run(h, int(ndte,c_int32_t)), 'evp1d_run')
+  endif
   call evp1d_check(evp1d_synchronize(h), 'synchronize')
   call system_clock(t1)
   secs = real(t1 - t0, dbl) / real(rate, dbl)
@@ -172,6 +191,15 @@ program evp1d_v2_driver
       if (ios /= 0) error stop 'Failed write output_stepu_v2g_1d.bin'
     enddo
     close(lun)
+  endif
+  if (lgrad) then
+    open(newunit=lun, file='output_gradient_v2g_1d.bin', form='unformatted', access='stream', &
+         action='write', status='replace', iostat=ios)
+    if (ios /= 0) error stop 'Failed open output_gradient_v2g_1d.bin'
+    write(lun, iostat=ios) loss, g(1:na,:)
+    if (ios /= 0) error stop 'Failed write output_gradient_v2g_1d.bin'
+    close(lun)
+    write(*,'(a,es24.16)') ' gradient: loss = ', loss
   endif
   write(*,'(a,i0,1x,i0,1x,i0)') ' Kernel dim (v2g - MI355X HIP): ', na, navel, ndte
 

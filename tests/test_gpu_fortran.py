@@ -48,6 +48,37 @@ def test_v2_driver_reproduces_reference_files(gpu, case, mode):
     assert O.md5(got_stepu) == O.md5(want_stepu)
 
 
+@pytest.mark.parametrize("mode", ["graph", "fused-pair"])
+def test_v2_driver_gradient_through_fortran_binding(gpu, mode):
+    """`evp1d_v2_driver <mode> grad` calls evp1d_adjoint_run through the
+    Fortran binding: the output files are still the reference's (the adjoint
+    run leaves the forward state), and the gradient file equals the
+    library's gradient taken through ctypes on the same inputs, target = the
+    input velocities (per-point gradients bit for bit; the loss, a sum in
+    slot order, to 1e-13)."""
+    from gpu_common import make_engine
+    case = "small_64x48"
+    st, ndte, want_stress, want_stepu = load_golden(case)
+    with tempfile.TemporaryDirectory() as d:
+        O.write_inputs(d, st, ndte)
+        r = _run(FBIN / "evp1d_v2_driver", d, mode, "grad")
+        assert "gradient: loss" in r.stdout
+        got_stress = (Path(d) / "output_stress_v2g_1d.bin").read_bytes()
+        got_stepu = (Path(d) / "output_stepu_v2g_1d.bin").read_bytes()
+        raw = np.fromfile(Path(d) / "output_gradient_v2g_1d.bin", dtype=np.float64)
+    assert O.md5(got_stress) == O.md5(want_stress)
+    assert O.md5(got_stepu) == O.md5(want_stepu)
+    na = st.na
+    assert raw.size == 1 + 4 * na
+    loss, gf = raw[0], raw[1:].reshape(4, na)
+    with make_engine(st, ndte, mode) as eng:
+        want = eng.adjoint_run(ndte, st.f["uvel"][:na], st.f["vvel"][:na])
+    for k, key in enumerate(("strength", "cdn_ocn", "forcex", "forcey")):
+        assert np.array_equal(gf[k], want[key]), key
+    assert np.any(gf[0] != 0.0)
+    assert abs(loss - want["loss"]) <= 1e-13 * abs(want["loss"])
+
+
 def test_v2_driver_evp_c0_md5(gpu):
     from evp1d import synth
     import evp1d
