@@ -166,14 +166,17 @@ def api_measure(workload: str, nrun: int) -> dict:
                 fh.write(g.fields[n].tobytes(order="F"))
             for n in layout.DYN_2D:
                 fh.write(g.fields[n].tobytes(order="F"))
-        for flag, key in ((".true.", "device_pack_ms"), (".false.", "host_pack_ms")):
-            (Path(d) / "evp_1d.nml").write_text(f"&evp1d_nml\n  ldevpack = {flag}\n/\n")
+        # lautotune: the first call tunes the fused form (evp1d_autotune_fused),
+        # later calls run the tuned form
+        for flag, tune, key in ((".true.", ".true.", "device_pack_tuned_ms"), (".true.", ".false.", "device_pack_ms"),
+                                (".false.", ".false.", "host_pack_ms")):
+            (Path(d) / "evp_1d.nml").write_text(f"&evp1d_nml\n  ldevpack = {flag}\n  lautotune = {tune}\n/\n")
             r = subprocess.run([str(exe)], cwd=d, capture_output=True, text=True, timeout=600)
             if r.returncode != 0:
                 raise RuntimeError(r.stdout[-500:] + r.stderr[-500:])
             ms = [float(x) for x in re.findall(r"dyn_evp1d_run\s+\d+:\s+([0-9.]+) ms", r.stdout)]
             out[key] = min(ms[1:]) if len(ms) > 1 else ms[0]   # the first call also builds the graph
-            log(f"[bench] api ldevpack={flag}: per call {ms} ms")
+            log(f"[bench] api ldevpack={flag} lautotune={tune}: per call {ms} ms")
     return out
 
 
