@@ -223,6 +223,7 @@ def adjoint_measure(s: dict, ndte: int, t_fwd_step: float, nsnap: int, ndir: int
         eng.set_tiling(fused_cfg["tile_w"], fused_cfg["tile_h"], fused_cfg["block"], fused_cfg["xcd"])
         eng.set_fused_inplace(fused_cfg["inplace"])
         eng.set_fused_depth(fused_cfg.get("depth", 1))
+        eng.set_tile_order(fused_cfg.get("order", 0))
     reset_state(eng, s, {"strength": 1.2 * s["fields"]["strength"]})
     eng.run(ndte)
     ut = eng.get_field("uvel")[:s["na"]]
@@ -481,6 +482,8 @@ def main() -> int:
                          "candidate (tile ladder x stress storage); 0: keep the defaults or --tiling/--inplace")
     ap.add_argument("--inplace", type=int, default=None, choices=(0, 1),
                     help="fused stress storage: 1 in place, 0 two buffers (implies --tune-fused 0)")
+    ap.add_argument("--order", type=int, default=None, choices=(0, 1),
+                    help="tile order of the fused forms: 0 rows of tiles, 1 Hilbert curve (evp1d_set_tile_order)")
     ap.add_argument("--depth", type=int, default=None, choices=(1, 2),
                     help="subcycles per fused launch: 2 the pair form (implies --tune-fused 0)")
     ap.add_argument("--tuning", default=None, metavar="SB,SX,UB,UX",
@@ -523,18 +526,21 @@ def main() -> int:
         eng.set_fused_inplace(args.inplace)
     if args.depth is not None:
         eng.set_fused_depth(args.depth)
+    if args.order is not None:
+        eng.set_tile_order(args.order)
     if args.autotune:
         log(f"[bench] autotune: {eng.autotune(args.autotune)}")
     tiling = eng.get_tiling()
     # the form evp1d_run takes: auto is the fused form whenever the lists tile (and ndte >= 4)
     fused = args.mode == "fused" or (args.mode == "auto" and tiling["ntiles"] > 0 and ndte >= 4)
     tuned_fused = None
-    if fused and args.tune_fused > 0 and not args.tiling and args.inplace is None and args.depth is None:
+    if (fused and args.tune_fused > 0 and not args.tiling and args.inplace is None and args.depth is None
+            and args.order is None):
         t0 = time.perf_counter()
         tuned_fused = eng.autotune_fused(args.tune_fused)
         log(f"[bench] autotune_fused ({time.perf_counter() - t0:.1f}s, setup): {tuned_fused}")
     tuning = eng.get_tuning()
-    tiling = dict(eng.get_tiling(), inplace=int(eng.get_fused_inplace()))
+    tiling = dict(eng.get_tiling(), inplace=int(eng.get_fused_inplace()), order=eng.get_tile_order())
     dinfo = eng.get_fused_depth()
     tiling["depth"] = 2 if dinfo["active"] else 1
     if dinfo["active"]:
@@ -579,8 +585,8 @@ def main() -> int:
         key = dom["name"]
         if key == "k_fused":
             key += "@inplace" if tiling.get("inplace") else "@twobuf"
-        elif key == "k_pair":   # measured per tile shape (the tuner's choice varies between boxes)
-            key += f"@{tiling['tile_w']}x{tiling['tile_h']}"
+        elif key == "k_pair":   # measured per tile shape and order (the tuner's choice varies between boxes)
+            key += f"@{tiling['tile_w']}x{tiling['tile_h']}" + ("@hilbert" if tiling.get("order") == 1 else "")
         if args.fma:   # counters were taken on the exact build
             key += "@fma"
         traffic = tab.get(key)
@@ -617,8 +623,9 @@ def main() -> int:
                    "ny": w["ny"], "seed": w["seed"], "na": na, "navel": s["navel"], "nT": nT, "nU": nU,
                    "ndte": ndte, "mode": args.mode, "form": "fused" if fused else "unfused", "tiling": tiling,
                    "tiling_chosen_by": ("evp1d_autotune_fused (setup, untimed)" if tuned_fused else
-                                        "--tiling/--inplace/--depth" if (args.tiling or args.inplace is not None
-                                                                         or args.depth is not None)
+                                        "--tiling/--inplace/--depth/--order" if (args.tiling or args.inplace is not None
+                                                                         or args.depth is not None
+                                                                         or args.order is not None)
                                         else "library default"),
                    "build": "fma" if args.fma else "exact", "tuning": tuning,
                    "us_per_subcycle": round(step_us, 3), "loop_GBps_algorithmic": round(whole, 1),

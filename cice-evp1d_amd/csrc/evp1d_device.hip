@@ -1023,6 +1023,7 @@ struct evp1d_handle {
   // pair form (two subcycles per fused launch): its tables, planned on the
   // current tiling when fused_depth is 2
   int fused_depth = 1;
+  int tile_order = 0;              // 0 rows of tiles, 1 Hilbert curve over the tiles
   // subcycles per checkpointing unit of a gradient with fewer stored states
   // than subcycles (0: chosen per call)
   int adj_segment = 0;
@@ -1325,7 +1326,7 @@ static int relayout(evp1d_handle *h) {
     struct Rung { int w, h, block, min_tiles; };
     static const Rung ladder[3] = {{64, 16, 512, 1024}, {32, 8, 256, 512}, {16, 8, 128, 0}};
     for (const Rung &r : ladder) {
-      h->tile_cache = evp1d::tile_ids(h->coords, r.w, r.h);
+      h->tile_cache = evp1d::tile_ids(h->coords, r.w, r.h, h->tile_order);
       h->tile_w = r.w;
       h->tile_h = r.h;
       h->tile_block = r.block;
@@ -1333,8 +1334,9 @@ static int relayout(evp1d_handle *h) {
     }
     h->tile_cache_ok = true;
   }
-  if (!h->tile_cache_ok || h->tile_cache.tile_w != h->tile_w || h->tile_cache.tile_h != h->tile_h) {
-    h->tile_cache = evp1d::tile_ids(h->coords, h->tile_w, h->tile_h);
+  if (!h->tile_cache_ok || h->tile_cache.tile_w != h->tile_w || h->tile_cache.tile_h != h->tile_h ||
+      h->tile_cache.order != h->tile_order) {
+    h->tile_cache = evp1d::tile_ids(h->coords, h->tile_w, h->tile_h, h->tile_order);
     h->tile_cache_ok = true;
   }
   const evp1d::TilePlan plan = evp1d::plan_tiles(na, h->navel, all ? idx : nullptr, h->skip_host.data(),
@@ -2172,6 +2174,24 @@ int evp1d_set_tiling(evp1d_handle *h, int32_t tile_w, int32_t tile_h, int32_t bl
   return EVP1D_OK;
 }
 
+int evp1d_set_tile_order(evp1d_handle *h, int32_t order) {
+  if (!h) return fail(EVP1D_EINVAL, "evp1d_set_tile_order: NULL handle");
+  if (order != 0 && order != 1) return fail(EVP1D_EINVAL, "evp1d_set_tile_order: order must be 0 or 1");
+  if (order != h->tile_order) {
+    h->tile_order = order;
+    h->layout_dirty = true;
+    h->tile_cache_ok = false;
+    h->graph_dirty = true;
+  }
+  return EVP1D_OK;
+}
+
+int evp1d_get_tile_order(evp1d_handle *h, int32_t *order) {
+  if (!h) return fail(EVP1D_EINVAL, "evp1d_get_tile_order: NULL handle");
+  if (order) *order = h->tile_order;
+  return EVP1D_OK;
+}
+
 int evp1d_get_tiling(evp1d_handle *h, int32_t *tile_w, int32_t *tile_h, int32_t *block, int32_t *xcd,
                      int32_t *ntiles) {
   if (!h) return fail(EVP1D_EINVAL, "evp1d_get_tiling: NULL handle");
@@ -2327,18 +2347,22 @@ int evp1d_autotune_fused(evp1d_handle *h, int32_t nsub, float *us) {
   RC_TRY(ensure(h));
   if (!h->fused_ok)
     return fail(EVP1D_ESTATE, "evp1d_autotune_fused: fused form unavailable (%s)", h->fused_why.c_str());
-  // depth 1 with both stress storages, then the pair form (two buffers)
+  // depth 1 with both stress storages, then the pair form (two buffers) in
+  // both tile orders (rows, Hilbert curve)
   struct Cand { int w, h, block, depth; };
   static const Cand cands[8] = {{64, 16, 512, 1}, {32, 16, 512, 1}, {32, 8, 256, 1}, {16, 8, 128, 1},
                                 {31, 15, 512, 2}, {62, 15, 512, 2}, {15, 15, 256, 2}, {62, 7, 512, 2}};
   const bool keep_auto = h->tiling_auto;
   const int keep_w = h->tile_w, keep_h = h->tile_h, keep_b = h->tile_block, keep_x = h->tile_xcd,
-            keep_ip = h->fused_inplace, keep_d = h->fused_depth;
+            keep_ip = h->fused_inplace, keep_d = h->fused_depth, keep_o = h->tile_order;
   float best = 1e30f;
-  int win = -1, win_ip = 0;
+  int win = -1, win_ip = 0, win_o = 0;
   int rc = EVP1D_OK;
-  for (int c = 0; c < 8 && !rc; ++c) {
+  for (int cc = 0; cc < 12 && !rc; ++cc) {
+    const int c = cc < 8 ? cc : 4 + (cc - 8);             // 8..11: the pair tiles again, Hilbert order
+    const int order = cc < 8 ? 0 : 1;
     rc = evp1d_set_fused_depth(h, cands[c].depth);
+    if (!rc) rc = evp1d_set_tile_order(h, order);
     if (!rc) rc = evp1d_set_tiling(h, cands[c].w, cands[c].h, cands[c].block, 1);
     if (!rc) rc = ensure(h);
     if (rc || !h->fused_ok) continue;                       // tile too large for LDS on these lists
@@ -2351,7 +2375,7 @@ int evp1d_autotune_fused(evp1d_handle *h, int32_t nsub, float *us) {
       float t = 0.f;
       if (!rc) rc = time_launches(h, 1, [&] { return issue_fused(h, nsub); }, &t);
       t /= (float)nsub;
-      if (!rc && t < best) { best = t; win = c; win_ip = ip; }
+      if (!rc && t < best) { best = t; win = c; win_ip = ip; win_o = order; }
     }
     rc = st.restore(h, rc, "evp1d_autotune_fused");
   }
@@ -2359,6 +2383,7 @@ int evp1d_autotune_fused(evp1d_handle *h, int32_t nsub, float *us) {
   if (rc || win < 0) {
     h->fused_inplace = keep_ip;
     (void)evp1d_set_fused_depth(h, keep_d);
+    (void)evp1d_set_tile_order(h, keep_o);
     if (keep_auto) {
       h->tiling_auto = true;
       h->layout_dirty = true;
@@ -2370,6 +2395,7 @@ int evp1d_autotune_fused(evp1d_handle *h, int32_t nsub, float *us) {
   }
   h->fused_inplace = win_ip;
   RC_TRY(evp1d_set_fused_depth(h, cands[win].depth));
+  RC_TRY(evp1d_set_tile_order(h, win_o));
   RC_TRY(evp1d_set_tiling(h, cands[win].w, cands[win].h, cands[win].block, 1));
   RC_TRY(ensure(h));
   RC_TRY(ensure_fused_buffers(h));

@@ -124,10 +124,31 @@ GridCoords grid_coords(int na, int navel, const int32_t *const idx[6]) {
   return c;
 }
 
-TileIds tile_ids(const GridCoords &c, int tile_w, int tile_h) {
+// Position of (x, y) along the Hilbert curve that fills the n x n square (n a
+// power of two): the classic bit-pair recursion (Hilbert 1891; the iterative
+// form of Butz 1971), which visits each quadrant in the order the curve enters
+// it and turns the coordinates so the sub-curve starts where the last ended.
+static int64_t hilbert_index(int64_t n, int64_t x, int64_t y) {
+  int64_t d = 0;
+  for (int64_t s = n / 2; s > 0; s /= 2) {
+    const int64_t rx = (x & s) > 0 ? 1 : 0, ry = (y & s) > 0 ? 1 : 0;
+    d += s * s * ((3 * rx) ^ ry);
+    if (ry == 0) {
+      if (rx == 1) {
+        x = n - 1 - x;
+        y = n - 1 - y;
+      }
+      std::swap(x, y);
+    }
+  }
+  return d;
+}
+
+TileIds tile_ids(const GridCoords &c, int tile_w, int tile_h, int order) {
   TileIds t;
   t.tile_w = tile_w;
   t.tile_h = tile_h;
+  t.order = order;
   if (!c.ok) {
     t.why = c.why;
     return t;
@@ -136,13 +157,24 @@ TileIds tile_ids(const GridCoords &c, int tile_w, int tile_h) {
     t.why = "tile size < 1";
     return t;
   }
+  if (order != 0 && order != 1) {
+    t.why = "tile order must be 0 or 1";
+    return t;
+  }
   const int na = (int)c.gx.size();
-  // dense tile ids in (row of tiles, column of tiles) order
+  // dense tile ids in (row of tiles, column of tiles) order, or along a
+  // Hilbert curve over the square of side hn that holds the tile grid
   const int64_t ntx = (c.width + tile_w - 1) / tile_w, nty = (c.height + tile_h - 1) / tile_h;
-  auto key_of = [&](int q) { return (int64_t)(c.gy[(size_t)q] / tile_h) * ntx + c.gx[(size_t)q] / tile_w; };
+  int64_t hn = 1;
+  while (hn < ntx || hn < nty) hn *= 2;
+  const int64_t nkey = order == 1 ? hn * hn : ntx * nty;
+  auto key_of = [&](int q) {
+    const int64_t tx = c.gx[(size_t)q] / tile_w, ty = c.gy[(size_t)q] / tile_h;
+    return order == 1 ? hilbert_index(hn, tx, ty) : ty * ntx + tx;
+  };
   t.tid.resize((size_t)na);
-  if (ntx * nty <= 4 * (int64_t)na + 1024) {   // a table over all tile positions
-    std::vector<int32_t> dense((size_t)(ntx * nty), -1);
+  if (nkey <= 4 * (int64_t)na + 1024 && nkey <= 4 * ntx * nty + 1024) {   // a table over all tile positions
+    std::vector<int32_t> dense((size_t)nkey, -1);
     for (int q = 0; q < na; ++q) dense[(size_t)key_of(q)] = 0;
     int32_t n = 0;
     for (int32_t &d : dense)

@@ -71,11 +71,12 @@ GridCoords grid_coords(int na, int navel, const int32_t *const idx[6]);
 // set) when the lists are not those of a grid.
 struct TileIds {
   int tile_w = 0, tile_h = 0;
+  int order = 0;              // tile launch order: 0 rows of tiles, 1 Hilbert curve over the tiles
   int ntiles = 0;
   std::vector<int32_t> tid;   // caller position -> tile, in launch order
   std::string why;
 };
-TileIds tile_ids(const GridCoords &c, int tile_w, int tile_h);
+TileIds tile_ids(const GridCoords &c, int tile_w, int tile_h, int order = 0);
 TileIds tile_ids(int na, int navel, const int32_t *const idx[6], int tile_w, int tile_h);
 
 // idx: ee, ne, se, nw, sw, sse as 0-based caller positions (values in

@@ -58,7 +58,7 @@ EXPORTS = (
     "evp1d_get_adjoint_segment",
     "evp1d_set_tiling", "evp1d_get_tiling", "evp1d_tile_plan",
     "evp1d_set_fused_inplace", "evp1d_get_fused_inplace", "evp1d_autotune_fused",
-    "evp1d_set_fused_depth", "evp1d_get_fused_depth",
+    "evp1d_set_fused_depth", "evp1d_get_fused_depth", "evp1d_set_tile_order", "evp1d_get_tile_order",
     "evp1d_set_grid", "evp1d_put_2d", "evp1d_get_2d", "evp1d_host_alloc", "evp1d_host_free",
     "evp2d_create", "evp2d_destroy", "evp2d_set_params", "evp2d_set_cells", "evp2d_set_halo",
     "evp2d_set_field", "evp2d_get_field", "evp2d_run", "evp2d_stress", "evp2d_stepu", "evp2d_halo",
@@ -146,6 +146,8 @@ def load(fma: bool = False, path: str | os.PathLike | None = None) -> ctypes.CDL
         "evp1d_adjoint_run": [_H, i32, i32, _DP, _DP, _DP, _DP, _DP, _DP, _DP],
         "evp1d_loss": [_H, _DP, _DP, _DP],
         "evp1d_set_adjoint_segment": [_H, i32],
+        "evp1d_set_tile_order": [_H, i32],
+        "evp1d_get_tile_order": [_H, _IP],
         "evp1d_get_adjoint_segment": [_H, _IP],
         "evp1d_tangent_run": [_H, i32, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP],
         "evp1d_set_tiling": [_H, i32, i32, i32, i32],
@@ -418,6 +420,16 @@ class Engine:
         """Subcycles per fused launch: 1, or 2 (the pair form; evp1d_set_fused_depth)."""
         _check(self.L, self.L.evp1d_set_fused_depth(self.h, int(depth)))
 
+    def set_tile_order(self, order: int) -> None:
+        """Tile visiting order of the fused forms: 0 rows of tiles, 1 Hilbert
+        curve over the tiles (evp1d_set_tile_order)."""
+        _check(self.L, self.L.evp1d_set_tile_order(self.h, int(order)))
+
+    def get_tile_order(self) -> int:
+        o = ctypes.c_int32()
+        _check(self.L, self.L.evp1d_get_tile_order(self.h, ctypes.byref(o)))
+        return o.value
+
     def set_adjoint_segment(self, seg: int) -> None:
         """Subcycles per checkpointing unit of adjoint_run with fewer stored
         states than subcycles (evp1d_set_adjoint_segment; 0 = automatic)."""
@@ -440,7 +452,7 @@ class Engine:
         us = ctypes.c_float()
         _check(self.L, self.L.evp1d_autotune_fused(self.h, nsub, ctypes.byref(us)))
         return dict(self.get_tiling(), inplace=self.get_fused_inplace(), depth=self.get_fused_depth()["depth"],
-                    us=us.value)
+                    order=self.get_tile_order(), us=us.value)
 
     # lifetime ------------------------------------------------------------
     def close(self) -> None:

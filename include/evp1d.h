@@ -249,15 +249,22 @@ int evp1d_get_fused_inplace(evp1d_handle *h, int32_t *inplace);
  * (evp1d_set_halo), fused runs use depth 1.  evp1d_get_fused_depth reports
  * the setting and, in *active, whether runs use the pair form (0/1) and, in
  * *block, its workgroup size.  Results are identical.                       */
+/* Order in which the fused forms visit their tiles, and so the slot order of
+ * the points (no counterpart in the reference): 0 (default) rows of tiles,
+ * 1 along a Hilbert curve over the tile grid (a space-filling-curve
+ * ordering at tile granularity; with the XCD-aware order each XCD then takes
+ * a compact block of tiles instead of a band).  Results are identical.     */
+int evp1d_set_tile_order(evp1d_handle *h, int32_t order);
+int evp1d_get_tile_order(evp1d_handle *h, int32_t *order);
 int evp1d_set_fused_depth(evp1d_handle *h, int32_t depth);
 int evp1d_get_fused_depth(evp1d_handle *h, int32_t *depth, int32_t *active, int32_t *block);
 /* Times one fused run of nsub subcycles (after an untimed one) for each tile
  * of 64x16/512, 32x16/512, 32x8/256, 16x8/128 (XCD-aware order) that fits
  * the lists, each with both stress storages, and the pair form on tiles of
- * 31x15, 62x15, 15x15 and 62x7, on the handle's own data (the state is saved
- * and restored), keeps the fastest (as evp1d_set_tiling,
- * evp1d_set_fused_inplace and evp1d_set_fused_depth would) and reports its
- * time per subcycle in microseconds.                                        */
+ * 31x15, 62x15, 15x15 and 62x7 in both tile orders, on the handle's own data
+ * (the state is saved and restored), keeps the fastest (as evp1d_set_tiling,
+ * evp1d_set_fused_inplace, evp1d_set_fused_depth and evp1d_set_tile_order
+ * would) and reports its time per subcycle in microseconds.                 */
 int evp1d_autotune_fused(evp1d_handle *h, int32_t nsub, float *us);
 
 /* Host-only tiling of a 1D list (no device use; for inspection and tests).

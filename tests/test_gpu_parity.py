@@ -275,6 +275,32 @@ def test_fused_tilings_match_oracle(gpu, tile, nsub, inplace):
         assert_state_bitexact(fetch_state(eng, st), want)
 
 
+@pytest.mark.parametrize("mode", ["fused", "fused-inplace", "fused-pair", "graph"])
+@pytest.mark.parametrize("nsub", [1, 2, 7])
+def test_hilbert_tile_order_matches_oracle(gpu, mode, nsub):
+    """Tiles visited along a Hilbert curve (evp1d_set_tile_order(h, 1)): a
+    different slot order, the same results bit for bit; switching back and
+    forth between runs keeps the state."""
+    st = synthetic_state(90, 70, seed=63)
+    lib, p = oracle_params(st, nsub)
+    want = st.copy()
+    lib.run(p, want, nsub)
+    _log(f"hilbert mode={mode} nsub={nsub}")
+    with make_engine(st, nsub, mode) as eng:
+        eng.set_tile_order(1)
+        assert eng.get_tile_order() == 1
+        eng.run(nsub)
+        assert_state_bitexact(fetch_state(eng, st), want)
+        eng.set_tile_order(0)
+        eng.run(nsub)
+        lib.run(p, want, nsub)
+        assert_state_bitexact(fetch_state(eng, st), want)
+        eng.set_tile_order(1)
+        eng.run(nsub)
+        lib.run(p, want, nsub)
+        assert_state_bitexact(fetch_state(eng, st), want)
+
+
 @pytest.mark.parametrize("mode", ["fused", "fused-inplace", "fused-pair"])
 def test_fused_sees_fields_set_between_runs(gpu, mode):
     """str1..8 at points stress never writes (read by stepu from memory),

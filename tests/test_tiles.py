@@ -249,3 +249,34 @@ def test_pair_plan_schedule_and_check(tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     assert "NOT CAUGHT" not in r.stdout and "good: ''" in r.stdout
     assert r.stdout.count(": 0 differing outputs") == 5
+
+
+def test_hilbert_tile_order(tmp_path):
+    """Build tests/cpp/tile_order_check.cpp against the planner source: the
+    Hilbert tile order (tile_ids order 1) is a continuous curve of
+    4-neighbours on power-of-two grids, keeps tile membership, and on a real
+    grid its tiling and pair tables pass the host checks."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    repo = Path(__file__).resolve().parent.parent
+    csrc = repo / "cice-evp1d_amd" / "csrc"
+    exe = tmp_path / "order"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-pthread", f"-I{csrc}", str(repo / "tests" / "cpp" / "tile_order_check.cpp"),
+                    str(csrc / "evp1d_tiles.cpp"), "-o", str(exe)], check=True, capture_output=True)
+    g = synth.make_grid(300, 200, 73)
+    ix = evp1d.build_index(g.tmask, g.nghost)
+    from evp1d import layout
+    s = layout.pack(g, ix)
+    na, navel = s["na"], s["navel"]
+    skip = (s["skipT"] != 0).astype(np.uint8) | ((s["skipU"] != 0).astype(np.uint8) << 1)
+    with open(tmp_path / "in.bin", "wb") as fh:
+        fh.write(np.array([na, navel], np.int32).tobytes())
+        for n in evp1d.NBR:
+            fh.write((s["idx"][n].astype(np.int32) - 1).tobytes())
+        fh.write(skip.tobytes())
+    r = subprocess.run([str(exe), str(tmp_path / "in.bin")], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "all good" in r.stdout and "order 1:" in r.stdout
