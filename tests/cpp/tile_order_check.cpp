@@ -3,11 +3,13 @@
 //     are a permutation of 0..n*n-1 and consecutive ids are 4-neighbours,
 //     the defining property of the Hilbert curve (a wrong turn breaks it),
 //     and the curve starts at (0, 0) and ends at (n - 1, 0);
-//  2. on a non-square grid the ids are dense and every tile's ids are those
-//     of the row-major order permuted (same tile membership);
+//  2. on a non-square grid, and on a sparse frame of two far-apart
+//     components (the sorted-key path), the ids are dense and every tile's
+//     ids are those of the row-major order permuted (same tile membership);
 //  3. on a real grid (input file as for pair_plan_check.cpp) the Hilbert
 //     ordered plan and its pair tables pass check_plan / check_pairs.
 #include "evp1d_tiles.hpp"
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -69,6 +71,33 @@ int main(int argc, char **argv) {
     for (int m : map) if (m >= 0 && m < t.ntiles) seen[(size_t)m]++;
     for (int k = 0; k < t.ntiles; ++k) CHECK(seen[(size_t)k] == 1, "Hilbert id %d not used once", k);
     CHECK(evp1d::tile_ids(c, tw, th, 2).ntiles == 0, "order 2 accepted");
+  }
+  {
+    // a sparse frame (two small components far apart, as a batch lays out
+    // its members): the sorted-key path, same tile membership in both orders
+    evp1d::GridCoords c;
+    c.ok = true;
+    c.width = 200000;
+    c.height = 40;
+    for (int y = 0; y < 40; ++y)
+      for (int x = 0; x < 30; ++x) {
+        c.gx.push_back(x);
+        c.gy.push_back(y);
+        c.gx.push_back(199970 + x);
+        c.gy.push_back(y);
+      }
+    const evp1d::TileIds r = evp1d::tile_ids(c, 8, 8, 0), t = evp1d::tile_ids(c, 8, 8, 1);
+    CHECK(r.ntiles == t.ntiles && t.ntiles == 2 * 4 * 5, "sparse ntiles %d %d", r.ntiles, t.ntiles);
+    std::vector<int> map((size_t)std::max(r.ntiles, 1), -1), seen((size_t)std::max(t.ntiles, 1), 0);
+    for (size_t q = 0; q < c.gx.size(); ++q) {
+      int &m = map[(size_t)r.tid[q]];
+      if (m < 0) {
+        m = t.tid[q];
+        if (m >= 0 && m < t.ntiles) seen[(size_t)m]++;
+      }
+      CHECK(m == t.tid[q], "sparse cell %zu: tile membership differs between orders", q);
+    }
+    for (int k = 0; k < t.ntiles; ++k) CHECK(seen[(size_t)k] == 1, "sparse Hilbert id %d not used once", k);
   }
   if (argc >= 2) {
     FILE *f = std::fopen(argv[1], "rb");

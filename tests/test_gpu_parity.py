@@ -234,6 +234,31 @@ def test_batch_members_match_independent_oracle_runs(gpu):
                 assert_state_bitexact(got, wants[k])
 
 
+@pytest.mark.parametrize("mode", ["fused", "fused-pair"])
+def test_batch_hilbert_order_matches_independent_oracle_runs(gpu, mode):
+    """A batch in the Hilbert tile order (members are separate components of
+    one frame): every member bit-identical to its own oracle run."""
+    sizes = [(30, 22, 91), (64, 40, 92), (12, 9, 93), (50, 70, 94)]
+    states = [synthetic_state(nx, ny, seed, force_perturb=0.02) for nx, ny, seed in sizes]
+    ndte = 9
+    lib, p = oracle_params(states[0], ndte)
+    params = evp1d.set_evp_parameters(ndte)
+    _log(f"batch hilbert mode={mode}")
+    with evp1d.Engine.batch([(st.na, st.navel) for st in states], params) as eng:
+        set_mode(eng, mode)
+        eng.set_tile_order(1)
+        for k, st in enumerate(states):
+            eng.load_member(k, st.f, st.idx, st.skipT, st.skipU)
+        eng.run(ndte)
+        for k, st in enumerate(states):
+            want = st.copy()
+            lib.run(p, want, ndte)
+            got = st.copy()
+            for n in got.f:
+                got.f[n][:] = eng.get_field_member(k, n)
+            assert_state_bitexact(got, want)
+
+
 def test_autotune_keeps_results_and_state(gpu):
     st = synthetic_state(120, 90, seed=91)
     lib, p = oracle_params(st, 12)
