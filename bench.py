@@ -367,17 +367,32 @@ def ensemble_measure(nmem: int, steps: int, autotune: int, tune_fused: int = 0) 
     for e in singles:
         e.synchronize()
     t_lock = time.perf_counter() - t0
+    # member after member in the unfused form (the reference's two kernels,
+    # stress then stepu, as one graph per member)
+    for e in singles:
+        e.set_mode("graph")
+        e.run(ndte)
+    for e in singles:
+        e.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for e in singles:
+            e.run(ndte)
+            e.synchronize()
+    t_seq_graph = (time.perf_counter() - t0) / steps
     for e in singles:
         e.close()
     out = {"members": nmem, "total_points": tot, "ndte": ndte,
            "batched_s": round(t_batch, 4), "sequential_s": round(t_seq, 4), "lockstep_s": round(t_lock, 4),
+           "sequential_unfused_s": round(t_seq_graph, 4), "speedup_vs_sequential_unfused": round(t_seq_graph / t_batch, 3),
            "batched_pts_sub_per_s": tot * ndte / t_batch, "sequential_pts_sub_per_s": tot * ndte / t_seq,
            "lockstep_pts_sub_per_s": tot * ndte / t_lock,
            "speedup": round(t_seq / t_batch, 3), "speedup_vs_lockstep": round(t_lock / t_batch, 3),
            "tuning": tuning, "fused_batch": fused_batch,
            "members_on_pair_form": depths.count(2) if depths else None}
     log(f"[bench] ensemble: batched {t_batch:.3f}s, member after member {t_seq:.3f}s (x{t_seq / t_batch:.2f}), "
-        f"lockstep {t_lock:.3f}s (x{t_lock / t_batch:.2f})")
+        f"lockstep {t_lock:.3f}s (x{t_lock / t_batch:.2f}), member after member unfused {t_seq_graph:.3f}s "
+        f"(x{t_seq_graph / t_batch:.2f})")
     return out
 
 
