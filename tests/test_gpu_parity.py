@@ -326,6 +326,21 @@ def test_hilbert_tile_order_matches_oracle(gpu, mode, nsub):
         assert_state_bitexact(fetch_state(eng, st), want)
 
 
+def test_setting_errors_are_reported(gpu):
+    """Out-of-range settings fail with EVP1D_EINVAL and change nothing."""
+    st = synthetic_state(20, 16, seed=64)
+    with make_engine(st, 3) as eng:
+        for bad in (-1, 2):
+            with pytest.raises(evp1d.Evp1dError):
+                eng.set_tile_order(bad)
+        assert eng.get_tile_order() == 0
+        with pytest.raises(evp1d.Evp1dError):
+            eng.set_adjoint_segment(-1)
+        assert eng.get_adjoint_segment() == 0
+        eng.set_adjoint_segment(7)
+        assert eng.get_adjoint_segment() == 7
+
+
 @pytest.mark.parametrize("mode", ["fused", "fused-inplace", "fused-pair"])
 def test_fused_sees_fields_set_between_runs(gpu, mode):
     """str1..8 at points stress never writes (read by stepu from memory),
