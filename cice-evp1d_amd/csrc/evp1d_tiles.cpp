@@ -372,10 +372,10 @@ TilePlan plan_tiles(int na, int navel, const int32_t *const idx[6], const uint8_
 
   const int t_end = p.cnt[0] + p.cnt[1], u_begin = p.cnt[0];
   const int nu = p.cnt[1] + p.cnt[2];
-  p.uloc.assign((size_t)4 * nu, 0);
+  p.uloc.resize((size_t)4 * nu);   // every entry is written below when the plan succeeds
   std::vector<TileDesc> desc((size_t)nt);
   std::vector<TileWork> work((size_t)nt);
-  std::vector<int32_t> lidx((size_t)t_end, 0);   // owned slot -> entry in its tile's table
+  Table<int32_t> lidx((size_t)t_end);   // owned slot -> entry in its tile's table (written before read)
   std::vector<SlotMap> maps;
   maps.reserve((size_t)plan_threads());
   for (int w = 0; w < plan_threads(); ++w) maps.emplace_back();
@@ -466,12 +466,32 @@ TilePlan plan_tiles(int na, int navel, const int32_t *const idx[6], const uint8_
   }
   // shared ids: the flagged stress slots in slot order (exactly the slots of
   // the halo lists; the check re-derives that), per-entry and per-slot maps
-  p.sid.assign((size_t)t_end, -1);
-  for (int s = 0; s < t_end; ++s)
-    if (shf[(size_t)p.perm[(size_t)s]]) {
-      p.sid[(size_t)s] = (int32_t)p.shared.size();
-      p.shared.push_back(s);
-    }
+  // (two passes over chunks of slots: count the flagged ones, then number
+  // them in slot order)
+  {
+    const int Ts = std::max(1, std::min(plan_threads(), t_end / (1 << 15)));
+    auto lo_of = [&](int w) { return (int64_t)t_end * w / Ts; };
+    std::vector<int64_t> first((size_t)Ts + 1, 0);
+    run_workers(Ts, [&](int w) {
+      int64_t c = 0;
+      for (int64_t s = lo_of(w); s < lo_of(w + 1); ++s) c += shf[(size_t)p.perm[(size_t)s]] ? 1 : 0;
+      first[(size_t)w + 1] = c;
+    });
+    for (int w = 0; w < Ts; ++w) first[(size_t)w + 1] += first[(size_t)w];
+    p.sid.resize((size_t)t_end);
+    p.shared.resize((size_t)first[(size_t)Ts]);
+    run_workers(Ts, [&](int w) {
+      int64_t id = first[(size_t)w];
+      for (int64_t s = lo_of(w); s < lo_of(w + 1); ++s) {
+        if (shf[(size_t)p.perm[(size_t)s]]) {
+          p.sid[(size_t)s] = (int32_t)id;
+          p.shared[(size_t)id++] = (int32_t)s;
+        } else {
+          p.sid[(size_t)s] = -1;
+        }
+      }
+    });
+  }
   p.hid.resize(p.list.size());
   par_chunks((int64_t)p.list.size(), 1 << 15, [&](int64_t lo, int64_t hi, int) {
     for (int64_t k = lo; k < hi; ++k) p.hid[(size_t)k] = p.sid[(size_t)p.list[(size_t)k]];

@@ -19,10 +19,38 @@
 // (ice_dyn_evp1d.F90:766-772).  Connected components get disjoint frames.
 #pragma once
 #include <cstdint>
+#include <memory>
+#include <new>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace evp1d {
+
+// std::vector whose resize leaves new elements uninitialised, for the large
+// planner tables that are written in full (in parallel) before any read: the
+// value-initialisation of std::vector would be a serial pass over them.
+template <class T>
+struct UninitAlloc : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    using other = UninitAlloc<U>;
+  };
+  UninitAlloc() = default;
+  template <class U>
+  UninitAlloc(const UninitAlloc<U> &) {}
+  template <class U>
+  void construct(U *p) noexcept {
+    ::new (static_cast<void *>(p)) U;
+  }
+  template <class U, class... A>
+  void construct(U *p, A &&...a) {
+    ::new (static_cast<void *>(p)) U(std::forward<A>(a)...);
+  }
+};
+template <class T>
+using Table = std::vector<T, UninitAlloc<T>>;
+
 
 // Slot ranges of one tile; entries of its LDS table are numbered
 // [owned stress-only | owned stress-and-stepu | halo stress | constants].
@@ -45,12 +73,12 @@ struct TilePlan {
   std::vector<TileDesc> tiles;        // tiles with work, in launch order
   std::vector<int32_t> list;          // halo stress slots, per tile
   std::vector<int32_t> klist;         // constant slots, per tile
-  std::vector<uint16_t> uloc;         // 4 per stepu slot (slot - cnt[0]): own, nw, sse, sw entries
+  Table<uint16_t> uloc;               // 4 per stepu slot (slot - cnt[0]): own, nw, sse, sw entries
   // Stresses of shared points (halo of some tile) live, during a fused run,
   // in a double-buffered copy indexed by a shared id: every reader sees the
   // previous subcycle's values while their owner writes the next ones.
-  std::vector<int32_t> shared;        // shared id -> slot (sorted slots of all halo entries)
-  std::vector<int32_t> sid;           // stress slot -> shared id, or -1
+  Table<int32_t> shared;              // shared id -> slot (sorted slots of all halo entries)
+  Table<int32_t> sid;                 // stress slot -> shared id, or -1
   std::vector<int32_t> hid;           // per halo entry (parallel to list): shared id
   int max_entries = 0;
 };
@@ -115,12 +143,12 @@ struct PairPlan {
   bool ok = false;
   std::string why;
   std::vector<PairTile> tiles;        // parallel to TilePlan::tiles
-  std::vector<int32_t> xlist, klist, elist;
+  Table<int32_t> xlist, klist, elist;
   // 4 per U1 entry: str table entries of own, nw, sse, sw
-  std::vector<uint16_t> uloc;
+  Table<uint16_t> uloc;
   // 4 per T2 entry: velocity table entries of cc, ee, ne, se; kNoEntry when
   // that velocity never changes (not an active U point: read from memory)
-  std::vector<uint16_t> vloc;
+  Table<uint16_t> vloc;
   int cap_s = 0, cap_v = 0;           // largest str / velocity table
   int max_t2 = 0;                     // largest T2 (level-2 stresses, one per thread)
   static constexpr uint16_t kNoEntry = 0xffff;
