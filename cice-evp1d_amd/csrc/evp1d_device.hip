@@ -43,6 +43,7 @@
 #include "evp1d_adjoint.hpp"
 #include "evp1d_tiles.hpp"
 #include "evp1d_point.hpp"
+#include "evp1d_xcd.hpp"
 
 #ifndef EVP1D_FMA
 #define EVP1D_FMA 0
@@ -197,19 +198,10 @@ __device__ __forceinline__ void halo_at(const DevView &d, int k) {
 
 constexpr int kBlock = EVP1D_BLOCK;   // block size of the helper kernels and the persistent form
 
-// Workgroups of one launch are dispatched round-robin over the 8 XCDs of the
-// chip (workgroup b runs on XCD b % 8), and each XCD has its own L2.  With
-// `xcd` set, the workgroup with hardware index b processes logical tile L(b),
-// a bijection that hands each XCD one contiguous eighth of the slot range, so
-// the neighbour rows a tile gathers from (the previous row for stress, the
-// next row for stepu) were streamed through the same L2.
-constexpr int kXcds = 8;
-__device__ __forceinline__ int tile_of(int b, int n, bool xcd) {
-  if (!xcd) return b;
-  const int q = n / kXcds, r = n % kXcds;
-  const int x = b % kXcds, k = b / kXcds;
-  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
-}
+// tile_of (evp1d_xcd.hpp): the XCD-aware workgroup-to-tile mapping; for the
+// unfused kernels a "tile" is a block of B slots, so each XCD streams one
+// contiguous eighth of the slot range (the previous row for stress, the next
+// row for stepu come through the same L2).
 
 template <int B, bool X>
 __global__ __launch_bounds__(B) void k_stress(DevView d) {

@@ -38,6 +38,7 @@
 
 #include "../../include/evp1d.h"
 #include "evp1d_point.hpp"
+#include "evp1d_xcd.hpp"
 
 extern "C" int evp1d__set_error(int code, const char *msg);
 
@@ -177,7 +178,7 @@ struct Fused2 {
 
 __global__ __launch_bounds__(kB2F) void k2_fused(View2 v, Fused2 f, int last) {
   __shared__ double2 lds[4 * kCap2];
-  const Tile2 t = f.tiles[blockIdx.x];
+  const Tile2 t = f.tiles[tile_of(blockIdx.x, gridDim.x, true)];   // each XCD a contiguous band of tiles
   for (int k = threadIdx.x; k < 4 * kCap2; k += kB2F) lds[k] = make_double2(0.0, 0.0);
   __syncthreads();
   const int nT = t.nto + t.nth;
