@@ -11,7 +11,11 @@ Checks:
   * central finite differences (eps = 1e-6 relative per component) agree
     with <grad, d> to 1e-6 relative;
   * the gradient does not depend on the number of stored states, from one
-    stored state (recompute everything) to all of them (no recomputation).
+    stored state (recompute everything) to all of them (no recomputation),
+    nor on the checkpointing unit (single subcycles, or segments reversed
+    through a stored velocity trajectory), in the graph and pair forms;
+  * the pair-form forward sweep (two stored subcycles per launch, odd and
+    tiny nsub) gives the same gradient as one subcycle per launch.
 """
 import numpy as np
 import pytest
