@@ -118,18 +118,22 @@ def cpu_baseline(workload: str, s: dict, ndte_sample: int, threads: int) -> dict
         st.idx[k][:] = s["idx"][k]
     st.skipT[:] = s["skipT"]
     st.skipU[:] = s["skipU"]
-    exe = O.REFDIR / "evp_v2b_omp"
-    if exe.exists():
+    # the reference's performance flag set (f2: OpenMP, -O3, vector ISA, contraction on;
+    # compiler_oneapi.mk:18) as flang has it, else its OpenMP build without contraction
+    exe = next((O.REFDIR / b for b in ("evp_v2b_f2", "evp_v2b_omp") if (O.REFDIR / b).exists()), None)
+    if exe is not None:
         with tempfile.TemporaryDirectory() as d:
             O.write_inputs(d, st, ndte_sample)
             t = time.time()
-            r = O.run_reference(Path(d), "evp_v2b_omp", threads=threads, timeout=600,
+            r = O.run_reference(Path(d), exe.name, threads=threads, timeout=600,
                                 env_extra={"OMP_PROC_BIND": "close", "OMP_PLACES": "cores"})
             wall = time.time() - t
         m = re.search(r"Benchmark took:\s+([0-9.]+)\s+seconds", r.stdout)
         secs = float(m.group(1)) if m else wall
         kind = "reference"
-        what = (f"reference src/evp_v2b.F90 (flang -O2 -fopenmp), timer 'bench' around the "
+        flags = ("flang -O3 -fopenmp -march=x86-64-v3 (contraction on)" if exe.name == "evp_v2b_f2"
+                 else "flang -O2 -fopenmp -ffp-contract=off")
+        what = (f"reference src/evp_v2b.F90 ({flags}), timer 'bench' around the "
                 f"subcycle loop, {workload} grid, {ndte_sample} subcycles")
     else:
         lib = O.OracleLib(omp=True)
@@ -141,7 +145,7 @@ def cpu_baseline(workload: str, s: dict, ndte_sample: int, threads: int) -> dict
         kind = "port"
         what = f"oracle C restatement (gcc -O3 -fopenmp), {workload} grid, {ndte_sample} subcycles"
     return {"value": s["na"] * ndte_sample / secs, "unit": "ice-pts*subcycles/s", "cores": threads,
-            "kind": kind, "sample": what, "seconds": round(secs, 3)}
+            "kind": kind, "sample": what, "seconds": round(secs, 3), "host": host_cpu()}
 
 
 def api_measure(workload: str, nrun: int) -> dict:
@@ -210,6 +214,60 @@ def time_steps(eng, ndte: int, steps: int, warmup: int) -> float:
     wall = time.perf_counter() - t0
     log(f"[bench] timed {steps} steps: wall {wall:.3f}s, events {ev_ms / 1e3:.3f}s")
     return max(wall, ev_ms / 1e3)
+
+
+OUT_FIELDS = ("stressp_1", "stressp_2", "stressp_3", "stressp_4", "stressm_1", "stressm_2", "stressm_3",
+              "stressm_4", "stress12_1", "stress12_2", "stress12_3", "stress12_4", "uvel", "vvel",
+              "strintxU", "strintyU", "Cb")
+
+
+def output_fields(eng, s: dict) -> dict:
+    """The fields after one ndte-subcycle run from the initial state (the
+    engine is left in that state)."""
+    reset_state(eng, s)
+    eng.run(eng.get_params().ndte)
+    eng.synchronize()
+    return {n: eng.get_field(n) for n in OUT_FIELDS}
+
+
+def fma_measure(s: dict, ndte: int, steps: int, warmup: int, tune_nsub: int, exact: dict) -> dict:
+    """The contraction-on build (libevp1d_fma.so) on the same workload, tuned
+    on its own (evp1d_autotune_fused), timed like the exact build; its
+    largest norm-wise difference max|d| / max|exact| over the output fields
+    after one run from the initial state."""
+    eng = new_engine(s, ndte, "auto", fma=True)
+    tuned = eng.autotune_fused(tune_nsub) if tune_nsub > 0 else None
+    log(f"[bench] fma: autotune_fused {tuned}")
+    secs = time_steps(eng, ndte, steps, warmup)
+    got = output_fields(eng, s)
+    eng.close()
+    err = 0.0
+    for n, a in exact.items():
+        r = np.max(np.abs(a))
+        d = np.max(np.abs(got[n] - a))
+        err = max(err, d / r if r > 0 else d)
+    out = {"value": s["na"] * ndte * steps / secs, "unit": "ice-pts*subcycles/s",
+           "us_per_subcycle": round(secs / (steps * ndte) * 1e6, 3), "ms_per_step": secs / steps * 1e3,
+           "max_normwise_diff_vs_exact": err, "tiling": tuned}
+    log(f"[bench] fma: {out['us_per_subcycle']} us/subcycle, max norm-wise diff vs exact {err:.2e}")
+    return out
+
+
+def host_cpu() -> dict:
+    """Model, sockets and NUMA nodes of this host (lscpu)."""
+    import subprocess
+    info = {"logical_cpus": os.cpu_count()}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+    except Exception:
+        return info
+    for key, name in (("Model name", "model"), ("Socket(s)", "sockets"), ("NUMA node(s)", "numa_nodes"),
+                      ("Core(s) per socket", "cores_per_socket"), ("Thread(s) per core", "threads_per_core")):
+        m = re.search(rf"^{re.escape(key)}:\s*(.+)$", txt, re.M)
+        if m:
+            v = m.group(1).strip()
+            info[name] = int(v) if v.isdigit() else v
+    return info
 
 
 def adjoint_measure(s: dict, ndte: int, t_fwd_step: float, nsnap: int, ndir: int, ndot: int,
@@ -481,9 +539,10 @@ def main() -> int:
     ap.add_argument("--cpu-subcycles", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-fma", action="store_true", help="skip timing the contraction-on build beside the exact one")
     ap.add_argument("--adjoint-snapshots", type=int, default=32)
-    ap.add_argument("--fd-directions", type=int, default=3)
-    ap.add_argument("--dot-directions", type=int, default=2)
+    ap.add_argument("--fd-directions", type=int, default=50)
+    ap.add_argument("--dot-directions", type=int, default=10)
     ap.add_argument("--path", default="1d", choices=("1d", "v0"),
                     help="v0: time the v0 2D block path (evp2d_*) on the workload instead of the 1D engine")
     ap.add_argument("--api", action="store_true",
@@ -563,8 +622,9 @@ def main() -> int:
     log(f"[bench] mode {args.mode} ({'fused' if fused else 'unfused'}), tiling {tiling}, tuning {tuning}")
     secs = time_steps(eng, ndte, args.steps, args.warmup)
     value = na * ndte * args.steps / secs
+    step_us = secs / (args.steps * ndte) * 1e6
 
-    # per-kernel average launch durations (HIP events, the library's stream)
+    # each kernel alone (HIP events on the library's stream, launched one by one)
     B = algorithmic_bytes(na, nT, nU)
     B["fused"] = B["subcycle"]
     kernels = {}
@@ -576,20 +636,27 @@ def main() -> int:
         avg_us = ms * 1e3 / args.kernel_reps
         kernels[which] = {"avg_us": round(avg_us, 3), "bytes": B[which],
                           "GBps": round(B[which] / (avg_us * 1e-6) / 1e9, 1)}
-        log(f"[bench] {which}: {avg_us:.2f} us/launch, {kernels[which]['GBps']} GB/s algorithmic")
+        log(f"[bench] {which}: {avg_us:.2f} us/launch alone, {kernels[which]['GBps']} GB/s algorithmic")
+    exact_out = output_fields(eng, s) if (fused and not args.fma and not args.no_fma) else None
     eng.close()
-    step_us = secs / (args.steps * ndte) * 1e6
+    # The dominant kernel's average launch duration INSIDE the timed loop:
+    # the loop's HIP-event time divided by the number of its launches.  It
+    # is an upper bound of the kernel's own duration (each run's state
+    # copies and packing, about 1 % of it, are included), so launches x
+    # avg_us equals ms_per_step.
+    run_us = secs / args.steps * 1e6
     if args.mode == "persistent":
-        dom = {"name": "k_persistent", "bytes": B["subcycle"] * ndte, "avg_us": secs / args.steps * 1e6}
+        dom = {"name": "k_persistent", "bytes": B["subcycle"] * ndte, "avg_us": run_us, "launches_per_step": 1}
     elif fused and tiling["depth"] == 2:
-        # one k_pair launch = two subcycles (run_individual counts subcycles)
-        dom = {"name": "k_pair", "bytes": 2 * B["subcycle"], "avg_us": 2 * kernels["fused"]["avg_us"],
-               "subcycles_per_launch": 2}
+        nl = ndte / 2.0     # one k_pair launch = two subcycles (an odd ndte adds one one-subcycle launch)
+        dom = {"name": "k_pair", "bytes": 2 * B["subcycle"], "avg_us": run_us / nl,
+               "subcycles_per_launch": 2, "launches_per_step": nl}
     elif fused:
-        dom = {"name": "k_fused", "bytes": B["fused"], "avg_us": kernels["fused"]["avg_us"]}
-    else:
-        name = max(kernels, key=lambda k: kernels[k]["avg_us"])
-        dom = {"name": "k_" + name, "bytes": B[name], "avg_us": kernels[name]["avg_us"]}
+        dom = {"name": "k_fused", "bytes": B["fused"], "avg_us": run_us / ndte, "launches_per_step": ndte}
+    else:   # two kernels per subcycle: the slower one, timed alone
+        name = max(("stress", "stepu"), key=lambda k: kernels[k]["avg_us"])
+        dom = {"name": "k_" + name, "bytes": B[name], "avg_us": kernels[name]["avg_us"], "launches_per_step": ndte,
+               "timed": "alone (run_individual)"}
     achieved = dom["bytes"] / (dom["avg_us"] * 1e-6) / 1e9
     traffic = None
     tf = REPO / "profiles" / f"traffic_{args.workload}.json"
@@ -607,8 +674,27 @@ def main() -> int:
         traffic = tab.get(key)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom["name"],
-                "bytes_per_launch": dom["bytes"], "subcycles_per_launch": dom.get("subcycles_per_launch", 1)}
-    whole = B["subcycle"] / (step_us * 1e-6) / 1e9
+                "bytes_per_launch": dom["bytes"], "avg_us": round(dom["avg_us"], 3),
+                "avg_us_timed": dom.get("timed", "inside the timed loop (loop event time / launches)"),
+                "launches_per_step": dom["launches_per_step"],
+                "subcycles_per_launch": dom.get("subcycles_per_launch", 1)}
+    if traffic:
+        # the bytes the kernel really moves through HBM (counters) over the same duration
+        phys = traffic / (dom["avg_us"] * 1e-6) / 1e9
+        roofline.update({"achieved_physical": round(phys, 1), "frac_physical": round(phys / HBM_PEAK_GBS, 4),
+                         "traffic_over_algorithmic": round(traffic / dom["bytes"], 3)})
+    if dom["name"] == "k_pair":
+        roofline["note"] = ("frac counts the reference's bytes of two subcycles; the pair form keeps "
+                            "str1..8 and the first subcycle's state on chip, so it moves fewer bytes than "
+                            "that count and frac_physical is the kernel's distance to HBM bandwidth")
+
+    fma = None
+    if exact_out is not None:
+        try:
+            fma = fma_measure(s, ndte, args.steps, args.warmup, args.tune_fused, exact_out)
+        except Exception as e:   # reported, never fatal for the exact figure
+            log(f"[bench] fma measurement failed: {e!r}")
+            fma = {"error": repr(e)[:200]}
 
     adjoint = None
     if not args.no_adjoint and not args.fma:
@@ -643,10 +729,11 @@ def main() -> int:
                                                                          or args.order is not None)
                                         else "library default"),
                    "build": "fma" if args.fma else "exact", "tuning": tuning,
-                   "us_per_subcycle": round(step_us, 3), "loop_GBps_algorithmic": round(whole, 1),
-                   "loop_frac_of_peak": round(whole / HBM_PEAK_GBS, 4)},
+                   "us_per_subcycle": round(step_us, 3),
+                   "loop_GBps_algorithmic": round(B["subcycle"] / (step_us * 1e-6) / 1e9, 1)},
         "kernels": kernels,
         "roofline": roofline,
+        "fma": fma,
         "adjoint": adjoint,
         "cpu_baseline": cpu,
     }

@@ -297,27 +297,55 @@ _IN_ORDER = ("strength", "uvel", "vvel", "dxT", "dyT", "uarear", "cdn_ocn", "aiu
              "Tbu") + STRESS_FIELDS
 
 
+def input_chunks(st: State1D) -> dict:
+    """The byte chunks of input_double_1d_v2.bin, input_integer_1d.bin and
+    input_logical_1d.bin, in file order (readin_1d_v2, vars.F90:2101-2135).
+    Logicals are 4-byte 0/1."""
+    na, navel = st.na, st.navel
+
+    def doubles():
+        for n in _IN_ORDER:
+            a = st.f[n]
+            yield np.ascontiguousarray(a[:navel] if n in ("uvel", "vvel") else a[:na]).tobytes()
+        s = st.scalars
+        yield np.array([s["capping"], s["e_factor"], s["epp2i"], s["deltaminEVP"]]).tobytes()
+        for n in ("HTE", "HTN", "HTEm1", "HTNm1"):
+            yield np.ascontiguousarray(st.f[n][:na]).tobytes()
+
+    def integers():
+        for n in NBR:
+            yield st.idx[n].astype("<i4").tobytes()
+
+    def logicals():
+        yield st.skipU.astype("<i4").tobytes()
+        yield st.skipT.astype("<i4").tobytes()
+
+    return {"input_double_1d_v2.bin": doubles, "input_integer_1d.bin": integers,
+            "input_logical_1d.bin": logicals}
+
+
+def input_md5s(st: State1D) -> dict:
+    """md5 of each input file write_inputs would write, computed in memory."""
+    out = {}
+    for name, gen in input_chunks(st).items():
+        h = hashlib.md5()
+        for b in gen():
+            h.update(b)
+        out[name] = h.hexdigest()
+    return out
+
+
 def write_inputs(d: Path, st: State1D, ndte: int, nx_block: int = -1, ny_block: int = -1) -> None:
     """input_double_1d_v2.bin, input_integer_1d.bin, input_logical_1d.bin and
     kernel.nml as readin_1d_v2 / read_nml consume them (vars.F90:2101-2135,
-    19-69).  Logicals are written as 4-byte 0/1."""
+    19-69)."""
     d = Path(d)
     d.mkdir(parents=True, exist_ok=True)
-    na, navel = st.na, st.navel
-    with open(d / "input_double_1d_v2.bin", "wb") as fh:
-        for n in _IN_ORDER:
-            a = st.f[n]
-            fh.write(np.ascontiguousarray(a[:navel] if n in ("uvel", "vvel") else a[:na]).tobytes())
-        s = st.scalars
-        fh.write(np.array([s["capping"], s["e_factor"], s["epp2i"], s["deltaminEVP"]]).tobytes())
-        for n in ("HTE", "HTN", "HTEm1", "HTNm1"):
-            fh.write(st.f[n][:na].tobytes())
-    with open(d / "input_integer_1d.bin", "wb") as fh:
-        for n in NBR:
-            fh.write(st.idx[n].astype("<i4").tobytes())
-    with open(d / "input_logical_1d.bin", "wb") as fh:
-        fh.write(st.skipU.astype("<i4").tobytes())
-        fh.write(st.skipT.astype("<i4").tobytes())
+    na = st.na
+    for name, gen in input_chunks(st).items():
+        with open(d / name, "wb") as fh:
+            for b in gen():
+                fh.write(b)
     nb = int(na - np.count_nonzero(st.skipU))
     (d / "kernel.nml").write_text(
         "&kernel_nml\n"

@@ -49,15 +49,14 @@ def build_state(kw, scal):
 
 
 def input_md5(st, ndte):
-    with tempfile.TemporaryDirectory(dir=REPO / "_work") as d:
+    with tempfile.TemporaryDirectory() as d:
         O.write_inputs(d, st, ndte)
         return {f: O.md5((Path(d) / f).read_bytes()) for f in
                 ("input_double_1d_v2.bin", "input_integer_1d.bin", "input_logical_1d.bin")}
 
 
 def run_case(st, ndte):
-    (REPO / "_work").mkdir(exist_ok=True)
-    with tempfile.TemporaryDirectory(dir=REPO / "_work") as d:
+    with tempfile.TemporaryDirectory() as d:
         O.write_inputs(d, st, ndte)
         O.run_reference(Path(d), "evp_v2b_f0", timeout=7200)
         a = (Path(d) / "output_stress_v2b_1d.bin").read_bytes()
@@ -95,11 +94,13 @@ def md5_workload(name):
 
 if __name__ == "__main__":
     gj = HERE / "golden.json"
-    golden = json.loads(gj.read_text()) if gj.exists() else {}
     names = sys.argv[1:] or list(SMALL) + ["evp_c0"]
     for n in names:
         if n in SMALL:
             save_small(n)
         else:
-            golden[n] = md5_workload(n)
+            rec = md5_workload(n)
+            # re-read before writing: several workloads may run in parallel processes
+            golden = json.loads(gj.read_text()) if gj.exists() else {}
+            golden[n] = rec
             gj.write_text(json.dumps(golden, indent=1, sort_keys=True) + "\n")

@@ -49,8 +49,7 @@ def build(kw, nblocks, scal):
 
 
 def run_reference_v0(st, ndte):
-    (REPO / "_work").mkdir(exist_ok=True)
-    with tempfile.TemporaryDirectory(dir=REPO / "_work") as d:
+    with tempfile.TemporaryDirectory() as d:
         O.write_inputs_v0(d, st, ndte)
         exe = O.REFDIR / "evp_v0_f0"
         subprocess.run([str(exe)], cwd=d, capture_output=True, text=True, timeout=7200, check=True)

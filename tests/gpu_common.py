@@ -104,3 +104,11 @@ def synthetic_state(nx: int, ny: int, seed: int, **kw) -> O.State1D:
     g = synth.make_grid(nx, ny, seed, **kw)
     ix = evp1d.build_index(g.tmask, g.nghost)
     return O.pack_dyn(g, ix)
+
+
+def ensemble_member_grid(k: int):
+    """Synthetic 2D state of ensemble member k (numpy only: safe to run in a
+    worker process that never touches the GPU)."""
+    from evp1d import synth
+    m = synth.ensemble_member(k)
+    return synth.make_grid(m["nx"], m["ny"], m["seed"], force_perturb=0.02)

@@ -1,0 +1,278 @@
+"""Parity at the full sizes of the benchmark workloads.
+
+* 1280x1520, 1000 subcycles (na = 1,361,920): both output files' md5 equal
+  the flang-built reference f0's (tests/golden/golden.json, made by
+  tests/golden/make_golden.py evp_c1) in the library default form, the
+  two-kernel graph, the persistent kernel, the pair form on 31x15 tiles in
+  Hilbert order (the form the benchmark's tuner picks) and whatever
+  evp1d_autotune_fused picks on the box; the contraction-on build within
+  1e-12 norm-wise of the oracle in the graph and pair forms.
+* 2560x3040, 1000 subcycles (na = 5,447,680): md5 equal to the reference f0's
+  in the pair form, the one-subcycle fused form and the persistent kernel.
+* 64 ensemble members in one batched handle (25.2 M points, 1000 subcycles):
+  every member's output md5 equals its own oracle run
+  (tests/golden/make_golden_ensemble.py).
+* The gradient at 1280x1520 (pair-form forward, 1000 subcycles): 32 stored
+  states and the whole stored trajectory give the same bits; central finite
+  differences and the tangent-linear dot product agree.
+
+The regenerated inputs are checked first against the md5 of the input files
+the reference ran on, so the 1D list construction and packing are pinned at
+these sizes too.  Reference call sites: src/evp_v2b.F90:91-110 (loop),
+src/vars.F90:2101-2135 and 2233-2304 (input and output files).
+"""
+from __future__ import annotations
+
+import json
+import multiprocessing as mp
+import os
+import sys
+import time
+from concurrent.futures import ProcessPoolExecutor
+
+import numpy as np
+import pytest
+
+import evp1d
+from oracle import oracle as O
+from gpu_common import GOLD, ensemble_member_grid, fetch_state
+
+pytestmark = pytest.mark.gpu
+GOLDEN = json.loads((GOLD / "golden.json").read_text())
+OUT_FIELDS = O.STRESS_FIELDS + ("strintxU", "strintyU", "uvel", "vvel")
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def workload_state(name: str) -> O.State1D:
+    """The workload's 1D state built by the library's own index builder, with
+    the input files' md5 checked against the ones the reference consumed."""
+    from evp1d import synth
+    rec = GOLDEN[name]
+    t = time.time()
+    g = synth.make_grid(rec["nx"], rec["ny"], rec["seed"])
+    st = O.pack_dyn(g, evp1d.build_index(g.tmask, g.nghost))
+    assert (st.na, st.navel) == (rec["na"], rec["navel"])
+    assert O.input_md5s(st) == rec["inputs"]
+    _log(f"{name}: na={st.na} navel={st.navel} built and inputs checked in {time.time() - t:.1f}s")
+    return st
+
+
+@pytest.fixture(scope="module")
+def c1(gpu):
+    return workload_state("evp_c1")
+
+
+@pytest.fixture(scope="module")
+def c2(gpu):
+    return workload_state("evp_c2")
+
+
+def set_form(eng: evp1d.Engine, form: str) -> None:
+    if form == "auto":
+        return
+    if form in ("graph", "persistent"):
+        eng.set_mode(form)
+    elif form == "fused-64x16":
+        eng.set_mode("fused")
+        eng.set_tiling(64, 16, 512, 1)
+    elif form == "pair-31x15-hilbert":
+        eng.set_mode("fused")
+        eng.set_tiling(31, 15, 512, 1)
+        eng.set_fused_depth(2)
+        eng.set_tile_order(1)
+    elif form == "tuned":
+        _log(f"autotune_fused: {eng.autotune_fused(100)}")
+    else:
+        raise ValueError(form)
+
+
+def run_form(st: O.State1D, ndte: int, form: str, fma: bool = False) -> O.State1D:
+    p = evp1d.set_evp_parameters(ndte, fma=fma, **st.scalars)
+    with evp1d.Engine(st.na, st.navel, p, fma=fma) as eng:
+        eng.load_state(st.f, st.idx, st.skipT, st.skipU)
+        set_form(eng, form)
+        if form == "pair-31x15-hilbert":
+            d = eng.get_fused_depth()
+            assert d["active"], d
+        t = time.time()
+        eng.run(ndte)
+        eng.synchronize()
+        _log(f"  {form}{' (fma)' if fma else ''}: {ndte} subcycles in {time.time() - t:.3f}s")
+        return fetch_state(eng, st)
+
+
+def check_md5(got: O.State1D, rec: dict) -> None:
+    a, b = O.output_bytes(got)
+    assert O.md5(a) == rec["output_stress_md5"], "output_stress md5 differs from the reference's"
+    assert O.md5(b) == rec["output_stepu_md5"], "output_stepu md5 differs from the reference's"
+
+
+@pytest.mark.timeout(400)
+@pytest.mark.parametrize("form", ["auto", "graph", "persistent", "pair-31x15-hilbert", "tuned"])
+def test_c1_md5(c1, form):
+    """1280x1520, 1000 subcycles: md5 of both output files equal reference f0's."""
+    _log(f"evp_c1 form={form}")
+    rec = GOLDEN["evp_c1"]
+    check_md5(run_form(c1, rec["ndte"], form), rec)
+
+
+def _normwise(a, b):
+    d = np.max(np.abs(a - b)) if a.size else 0.0
+    r = np.max(np.abs(b)) if b.size else 0.0
+    return d / r if r > 0 else d
+
+
+@pytest.mark.timeout(400)
+def test_c1_fma_within_1e12(c1):
+    """libevp1d_fma.so at 1280x1520 over 1000 subcycles: every field within
+    1e-12 norm-wise (max|d| / max|ref|) of the oracle (OpenMP build, the C
+    restatement of src/bench_v2b.F90), in the graph and the pair form."""
+    ndte = GOLDEN["evp_c1"]["ndte"]
+    lib = O.OracleLib(omp=True)
+    want = c1.copy()
+    t = time.time()
+    lib.run(lib.params(ndte, **c1.scalars), want, ndte)
+    _log(f"evp_c1 oracle (OpenMP, {os.environ.get('OMP_NUM_THREADS', '?')} threads): {time.time() - t:.1f}s")
+    check_md5(want, GOLDEN["evp_c1"])     # the oracle itself is the reference here
+    for form in ("graph", "pair-31x15-hilbert"):
+        got = run_form(c1, ndte, form, fma=True)
+        errs = {k: _normwise(got.f[k], want.f[k]) for k in want.f}
+        worst = max(errs, key=errs.get)
+        _log(f"  fma {form}: worst field {worst} {errs[worst]:.3e}")
+        assert errs[worst] <= 1e-12, (form, worst, errs[worst])
+
+
+@pytest.mark.timeout(400)
+@pytest.mark.parametrize("form", ["pair-31x15-hilbert", "fused-64x16", "persistent"])
+def test_c2_md5(c2, form):
+    """2560x3040, 1000 subcycles: md5 of both output files equal reference f0's."""
+    _log(f"evp_c2 form={form}")
+    rec = GOLDEN["evp_c2"]
+    check_md5(run_form(c2, rec["ndte"], form), rec)
+
+
+class _Out:
+    """The arrays O.output_bytes reads."""
+
+    def __init__(self, na, skipU):
+        self.na, self.skipU, self.f = na, skipU, {}
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("form", ["pair-31x15-hilbert"])
+def test_c3_batch_member_md5(gpu, form):
+    """64 members (nx, ny ~ U[200, 1400], own masks and fields, forcing +
+    N(0, 0.02)) in one batched handle, 1000 subcycles: each member's output
+    md5 equals its own oracle run's."""
+    rec = GOLDEN["evp_c3"]
+    mems = rec["members"]
+    ndte = rec["ndte"]
+    params = evp1d.set_evp_parameters(ndte)
+    t = time.time()
+    skipU = {}
+    with evp1d.Engine.batch([(m["na"], m["navel"]) for m in mems], params) as eng:
+        set_form(eng, form)
+        nproc = max(1, min(16, os.cpu_count() or 1))
+        ctx = mp.get_context("spawn")
+        with ProcessPoolExecutor(max_workers=nproc, mp_context=ctx) as ex:
+            for k, g in enumerate(ex.map(ensemble_member_grid, range(len(mems)))):
+                m = mems[k]
+                assert (g.nx, g.ny) == (m["nx"], m["ny"])
+                st = O.pack_dyn(g, evp1d.build_index(g.tmask, g.nghost))
+                assert (st.na, st.navel) == (m["na"], m["navel"]), k
+                assert O.input_md5s(st) == m["inputs"], k
+                eng.load_member(k, st.f, st.idx, st.skipT, st.skipU)
+                skipU[k] = st.skipU.copy()
+                del st, g
+        _log(f"evp_c3: {len(mems)} members, {rec['total_points']} points built and loaded in "
+             f"{time.time() - t:.1f}s")
+        if form == "pair-31x15-hilbert":
+            assert eng.get_fused_depth()["active"]
+        t = time.time()
+        eng.run(ndte)
+        eng.synchronize()
+        _log(f"  batched {form}: {ndte} subcycles in {time.time() - t:.3f}s")
+        bad = []
+        for k, m in enumerate(mems):
+            o = _Out(m["na"], skipU[k])
+            for n in OUT_FIELDS:
+                o.f[n] = eng.get_field_member(k, n)
+            a, b = O.output_bytes(o)
+            if (O.md5(a), O.md5(b)) != (m["output_stress_md5"], m["output_stepu_md5"]):
+                bad.append(k)
+    assert not bad, f"members whose output md5 differs from their oracle run: {bad}"
+
+
+@pytest.mark.timeout(400)
+def test_c4_gradient_at_c1(c1):
+    """Gradient of L = 1/2 sum_{skipU=0} |u_N - u*|^2 (u* from a run with
+    strength x 1.2) at 1280x1520 over 1000 subcycles, pair-form forward:
+    32 stored states and the whole trajectory give identical bits; central
+    FD (eps 1e-6 relative) within 1e-6 on 3 directions; tangent-linear
+    dot product within 1e-10 on 2 directions."""
+    ndte = GOLDEN["evp_c1"]["ndte"]
+    st = c1
+    f = st.f
+    p = evp1d.set_evp_parameters(ndte, **st.scalars)
+    dyn = O.STRESS_FIELDS + ("uvel", "vvel", "strintxU", "strintyU", "Cb") + tuple(f"str{k}" for k in range(1, 9))
+
+    with evp1d.Engine(st.na, st.navel, p) as eng:
+        eng.load_state(st.f, st.idx, st.skipT, st.skipU)
+        set_form(eng, "pair-31x15-hilbert")
+
+        def reset(over=None):
+            for n in dyn:
+                eng.set_field(n, f[n])
+            for n in ("strength", "cdn_ocn", "forcexU", "forceyU"):
+                eng.set_field(n, (over or {}).get(n, f[n]))
+
+        reset({"strength": 1.2 * f["strength"]})
+        eng.run(ndte)
+        ut = eng.get_field("uvel")[:st.na].copy()
+        vt = eng.get_field("vvel")[:st.na].copy()
+        grads = {}
+        for ns in (ndte, 32):
+            reset()
+            t = time.time()
+            grads[ns] = eng.adjoint_run(ndte, ut, vt, nsnap=ns)
+            _log(f"  gradient, {ns} stored states: {time.time() - t:.3f}s")
+        for k in ("strength", "cdn_ocn", "forcex", "forcey"):
+            assert np.array_equal(grads[32][k].view(np.uint64), grads[ndte][k].view(np.uint64)), k
+        g = grads[ndte]
+        rng = np.random.default_rng(4321)
+
+        def direction():
+            return {"strength": rng.standard_normal(st.na) * f["strength"],
+                    "cdn_ocn": np.full(st.na, rng.standard_normal()) * f["cdn_ocn"],
+                    "forcexU": rng.standard_normal(st.na) * np.abs(f["forcexU"]),
+                    "forceyU": rng.standard_normal(st.na) * np.abs(f["forceyU"])}
+
+        def dot(d):
+            return (np.dot(g["strength"], d["strength"]) + np.dot(g["cdn_ocn"], d["cdn_ocn"]) +
+                    np.dot(g["forcex"], d["forcexU"]) + np.dot(g["forcey"], d["forceyU"]))
+
+        h = 1e-6
+        for _ in range(3):
+            d = direction()
+            vals = []
+            for sgn in (1.0, -1.0):
+                reset({n: f[n] + sgn * h * d[n] for n in d})
+                eng.run(ndte)
+                vals.append(eng.loss(ut, vt))
+            fd = (vals[0] - vals[1]) / (2 * h)
+            ad = dot(d)
+            err = abs(fd - ad) / max(abs(fd), abs(ad))
+            _log(f"  FD {fd:.10e} adjoint {ad:.10e} rel err {err:.2e}")
+            assert err <= 1e-6
+        for _ in range(2):
+            d = direction()
+            reset()
+            _, dl = eng.tangent_run(ndte, ut, vt, d["strength"], d["cdn_ocn"], d["forcexU"], d["forceyU"])
+            ad = dot(d)
+            err = abs(dl - ad) / max(abs(dl), abs(ad))
+            _log(f"  tangent {dl:.14e} adjoint {ad:.14e} rel err {err:.2e}")
+            assert err <= 1e-10
