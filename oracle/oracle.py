@@ -341,7 +341,7 @@ def write_inputs(d: Path, st: State1D, ndte: int, nx_block: int = -1, ny_block: 
     19-69)."""
     d = Path(d)
     d.mkdir(parents=True, exist_ok=True)
-    na = st.na
+    na, navel = st.na, st.navel
     for name, gen in input_chunks(st).items():
         with open(d / name, "wb") as fh:
             for b in gen():

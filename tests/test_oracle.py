@@ -169,3 +169,21 @@ def test_golden_vectors_catch_planted_mistake(tmp_path, name):
         if O.output_bytes(st) != (want_stress, want_stepu):
             caught = True
     assert caught, f"mutation {name} not detected by the golden vectors"
+
+
+def test_input_files_round_trip_and_md5(tmp_path):
+    """write_inputs writes the reference's three input files and kernel.nml;
+    input_md5s (used by the full-size GPU tests to check regenerated inputs)
+    gives the md5 of the same bytes, and read_inputs reads them back."""
+    st, ndte, _, _ = load_golden("ragged_40x7")
+    O.write_inputs(tmp_path, st, ndte)
+    for f, m in O.input_md5s(st).items():
+        assert O.md5((tmp_path / f).read_bytes()) == m, f
+    nml = (tmp_path / "kernel.nml").read_text()
+    assert f"navel = {st.navel}" in nml and f"na = {st.na}" in nml and f"ndte = {ndte}" in nml
+    back = O.read_inputs(tmp_path, st.na, st.navel)
+    for k in ("strength", "uvel", "vvel", "HTNm1", "stress12_4", "Tbu"):
+        assert np.array_equal(back.f[k], st.f[k]), k
+    for k in O.NBR:
+        assert np.array_equal(back.idx[k], st.idx[k]), k
+    assert np.array_equal(back.skipT, st.skipT) and np.array_equal(back.skipU, st.skipU)
