@@ -1297,7 +1297,11 @@ static int install_pairs(evp1d_handle *h, const evp1d::TilePlan &plan, const int
 // Rebuild the slot order (class, then fused tile, then caller position) from
 // the skip flags and the neighbour lists, and move every resident array into
 // it.  Without a complete set of lists the order is class, caller position.
+static int relayout_impl(evp1d_handle *h);
 static int relayout(evp1d_handle *h) {
+  return evp1d::no_throw("slot layout", [h] { return relayout_impl(h); });
+}
+static int relayout_impl(evp1d_handle *h) {
   const int na = h->na;
   const int32_t *idx[6];
   bool all = na > 0;

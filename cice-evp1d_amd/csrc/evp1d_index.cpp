@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/evp1d.h"
+#include "evp1d_tiles.hpp"
 
 // error channel shared with the device translation unit
 extern "C" int evp1d__set_error(int code, const char *msg);
@@ -95,6 +96,7 @@ extern "C" {
 
 int evp1d_index_count(int32_t nx, int32_t ny, int32_t nghost, const int32_t *tmask, int32_t *na,
                       int32_t *navel) {
+  return evp1d::no_throw("evp1d_index_count", [&]() -> int {
   Lists L;
   int64_t na_all = 0;
   int rc = collect(nx, ny, nghost, tmask, L, na_all);
@@ -103,11 +105,13 @@ int evp1d_index_count(int32_t nx, int32_t ny, int32_t nghost, const int32_t *tma
   if (na) *na = (int32_t)L.in.size();
   if (navel) *navel = (int32_t)(L.in.size() + g.size());
   return EVP1D_OK;
+  });
 }
 
 int evp1d_index_build(int32_t nx, int32_t ny, int32_t nghost, const int32_t *tmask, int32_t na,
                       int32_t navel, int32_t *indxti, int32_t *indxtj, int32_t *indxtij, int32_t *ee,
                       int32_t *ne, int32_t *se, int32_t *nw, int32_t *sw, int32_t *sse) {
+  return evp1d::no_throw("evp1d_index_build", [&]() -> int {
   Lists L;
   int64_t na_all = 0;
   int rc = collect(nx, ny, nghost, tmask, L, na_all);
@@ -144,6 +148,7 @@ int evp1d_index_build(int32_t nx, int32_t ny, int32_t nghost, const int32_t *tma
     }
   }
   return EVP1D_OK;
+  });
 }
 
 }  // extern "C"

@@ -212,22 +212,43 @@ int plan_threads() {
 
 // Run body(w) for w = 0..T-1, on T-1 new threads and the caller; when a
 // thread cannot be started (a process thread limit) its share runs on the
-// caller, so the work is always done and no exception leaves the library.
+// caller, so the work is always done.  An exception thrown by a share (a
+// host allocation failure) is caught on its thread and the first one is
+// rethrown on the caller after every thread has joined; the C entry points
+// turn it into a status code (no_throw).
 template <class B>
 void run_workers(int T, B body) {
+  std::vector<std::exception_ptr> err((size_t)T);
+  auto guarded = [&body, &err](int w) {
+    try {
+      body(w);
+    } catch (...) {
+      err[(size_t)w] = std::current_exception();
+    }
+  };
   std::vector<std::thread> th;
   std::vector<int> left;
-  th.reserve((size_t)T);
+  try {
+    th.reserve((size_t)T);
+    left.reserve((size_t)T);
+  } catch (...) {
+    for (int w = 0; w < T; ++w) guarded(w);   // no room for the bookkeeping: all on the caller
+    for (const auto &e : err)
+      if (e) std::rethrow_exception(e);
+    return;
+  }
   for (int w = 1; w < T; ++w) {
     try {
-      th.emplace_back([&body, w] { body(w); });
+      th.emplace_back([&guarded, w] { guarded(w); });
     } catch (...) {
       left.push_back(w);
     }
   }
-  body(0);
-  for (int w : left) body(w);
+  guarded(0);
+  for (int w : left) guarded(w);
   for (auto &t : th) t.join();
+  for (const auto &e : err)
+    if (e) std::rethrow_exception(e);
 }
 
 // f(lo, hi, w) over [0, n) in contiguous chunks, one per worker w, in order.
@@ -893,14 +914,13 @@ std::string check_pairs(const PairPlan &q, const TilePlan &p, int na, int navel,
 
 #include "../../include/evp1d.h"
 
-extern "C" int evp1d__set_error(int code, const char *msg);
-
 extern "C" int evp1d_tile_plan(int32_t na, int32_t navel, const int32_t *ee, const int32_t *ne, const int32_t *se,
                                const int32_t *nw, const int32_t *sw, const int32_t *sse, const int32_t *skipT,
                                const int32_t *skipU, int32_t tile_w, int32_t tile_h, int32_t max_entries,
                                int32_t *perm, int32_t *counts, int32_t *ntiles, int32_t *max_used,
                                int32_t *tiles, int32_t tiles_cap, int32_t *list, int32_t list_cap, int32_t *nlist,
                                int32_t *klist, int32_t klist_cap, int32_t *nklist, int32_t *uloc) {
+  return evp1d::no_throw("evp1d_tile_plan", [&]() -> int {
   if (na < 0 || navel < na) return evp1d__set_error(EVP1D_EINVAL, "evp1d_tile_plan: need 0 <= na <= navel");
   const int32_t *in[6] = {ee, ne, se, nw, sw, sse};
   for (int k = 0; k < 6; ++k)
@@ -944,4 +964,5 @@ extern "C" int evp1d_tile_plan(int32_t na, int32_t navel, const int32_t *ee, con
   if (uloc && p.ok)
     for (size_t k = 0; k < p.uloc.size(); ++k) uloc[k] = p.uloc[k];
   return EVP1D_OK;
+  });
 }

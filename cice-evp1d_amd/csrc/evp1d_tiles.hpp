@@ -19,13 +19,38 @@
 // (ice_dyn_evp1d.F90:766-772).  Connected components get disjoint frames.
 #pragma once
 #include <cstdint>
+#include <cstdio>
+#include <exception>
 #include <memory>
 #include <new>
 #include <string>
 #include <utility>
 #include <vector>
 
+extern "C" int evp1d__set_error(int code, const char *msg);
+
 namespace evp1d {
+
+// Runs f() (which returns a status code) so that no C++ exception crosses the
+// C ABI: a host allocation failure becomes EVP1D_ENOMEM (4), any other
+// exception EVP1D_EINVAL (1), with the message kept for evp1d_last_error.
+// The message is formatted into a fixed buffer (nothing is allocated).
+template <class F>
+int no_throw(const char *who, F &&f) {
+  char buf[256];
+  try {
+    return f();
+  } catch (const std::bad_alloc &) {
+    std::snprintf(buf, sizeof buf, "%s: host memory allocation failed", who);
+    return evp1d__set_error(4, buf);
+  } catch (const std::exception &e) {
+    std::snprintf(buf, sizeof buf, "%s: %s", who, e.what());
+    return evp1d__set_error(1, buf);
+  } catch (...) {
+    std::snprintf(buf, sizeof buf, "%s: unexpected exception", who);
+    return evp1d__set_error(1, buf);
+  }
+}
 
 // std::vector whose resize leaves new elements uninitialised, for the large
 // planner tables that are written in full (in parallel) before any read: the

@@ -38,6 +38,7 @@
 
 #include "../../include/evp1d.h"
 #include "evp1d_point.hpp"
+#include "evp1d_tiles.hpp"
 #include "evp1d_xcd.hpp"
 
 extern "C" int evp1d__set_error(int code, const char *msg);
@@ -437,9 +438,14 @@ int build_tiles2(evp2d_handle *h) {
   return EVP1D_OK;
 }
 
-// upload the concatenated lists (and the halo tables) when they changed
+// upload the concatenated lists (and the halo tables) when they changed;
+// a host allocation failure while building them returns EVP1D_ENOMEM
+int ensure2_impl(evp2d_handle *h);
 int ensure2(evp2d_handle *h) {
   if (!h->cells_dirty) return EVP1D_OK;
+  return evp1d::no_throw("evp2d lists", [h] { return ensure2_impl(h); });
+}
+int ensure2_impl(evp2d_handle *h) {
   HIP2(hipStreamSynchronize(h->stream));
   drop_graph2(h);
   free_lists2(h);

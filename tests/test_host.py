@@ -210,3 +210,47 @@ def test_product_pack_equals_reference_pack():
     ocean = g.tmask
     for k in ("uvel", "stressp_2", "forcexU"):
         assert np.array_equal(back[k][ocean], g.fields[k][ocean]), k
+
+
+_OOM_CHILD = r"""
+import ctypes, resource, sys
+# blocks above 128 KiB from mmap, returned to the system when freed, so the
+# mapped size below is what the process really holds (M_MMAP_THRESHOLD = -3)
+ctypes.CDLL("libc.so.6").mallopt(-3, 128 * 1024)
+sys.path[:0] = sys.argv[1:3]
+import numpy as np
+import evp1d
+from evp1d import synth
+def lists(nx, ny):
+    g = synth.make_grid(nx, ny, 5)
+    ix = evp1d.build_index(g.tmask, g.nghost)
+    return ix, np.zeros(int(ix["na"]), np.int32), int(ix["navel"])
+small, big = lists(60, 50), lists(2000, 1600)
+evp1d.tile_plan(small[0], small[1], small[1], small[2])   # library loaded, planner works
+ix, skip, navel = big
+# the address space from now on: what is mapped plus a little, so the
+# planner's tables for the large lists (hundreds of MB) cannot be allocated
+with open("/proc/self/statm") as fh:
+    vm = int(fh.read().split()[0]) * resource.getpagesize()
+resource.setrlimit(resource.RLIMIT_AS, (vm + (8 << 20), resource.RLIM_INFINITY))
+try:
+    evp1d.tile_plan(ix, skip, skip, navel)
+    print("NO-ERROR")
+except evp1d.Evp1dError as e:
+    print("CODE", e.code, str(e)[:120])
+"""
+
+
+def test_planner_host_allocation_failure_is_a_status_code(tmp_path):
+    """A host allocation failure inside the tile planner (on its worker
+    threads or the caller) comes back as EVP1D_ENOMEM through the C ABI; it
+    does not terminate the process.  The child process caps its own address
+    space after planning small lists, then plans large ones."""
+    import subprocess
+    import sys
+    from pathlib import Path
+    repo = Path(__file__).resolve().parent.parent
+    r = subprocess.run([sys.executable, "-c", _OOM_CHILD, str(repo / "cice-evp1d_amd"), str(repo)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, (r.returncode, r.stdout[-500:], r.stderr[-1500:])
+    assert "CODE 4" in r.stdout, r.stdout + r.stderr[-800:]
