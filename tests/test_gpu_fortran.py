@@ -7,6 +7,11 @@
   dyn_evp1d_finalize on a 2D state; its 2D results must equal the oracle run
   through the reference's own 1D construction and packing
   (oracle.index_reference / pack_dyn, integration/ice_dyn_evp1d.F90).
+
+The drivers run as child processes.  Their persistent mode makes a
+cooperative launch, which needs the GPU's global wave-sync resource; one
+process holds it at a time, so no test module that runs before this one may
+launch the persistent kernel in the test process itself.
 """
 import json
 import subprocess

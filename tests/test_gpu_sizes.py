@@ -18,7 +18,12 @@
 
 The regenerated inputs are checked first against the md5 of the input files
 the reference ran on, so the 1D list construction and packing are pinned at
-these sizes too.  Reference call sites: src/evp_v2b.F90:91-110 (loop),
+these sizes too.
+
+This module runs the persistent (cooperative) kernel in the test process.  A
+cooperative launch takes the GPU's global wave-sync resource, which one
+process holds at a time, so the module is named to run after the tests that
+start the Fortran drivers (test_gpu_fortran.py) as child processes.  Reference call sites: src/evp_v2b.F90:91-110 (loop),
 src/vars.F90:2101-2135 and 2233-2304 (input and output files).
 """
 from __future__ import annotations
