@@ -61,6 +61,7 @@ using evp1d::Scalars;
 static thread_local int g_err_code = 0;
 static thread_local char g_err_msg[512] = "";
 
+static int scratch_setup(int dev, bool quiet);   // evp1d_loss_noalloc's scratch address
 static int fail(int code, const char *fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
@@ -1778,6 +1779,10 @@ static int create_impl(int32_t nmem, const int32_t *mna, const int32_t *mnavel, 
     if (e != hipSuccess && rc == EVP1D_OK) rc = fail(EVP1D_EHIP, "%s: %s", what, hipGetErrorString(e));
   };
   chk(hipGetDevice(&h->device), "hipGetDevice");
+  // this device's address of the fixed scratch of evp1d_loss_noalloc (looked
+  // up once per device, the device being current); evp1d_loss_noalloc
+  // retries and reports a failure
+  if (!rc) (void)scratch_setup(h->device, true);
   chk(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
   chk(hipEventCreate(&h->ev0), "hipEventCreate");
   chk(hipEventCreate(&h->ev1), "hipEventCreate");
@@ -3016,6 +3021,108 @@ int evp1d_loss(evp1d_handle *h, const double *u_target, const double *v_target, 
   (void)hipFree(buf);
   if (rc) return rc;
   for (double x : t) *loss += x;
+  return EVP1D_OK;
+}
+
+// ---------------------------------------------------------------------------
+// evp1d_loss_noalloc: evp1d_loss with a fixed device scratch array instead of
+// per-call allocations.  `scratch` holds the two targets in caller order
+// (kLossMaxPoints doubles each), then one partial sum per block of the
+// reduction, then the result.  Host code never passes `scratch` itself to a
+// kernel or a copy: each device's own address of it comes from
+// hipGetSymbolAddress (scratch_ptr) and only that address is used.  Every call
+// on a device shares that device's array; calls that overlap on one device
+// race.
+// ---------------------------------------------------------------------------
+constexpr long kLossMaxPoints = 1L << 22;   // 4,194,304 points: 64 MiB of targets
+constexpr int kLossBlock = 256;
+constexpr long kLossMaxBlocks = (kLossMaxPoints + kLossBlock - 1) / kLossBlock;
+constexpr long kLossScratch = 2 * kLossMaxPoints + kLossMaxBlocks + 1;
+static __device__ __align__(256) double scratch[kLossScratch];
+
+constexpr int kMaxDevices = 64;
+static double *g_scratch_dev[kMaxDevices];   // per device: the address of its `scratch`
+
+// Look up the current device's address of `scratch` once (the caller has
+// made device `dev` current).  quiet: leave the error message alone.
+static int scratch_setup(int dev, bool quiet) {
+  if (dev < 0 || dev >= kMaxDevices)
+    return quiet ? EVP1D_EINVAL : fail(EVP1D_EINVAL, "scratch: device %d out of range", dev);
+  if (__atomic_load_n(&g_scratch_dev[dev], __ATOMIC_ACQUIRE)) return EVP1D_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return quiet ? EVP1D_EHIP
+                 : fail(EVP1D_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+  __atomic_store_n(&g_scratch_dev[dev], static_cast<double *>(p), __ATOMIC_RELEASE);
+  return EVP1D_OK;
+}
+
+// per block: the sum of the loss terms of its slots, in a fixed tree order;
+// the target of slot s is the caller's element perm[s]
+__global__ __launch_bounds__(kLossBlock) void k_loss_partial(DevView d, const int32_t *perm, const double *ut,
+                                                              const double *vt, double *partial) {
+  __shared__ double red[kLossBlock];
+  const int s = blockIdx.x * kLossBlock + threadIdx.x;
+  double t = 0.0;
+  if (s >= d.u_begin && s < d.u_end && s < d.na) {
+    const double2 v = d.vel[s];
+    const int q = perm[s];
+    const double ex = v.x - ut[q], ey = v.y - vt[q];
+    t = 0.5 * (ex * ex + ey * ey);
+  }
+  red[threadIdx.x] = t;
+  __syncthreads();
+  for (int w = kLossBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// one block: out[0] = the sum of partial[0..n) (each thread a strided run,
+// then a tree)
+__global__ __launch_bounds__(kLossBlock) void k_loss_final(const double *partial, long n, double *out) {
+  __shared__ double red[kLossBlock];
+  double t = 0.0;
+  for (long k = threadIdx.x; k < n; k += kLossBlock) t += partial[k];
+  red[threadIdx.x] = t;
+  __syncthreads();
+  for (int w = kLossBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+int evp1d_loss_noalloc(evp1d_handle *h, const double *u_target, const double *v_target, double *loss) {
+  if (!h || !loss) return fail(EVP1D_EINVAL, "evp1d_loss_noalloc: NULL argument");
+  if (h->na > kLossMaxPoints)
+    return fail(EVP1D_EINVAL, "evp1d_loss_noalloc: na = %d exceeds the fixed scratch of %ld points; use evp1d_loss",
+                h->na, kLossMaxPoints);
+  if (h->na > 0 && (!u_target || !v_target)) return fail(EVP1D_EINVAL, "evp1d_loss_noalloc: NULL target");
+  if (!ready(h)) return fail(EVP1D_ESTATE, "evp1d_loss_noalloc: neighbour lists not set");
+  // bringing the layout up to date may allocate: that is left to the other calls
+  if (h->layout_dirty || h->idx_dirty || h->perm_dev_dirty)
+    return fail(EVP1D_ESTATE, "evp1d_loss_noalloc: the slot layout is out of date; run evp1d_run or evp1d_loss "
+                              "once after changing lists or skip flags");
+  *loss = 0.0;
+  if (h->na == 0) return EVP1D_OK;
+  RC_TRY(scratch_setup(h->device, false));
+  double *sp = __atomic_load_n(&g_scratch_dev[h->device], __ATOMIC_ACQUIRE);
+  double *ut = sp, *vt = sp + kLossMaxPoints, *partial = sp + 2 * kLossMaxPoints;
+  double *out = partial + kLossMaxBlocks;
+  const size_t bytes = (size_t)h->na * sizeof(double);
+  HIP_TRY(hipMemcpyAsync(ut, u_target, bytes, hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(hipMemcpyAsync(vt, v_target, bytes, hipMemcpyHostToDevice, h->stream));
+  const long nb = (h->na + kLossBlock - 1) / kLossBlock;
+  hipLaunchKernelGGL(k_loss_partial, dim3((unsigned)nb), dim3(kLossBlock), 0, h->stream, h->v,
+                     (const int32_t *)h->perm_dev, (const double *)ut, (const double *)vt, partial);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(kLossBlock), 0, h->stream, (const double *)partial, nb, out);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(loss, out, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
   return EVP1D_OK;
 }
 

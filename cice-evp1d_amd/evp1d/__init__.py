@@ -54,7 +54,7 @@ EXPORTS = (
     "evp1d_create_batch", "evp1d_get_members", "evp1d_member_info", "evp1d_set_field_member",
     "evp1d_get_field_member", "evp1d_set_index_member", "evp1d_set_skip_member",
     "evp1d_autotune", "evp1d_set_tuning", "evp1d_get_tuning",
-    "evp1d_adjoint_run", "evp1d_loss", "evp1d_tangent_run", "evp1d_set_adjoint_segment",
+    "evp1d_adjoint_run", "evp1d_loss", "evp1d_loss_noalloc", "evp1d_tangent_run", "evp1d_set_adjoint_segment",
     "evp1d_get_adjoint_segment",
     "evp1d_set_tiling", "evp1d_get_tiling", "evp1d_tile_plan",
     "evp1d_set_fused_inplace", "evp1d_get_fused_inplace", "evp1d_autotune_fused",
@@ -145,6 +145,7 @@ def load(fma: bool = False, path: str | os.PathLike | None = None) -> ctypes.CDL
         "evp1d_get_tuning": [_H, _IP, _IP, _IP, _IP],
         "evp1d_adjoint_run": [_H, i32, i32, _DP, _DP, _DP, _DP, _DP, _DP, _DP],
         "evp1d_loss": [_H, _DP, _DP, _DP],
+        "evp1d_loss_noalloc": [_H, _DP, _DP, _DP],
         "evp1d_set_adjoint_segment": [_H, i32],
         "evp1d_set_tile_order": [_H, i32],
         "evp1d_get_tile_order": [_H, _IP],
@@ -342,6 +343,15 @@ class Engine:
         vt = np.ascontiguousarray(v_target, dtype=np.float64)
         out = ctypes.c_double()
         _check(self.L, self.L.evp1d_loss(self.h, _dp(ut), _dp(vt), ctypes.byref(out)))
+        return out.value
+
+    def loss_noalloc(self, u_target, v_target) -> float:
+        """evp1d_loss_noalloc: the same L without allocations (fixed device
+        scratch, na <= 4,194,304, slot layout already up to date)."""
+        ut = np.ascontiguousarray(u_target, dtype=np.float64)
+        vt = np.ascontiguousarray(v_target, dtype=np.float64)
+        out = ctypes.c_double()
+        _check(self.L, self.L.evp1d_loss_noalloc(self.h, _dp(ut), _dp(vt), ctypes.byref(out)))
         return out.value
 
     def adjoint_run(self, nsub: int, u_target, v_target, nsnap: int = 0) -> dict:

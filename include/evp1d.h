@@ -310,6 +310,18 @@ int evp1d_adjoint_run(evp1d_handle *h, int32_t nsub, int32_t nsnap, const double
                       const double *v_target, double *loss, double *g_strength, double *g_cdn_ocn,
                       double *g_forcex, double *g_forcey);
 int evp1d_loss(evp1d_handle *h, const double *u_target, const double *v_target, double *loss);
+/* evp1d_loss without allocating (no counterpart in the reference): for
+ * callers that cannot allocate device or host memory, such as tight
+ * optimisation or serving loops.  The targets are staged in a fixed device
+ * array of the library (64 MiB per GPU it runs on, held until the process
+ * ends), so na may be at most 4,194,304 (EVP1D_EINVAL beyond).  The handle's
+ * slot layout must be up to date (EVP1D_ESTATE after lists or skip flags
+ * changed until evp1d_run, evp1d_loss or another call that uses the layout
+ * has run).  L is summed in a fixed tree order on the device, so it can
+ * differ from evp1d_loss's serial sum in the last bits.  All calls on one GPU
+ * share the fixed array: calls that overlap on one GPU (other host threads or
+ * handles) race.                                                            */
+int evp1d_loss_noalloc(evp1d_handle *h, const double *u_target, const double *v_target, double *loss);
 /* Checkpointing unit of evp1d_adjoint_run when 0 < nsnap < nsub (no
  * counterpart in the reference).  The run stores full states (stresses and
  * velocities) at unit starts by binomial checkpointing over units of seg
