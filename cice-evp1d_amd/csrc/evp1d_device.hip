@@ -2348,20 +2348,29 @@ int evp1d_autotune_fused(evp1d_handle *h, int32_t nsub, float *us) {
   RC_TRY(ensure(h));
   if (!h->fused_ok)
     return fail(EVP1D_ESTATE, "evp1d_autotune_fused: fused form unavailable (%s)", h->fused_why.c_str());
-  // depth 1 with both stress storages, then the pair form (two buffers) in
-  // both tile orders (rows, Hilbert curve)
-  struct Cand { int w, h, block, depth; };
-  static const Cand cands[8] = {{64, 16, 512, 1}, {32, 16, 512, 1}, {32, 8, 256, 1}, {16, 8, 128, 1},
-                                {31, 15, 512, 2}, {62, 15, 512, 2}, {15, 15, 256, 2}, {62, 7, 512, 2}};
+  // depth 1 with both stress storages (tiles in rows), then the pair form
+  // (two buffers), mostly along the Hilbert curve.  The pair form's
+  // workgroup holds the largest number of level-2 stress points of any tile
+  // (256, 512 or 1024 threads), so the fastest pair tiles are the largest
+  // whose busiest tile still fits a workgroup size: on the synthetic ice
+  // fields 31x19, 27x23, 39x15 and 33x17 for 512 threads (a tile's (W+1) x
+  // (H+1) bound exceeds 512, its ice-covered points do not), 62x19 and 62x15
+  // for 1024, 15x19 and 23x11 for 256; 31x15 and 15x15 fit by their bound.
+  struct Cand { int w, h, block, depth, order; };
+  static const Cand cands[] = {
+      {64, 16, 512, 1, 0}, {32, 16, 512, 1, 0}, {32, 8, 256, 1, 0}, {16, 8, 128, 1, 0},
+      {31, 19, 512, 2, 1}, {27, 23, 512, 2, 1}, {39, 15, 512, 2, 1}, {33, 17, 512, 2, 1}, {31, 15, 512, 2, 1},
+      {62, 19, 512, 2, 1}, {62, 15, 512, 2, 1}, {15, 19, 256, 2, 1}, {23, 11, 256, 2, 1}, {15, 15, 256, 2, 1},
+      {62, 7, 512, 2, 1}, {31, 19, 512, 2, 0}, {31, 15, 512, 2, 0}, {15, 15, 256, 2, 0}};
+  constexpr int ncand = (int)(sizeof(cands) / sizeof(cands[0]));
   const bool keep_auto = h->tiling_auto;
   const int keep_w = h->tile_w, keep_h = h->tile_h, keep_b = h->tile_block, keep_x = h->tile_xcd,
             keep_ip = h->fused_inplace, keep_d = h->fused_depth, keep_o = h->tile_order;
   float best = 1e30f;
   int win = -1, win_ip = 0, win_o = 0;
   int rc = EVP1D_OK;
-  for (int cc = 0; cc < 12 && !rc; ++cc) {
-    const int c = cc < 8 ? cc : 4 + (cc - 8);             // 8..11: the pair tiles again, Hilbert order
-    const int order = cc < 8 ? 0 : 1;
+  for (int c = 0; c < ncand && !rc; ++c) {
+    const int order = cands[c].order;
     rc = evp1d_set_fused_depth(h, cands[c].depth);
     if (!rc) rc = evp1d_set_tile_order(h, order);
     if (!rc) rc = evp1d_set_tiling(h, cands[c].w, cands[c].h, cands[c].block, 1);

@@ -80,13 +80,40 @@ struct StressOutT {
 };
 using StressOut = StressOutT<double>;
 
-// visc_replpress (bench_v2b.F90:415-437)
+// The second term of tmpcalc in visc_replpress,
+// (c1 - capping) * (strength / (Delta + DminArea)), without its division
+// where that gives the same IEEE value.  `fast` (capping_fast, once per
+// point) holds when capping = 1, DminArea > 0 and strength is finite with
+// |strength| <= DminArea * 2^1000.  The term is then +0 * q.  Delta is a
+// square root, so unless it is a NaN it is +-0, positive or +inf; the
+// divisor is then at least DminArea > 0 and q is finite (|q| <= 2^1000) with
+// strength's sign, so +0 * q is the zero with strength's sign.  A NaN Delta
+// (or any other case) takes the division.  The tangent-linear type always
+// takes the division.
+template <class R>
+__host__ __device__ inline bool capping_fast(const Scalars &, R, R) {
+  return false;
+}
+__host__ __device__ inline bool capping_fast(const Scalars &s, double strength, double DminArea) {
+  return s.capping == c1 && DminArea > 0.0 && fabs(strength) <= fmin(__DBL_MAX__, DminArea * 0x1p1000);
+}
+template <class R>
+__host__ __device__ inline R capping_term(const Scalars &s, bool, R strength, R DminArea, R Delta) {
+  return (c1 - s.capping) * (strength / (Delta + DminArea));
+}
+__host__ __device__ inline double capping_term(const Scalars &s, bool fast, double strength, double DminArea,
+                                               double Delta) {
+  if (fast && Delta == Delta) return copysign(0.0, strength);
+  return (c1 - s.capping) * (strength / (Delta + DminArea));
+}
+
+// visc_replpress (bench_v2b.F90:415-437); fast: capping_fast(s, strength, DminArea)
 template <class R>
 __host__ __device__ inline void visc_replpress(const Scalars &s, R strength, R DminArea,
                                                R Delta, R &zetax2, R &etax2,
-                                               R &rep_prs) {
+                                               R &rep_prs, bool fast = false) {
   const R dmax = (Delta > DminArea) ? Delta : DminArea;   // max(Delta, DminArea)
-  const R tmpcalc = s.capping * (strength / dmax) + (c1 - s.capping) * (strength / (Delta + DminArea));
+  const R tmpcalc = s.capping * (strength / dmax) + capping_term(s, fast, strength, DminArea, Delta);
   zetax2 = (c1 + s.Ktens) * tmpcalc;
   rep_prs = (c1 - s.Ktens) * tmpcalc * Delta;
   etax2 = s.epp2i * zetax2;
@@ -131,10 +158,11 @@ __host__ __device__ inline void stress_core(const Scalars &s,
 
   R zetax2ne, etax2ne, rep_prsne, zetax2nw, etax2nw, rep_prsnw;
   R zetax2sw, etax2sw, rep_prssw, zetax2se, etax2se, rep_prsse;
-  visc_replpress(s, strength, DminTarea, Deltane, zetax2ne, etax2ne, rep_prsne);
-  visc_replpress(s, strength, DminTarea, Deltanw, zetax2nw, etax2nw, rep_prsnw);
-  visc_replpress(s, strength, DminTarea, Deltasw, zetax2sw, etax2sw, rep_prssw);
-  visc_replpress(s, strength, DminTarea, Deltase, zetax2se, etax2se, rep_prsse);
+  const bool fast = capping_fast(s, strength, DminTarea);
+  visc_replpress(s, strength, DminTarea, Deltane, zetax2ne, etax2ne, rep_prsne, fast);
+  visc_replpress(s, strength, DminTarea, Deltanw, zetax2nw, etax2nw, rep_prsnw, fast);
+  visc_replpress(s, strength, DminTarea, Deltasw, zetax2sw, etax2sw, rep_prssw, fast);
+  visc_replpress(s, strength, DminTarea, Deltase, zetax2se, etax2se, rep_prsse, fast);
 
   // stress relaxation (bench_v2b.F90:187-212)
   const double keep = c1 - s.arlx1i * s.revp;

@@ -261,8 +261,10 @@ int evp1d_get_fused_depth(evp1d_handle *h, int32_t *depth, int32_t *active, int3
 /* Times one fused run of nsub subcycles (after an untimed one) for each tile
  * of 64x16/512, 32x16/512, 32x8/256, 16x8/128 (XCD-aware order) that fits
  * the lists, each with both stress storages, and the pair form on tiles of
- * 31x15, 62x15, 15x15 and 62x7 in both tile orders, on the handle's own data
- * (the state is saved and restored), keeps the fastest (as evp1d_set_tiling,
+ * 31x19, 27x23, 39x15, 33x17, 31x15, 62x19, 62x15, 15x19, 23x11, 15x15 and
+ * 62x7 along the Hilbert curve and 31x19, 31x15, 15x15 in rows of tiles, on
+ * the handle's own data (the state is saved and restored), keeps the
+ * fastest (as evp1d_set_tiling,
  * evp1d_set_fused_inplace, evp1d_set_fused_depth and evp1d_set_tile_order
  * would) and reports its time per subcycle in microseconds.                 */
 int evp1d_autotune_fused(evp1d_handle *h, int32_t nsub, float *us);

@@ -254,3 +254,19 @@ def test_planner_host_allocation_failure_is_a_status_code(tmp_path):
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, (r.returncode, r.stdout[-500:], r.stderr[-1500:])
     assert "CODE 4" in r.stdout, r.stdout + r.stderr[-800:]
+
+
+def test_capping_fast_path_is_bit_identical(tmp_path):
+    """visc_replpress skips the division of its (1 - capping) term where the
+    term is a signed zero; tests/cpp/capping_term_check.cpp compares every
+    output bit with the division on edge values (signed zeros, subnormals,
+    huge, infinities, NaN) and random values, for capping 1, 0.5, 0 and 2."""
+    import subprocess
+    from pathlib import Path
+    repo = Path(__file__).resolve().parent.parent
+    exe = tmp_path / "capping_term_check"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                    f"-I{repo / 'cice-evp1d_amd' / 'csrc'}", str(repo / "tests" / "cpp" / "capping_term_check.cpp"),
+                    "-o", str(exe)], check=True, capture_output=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.startswith("ok"), r.stdout + r.stderr
