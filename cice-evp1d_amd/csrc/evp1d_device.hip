@@ -437,6 +437,12 @@ __global__ void k_pack_const(DevView d, double2 *pc, long lp) {
 // threads then fill 16 waves per CU.  TRAJ: the gradient's forward sweep,
 // which also stores both subcycles' stepu-slot velocities (f.tr_mid,
 // f.tr_end); the values computed are the same.
+//
+// Each barrier waits for LDS only, so global loads issued before it stay in
+// flight across it: every thread loads the memory inputs of its first entry
+// of the next phase (level-1 stepu, level-2 stress, level-2 stepu) before
+// the barrier that ends the current phase, and that phase starts on values
+// already in registers instead of one memory round trip after the barrier.
 template <int B, bool X, bool TRAJ = false>
 __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_pair(DevView d, PairView f, int last) {
   extern __shared__ double2 lds[];
@@ -459,16 +465,27 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
   };
   const double2 *pc = f.pc;
   const long lp = f.lp;
-  auto stepu = [&](int k, int q, double2 v, evp1d::StepuOut &o) {
-    const uint2 w = f.uloc[pt.u0 + k];
-    const double2 a = lds[w.x & 0xffffu];
-    const double2 b = lds[cap + (w.x >> 16)];
-    const double2 c = lds[2 * cap + (w.y & 0xffffu)];
-    const double2 e = lds[3 * cap + (w.y >> 16)];
-    const double2 f0 = pc[3 * lp + q], f1 = pc[4 * lp + q], f2 = pc[5 * lp + q], f3 = pc[6 * lp + q];
-    const double2 f4 = pc[7 * lp + q], f5 = pc[8 * lp + q], f6 = pc[9 * lp + q];
-    evp1d::stepu_point(d.s, v.x, v.y, f0.x, f0.y, f1.x, f1.y, f2.x, f2.y, f3.x, f3.y, f4.x, f4.y, f5.x, f6.x, f6.y,
-                       f5.y, a.x, b.x, c.x, e.x, a.y, c.y, b.y, e.y, o);
+  // memory inputs of one stepu evaluation: its str table entries and the
+  // seven packed constant pairs of slot q
+  struct UIn {
+    uint2 w;
+    double2 c[7];
+  };
+  auto uload = [&](int k, int q) {
+    UIn u;
+    u.w = f.uloc[pt.u0 + k];
+#pragma unroll
+    for (int c = 0; c < 7; ++c) u.c[c] = pc[(3 + c) * lp + q];
+    return u;
+  };
+  auto stepu = [&](const UIn &u, double2 v, evp1d::StepuOut &o) {
+    const double2 a = lds[u.w.x & 0xffffu];
+    const double2 b = lds[cap + (u.w.x >> 16)];
+    const double2 c = lds[2 * cap + (u.w.y & 0xffffu)];
+    const double2 e = lds[3 * cap + (u.w.y >> 16)];
+    const double2 *k = u.c;
+    evp1d::stepu_point(d.s, v.x, v.y, k[0].x, k[0].y, k[1].x, k[1].y, k[2].x, k[2].y, k[3].x, k[3].y, k[4].x,
+                       k[4].y, k[5].x, k[6].x, k[6].y, k[5].y, a.x, b.x, c.x, e.x, a.y, c.y, b.y, e.y, o);
   };
   // stress at slot g from the velocities cc, ee, ne, se and the stresses sig
   auto stress = [&](int g, double2 cc, double2 ve, double2 vn, double2 vs, const double *sig, evp1d::StressOut &o) {
@@ -522,28 +539,65 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
   // so no other evaluation overlaps the lifetime of its stresses
   for (int k = me + B; k < nE; k += B) level1(k);
   if (me < nE) level1(me);
+  // level 1, stepu, entry me: inputs loaded before the barrier
+  int q1 = 0;
+  UIn u1;
+  double2 v1;
+  if (me < nU) {
+    q1 = uslot(me);
+    u1 = uload(me, q1);
+    v1 = d.vel[q1];
+  }
   __syncthreads();
   // level 1, stepu: every U1 entry into the velocity table
-  for (int k = me; k < nU; k += B) {
-    const int q = uslot(k);
+  if (me < nU) {
     evp1d::StepuOut o;
-    stepu(k, q, d.vel[q], o);
+    stepu(u1, v1, o);
+    vt[me] = make_double2(o.uvel, o.vvel);
+    if (TRAJ && me < nUo) f.tr_mid[q1 - d.u_begin] = vt[me];
+  }
+  for (int k = me + B; k < nU; k += B) {
+    const int q = uslot(k);
+    const UIn u = uload(k, q);
+    evp1d::StepuOut o;
+    stepu(u, d.vel[q], o);
     vt[k] = make_double2(o.uvel, o.vvel);
     if (TRAJ && k < nUo) f.tr_mid[q - d.u_begin] = vt[k];
+  }
+  // level 2, stress of entry me: memory inputs loaded before the barrier
+  // (velocities that are no U1 entry never change, kNoEntry: from memory)
+  constexpr unsigned kNo = evp1d::PairPlan::kNoEntry;
+  int g2 = 0;
+  uint2 w2 = make_uint2(0u, 0u);
+  double2 c2[4], m2[3];
+  double st2 = 0.0;
+  if (me < nS) {
+    g2 = tslot(me);
+    w2 = f.vloc[pt.v0 + me];
+    m2[0] = pc[g2];
+    m2[1] = pc[lp + g2];
+    m2[2] = pc[2 * lp + g2];
+    st2 = d.strength[g2];
+    c2[0] = d.vel[g2];
+    if ((w2.x >> 16) == kNo || (w2.y & 0xffffu) == kNo || (w2.y >> 16) == kNo) {
+      const int4 n = f.nbr[g2];
+      c2[1] = d.vel[n.x];
+      c2[2] = d.vel[n.y];
+      c2[3] = d.vel[n.z];
+    }
   }
   __syncthreads();
   // level 2, stress: owned and halo entries, one per thread
   if (me < nS) {
-    const int g = tslot(me);
-    const uint2 w = f.vloc[pt.v0 + me];
-    const unsigned e0 = w.x & 0xffffu, e1 = w.x >> 16, e2 = w.y & 0xffffu, e3 = w.y >> 16;
-    constexpr unsigned kNo = evp1d::PairPlan::kNoEntry;
-    const double2 cc = e0 == kNo ? d.vel[g] : vt[e0];
-    const double2 ve = e1 == kNo ? d.vel[f.nbr[g].x] : vt[e1];
-    const double2 vn = e2 == kNo ? d.vel[f.nbr[g].y] : vt[e2];
-    const double2 vs = e3 == kNo ? d.vel[f.nbr[g].z] : vt[e3];
+    const unsigned e0 = w2.x & 0xffffu, e1 = w2.x >> 16, e2 = w2.y & 0xffffu, e3 = w2.y >> 16;
+    const double2 cc = e0 == kNo ? c2[0] : vt[e0];
+    const double2 ve = e1 == kNo ? c2[1] : vt[e1];
+    const double2 vn = e2 == kNo ? c2[2] : vt[e2];
+    const double2 vs = e3 == kNo ? c2[3] : vt[e3];
     evp1d::StressOut o;
-    stress(g, cc, ve, vn, vs, sg, o);
+    evp1d::stress_point(d.s, cc.x, cc.y, ve.x, ve.y, vs.x, vs.y, vn.x, vn.y, m2[0].x, m2[0].y, st2, m2[1].x,
+                        m2[1].y, m2[2].x, m2[2].y, sg, o);
+    const int g = g2;
     if (me < nO) {
       f.sg_out[g] = make_double2(o.sp[0], o.sp[1]);
       f.sg_out[f.ls + g] = make_double2(o.sp[2], o.sp[3]);
@@ -560,12 +614,18 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
     }
     put(me, o);
   }
+  // level 2, stepu of entry me: inputs loaded before the barrier
+  int q2 = 0;
+  UIn u2;
+  if (me < nUo) {
+    q2 = uslot(me);
+    u2 = uload(me, q2);
+  }
   __syncthreads();
   // level 2, stepu: the owned U points
-  for (int k = me; k < nUo; k += B) {
-    const int q = uslot(k);
+  auto level2u = [&](int k, int q, const UIn &u) {
     evp1d::StepuOut o;
-    stepu(k, q, vt[k], o);
+    stepu(u, vt[k], o);
     f.vel_out[q] = make_double2(o.uvel, o.vvel);
     if (TRAJ && f.tr_end) f.tr_end[q - d.u_begin] = make_double2(o.uvel, o.vvel);
     if (last) {
@@ -573,6 +633,11 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
       d.strinty[q] = o.strinty;
       d.cb[q] = o.Cb;
     }
+  };
+  if (me < nUo) level2u(me, q2, u2);
+  for (int k = me + B; k < nUo; k += B) {
+    const int q = uslot(k);
+    level2u(k, q, uload(k, q));
   }
 }
 
