@@ -365,9 +365,11 @@ __global__ __launch_bounds__(B) void k_fused(DevView d, FusedView f, int last) {
 // is thread k (the plan guarantees at most B of them), so its level-1
 // stresses stay in that thread's registers.  The 20 constant fields come
 // from a copy packed in pairs at the start of the run (k_pack_const), the
-// stresses live during the run in pairs too (k_pack_sig / k_unpack_sig), and
-// ee, ne, se of each stress point in one int4 (k_pack_nbr): one 16-byte load
-// per pair, since the kernel's time follows its load instructions.
+// stresses live during the run in pairs too (k_pack_sig / k_unpack_sig): one
+// 16-byte load per pair, since the kernel's time follows its load
+// instructions.  Each level-1 stress entry's slot and ee, ne, se neighbour
+// slots come from one per-tile int4 (PairPlan::tnbr), so a halo or T1-only
+// entry reaches its velocities after two dependent loads, not three.
 struct PairView {
   const evp1d::PairTile *tiles;
   const int32_t *hlist, *xlist, *elist;
@@ -383,7 +385,7 @@ struct PairView {
   const double2 *sg_in;     // [6][ls]: stresses in pairs (p1 p2, p3 p4, m1 m2, m3 m4, 12_1 12_2, 12_3 12_4)
   double2 *sg_out;
   long ls;
-  const int4 *nbr;          // per stress slot: ee, ne, se
+  const int4 *tnbr;         // per level-1 stress entry (PairTile::n0 + k): slot, ee, ne, se
   // gradient trajectory (k_pair<.., TRAJ>): the velocities of the stepu
   // slots after the launch's first subcycle (tr_mid) and after its second
   // (tr_end, null when the handle's own buffer is the only copy wanted),
@@ -498,8 +500,8 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
   // level 1, stress (or constants) of entry k
   auto level1 = [&](int k) {
     if (k < nT) {
-      const int g = tslot(k);
-      const int4 n = f.nbr[g];
+      const int4 n = f.tnbr[pt.n0 + k];   // slot and ee, ne, se of the entry
+      const int g = k < nO ? tslot(k) : n.x;
       double sig[12];
 #pragma unroll
       for (int c = 0; c < 6; ++c) {
@@ -508,7 +510,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
         sig[2 * c + 1] = v.y;
       }
       evp1d::StressOut o;
-      stress(g, d.vel[g], d.vel[n.x], d.vel[n.y], d.vel[n.z], sig, o);
+      stress(g, d.vel[g], d.vel[n.y], d.vel[n.z], d.vel[n.w], sig, o);
       if (k == me) {
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
@@ -572,7 +574,8 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
   double2 c2[4], m2[3];
   double st2 = 0.0;
   if (me < nS) {
-    g2 = tslot(me);
+    const int4 n = f.tnbr[pt.n0 + me];
+    g2 = me < nO ? tslot(me) : n.x;
     w2 = f.vloc[pt.v0 + me];
     m2[0] = pc[g2];
     m2[1] = pc[lp + g2];
@@ -580,10 +583,9 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
     st2 = d.strength[g2];
     c2[0] = d.vel[g2];
     if ((w2.x >> 16) == kNo || (w2.y & 0xffffu) == kNo || (w2.y >> 16) == kNo) {
-      const int4 n = f.nbr[g2];
-      c2[1] = d.vel[n.x];
-      c2[2] = d.vel[n.y];
-      c2[3] = d.vel[n.z];
+      c2[1] = d.vel[n.y];
+      c2[2] = d.vel[n.z];
+      c2[3] = d.vel[n.w];
     }
   }
   __syncthreads();
@@ -1092,6 +1094,7 @@ struct evp1d_handle {
   evp1d::PairTile *ptiles_dev = nullptr;
   int32_t *pxlist_dev = nullptr, *pelist_dev = nullptr, *pklist_dev = nullptr;
   uint2 *puloc_dev = nullptr, *pvloc_dev = nullptr;
+  int4 *ptnbr_dev = nullptr;
   double2 *pcstr_dev = nullptr;
   double2 *pconst = nullptr;       // constant fields in pairs, [kPairConst][na], packed per run
   double *sig_pair = nullptr;      // stresses in pairs during a pair run (with sig_alt: two buffers)
@@ -1141,13 +1144,14 @@ static void drop_graph(evp1d_handle *h) {
 }
 
 static void free_pairs(evp1d_handle *h) {
-  void *bufs[7] = {h->ptiles_dev, h->pxlist_dev, h->pelist_dev, h->pklist_dev,
-                   h->puloc_dev,  h->pvloc_dev,  h->pcstr_dev};
+  void *bufs[8] = {h->ptiles_dev, h->pxlist_dev, h->pelist_dev, h->pklist_dev,
+                   h->puloc_dev,  h->pvloc_dev,  h->pcstr_dev,  h->ptnbr_dev};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   h->ptiles_dev = nullptr;
   h->pxlist_dev = h->pelist_dev = h->pklist_dev = nullptr;
   h->puloc_dev = h->pvloc_dev = nullptr;
+  h->ptnbr_dev = nullptr;
   h->pcstr_dev = nullptr;
   h->pair_ok = false;
   h->pair_block = h->pair_cap = h->pair_capv = 0;
@@ -1349,6 +1353,7 @@ static int install_pairs(evp1d_handle *h, const evp1d::TilePlan &plan, const int
   RC_TRY(upload_table(h, &h->pklist_dev, q.klist.data(), q.klist.size()));
   RC_TRY(upload_table(h, &h->puloc_dev, reinterpret_cast<const uint2 *>(q.uloc.data()), q.uloc.size() / 4));
   RC_TRY(upload_table(h, &h->pvloc_dev, reinterpret_cast<const uint2 *>(q.vloc.data()), q.vloc.size() / 4));
+  RC_TRY(upload_table(h, &h->ptnbr_dev, reinterpret_cast<const int4 *>(q.tnbr.data()), q.tnbr.size() / 4));
   h->pk_total = (long)q.klist.size();
   HIP_TRY(hipMalloc((void **)&h->pcstr_dev, std::max<size_t>((size_t)4 * h->pk_total, 1) * sizeof(double2)));
   HIP_TRY(hipStreamSynchronize(h->stream));
@@ -1578,7 +1583,6 @@ static int ensure_fused_buffers(evp1d_handle *h) {
     HIP_TRY(hipMalloc((void **)&h->pconst, (size_t)kPairConst * std::max(h->na, 1) * sizeof(double2)));
   if (pair_form(h) && !h->sig_pair)
     HIP_TRY(hipMalloc((void **)&h->sig_pair, (size_t)12 * h->ldsig * sizeof(double)));
-  if (pair_form(h) && !h->nbr4) HIP_TRY(hipMalloc((void **)&h->nbr4, (size_t)std::max(h->na, 1) * sizeof(int4)));
   return EVP1D_OK;
 }
 
@@ -1655,13 +1659,9 @@ static int issue_fused(evp1d_handle *h, int nsub, double2 *traj = nullptr) {
   const long t_end = h->v.t_end;
   PairView pv{h->ptiles_dev, h->hlist_dev, h->pxlist_dev, h->pelist_dev, h->puloc_dev, h->pvloc_dev,
               h->pcstr_dev,  h->cnz_dev,   h->pk_total,   nullptr,       h->pair_cap,  h->pconst,
-              (long)h->na,   nullptr,      nullptr,       h->ldsig,      h->nbr4};
+              (long)h->na,   nullptr,      nullptr,       h->ldsig,      h->ptnbr_dev};
   if (pair && h->na > 0) {
     hipLaunchKernelGGL(k_pack_const, dim3(blocks_for(h->na)), dim3(kBlock), 0, h->stream, X, h->pconst, (long)h->na);
-    HIP_TRY(hipGetLastError());
-  }
-  if (pair && t_end > 0) {
-    hipLaunchKernelGGL(k_pack_nbr, dim3(blocks_for((int)t_end)), dim3(kBlock), 0, h->stream, X, h->nbr4, t_end);
     HIP_TRY(hipGetLastError());
   }
   double2 *P[2] = {reinterpret_cast<double2 *>(h->sig_pair), reinterpret_cast<double2 *>(h->sig_alt)};

@@ -680,7 +680,7 @@ std::vector<int32_t> inverse_perm(const TilePlan &p, int na) {
 }
 
 struct PairWork {
-  std::vector<int32_t> x, k, e;
+  std::vector<int32_t> x, k, e, tn;
   std::vector<uint16_t> uloc, vloc;
   int ns = 0, ne_s = 0, nu = 0;
   std::string err;
@@ -753,6 +753,12 @@ PairPlan plan_pairs(const TilePlan &p, int na, int navel, const int32_t *const i
       }
     }
     const int nT = nS + (int)wk.x.size(), nE = nT + (int)wk.k.size();
+    wk.tn.resize((size_t)4 * nT);
+    for (int k = 0; k < nT; ++k) {
+      const int32_t g = k < nS ? tslot(k) : wk.x[(size_t)(k - nS)], q = p.perm[(size_t)g];
+      wk.tn[(size_t)4 * k] = g;
+      for (int c = 0; c < 3; ++c) wk.tn[(size_t)4 * k + 1 + c] = slot(idx[c][q]);
+    }
     wk.ns = nS;
     wk.ne_s = nE;
     wk.nu = nU;
@@ -781,7 +787,7 @@ PairPlan plan_pairs(const TilePlan &p, int na, int navel, const int32_t *const i
     }
   }
   r.tiles.resize((size_t)nt);
-  size_t nx = 0, nk = 0, ne = 0, nu = 0, nv = 0;
+  size_t nx = 0, nk = 0, ne = 0, nu = 0, nv = 0, nn = 0;
   for (int t = 0; t < nt; ++t) {
     const PairWork &wk = work[(size_t)t];
     PairTile &pt = r.tiles[(size_t)t];
@@ -794,6 +800,8 @@ PairPlan plan_pairs(const TilePlan &p, int na, int navel, const int32_t *const i
     pt.ne = (int32_t)wk.e.size();
     pt.u0 = (int32_t)nu;
     pt.v0 = (int32_t)nv;
+    pt.n0 = (int32_t)nn;
+    nn += wk.tn.size() / 4;
     nx += wk.x.size();
     nk += wk.k.size();
     ne += wk.e.size();
@@ -803,7 +811,7 @@ PairPlan plan_pairs(const TilePlan &p, int na, int navel, const int32_t *const i
     r.cap_v = std::max(r.cap_v, wk.nu);
     r.max_t2 = std::max(r.max_t2, wk.ns);
   }
-  if (nu > 0x7fffffff || nv > 0x7fffffff) {
+  if (nu > 0x7fffffff || nv > 0x7fffffff || nn > 0x7fffffff) {
     r.why = "tables too large";
     r.tiles.clear();
     return r;
@@ -813,6 +821,7 @@ PairPlan plan_pairs(const TilePlan &p, int na, int navel, const int32_t *const i
   r.elist.resize(ne);
   r.uloc.resize(4 * nu);
   r.vloc.resize(4 * nv);
+  r.tnbr.resize(4 * nn);
   par_items(nt, [&](int t, int) {
     const PairWork &wk = work[(size_t)t];
     const PairTile &pt = r.tiles[(size_t)t];
@@ -821,6 +830,7 @@ PairPlan plan_pairs(const TilePlan &p, int na, int navel, const int32_t *const i
     std::copy(wk.e.begin(), wk.e.end(), r.elist.begin() + pt.e0);
     std::copy(wk.uloc.begin(), wk.uloc.end(), r.uloc.begin() + (size_t)4 * pt.u0);
     std::copy(wk.vloc.begin(), wk.vloc.end(), r.vloc.begin() + (size_t)4 * pt.v0);
+    std::copy(wk.tn.begin(), wk.tn.end(), r.tnbr.begin() + (size_t)4 * pt.n0);
   });
   r.ok = true;
   return r;
@@ -853,6 +863,7 @@ std::string check_pairs(const PairPlan &q, const TilePlan &p, int na, int navel,
     if (pt.ne < 0 || pt.e0 < 0 || (size_t)pt.e0 + pt.ne > q.elist.size()) { e = "U1-only list range"; return; }
     if (pt.u0 < 0 || (size_t)4 * ((size_t)pt.u0 + nU) > q.uloc.size()) { e = "uloc range"; return; }
     if (pt.v0 < 0 || (size_t)4 * ((size_t)pt.v0 + nS) > q.vloc.size()) { e = "vloc range"; return; }
+    if (pt.n0 < 0 || (size_t)4 * ((size_t)pt.n0 + nT) > q.tnbr.size()) { e = "tnbr range"; return; }
     if (nS > q.max_t2 || nE > q.cap_s || nU > q.cap_v) { e = "tile exceeds the plan's limits"; return; }
     std::vector<int32_t> &sm = smaps[(size_t)w], &vm = vmaps[(size_t)w];
     sm.assign((size_t)nE, -1);
@@ -888,6 +899,15 @@ std::string check_pairs(const PairPlan &q, const TilePlan &p, int na, int navel,
           e = "stepu entry " + std::to_string(k) + " of tile " + std::to_string(t) + ": bad str entry";
           return;
         }
+      }
+    }
+    // every level-1 stress entry's slot and ee, ne, se neighbour slots
+    for (int k = 0; k < nT; ++k) {
+      const int32_t g = sm[(size_t)k], r0 = p.perm[(size_t)g];
+      const int32_t *tn = &q.tnbr[(size_t)4 * (pt.n0 + k)];
+      if (tn[0] != g || tn[1] != slot(idx[0][r0]) || tn[2] != slot(idx[1][r0]) || tn[3] != slot(idx[2][r0])) {
+        e = "stress entry " + std::to_string(k) + " of tile " + std::to_string(t) + ": bad neighbour slots";
+        return;
       }
     }
     // every level-2 stress read names the level-1 velocity of an active U

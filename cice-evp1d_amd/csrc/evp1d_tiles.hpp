@@ -162,6 +162,7 @@ struct PairTile {
   int32_t q0, nq;           // constant slots (str1..8 never written): PairPlan::klist[q0, q0 + nq)
   int32_t e0, ne;           // U1-only stepu slots: PairPlan::elist[e0, e0 + ne)
   int32_t u0, v0;           // first uloc entry (per U1 entry), first vloc entry (per T2 entry)
+  int32_t n0;               // first tnbr entry (per stress entry of level 1, T2 entries first)
 };
 
 struct PairPlan {
@@ -174,6 +175,10 @@ struct PairPlan {
   // 4 per T2 entry: velocity table entries of cc, ee, ne, se; kNoEntry when
   // that velocity never changes (not an active U point: read from memory)
   Table<uint16_t> vloc;
+  // 4 per level-1 stress entry (T2 then T1-only): its slot and the slots of
+  // its ee, ne, se neighbours, so one load gives every address the entry's
+  // stress gathers
+  Table<int32_t> tnbr;
   int cap_s = 0, cap_v = 0;           // largest str / velocity table
   int max_t2 = 0;                     // largest T2 (level-2 stresses, one per thread)
   static constexpr uint16_t kNoEntry = 0xffff;

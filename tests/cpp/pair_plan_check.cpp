@@ -97,9 +97,12 @@ int main(int argc, char **argv) {
       std::vector<u64> S(nE), sg(nS), V(nU);
       for (int k = 0; k < nE; ++k) {
         if (k < nT) {
-          const int g = tslot(k);
+          // as the kernel: the slot of owned entries by position, of the
+          // others and every entry's ee, ne, se from the tnbr table
+          const int32_t *tn = &q.tnbr[4 * ((size_t)pt.n0 + k)];
+          const int g = k < nO ? tslot(k) : tn[0];
           u64 s1;
-          tstress(sig[g], vel[g], vel[nb[0][g]], vel[nb[1][g]], vel[nb[2][g]], s1, S[k]);
+          tstress(sig[g], vel[g], vel[tn[1]], vel[tn[2]], vel[tn[3]], s1, S[k]);
           if (k < nS) sg[k] = s1;
         } else {
           S[k] = str[q.klist[pt.q0 + (k - nT)]];
@@ -152,6 +155,13 @@ int main(int argc, char **argv) {
          const int t = first_tile(q, [](const evp1d::PairTile &x) { return x.nx > 0 && x.t.b1 > x.t.b0; });
          const auto &d = q.tiles[t].t;
          q.uloc[4 * (size_t)q.tiles[t].u0 + 1] = (uint16_t)(d.a1 - d.a0 + d.b1 - d.b0 + d.nh);
+       }},
+      {"tnbr neighbour slot changed", [](evp1d::PairPlan &q) { q.tnbr[4 * 300 + 2] += 1; }},
+      {"tnbr entry slot changed", [&](evp1d::PairPlan &q) {
+         const int t = first_tile(q, [](const evp1d::PairTile &x) { return x.nx > 0; });
+         const auto &d = q.tiles[t].t;
+         const int k = d.a1 - d.a0 + d.b1 - d.b0 + d.nh;   // the first T1-only entry
+         q.tnbr[4 * ((size_t)q.tiles[t].n0 + k)] = q.tnbr[4 * ((size_t)q.tiles[t].n0 + k) + 1];
        }},
       {"T2 above the thread limit", [](evp1d::PairPlan &q) { q.max_t2 -= 1; }},
       {"T1-only range past the list", [](evp1d::PairPlan &q) { q.tiles.back().nx += 1000000; }},
