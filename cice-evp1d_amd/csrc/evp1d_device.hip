@@ -489,14 +489,12 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
     evp1d::stepu_point(d.s, v.x, v.y, k[0].x, k[0].y, k[1].x, k[1].y, k[2].x, k[2].y, k[3].x, k[3].y, k[4].x,
                        k[4].y, k[5].x, k[6].x, k[6].y, k[5].y, a.x, b.x, c.x, e.x, a.y, c.y, b.y, e.y, o);
   };
-  // stress at slot g from the velocities cc, ee, ne, se and the stresses sig
-  auto stress = [&](int g, double2 cc, double2 ve, double2 vn, double2 vs, const double *sig, evp1d::StressOut &o) {
-    const double2 m0 = pc[g], m1 = pc[lp + g], m2 = pc[2 * lp + g];
-    evp1d::stress_point(d.s, cc.x, cc.y, ve.x, ve.y, vs.x, vs.y, vn.x, vn.y, m0.x, m0.y, d.strength[g], m1.x, m1.y,
-                        m2.x, m2.y, sig, o);
-  };
   const int me = threadIdx.x;
   double sg[12];   // level-1 stresses of entry `me` when me < nS
+  // entry `me`'s metric pairs and strength, loaded at level 1 and used again
+  // at level 2 (the same slot)
+  double2 m2[3];
+  double st2 = 0.0;
   // level 1, stress (or constants) of entry k
   auto level1 = [&](int k) {
     if (k < nT) {
@@ -509,9 +507,17 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
         sig[2 * c] = v.x;
         sig[2 * c + 1] = v.y;
       }
+      const double2 a0 = pc[g], a1 = pc[lp + g], a2 = pc[2 * lp + g];
+      const double st = d.strength[g];
       evp1d::StressOut o;
-      stress(g, d.vel[g], d.vel[n.y], d.vel[n.z], d.vel[n.w], sig, o);
+      const double2 cc = d.vel[g], ve = d.vel[n.y], vn = d.vel[n.z], vs = d.vel[n.w];
+      evp1d::stress_point(d.s, cc.x, cc.y, ve.x, ve.y, vs.x, vs.y, vn.x, vn.y, a0.x, a0.y, st, a1.x, a1.y, a2.x,
+                          a2.y, sig, o);
       if (k == me) {
+        m2[0] = a0;
+        m2[1] = a1;
+        m2[2] = a2;
+        st2 = st;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
           sg[c] = o.sp[c];
@@ -571,16 +577,11 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
   constexpr unsigned kNo = evp1d::PairPlan::kNoEntry;
   int g2 = 0;
   uint2 w2 = make_uint2(0u, 0u);
-  double2 c2[4], m2[3];
-  double st2 = 0.0;
+  double2 c2[4];
   if (me < nS) {
     const int4 n = f.tnbr[pt.n0 + me];
     g2 = me < nO ? tslot(me) : n.x;
     w2 = f.vloc[pt.v0 + me];
-    m2[0] = pc[g2];
-    m2[1] = pc[lp + g2];
-    m2[2] = pc[2 * lp + g2];
-    st2 = d.strength[g2];
     c2[0] = d.vel[g2];
     if ((w2.x >> 16) == kNo || (w2.y & 0xffffu) == kNo || (w2.y >> 16) == kNo) {
       c2[1] = d.vel[n.y];
