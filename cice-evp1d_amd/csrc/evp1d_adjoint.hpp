@@ -8,7 +8,8 @@
 // Branches follow the forward values: max(Delta, DminArea) differentiates the
 // branch taken, sqrt at 0 and |.| at 0 take derivative 0, sign(1, fm) is
 // constant.  Derivatives are exact products of the partials (no finite
-// differences); rounding differs from the forward, so no bit-exactness claim.
+// differences); rounding differs from the forward (quotients by one value
+// are taken as products with its reciprocal), so no bit-exactness claim.
 #pragma once
 
 #include "evp1d_point.hpp"
@@ -41,28 +42,31 @@ __host__ __device__ inline void stepu_adjoint(const Scalars &s,
   const double vrel = A * R;
   const double S = sqrt(uold * uold + vold * vold);
   const double den = S + s.u0;
-  const double Cb = Tbu / den;
+  const double rden = c1 / den;
+  const double Cb = Tbu * rden;
   const double sg = copysign(c1, fm);
   const double cca = (s.brlx + s.revp) * umassdti + vrel * s.cosw + Cb;
   const double ccb = fm + sg * vrel * s.sinw;
   const double ab2 = cca * cca + ccb * ccb;
+  const double rab2 = c1 / ab2;
   const double cc1 = cca * u - ccb * v;
   const double cc2 = ccb * u + cca * v;
   // u = (cca cc1 + ccb cc2)/ab2, v = (cca cc2 - ccb cc1)/ab2, ab2 = cca^2 + ccb^2
-  const double cc1_b = (u_b * cca - v_b * ccb) / ab2;
-  const double cc2_b = (u_b * ccb + v_b * cca) / ab2;
-  const double cca_b = (u_b * (cc1 - 2.0 * cca * u) + v_b * (cc2 - 2.0 * cca * v)) / ab2;
-  const double ccb_b = (u_b * (cc2 - 2.0 * ccb * u) - v_b * (cc1 + 2.0 * ccb * v)) / ab2;
+  const double cc1_b = (u_b * cca - v_b * ccb) * rab2;
+  const double cc2_b = (u_b * ccb + v_b * cca) * rab2;
+  const double cca_b = (u_b * (cc1 - 2.0 * cca * u) + v_b * (cc2 - 2.0 * cca * v)) * rab2;
+  const double ccb_b = (u_b * (cc2 - 2.0 * ccb * u) - v_b * (cc1 + 2.0 * ccb * v)) * rab2;
   const double vrel_b = cca_b * s.cosw + ccb_b * sg * s.sinw + cc1_b * waterx + cc2_b * watery;
   const double Cb_b = cca_b;
   double uold_b = cc1_b * umassdti * s.brlx;
   double vold_b = cc2_b * umassdti * s.brlx;
   if (R > 0.0) {                          // vrel = A |(uocn - uold, vocn - vold)|
-    uold_b -= vrel_b * A * du / R;
-    vold_b -= vrel_b * A * dv / R;
+    const double t = vrel_b * A / R;
+    uold_b -= t * du;
+    vold_b -= t * dv;
   }
   if (S > 0.0) {                          // Cb = Tbu / (|(uold, vold)| + u0)
-    const double t = -Cb_b * Tbu / (den * den) / S;
+    const double t = -Cb_b * Cb * rden / S;
     uold_b += t * uold;
     vold_b += t * vold;
   }
@@ -158,6 +162,7 @@ __host__ __device__ inline double stress_adjoint(const Scalars &s,
   combos_adjoint(dxT, dyT, dxhy, dyhx, str_b, sb);
 
   const double keep = c1 - s.arlx1i * s.revp;
+  const bool cap1 = s.capping == c1;
   double strength_b = 0.0;
   double ucc = 0, vcc = 0, uee = 0, vee = 0, use = 0, vse = 0, une = 0, vne = 0;
 #pragma unroll
@@ -182,10 +187,26 @@ __host__ __device__ inline double stress_adjoint(const Scalars &s,
       sh = -cym * vvel_se - dyT * vvel_ne - cxp * uvel_se + dxT * uvel_cc;
     }
     const double D = sqrt(dv * dv + s.e_factor * (tn * tn + sh * sh));
-    // forward: tmpcalc, zetax2, etax2, rep_prs (visc_replpress)
+    // forward: tmpcalc, zetax2, etax2, rep_prs (visc_replpress), with
+    // T_s = dT/dstrength, T_D = dT/dDelta and rD = 1/Delta (0 at Delta = 0).
+    // capping = 1 (the default): T = strength / max(Delta, DminArea), so one
+    // reciprocal gives T, T_s, T_D and, where Delta > DminArea, rD.
     const double mx = (D > DminTarea) ? D : DminTarea;
-    const double dsum = D + DminTarea;
-    const double T = s.capping * (strength / mx) + (c1 - s.capping) * (strength / dsum);
+    double T, T_s, T_D, rD;
+    if (cap1) {
+      const double rmx = c1 / mx;
+      T = strength * rmx;
+      T_s = rmx;
+      T_D = (D > DminTarea) ? -T * rmx : 0.0;
+      rD = (D > DminTarea) ? rmx : (D > 0.0 ? c1 / D : 0.0);
+    } else {
+      const double dsum = D + DminTarea;
+      T = s.capping * (strength / mx) + (c1 - s.capping) * (strength / dsum);
+      T_s = s.capping / mx + (c1 - s.capping) / dsum;
+      T_D = s.capping * ((D > DminTarea) ? -strength / (D * D) : 0.0) +
+            (c1 - s.capping) * (-strength / (dsum * dsum));
+      rD = D > 0.0 ? c1 / D : 0.0;
+    }
     const double zeta = (c1 + s.Ktens) * T;
     const double eta = s.epp2i * zeta;
     // sp = (sp0*keep + arlx1i*(zeta*div - rep))*denom1, rep = (1-K)*T*D
@@ -203,14 +224,13 @@ __host__ __device__ inline double stress_adjoint(const Scalars &s,
     const double zeta_b = zeta_b0 + s.epp2i * eta_b;
     const double T_b = (c1 + s.Ktens) * zeta_b + (c1 - s.Ktens) * D * rep_b;
     double D_b = (c1 - s.Ktens) * T * rep_b;
-    strength_b += T_b * (s.capping / mx + (c1 - s.capping) / dsum);
-    const double dT_dD = s.capping * ((D > DminTarea) ? -strength / (D * D) : 0.0) +
-                         (c1 - s.capping) * (-strength / (dsum * dsum));
-    D_b += T_b * dT_dD;
+    strength_b += T_b * T_s;
+    D_b += T_b * T_D;
     if (D > 0.0) {
-      dv_b += D_b * dv / D;
-      tn_b += D_b * s.e_factor * tn / D;
-      sh_b += D_b * s.e_factor * sh / D;
+      const double w = D_b * rD;
+      dv_b += w * dv;
+      tn_b += w * s.e_factor * tn;
+      sh_b += w * s.e_factor * sh;
     }
     // reverse of this corner's strain rates (linear in the 8 velocities)
     if (c == 0) {
