@@ -369,18 +369,40 @@ def adjoint_measure(s: dict, ndte: int, t_fwd_step: float, nsnap: int, ndir: int
             "grad_cdn_ocn_scalar": float(np.sum(g["cdn_ocn"]))}
 
 
-def ensemble_measure(nmem: int, steps: int, autotune: int, tune_fused: int = 0) -> dict:
-    """64-member ensemble: one batched handle against member after member
-    and lockstep launches.  tune_fused > 0: the batch and every single member
-    first pick their fused form with evp1d_autotune_fused (tile ladder,
-    stress storage, one or two subcycles per launch) on their own data."""
-    import evp1d
+def _ensemble_grid(k: int):
+    """Synthetic 2D state of ensemble member k (worker processes: numpy only)."""
     from evp1d import synth
+    m = synth.ensemble_member(k)
+    return synth.make_grid(m["nx"], m["ny"], m["seed"], force_perturb=0.02)
+
+
+def ensemble_states(nmem: int) -> list:
+    """The members' 1D states; the 2D fields are generated in worker
+    processes (at most 16), the lists and packing in this one."""
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    import evp1d
+    from evp1d import layout
+    out = []
+    workers = max(1, min(16, os.cpu_count() or 1))
+    with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+        for g in ex.map(_ensemble_grid, range(nmem)):
+            s = layout.pack(g, evp1d.build_index(g.tmask, g.nghost))
+            s["nT"] = int((s["skipT"] == 0).sum())
+            s["nU"] = int((s["skipU"] == 0).sum())
+            out.append(s)
+    return out
+
+
+def ensemble_measure(nmem: int, steps: int, autotune: int, tune_fused: int = 0, tune_members: bool = True) -> dict:
+    """64-member ensemble: one batched handle against member after member
+    and lockstep launches.  tune_fused > 0: the batch first picks its fused
+    form with evp1d_autotune_fused (tile ladder, stress storage, one or two
+    subcycles per launch) on its own data; every single member does the same
+    (tune_members) or takes the batch's choice."""
+    import evp1d
     t = time.time()
-    members = []
-    for k in range(nmem):
-        m = synth.ensemble_member(k)
-        members.append(make_state(m["nx"], m["ny"], m["seed"], force_perturb=0.02))
+    members = ensemble_states(nmem)
     sizes = [(s["na"], s["navel"]) for s in members]
     tot = sum(a for a, _ in sizes)
     log(f"[bench] ensemble: {nmem} members, na from {min(a for a, _ in sizes)} to "
@@ -398,10 +420,17 @@ def ensemble_measure(nmem: int, steps: int, autotune: int, tune_fused: int = 0) 
     batch.close()
     singles = [new_engine(s, ndte) for s in members]
     depths = []
-    if tune_fused > 0:
+    if tune_fused > 0 and tune_members:
         for e in singles:
             depths.append(e.autotune_fused(tune_fused)["depth"])
         log(f"[bench] ensemble: members tuned, {depths.count(2)} of {nmem} on two subcycles per launch")
+    elif fused_batch:
+        for e in singles:
+            e.set_tiling(fused_batch["tile_w"], fused_batch["tile_h"], fused_batch["block"], fused_batch["xcd"])
+            e.set_fused_inplace(fused_batch["inplace"])
+            e.set_fused_depth(fused_batch["depth"])
+            e.set_tile_order(fused_batch["order"])
+            depths.append(e.get_fused_depth()["depth"] if e.get_fused_depth()["active"] else 1)
     for e in singles:
         e.run(ndte)
     for e in singles:
@@ -447,6 +476,8 @@ def ensemble_measure(nmem: int, steps: int, autotune: int, tune_fused: int = 0) 
            "lockstep_pts_sub_per_s": tot * ndte / t_lock,
            "speedup": round(t_seq / t_batch, 3), "speedup_vs_lockstep": round(t_lock / t_batch, 3),
            "tuning": tuning, "fused_batch": fused_batch,
+           "members_tuned": "each member (evp1d_autotune_fused)" if (tune_fused > 0 and tune_members) else
+                            "the batch's choice" if fused_batch else "library default",
            "members_on_pair_form": depths.count(2) if depths else None}
     log(f"[bench] ensemble: batched {t_batch:.3f}s, member after member {t_seq:.3f}s (x{t_seq / t_batch:.2f}), "
         f"lockstep {t_lock:.3f}s (x{t_lock / t_batch:.2f}), member after member unfused {t_seq_graph:.3f}s "
@@ -540,6 +571,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-fma", action="store_true", help="skip timing the contraction-on build beside the exact one")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the 64-member batched-vs-sequential comparison in the default run")
     ap.add_argument("--adjoint-snapshots", type=int, default=32)
     ap.add_argument("--fd-directions", type=int, default=50)
     ap.add_argument("--dot-directions", type=int, default=10)
@@ -705,6 +738,15 @@ def main() -> int:
             log(f"[bench] adjoint measurement failed: {e!r}")
             adjoint = {"error": repr(e)[:200]}
 
+    ensemble = None
+    if not args.no_ensemble and not args.fma and args.workload == "evp_c1":
+        try:
+            # the batch tunes; the member-after-member runs take its form
+            ensemble = ensemble_measure(64, 1, 0, args.tune_fused, tune_members=False)
+        except Exception as e:   # reported, never fatal for the forward figure
+            log(f"[bench] ensemble measurement failed: {e!r}")
+            ensemble = {"error": repr(e)[:200]}
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -735,6 +777,7 @@ def main() -> int:
         "roofline": roofline,
         "fma": fma,
         "adjoint": adjoint,
+        "ensemble": ensemble,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
