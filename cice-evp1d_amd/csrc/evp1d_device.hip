@@ -386,6 +386,7 @@ struct PairView {
   double2 *sg_out;
   long ls;
   const int4 *tnbr;         // per level-1 stress entry (PairTile::n0 + k): slot, ee, ne, se
+  const int32_t *fix;       // fixed copy-list sources of level-2 ghost reads (PairPlan::fix)
   // gradient trajectory (k_pair<.., TRAJ>): the velocities of the stepu
   // slots after the launch's first subcycle (tr_mid) and after its second
   // (tr_end, null when the handle's own buffer is the only copy wanted),
@@ -445,7 +446,7 @@ __global__ void k_pack_const(DevView d, double2 *pc, long lp) {
 // of the next phase (level-1 stepu, level-2 stress, level-2 stepu) before
 // the barrier that ends the current phase, and that phase starts on values
 // already in registers instead of one memory round trip after the barrier.
-template <int B, bool X, bool TRAJ = false>
+template <int B, bool X, bool TRAJ = false, bool FIX = false>
 __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_pair(DevView d, PairView f, int last) {
   extern __shared__ double2 lds[];
   const int cap = f.cap;
@@ -574,7 +575,10 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
   }
   // level 2, stress of entry me: memory inputs loaded before the barrier
   // (velocities that are no U1 entry never change, kNoEntry: from memory)
-  constexpr unsigned kNo = evp1d::PairPlan::kNoEntry;
+  // (velocities that are no U1 entry never change, kNoEntry: from memory;
+  // a ghost on the copy list whose source is no U1 entry: kFixBase + j, the
+  // source's unchanging velocity or zero)
+  constexpr unsigned kNo = evp1d::PairPlan::kNoEntry, kFix = evp1d::PairPlan::kFixBase;
   int g2 = 0;
   uint2 w2 = make_uint2(0u, 0u);
   double2 c2[4];
@@ -583,20 +587,34 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
     g2 = me < nO ? tslot(me) : n.x;
     w2 = f.vloc[pt.v0 + me];
     c2[0] = d.vel[g2];
-    if ((w2.x >> 16) == kNo || (w2.y & 0xffffu) == kNo || (w2.y >> 16) == kNo) {
-      c2[1] = d.vel[n.y];
-      c2[2] = d.vel[n.z];
-      c2[3] = d.vel[n.w];
+    if (!FIX) {   // no copy list: every entry is a table entry or kNoEntry
+      if ((w2.x >> 16) == kNo || (w2.y & 0xffffu) == kNo || (w2.y >> 16) == kNo) {
+        c2[1] = d.vel[n.y];
+        c2[2] = d.vel[n.z];
+        c2[3] = d.vel[n.w];
+      }
+    } else {
+      const unsigned e[3] = {w2.x >> 16, w2.y & 0xffffu, w2.y >> 16};
+      const int ns[3] = {n.y, n.z, n.w};
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (e[c] == kNo) {
+          c2[1 + c] = d.vel[ns[c]];
+        } else if (e[c] >= kFix) {
+          const int s = f.fix[pt.f0 + (int)(e[c] - kFix)];
+          c2[1 + c] = s < 0 ? make_double2(0.0, 0.0) : d.vel[s];
+        }
+      }
     }
   }
   __syncthreads();
   // level 2, stress: owned and halo entries, one per thread
   if (me < nS) {
     const unsigned e0 = w2.x & 0xffffu, e1 = w2.x >> 16, e2 = w2.y & 0xffffu, e3 = w2.y >> 16;
-    const double2 cc = e0 == kNo ? c2[0] : vt[e0];
-    const double2 ve = e1 == kNo ? c2[1] : vt[e1];
-    const double2 vn = e2 == kNo ? c2[2] : vt[e2];
-    const double2 vs = e3 == kNo ? c2[3] : vt[e3];
+    const double2 cc = e0 >= kFix ? c2[0] : vt[e0];
+    const double2 ve = e1 >= kFix ? c2[1] : vt[e1];
+    const double2 vn = e2 >= kFix ? c2[2] : vt[e2];
+    const double2 vs = e3 >= kFix ? c2[3] : vt[e3];
     evp1d::StressOut o;
     evp1d::stress_point(d.s, cc.x, cc.y, ve.x, ve.y, vs.x, vs.y, vn.x, vn.y, m2[0].x, m2[0].y, st2, m2[1].x,
                         m2[1].y, m2[2].x, m2[2].y, sg, o);
@@ -1096,6 +1114,7 @@ struct evp1d_handle {
   int32_t *pxlist_dev = nullptr, *pelist_dev = nullptr, *pklist_dev = nullptr;
   uint2 *puloc_dev = nullptr, *pvloc_dev = nullptr;
   int4 *ptnbr_dev = nullptr;
+  int32_t *pfix_dev = nullptr;
   double2 *pcstr_dev = nullptr;
   double2 *pconst = nullptr;       // constant fields in pairs, [kPairConst][na], packed per run
   double *sig_pair = nullptr;      // stresses in pairs during a pair run (with sig_alt: two buffers)
@@ -1145,14 +1164,15 @@ static void drop_graph(evp1d_handle *h) {
 }
 
 static void free_pairs(evp1d_handle *h) {
-  void *bufs[8] = {h->ptiles_dev, h->pxlist_dev, h->pelist_dev, h->pklist_dev,
-                   h->puloc_dev,  h->pvloc_dev,  h->pcstr_dev,  h->ptnbr_dev};
+  void *bufs[9] = {h->ptiles_dev, h->pxlist_dev, h->pelist_dev, h->pklist_dev, h->puloc_dev,
+                   h->pvloc_dev,  h->pcstr_dev,  h->ptnbr_dev,  h->pfix_dev};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   h->ptiles_dev = nullptr;
   h->pxlist_dev = h->pelist_dev = h->pklist_dev = nullptr;
   h->puloc_dev = h->pvloc_dev = nullptr;
   h->ptnbr_dev = nullptr;
+  h->pfix_dev = nullptr;
   h->pcstr_dev = nullptr;
   h->pair_ok = false;
   h->pair_block = h->pair_cap = h->pair_capv = 0;
@@ -1333,13 +1353,23 @@ static int install_pairs(evp1d_handle *h, const evp1d::TilePlan &plan, const int
     h->pair_why = "pair form not selected";
     return EVP1D_OK;
   }
+  // the ghost copy list in slot space (ghost slots are their own slots)
+  std::vector<int32_t> hsrc(h->halo_src_host.size());
+  for (size_t k = 0; k < hsrc.size(); ++k) {
+    const int32_t s = h->halo_src_host[k];
+    hsrc[k] = s < 0 ? -1 : h->inv[(size_t)s];
+  }
+  evp1d::HaloList hl;
+  hl.n = (int)hsrc.size();
+  hl.dst = h->halo_dst_host.data();
+  hl.src = hsrc.data();
   const evp1d::PairPlan q = evp1d::plan_pairs(plan, h->na, h->navel, idx, h->skip_host.data(), kPairMaxLds / 64,
-                                              kPairMaxLds / 16, kPairBlocks[kNumPairBlocks - 1]);
+                                              kPairMaxLds / 16, kPairBlocks[kNumPairBlocks - 1], hl);
   if (!q.ok) {
     h->pair_why = q.why;
     return EVP1D_OK;
   }
-  const std::string bad = evp1d::check_pairs(q, plan, h->na, h->navel, idx, h->skip_host.data());
+  const std::string bad = evp1d::check_pairs(q, plan, h->na, h->navel, idx, h->skip_host.data(), hl);
   if (!bad.empty()) return fail(EVP1D_EINDEX, "pair plan: %s", bad.c_str());
   int b = 0;
   while (kPairBlocks[b] < q.max_t2) ++b;   // max_t2 <= the largest block (plan limit)
@@ -1355,6 +1385,7 @@ static int install_pairs(evp1d_handle *h, const evp1d::TilePlan &plan, const int
   RC_TRY(upload_table(h, &h->puloc_dev, reinterpret_cast<const uint2 *>(q.uloc.data()), q.uloc.size() / 4));
   RC_TRY(upload_table(h, &h->pvloc_dev, reinterpret_cast<const uint2 *>(q.vloc.data()), q.vloc.size() / 4));
   RC_TRY(upload_table(h, &h->ptnbr_dev, reinterpret_cast<const int4 *>(q.tnbr.data()), q.tnbr.size() / 4));
+  RC_TRY(upload_table(h, &h->pfix_dev, q.fix.data(), q.fix.size()));
   h->pk_total = (long)q.klist.size();
   HIP_TRY(hipMalloc((void **)&h->pcstr_dev, std::max<size_t>((size_t)4 * h->pk_total, 1) * sizeof(double2)));
   HIP_TRY(hipStreamSynchronize(h->stream));
@@ -1542,6 +1573,11 @@ typedef void (*PairFn)(DevView, PairView, int);
 static const PairFn kPairFns[kNumPairBlocks][2] = {{k_pair<256, false>, k_pair<256, true>},
                                                    {k_pair<512, false>, k_pair<512, true>},
                                                    {k_pair<1024, false>, k_pair<1024, true>}};
+// with a ghost copy list (level-2 reads of listed ghosts through PairPlan::fix)
+static const PairFn kPairFixFns[kNumPairBlocks][2] = {
+    {k_pair<256, false, false, true>, k_pair<256, true, false, true>},
+    {k_pair<512, false, false, true>, k_pair<512, true, false, true>},
+    {k_pair<1024, false, false, true>, k_pair<1024, true, false, true>}};
 // the gradient sweep's trajectory-storing form (XCD-aware tile order only)
 static const PairFn kPairTrajFns[kNumPairBlocks] = {k_pair<256, true, true>, k_pair<512, true, true>,
                                                     k_pair<1024, true, true>};
@@ -1550,9 +1586,8 @@ static int pair_block_index(int b) {
     if (kPairBlocks[i] == b) return i;
   return -1;
 }
-// the pair form runs when selected, planned, and no ghost copy-list needs
-// to run between the two subcycles of a launch
-static bool pair_form(const evp1d_handle *h) { return h->fused_depth == 2 && h->pair_ok && h->v.nhalo == 0; }
+// the pair form runs when selected and planned
+static bool pair_form(const evp1d_handle *h) { return h->fused_depth == 2 && h->pair_ok; }
 
 static int fused_block_index(int b) {
   for (int i = 0; i < 4; ++i)
@@ -1572,6 +1607,9 @@ static int ensure_fused_buffers(evp1d_handle *h) {
       HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   kFusedMaxEntries * 4 * (int)sizeof(double2)));
     for (auto &row : kPairFns)
+      for (PairFn fn : row)
+        HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, kPairMaxLds));
+    for (auto &row : kPairFixFns)
       for (PairFn fn : row)
         HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, kPairMaxLds));
     for (PairFn fn : kPairTrajFns)
@@ -1660,7 +1698,7 @@ static int issue_fused(evp1d_handle *h, int nsub, double2 *traj = nullptr) {
   const long t_end = h->v.t_end;
   PairView pv{h->ptiles_dev, h->hlist_dev, h->pxlist_dev, h->pelist_dev, h->puloc_dev, h->pvloc_dev,
               h->pcstr_dev,  h->cnz_dev,   h->pk_total,   nullptr,       h->pair_cap,  h->pconst,
-              (long)h->na,   nullptr,      nullptr,       h->ldsig,      h->ptnbr_dev};
+              (long)h->na,   nullptr,      nullptr,       h->ldsig,      h->ptnbr_dev,  h->pfix_dev};
   if (pair && h->na > 0) {
     hipLaunchKernelGGL(k_pack_const, dim3(blocks_for(h->na)), dim3(kBlock), 0, h->stream, X, h->pconst, (long)h->na);
     HIP_TRY(hipGetLastError());
@@ -1694,7 +1732,7 @@ static int issue_fused(evp1d_handle *h, int nsub, double2 *traj = nullptr) {
       pv.sg_in = P[pin];
       pv.sg_out = P[1 - pin];
       pin = 1 - pin;
-      PairFn fn = kPairFns[pair_block_index(h->pair_block)][h->tile_xcd ? 1 : 0];
+      PairFn fn = (h->v.nhalo > 0 ? kPairFixFns : kPairFns)[pair_block_index(h->pair_block)][h->tile_xcd ? 1 : 0];
       if (traj) {
         const int before = odd_first ? 2 * j - 3 : 2 * j - 2;   // subcycles done before this launch
         pv.tr_mid = traj + (size_t)(before + 1) * nu;
@@ -1703,6 +1741,10 @@ static int issue_fused(evp1d_handle *h, int nsub, double2 *traj = nullptr) {
       }
       if (h->fused_ntiles > 0) {
         hipLaunchKernelGGL(fn, dim3(h->fused_ntiles), dim3(h->pair_block), lds2, h->stream, in, pv, j == L ? 1 : 0);
+        HIP_TRY(hipGetLastError());
+      }
+      if (h->v.nhalo > 0) {   // the copy after the launch's second subcycle
+        hipLaunchKernelGGL(k_halo, dim3(blocks_for(h->v.nhalo)), dim3(kBlock), 0, h->stream, out);
         HIP_TRY(hipGetLastError());
       }
       continue;
@@ -2117,6 +2159,7 @@ int evp1d_set_halo(evp1d_handle *h, int32_t n, const int32_t *dst, const int32_t
   h->v.halo_dst = h->halo_dev;
   h->v.halo_src = h->halo_dev ? h->halo_dev + n : nullptr;
   h->idx_dirty = true;
+  h->layout_dirty = true;   // the pair form's level-2 reads of listed ghosts change
   return EVP1D_OK;
 }
 

@@ -680,16 +680,25 @@ std::vector<int32_t> inverse_perm(const TilePlan &p, int na) {
 }
 
 struct PairWork {
-  std::vector<int32_t> x, k, e, tn;
+  std::vector<int32_t> x, k, e, tn, fx;
   std::vector<uint16_t> uloc, vloc;
   int ns = 0, ne_s = 0, nu = 0;
   std::string err;
 };
 }  // namespace
 
+// ghost slot -> copy-list source slot (-1: zero), or -2 when not listed
+std::vector<int32_t> halo_map(const HaloList &halo, int na, int navel) {
+  std::vector<int32_t> m((size_t)std::max(navel - na, 0), -2);
+  for (int k = 0; k < halo.n; ++k) {
+    const int32_t d = halo.dst[k];
+    if (d >= na && d < navel) m[(size_t)(d - na)] = halo.src[k] < 0 ? -1 : halo.src[k];
+  }
+  return m;
+}
+
 PairPlan plan_pairs(const TilePlan &p, int na, int navel, const int32_t *const idx[6], const uint8_t *skip,
-                    int cap_s, int cap_v, int max_t2) {
-  (void)navel;
+                    int cap_s, int cap_v, int max_t2, const HaloList &halo) {
   (void)skip;
   PairPlan r;
   if (!p.ok || !idx) {
@@ -698,6 +707,7 @@ PairPlan plan_pairs(const TilePlan &p, int na, int navel, const int32_t *const i
   }
   const int t_end = p.cnt[0] + p.cnt[1], u_begin = p.cnt[0], u_end = t_end + p.cnt[2];
   const std::vector<int32_t> inv = inverse_perm(p, na);
+  const std::vector<int32_t> hmap = halo_map(halo, na, navel);
   auto slot = [&](int32_t q) { return q < na ? inv[(size_t)q] : q; };
   const int nt = (int)p.tiles.size();
   std::vector<PairWork> work((size_t)nt);
@@ -716,17 +726,30 @@ PairPlan plan_pairs(const TilePlan &p, int na, int navel, const int32_t *const i
     for (int k = 0; k < nS; ++k) sm.find_or_add(tslot(k), k, added);
     for (int k = 0; k < nUo; ++k) vm.find_or_add(k < nB ? d.b0 + k : d.c0 + (k - nB), k, added);
     // level-2 stresses gather cc, ee, ne, se: active U points come from the
-    // velocity table (U1-only entries in first-visit order), others from memory
+    // velocity table (U1-only entries in first-visit order), others from
+    // memory; a ghost on the copy list reads its source as the copy made
+    // after level 1 leaves it (the source's level-1 velocity, its unchanging
+    // velocity, or zero)
     wk.vloc.resize((size_t)4 * nS);
     for (int k = 0; k < nS; ++k) {
       const int32_t g = tslot(k), q = p.perm[(size_t)g];
       const int32_t src[4] = {g, slot(idx[0][q]), slot(idx[1][q]), slot(idx[2][q])};
       for (int c = 0; c < 4; ++c) {
         uint16_t en = PairPlan::kNoEntry;
-        if (src[c] >= u_begin && src[c] < u_end) {
-          const int v = vm.find_or_add(src[c], nUo + (int)wk.e.size(), added);
-          if (added) wk.e.push_back(src[c]);
+        int32_t s = src[c];
+        const bool ghost = s >= na && hmap[(size_t)(s - na)] != -2;
+        if (ghost) s = hmap[(size_t)(s - na)];
+        if (s >= u_begin && s < u_end) {
+          const int v = vm.find_or_add(s, nUo + (int)wk.e.size(), added);
+          if (added) wk.e.push_back(s);
           en = (uint16_t)std::min(v, 0xfffe);
+        } else if (ghost) {
+          int j = 0;
+          while (j < (int)wk.fx.size() && wk.fx[(size_t)j] != s) ++j;
+          if (j == (int)wk.fx.size()) wk.fx.push_back(s);
+          if (j >= 0xffff - PairPlan::kFixBase && wk.err.empty())
+            wk.err = "tile " + std::to_string(t) + " has too many copy-list sources";
+          en = (uint16_t)(PairPlan::kFixBase + std::min(j, 0xffff - PairPlan::kFixBase - 1));
         }
         wk.vloc[(size_t)4 * k + c] = en;
       }
@@ -768,7 +791,7 @@ PairPlan plan_pairs(const TilePlan &p, int na, int navel, const int32_t *const i
     if (wk.err.empty() && (nE > cap_s || nE > 0xfffe))
       wk.err = "tile " + std::to_string(t) + " needs " + std::to_string(nE) + " str entries (limit " +
                std::to_string(cap_s) + ")";
-    if (wk.err.empty() && (nU > cap_v || nU > 0xfffe))
+    if (wk.err.empty() && (nU > cap_v || nU >= PairPlan::kFixBase))
       wk.err = "tile " + std::to_string(t) + " needs " + std::to_string(nU) + " velocity entries (limit " +
                std::to_string(cap_v) + ")";
     if (wk.err.empty()) {
@@ -787,7 +810,7 @@ PairPlan plan_pairs(const TilePlan &p, int na, int navel, const int32_t *const i
     }
   }
   r.tiles.resize((size_t)nt);
-  size_t nx = 0, nk = 0, ne = 0, nu = 0, nv = 0, nn = 0;
+  size_t nx = 0, nk = 0, ne = 0, nu = 0, nv = 0, nn = 0, nf = 0;
   for (int t = 0; t < nt; ++t) {
     const PairWork &wk = work[(size_t)t];
     PairTile &pt = r.tiles[(size_t)t];
@@ -802,6 +825,9 @@ PairPlan plan_pairs(const TilePlan &p, int na, int navel, const int32_t *const i
     pt.v0 = (int32_t)nv;
     pt.n0 = (int32_t)nn;
     nn += wk.tn.size() / 4;
+    pt.f0 = (int32_t)nf;
+    pt.nf = (int32_t)wk.fx.size();
+    nf += wk.fx.size();
     nx += wk.x.size();
     nk += wk.k.size();
     ne += wk.e.size();
@@ -822,6 +848,7 @@ PairPlan plan_pairs(const TilePlan &p, int na, int navel, const int32_t *const i
   r.uloc.resize(4 * nu);
   r.vloc.resize(4 * nv);
   r.tnbr.resize(4 * nn);
+  r.fix.resize(nf);
   par_items(nt, [&](int t, int) {
     const PairWork &wk = work[(size_t)t];
     const PairTile &pt = r.tiles[(size_t)t];
@@ -831,19 +858,21 @@ PairPlan plan_pairs(const TilePlan &p, int na, int navel, const int32_t *const i
     std::copy(wk.uloc.begin(), wk.uloc.end(), r.uloc.begin() + (size_t)4 * pt.u0);
     std::copy(wk.vloc.begin(), wk.vloc.end(), r.vloc.begin() + (size_t)4 * pt.v0);
     std::copy(wk.tn.begin(), wk.tn.end(), r.tnbr.begin() + (size_t)4 * pt.n0);
+    std::copy(wk.fx.begin(), wk.fx.end(), r.fix.begin() + pt.f0);
   });
   r.ok = true;
   return r;
 }
 
 std::string check_pairs(const PairPlan &q, const TilePlan &p, int na, int navel, const int32_t *const idx[6],
-                        const uint8_t *skip) {
+                        const uint8_t *skip, const HaloList &halo) {
   (void)skip;
   if (!q.ok) return "";
   if (!p.ok || !idx) return "pair plan without a tiling";
   if (q.tiles.size() != p.tiles.size()) return "pair plan and tiling differ in tiles";
   const int t_end = p.cnt[0] + p.cnt[1], u_begin = p.cnt[0], u_end = t_end + p.cnt[2];
   const std::vector<int32_t> inv = inverse_perm(p, na);
+  const std::vector<int32_t> hmap = halo_map(halo, na, navel);
   auto slot = [&](int32_t r) { return r < na ? inv[(size_t)r] : r; };
   std::vector<std::string> err(q.tiles.size());
   std::vector<std::vector<int32_t>> smaps((size_t)plan_threads()), vmaps((size_t)plan_threads());
@@ -911,14 +940,24 @@ std::string check_pairs(const PairPlan &q, const TilePlan &p, int na, int navel,
       }
     }
     // every level-2 stress read names the level-1 velocity of an active U
-    // point, and only those
+    // point, and only those; a copy-list ghost is read as its source
+    if (pt.nf < 0 || pt.f0 < 0 || (size_t)pt.f0 + pt.nf > q.fix.size()) { e = "fixed-source range"; return; }
     for (int k = 0; k < nS; ++k) {
       const int32_t g = sm[(size_t)k], r0 = p.perm[(size_t)g];
       const int32_t want[4] = {g, slot(idx[0][r0]), slot(idx[1][r0]), slot(idx[2][r0])};
       for (int c = 0; c < 4; ++c) {
         const int en = q.vloc[(size_t)4 * (pt.v0 + k) + c];
-        const bool active = want[c] >= u_begin && want[c] < u_end;
-        if (en == PairPlan::kNoEntry ? active : (en >= nU || vm[(size_t)en] != want[c])) {
+        const bool ghost = want[c] >= na && hmap[(size_t)(want[c] - na)] != -2;
+        const int32_t s = ghost ? hmap[(size_t)(want[c] - na)] : want[c];
+        const bool active = s >= u_begin && s < u_end;
+        bool ok;
+        if (en == PairPlan::kNoEntry)
+          ok = !active && !ghost;
+        else if (en >= PairPlan::kFixBase)
+          ok = ghost && !active && en - PairPlan::kFixBase < pt.nf && q.fix[(size_t)pt.f0 + (en - PairPlan::kFixBase)] == s;
+        else
+          ok = active && en < nU && vm[(size_t)en] == s;
+        if (!ok) {
           e = "stress entry " + std::to_string(k) + " of tile " + std::to_string(t) + ": bad velocity entry";
           return;
         }

@@ -163,6 +163,7 @@ struct PairTile {
   int32_t e0, ne;           // U1-only stepu slots: PairPlan::elist[e0, e0 + ne)
   int32_t u0, v0;           // first uloc entry (per U1 entry), first vloc entry (per T2 entry)
   int32_t n0;               // first tnbr entry (per stress entry of level 1, T2 entries first)
+  int32_t f0, nf;           // fixed-source entries: PairPlan::fix[f0, f0 + nf)
 };
 
 struct PairPlan {
@@ -179,16 +180,30 @@ struct PairPlan {
   // its ee, ne, se neighbours, so one load gives every address the entry's
   // stress gathers
   Table<int32_t> tnbr;
+  // Ghost copy list (evp1d_set_halo): between the two subcycles of a launch
+  // a listed ghost slot holds the copy of its source.  A level-2 read of such
+  // a ghost names the source's level-1 velocity entry when the source is an
+  // active U point; otherwise vloc holds kFixBase + j and fix[f0 + j] is the
+  // source slot (a velocity no subcycle changes) or -1 (the copy is zero).
+  Table<int32_t> fix;
   int cap_s = 0, cap_v = 0;           // largest str / velocity table
   int max_t2 = 0;                     // largest T2 (level-2 stresses, one per thread)
   static constexpr uint16_t kNoEntry = 0xffff;
+  static constexpr uint16_t kFixBase = 0xc000;   // velocity table entries stay below it
+};
+
+// Ghost copy list in slot space: ghost slot dst[k] (>= na) takes the value
+// of slot src[k] (< na) or zero (src[k] < 0) after every subcycle.
+struct HaloList {
+  int n = 0;
+  const int32_t *dst = nullptr, *src = nullptr;
 };
 
 // cap_s, cap_v, max_t2: limits on the tables and on T2 (plan fails beyond).
 PairPlan plan_pairs(const TilePlan &p, int na, int navel, const int32_t *const idx[6], const uint8_t *skip,
-                    int cap_s, int cap_v, int max_t2);
+                    int cap_s, int cap_v, int max_t2, const HaloList &halo = HaloList());
 // Re-derives every table of a pair plan from the lists; "" when consistent.
 std::string check_pairs(const PairPlan &q, const TilePlan &p, int na, int navel, const int32_t *const idx[6],
-                        const uint8_t *skip);
+                        const uint8_t *skip, const HaloList &halo = HaloList());
 
 }  // namespace evp1d

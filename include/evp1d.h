@@ -245,8 +245,10 @@ int evp1d_get_fused_inplace(evp1d_handle *h, int32_t *inplace);
  * with one one-subcycle launch.  The pair form uses two stress buffers, needs
  * (tile_w+1)(tile_h+1) <= 1024 on the lists and its tables within 160 KiB of
  * LDS, and picks its own workgroup size (one thread per level-2 stress
- * point); where it does not fit, or a ghost copy-list is set
- * (evp1d_set_halo), fused runs use depth 1.  evp1d_get_fused_depth reports
+ * point); where it does not fit, fused runs use depth 1.  With a ghost
+ * copy list (evp1d_set_halo) the copy also runs between the two subcycles
+ * of a launch: a listed ghost read at the second subcycle takes its source's
+ * value after the first.  evp1d_get_fused_depth reports
  * the setting and, in *active, whether runs use the pair form (0/1) and, in
  * *block, its workgroup size.  Results are identical.                       */
 /* Order in which the fused forms visit their tiles, and so the slot order of

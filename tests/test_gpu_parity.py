@@ -95,11 +95,36 @@ def test_halo_copy_list(gpu):
     for mode in MODES:
         with make_engine(st, 9, mode) as eng:
             eng.set_halo(dst, src)
-            if mode == "fused-pair":   # no copy-list between the two subcycles of a launch
-                assert not eng.get_fused_depth()["active"]
+            if mode == "fused-pair":   # the copy list runs between the two subcycles of a launch
+                assert eng.get_fused_depth()["active"]
             eng.run(9)
             got = fetch_state(eng, st)
         assert_state_bitexact(got, want)
+
+
+@pytest.mark.parametrize("tile,order,nsub", [((31, 19), 1, 8), ((15, 15), 0, 9), ((5, 7), 1, 4)])
+def test_halo_copy_list_pair_form(gpu, tile, order, nsub):
+    """The pair form with a ghost copy list (the copy made after each
+    subcycle, the first of a launch's two included) on several tiles, tile
+    orders and subcycle counts: bit-identical to the oracle's halo run."""
+    st = synthetic_state(90, 70, seed=43)
+    rng = np.random.default_rng(7)
+    ghosts = np.arange(st.na + 1, st.navel + 1, dtype=np.int32)
+    dst = rng.choice(ghosts, size=2 * ghosts.size // 3, replace=False).astype(np.int32)
+    src = rng.integers(0, st.na + 1, size=dst.size).astype(np.int32)   # 0 = fill with zero
+    lib, p = oracle_params(st, nsub)
+    want = st.copy()
+    lib.run(p, want, nsub, halo=(dst, src))
+    _log(f"halo pair tile={tile} order={order} nsub={nsub}")
+    with make_engine(st, nsub, "fused") as eng:
+        eng.set_tiling(tile[0], tile[1], 512, 1)
+        eng.set_fused_depth(2)
+        eng.set_tile_order(order)
+        eng.set_halo(dst, src)
+        eng.run(nsub)
+        assert eng.get_fused_depth()["active"]
+        got = fetch_state(eng, st)
+    assert_state_bitexact(got, want)
 
 
 def test_evp_c0_md5(gpu):

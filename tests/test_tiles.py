@@ -248,7 +248,7 @@ def test_pair_plan_schedule_and_check(tmp_path):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "NOT CAUGHT" not in r.stdout and "good: ''" in r.stdout
-    assert r.stdout.count(": 0 differing outputs") == 5
+    assert r.stdout.count(": 0 differing outputs") == 10   # five tiles, with and without a copy list
 
 
 def test_hilbert_tile_order(tmp_path):
