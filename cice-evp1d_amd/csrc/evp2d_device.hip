@@ -169,13 +169,69 @@ struct Tile2 {
   int32_t t0, nto, nth;   // T entries [t0, t0 + nto) owned, then nth halo entries
   int32_t u0, nu;         // U entries [u0, u0 + nu)
 };
+// During a fused run the per-cell streams are packed by entry, so no cache
+// line is fetched for a cell the tile does not evaluate: per T entry (owned
+// then halo, tile by tile) its stress constants in five pairs, per U entry
+// its stepu constants in seven pairs, and per owned T cell its 12 stresses in
+// six pairs (a halo entry reads its owner's, hidx).  Velocities keep the
+// block layout (the stencil gathers read neighbours by offset).
 struct Fused2 {
   const Tile2 *tiles;
   const int32_t *tcell, *ucell;   // linear cells
   const uint16_t *tlds, *ulds;    // their LDS entries
+  const int32_t *hidx;            // per T entry: packed stress index (its owner's)
+  const double2 *tc, *uc;         // [5][et] and [7][eu] packed constants
+  long et, eu, no;
+  const double2 *sg_in;           // [6][no] stresses in pairs before the subcycle
+  double2 *sg_out;                //          and after it
   double2 *vel_out;
-  double *sig_out;
 };
+
+// stress constants of T entry e: (dxt, dyt), (strength, dmin), (cxp, cyp),
+// (cxm, cym), (dxhy, dyhx)
+__global__ void k2_pack_t(View2 v, const int32_t *tcell, double2 *tc, long et) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= et) return;
+  const long c = tcell[e];
+  tc[e] = make_double2(v.dxt[c], v.dyt[c]);
+  tc[et + e] = make_double2(v.strength[c], v.dmin[c]);
+  tc[2 * et + e] = make_double2(v.cxp[c], v.cyp[c]);
+  tc[3 * et + e] = make_double2(v.cxm[c], v.cym[c]);
+  tc[4 * et + e] = make_double2(v.dxhy[c], v.dyhx[c]);
+}
+// stepu constants of U entry e: (cw, aix), (uocn, vocn), (waterx, watery),
+// (forcex, forcey), (umassdti, fm), (uarear, tbu), (uvel_init, vvel_init)
+__global__ void k2_pack_u(View2 v, const int32_t *ucell, double2 *uc, long eu) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= eu) return;
+  const long c = ucell[e];
+  uc[e] = make_double2(v.cw[c], v.aix[c]);
+  uc[eu + e] = make_double2(v.uocn[c], v.vocn[c]);
+  uc[2 * eu + e] = make_double2(v.waterx[c], v.watery[c]);
+  uc[3 * eu + e] = make_double2(v.forcex[c], v.forcey[c]);
+  uc[4 * eu + e] = make_double2(v.umassdti[c], v.fm[c]);
+  uc[5 * eu + e] = make_double2(v.uarear[c], v.tbu[c]);
+  uc[6 * eu + e] = v.vinit[c];
+}
+// the stresses of the owned T cells (ocell, packed order) into pairs, and back
+__global__ void k2_pack_sig(const double *sig, long n, const int32_t *ocell, double2 *sg, long no) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= no) return;
+  const long c = ocell[p];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) sg[q * no + p] = make_double2(sig[2 * q * n + c], sig[(2 * q + 1) * n + c]);
+}
+__global__ void k2_unpack_sig(double *sig, long n, const int32_t *ocell, const double2 *sg, long no) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= no) return;
+  const long c = ocell[p];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const double2 s = sg[q * no + p];
+    sig[2 * q * n + c] = s.x;
+    sig[(2 * q + 1) * n + c] = s.y;
+  }
+}
 
 __global__ __launch_bounds__(kB2F) void k2_fused(View2 v, Fused2 f, int last) {
   __shared__ double2 lds[4 * kCap2];
@@ -184,27 +240,35 @@ __global__ __launch_bounds__(kB2F) void k2_fused(View2 v, Fused2 f, int last) {
   __syncthreads();
   const int nT = t.nto + t.nth;
   for (int k = threadIdx.x; k < nT; k += kB2F) {
-    const long c = f.tcell[t.t0 + k];
-    const int e = f.tlds[t.t0 + k];
+    const long e = t.t0 + k;
+    const long c = f.tcell[e];
+    const long p = f.hidx[e];
+    const int el = f.tlds[e];
     const double2 cc = v.vel[c];
     const double2 ve = v.vel[c - 1];
     const double2 vs = v.vel[c - v.nxb];
     const double2 vn = v.vel[c - v.nxb - 1];
     double sig[12];
 #pragma unroll
-    for (int q = 0; q < 12; ++q) sig[q] = v.sig[q * v.n + c];
+    for (int q = 0; q < 6; ++q) {
+      const double2 s = f.sg_in[q * f.no + p];
+      sig[2 * q] = s.x;
+      sig[2 * q + 1] = s.y;
+    }
+    const double2 m0 = f.tc[e], m1 = f.tc[f.et + e], m2 = f.tc[2 * f.et + e], m3 = f.tc[3 * f.et + e],
+                  m4 = f.tc[4 * f.et + e];
     evp1d::StressOut o;
-    evp1d::stress_core(v.s, cc.x, cc.y, ve.x, ve.y, vs.x, vs.y, vn.x, vn.y, v.dxt[c], v.dyt[c], v.strength[c],
-                       v.cxp[c], v.cyp[c], v.cxm[c], v.cym[c], v.dmin[c], v.dxhy[c], v.dyhx[c], sig, o);
+    evp1d::stress_core(v.s, cc.x, cc.y, ve.x, ve.y, vs.x, vs.y, vn.x, vn.y, m0.x, m0.y, m1.x, m2.x, m2.y, m3.x,
+                       m3.y, m1.y, m4.x, m4.y, sig, o);
     const double2 p15 = make_double2(o.str[0], o.str[4]), p27 = make_double2(o.str[1], o.str[6]);
     const double2 p36 = make_double2(o.str[2], o.str[5]), p48 = make_double2(o.str[3], o.str[7]);
     if (k < t.nto) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        f.sig_out[q * v.n + c] = o.sp[q];
-        f.sig_out[(4 + q) * v.n + c] = o.sm[q];
-        f.sig_out[(8 + q) * v.n + c] = o.s12[q];
-      }
+      f.sg_out[p] = make_double2(o.sp[0], o.sp[1]);
+      f.sg_out[f.no + p] = make_double2(o.sp[2], o.sp[3]);
+      f.sg_out[2 * f.no + p] = make_double2(o.sm[0], o.sm[1]);
+      f.sg_out[3 * f.no + p] = make_double2(o.sm[2], o.sm[3]);
+      f.sg_out[4 * f.no + p] = make_double2(o.s12[0], o.s12[1]);
+      f.sg_out[5 * f.no + p] = make_double2(o.s12[2], o.s12[3]);
       if (last) {
         v.s15[c] = p15;
         v.s27[c] = p27;
@@ -212,25 +276,26 @@ __global__ __launch_bounds__(kB2F) void k2_fused(View2 v, Fused2 f, int last) {
         v.s48[c] = p48;
       }
     }
-    lds[e] = p15;
-    lds[kCap2 + e] = p27;
-    lds[2 * kCap2 + e] = p36;
-    lds[3 * kCap2 + e] = p48;
+    lds[el] = p15;
+    lds[kCap2 + el] = p27;
+    lds[2 * kCap2 + el] = p36;
+    lds[3 * kCap2 + el] = p48;
   }
   __syncthreads();
   for (int k = threadIdx.x; k < t.nu; k += kB2F) {
-    const long c = f.ucell[t.u0 + k];
-    const int e = f.ulds[t.u0 + k];
-    const double2 a = lds[e];                        // (i, j): str1, str5
-    const double2 b = lds[kCap2 + e + 1];            // (i+1, j): str2, str7
-    const double2 g = lds[2 * kCap2 + e + kTW + 1];  // (i, j+1): str3, str6
-    const double2 h = lds[3 * kCap2 + e + kTW + 2];  // (i+1, j+1): str4, str8
+    const long e = t.u0 + k;
+    const long c = f.ucell[e];
+    const int el = f.ulds[e];
+    const double2 a = lds[el];                        // (i, j): str1, str5
+    const double2 b = lds[kCap2 + el + 1];            // (i+1, j): str2, str7
+    const double2 g = lds[2 * kCap2 + el + kTW + 1];  // (i, j+1): str3, str6
+    const double2 h = lds[3 * kCap2 + el + kTW + 2];  // (i+1, j+1): str4, str8
     const double2 vv = v.vel[c];
-    const double2 vi = v.vinit[c];
+    const double2 k0 = f.uc[e], k1 = f.uc[f.eu + e], k2 = f.uc[2 * f.eu + e], k3 = f.uc[3 * f.eu + e];
+    const double2 k4 = f.uc[4 * f.eu + e], k5 = f.uc[5 * f.eu + e], k6 = f.uc[6 * f.eu + e];
     evp1d::StepuOut o;
-    evp1d::stepu_point(v.s, vv.x, vv.y, v.cw[c], v.aix[c], v.uocn[c], v.vocn[c], v.waterx[c], v.watery[c],
-                       v.forcex[c], v.forcey[c], v.umassdti[c], v.fm[c], v.uarear[c], vi.x, vi.y, v.tbu[c],
-                       a.x, b.x, g.x, h.x, a.y, g.y, b.y, h.y, o);
+    evp1d::stepu_point(v.s, vv.x, vv.y, k0.x, k0.y, k1.x, k1.y, k2.x, k2.y, k3.x, k3.y, k4.x, k4.y, k5.x, k6.x,
+                       k6.y, k5.y, a.x, b.x, g.x, h.x, a.y, g.y, b.y, h.y, o);
     f.vel_out[c] = make_double2(o.uvel, o.vvel);
     if (last) {
       v.strintx[c] = o.strintx;
@@ -292,9 +357,12 @@ struct evp2d_handle {
   Tile2 *tiles_dev = nullptr;
   int32_t *ftcell_dev = nullptr, *fucell_dev = nullptr;
   uint16_t *ftlds_dev = nullptr, *fulds_dev = nullptr;
+  int32_t *hidx_dev = nullptr, *ocell_dev = nullptr;   // per T entry: packed stress index; per index: cell
+  double2 *tc_dev = nullptr, *uc_dev = nullptr;        // packed constants (per run)
+  double2 *sg_dev[2] = {nullptr, nullptr};             // packed stresses during a run
+  long et = 0, eu = 0, no = 0;                          // T entries, U entries, owned T cells
   int ntiles = 0;
   double2 *vel_alt = nullptr;
-  double *sig_alt = nullptr;
   // captured subcycle loop
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -313,14 +381,18 @@ void drop_graph2(evp2d_handle *h) {
 }
 
 void free_lists2(evp2d_handle *h) {
-  void *bufs[10] = {h->tcell_dev, h->ucell_dev, h->fill_dev, h->dst_dev, h->src_dev,
-                    h->tiles_dev, h->ftcell_dev, h->fucell_dev, h->ftlds_dev, h->fulds_dev};
+  void *bufs[16] = {h->tcell_dev, h->ucell_dev, h->fill_dev, h->dst_dev, h->src_dev,   h->tiles_dev,
+                    h->ftcell_dev, h->fucell_dev, h->ftlds_dev, h->fulds_dev, h->hidx_dev, h->ocell_dev,
+                    h->tc_dev,    h->uc_dev,     h->sg_dev[0], h->sg_dev[1]};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   h->tcell_dev = h->ucell_dev = h->fill_dev = h->dst_dev = h->src_dev = nullptr;
   h->tiles_dev = nullptr;
   h->ftcell_dev = h->fucell_dev = nullptr;
   h->ftlds_dev = h->fulds_dev = nullptr;
+  h->hidx_dev = h->ocell_dev = nullptr;
+  h->tc_dev = h->uc_dev = h->sg_dev[0] = h->sg_dev[1] = nullptr;
+  h->et = h->eu = h->no = 0;
   h->ntiles = 0;
 }
 
@@ -330,7 +402,6 @@ void release2(evp2d_handle *h) {
   drop_graph2(h);
   free_lists2(h);
   if (h->vel_alt) (void)hipFree(h->vel_alt);
-  if (h->sig_alt) (void)hipFree(h->sig_alt);
   for (void *a : h->allocs) (void)hipFree(a);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -387,11 +458,22 @@ int build_tiles2(evp2d_handle *h) {
     }
   }
   std::vector<Tile2> tiles;
-  std::vector<int32_t> tcell, ucell;
+  std::vector<int32_t> tcell, ucell, ocell;
   std::vector<uint16_t> tlds, ulds;
+  // packed stress index of every owned T cell: its position among the owned
+  // entries, tile by tile
+  std::vector<int32_t> pidx((size_t)h->n, -1);
+  for (long t = 0; t < nt_all; ++t)
+    for (int32_t c : own[(size_t)t]) {
+      pidx[(size_t)c] = (int32_t)ocell.size();
+      ocell.push_back(c);
+    }
+  std::vector<int32_t> hidx;
   for (long t = 0; t < nt_all; ++t) {
     const size_t k = (size_t)t;
     if (own[k].empty() && uc[k].empty()) continue;
+    for (int32_t c : own[k]) hidx.push_back(pidx[(size_t)c]);
+    for (int32_t c : halo[k]) hidx.push_back(pidx[(size_t)c]);
     Tile2 d;
     d.t0 = (int32_t)tcell.size();
     d.nto = (int32_t)own[k].size();
@@ -433,6 +515,17 @@ int build_tiles2(evp2d_handle *h) {
   RC2(up32(&h->fucell_dev, ucell));
   RC2(up16(&h->ftlds_dev, tlds));
   RC2(up16(&h->fulds_dev, ulds));
+  for (int32_t p : hidx)   // every halo T cell is some tile's owned cell
+    if (p < 0) return fail2(EVP1D_EINDEX, "evp2d: a halo T entry has no owning tile");
+  RC2(up32(&h->hidx_dev, hidx));
+  RC2(up32(&h->ocell_dev, ocell));
+  h->et = (long)tcell.size();
+  h->eu = (long)ucell.size();
+  h->no = (long)ocell.size();
+  HIP2(hipMalloc((void **)&h->tc_dev, std::max<size_t>((size_t)5 * h->et, 1) * sizeof(double2)));
+  HIP2(hipMalloc((void **)&h->uc_dev, std::max<size_t>((size_t)7 * h->eu, 1) * sizeof(double2)));
+  for (int b = 0; b < 2; ++b)
+    HIP2(hipMalloc((void **)&h->sg_dev[b], std::max<size_t>((size_t)6 * h->no, 1) * sizeof(double2)));
   HIP2(hipStreamSynchronize(h->stream));   // the host vectors go out of scope
   h->ntiles = (int)tiles.size();
   return EVP1D_OK;
@@ -518,26 +611,41 @@ int launch_halo2_on(evp2d_handle *h, double2 *vel) {
   return EVP1D_OK;
 }
 
-// nsub fused subcycles.  The state starts in the handle's buffers (X), is
-// copied to the second set (Y), and the subcycles alternate so that the last
-// one writes X: subcycle k writes X when nsub - k is even.
+// nsub fused subcycles.  The velocities start in the handle's buffer (X),
+// are copied to the second one (Y), and the subcycles alternate so that the
+// last one writes X: subcycle k writes X when nsub - k is even.  The
+// constants and stresses are packed by entry first (k2_pack_*); the
+// stresses alternate between two packed buffers and are put back into the
+// handle's block arrays after the last subcycle.
 int issue_fused2(evp2d_handle *h, int nsub) {
   const View2 X = h->v;
   View2 Y = h->v;
   Y.vel = h->vel_alt;
-  Y.sig = h->sig_alt;
   HIP2(hipMemcpyAsync(Y.vel, X.vel, (size_t)h->n * sizeof(double2), hipMemcpyDeviceToDevice, h->stream));
-  HIP2(hipMemcpyAsync(Y.sig, X.sig, (size_t)12 * h->n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  if (h->et > 0) hipLaunchKernelGGL(k2_pack_t, dim3(blocks2(h->et)), dim3(kB2), 0, h->stream, X, h->ftcell_dev, h->tc_dev, h->et);
+  if (h->eu > 0) hipLaunchKernelGGL(k2_pack_u, dim3(blocks2(h->eu)), dim3(kB2), 0, h->stream, X, h->fucell_dev, h->uc_dev, h->eu);
+  if (h->no > 0)
+    hipLaunchKernelGGL(k2_pack_sig, dim3(blocks2(h->no)), dim3(kB2), 0, h->stream, (const double *)X.sig, h->n,
+                       (const int32_t *)h->ocell_dev, h->sg_dev[0], h->no);
+  HIP2(hipGetLastError());
+  int cur = 0;
   for (int k = 1; k <= nsub; ++k) {
     const bool to_x = ((nsub - k) % 2) == 0;
     const View2 &in = to_x ? Y : X;
     const View2 &out = to_x ? X : Y;
-    Fused2 f{h->tiles_dev, h->ftcell_dev, h->fucell_dev, h->ftlds_dev, h->fulds_dev, out.vel, out.sig};
+    Fused2 f{h->tiles_dev, h->ftcell_dev, h->fucell_dev, h->ftlds_dev, h->fulds_dev, h->hidx_dev,
+             h->tc_dev,    h->uc_dev,     h->et,         h->eu,        h->no,         h->sg_dev[cur],
+             h->sg_dev[1 - cur], out.vel};
     if (h->ntiles > 0)
       hipLaunchKernelGGL(k2_fused, dim3(h->ntiles), dim3(kB2F), 0, h->stream, in, f, k == nsub ? 1 : 0);
     HIP2(hipGetLastError());
     RC2(launch_halo2_on(h, out.vel));
+    cur = 1 - cur;
   }
+  if (h->no > 0)
+    hipLaunchKernelGGL(k2_unpack_sig, dim3(blocks2(h->no)), dim3(kB2), 0, h->stream, X.sig, h->n,
+                       (const int32_t *)h->ocell_dev, (const double2 *)h->sg_dev[cur], h->no);
+  HIP2(hipGetLastError());
   return EVP1D_OK;
 }
 
@@ -795,7 +903,6 @@ int evp2d_run(evp2d_handle *h, int32_t nsub) {
     drop_graph2(h);
     const bool fused = h->mode == EVP2D_MODE_FUSED;
     if (fused && !h->vel_alt) HIP2(hipMalloc((void **)&h->vel_alt, (size_t)h->n * sizeof(double2)));
-    if (fused && !h->sig_alt) HIP2(hipMalloc((void **)&h->sig_alt, (size_t)12 * h->n * sizeof(double)));
     HIP2(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     const int rc = fused ? issue_fused2(h, nsub) : issue2(h, nsub);
     hipGraph_t g = nullptr;
@@ -853,7 +960,6 @@ int evp2d_run_individual(evp2d_handle *h, int32_t which, int32_t nsub) {
   if (which == 2) {   // fused subcycles launched one by one (no graph)
     if (nsub <= 0) return EVP1D_OK;
     if (!h->vel_alt) HIP2(hipMalloc((void **)&h->vel_alt, (size_t)h->n * sizeof(double2)));
-    if (!h->sig_alt) HIP2(hipMalloc((void **)&h->sig_alt, (size_t)12 * h->n * sizeof(double)));
     return issue_fused2(h, nsub);
   }
   for (int k = 0; k < nsub; ++k) RC2(which == 0 ? launch_stress2(h) : launch_stepu2(h));
