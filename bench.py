@@ -199,6 +199,35 @@ def calibrate_copy(mib: int) -> int:
     return 0
 
 
+def stream_probe(mib: int, reps: int = 20) -> int:
+    """HBM ceiling at several read/write mixes: lib/libevp1d_probe.so streams
+    nr double2 arrays of mib MiB each in and nw out (independent coalesced
+    loads), timed with HIP events; prints one JSON line."""
+    import ctypes
+    path = REPO / "cice-evp1d_amd" / "lib" / "libevp1d_probe.so"
+    if not path.exists():
+        raise ImportError(f"{path} is not built: run __graft_entry__.build()")
+    L = ctypes.CDLL(str(path))
+    L.evp1d_probe_stream.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64,
+                                     ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+    L.evp1d_probe_stream.restype = ctypes.c_int
+    n = mib * (1 << 20) // 16 // 1024 * 1024
+    rows = []
+    for per in (1, 4):
+        for nr, nw in ((1, 0), (1, 1), (2, 2), (2, 1), (3, 1), (4, 1), (5, 1), (6, 1)):
+            us = ctypes.c_double()
+            rc = L.evp1d_probe_stream(nr, nw, per, n, reps, ctypes.byref(us))
+            if rc:
+                raise RuntimeError(f"evp1d_probe_stream({nr}, {nw}, {per}) returned {rc}")
+            b = (nr + nw) * n * 16
+            rows.append({"read_streams": nr, "write_streams": nw, "points_per_thread": per,
+                         "read_fraction": round(nr / (nr + nw), 3), "bytes": b, "us": round(us.value, 2),
+                         "GB_s": round(b / us.value / 1e3, 1)})
+            log(f"[probe] {nr} in {nw} out, {per} per thread: {us.value:.1f} us, {b / us.value / 1e3:.1f} GB/s")
+    print(json.dumps({"stream_probe_mib_per_stream": mib, "launches": reps, "mixes": rows}))
+    return 0
+
+
 def time_steps(eng, ndte: int, steps: int, warmup: int) -> float:
     for k in range(warmup):
         eng.run(ndte)
@@ -595,10 +624,14 @@ def main() -> int:
                     help="subcycles per fused launch: 2 the pair form (implies --tune-fused 0)")
     ap.add_argument("--tuning", default=None, metavar="SB,SX,UB,UX",
                     help="fixed launch tuning: stress block, stress xcd order, stepu block, stepu xcd order")
+    ap.add_argument("--stream-probe", type=int, default=0, metavar="MIB",
+                    help="HBM streaming probe at several read/write mixes, MIB per stream")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB 20 times (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.stream_probe:
+        return stream_probe(args.stream_probe)
     if args.calibrate_copy:
         return calibrate_copy(args.calibrate_copy)
     if args.gpus != 1:

@@ -2,6 +2,7 @@
 
 * ``lib/libevp1d.so``      HIP kernels + C ABI, exact build (-ffp-contract=off)
 * ``lib/libevp1d_fma.so``  same sources, contraction on (fused multiply-add)
+* ``lib/libevp1d_probe.so`` HBM streaming probe (bench.py --stream-probe)
 * ``lib/evp1d_fortran/``   Fortran host layer (ISO_C_BINDING module, the
                            evp1d driver API module and its driver programs),
                            built with flang when it is available
@@ -42,6 +43,7 @@ FLAVOURS = {
     "libevp1d_fma.so": ["-ffp-contract=fast", "-DEVP1D_FMA=1"],
 }
 HIP_SOURCES = ["evp1d_device.hip", "evp2d_device.hip", "evp1d_index.cpp", "evp1d_tiles.cpp"]
+PROBE = ("libevp1d_probe.so", ["evp1d_stream_probe.hip"])
 
 FORTRAN_MODULES = ["evp1d_capi.F90", "cice_standins.F90", "ice_dyn_evp1d.F90"]
 FORTRAN_PROGRAMS = {
@@ -66,6 +68,11 @@ def build_hip(verbose: bool = True) -> list[Path]:
         _run(cmd)
         os.replace(tmp, target)
         out.append(target)
+    name, srcs = PROBE
+    tmp = LIB / (name + ".tmp")
+    _run([HIPCC, *HIP_COMMON, "-o", tmp, *[CSRC / s for s in srcs]])
+    os.replace(tmp, LIB / name)
+    out.append(LIB / name)
     return out
 
 

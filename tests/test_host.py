@@ -31,6 +31,25 @@ def test_library_exports_every_declared_symbol(fma):
     assert set(declared) == set(evp1d.EXPORTS)
 
 
+def test_stream_probe_library_rejects_unsupported_mixes():
+    """The HBM streaming probe (bench.py --stream-probe) is built beside the
+    library and refuses a read/write mix it has no kernel for before any
+    HIP call (so this runs without a GPU)."""
+    L = ctypes.CDLL(str(REPO / "cice-evp1d_amd" / "lib" / "libevp1d_probe.so"))
+    f = L.evp1d_probe_stream
+    f.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
+                  ctypes.POINTER(ctypes.c_double)]
+    f.restype = ctypes.c_int
+    us = ctypes.c_double()
+    for nr, nw in ((0, 1), (7, 1), (1, 2), (3, 3)):
+        assert f(nr, nw, 1, 1 << 20, 1, ctypes.byref(us)) == -1, (nr, nw)
+    for per in (0, 2, 8):
+        assert f(1, 1, per, 1 << 20, 1, ctypes.byref(us)) == -1, per
+    assert f(1, 1, 4, (1 << 20) + 256, 1, ctypes.byref(us)) == -1   # not a multiple of 256 x 4
+    assert f(1, 1, 1, 0, 1, ctypes.byref(us)) == -1
+    assert f(1, 1, 1, 1 << 20, 0, ctypes.byref(us)) == -1
+
+
 def _c_prototypes():
     """name -> number of parameters of every function include/evp1d.h declares."""
     text = (REPO / "include" / "evp1d.h").read_text()
