@@ -446,12 +446,21 @@ __global__ void k_pack_const(DevView d, double2 *pc, long lp) {
 // of the next phase (level-1 stepu, level-2 stress, level-2 stepu) before
 // the barrier that ends the current phase, and that phase starts on values
 // already in registers instead of one memory round trip after the barrier.
-template <int B, bool X, bool TRAJ = false, bool FIX = false>
-__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_pair(DevView d, PairView f, int last) {
+// the buffers that alternate between launches: input velocities and
+// stresses, output velocities and stresses
+struct PairBufs {
+  const double2 *vel;
+  double2 *vel_out;
+  const double2 *sg_in;
+  double2 *sg_out;
+};
+
+template <int B, bool TRAJ, bool FIX>
+__device__ __forceinline__ void pair_tile(const DevView &d, const PairView &f, PairBufs bf, int last, int tix) {
   extern __shared__ double2 lds[];
   const int cap = f.cap;
   double2 *vt = lds + 4 * cap;
-  const evp1d::PairTile pt = f.tiles[tile_of(blockIdx.x, gridDim.x, X)];
+  const evp1d::PairTile pt = f.tiles[tix];
   const evp1d::TileDesc &t = pt.t;
   const int nA = t.a1 - t.a0, nB = t.b1 - t.b0, nO = nA + nB, nS = nO + t.nh, nT = nS + pt.nx, nE = nT + pt.nq;
   const int nUo = nB + (t.c1 - t.c0), nU = nUo + pt.ne;
@@ -504,14 +513,14 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
       double sig[12];
 #pragma unroll
       for (int c = 0; c < 6; ++c) {
-        const double2 v = f.sg_in[c * f.ls + g];
+        const double2 v = bf.sg_in[c * f.ls + g];
         sig[2 * c] = v.x;
         sig[2 * c + 1] = v.y;
       }
       const double2 a0 = pc[g], a1 = pc[lp + g], a2 = pc[2 * lp + g];
       const double st = d.strength[g];
       evp1d::StressOut o;
-      const double2 cc = d.vel[g], ve = d.vel[n.y], vn = d.vel[n.z], vs = d.vel[n.w];
+      const double2 cc = bf.vel[g], ve = bf.vel[n.y], vn = bf.vel[n.z], vs = bf.vel[n.w];
       evp1d::stress_point(d.s, cc.x, cc.y, ve.x, ve.y, vs.x, vs.y, vn.x, vn.y, a0.x, a0.y, st, a1.x, a1.y, a2.x,
                           a2.y, sig, o);
       if (k == me) {
@@ -555,7 +564,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
   if (me < nU) {
     q1 = uslot(me);
     u1 = uload(me, q1);
-    v1 = d.vel[q1];
+    v1 = bf.vel[q1];
   }
   __syncthreads();
   // level 1, stepu: every U1 entry into the velocity table
@@ -569,7 +578,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
     const int q = uslot(k);
     const UIn u = uload(k, q);
     evp1d::StepuOut o;
-    stepu(u, d.vel[q], o);
+    stepu(u, bf.vel[q], o);
     vt[k] = make_double2(o.uvel, o.vvel);
     if (TRAJ && k < nUo) f.tr_mid[q - d.u_begin] = vt[k];
   }
@@ -586,12 +595,12 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
     const int4 n = f.tnbr[pt.n0 + me];
     g2 = me < nO ? tslot(me) : n.x;
     w2 = f.vloc[pt.v0 + me];
-    c2[0] = d.vel[g2];
+    c2[0] = bf.vel[g2];
     if (!FIX) {   // no copy list: every entry is a table entry or kNoEntry
       if ((w2.x >> 16) == kNo || (w2.y & 0xffffu) == kNo || (w2.y >> 16) == kNo) {
-        c2[1] = d.vel[n.y];
-        c2[2] = d.vel[n.z];
-        c2[3] = d.vel[n.w];
+        c2[1] = bf.vel[n.y];
+        c2[2] = bf.vel[n.z];
+        c2[3] = bf.vel[n.w];
       }
     } else {
       const unsigned e[3] = {w2.x >> 16, w2.y & 0xffffu, w2.y >> 16};
@@ -599,10 +608,10 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         if (e[c] == kNo) {
-          c2[1 + c] = d.vel[ns[c]];
+          c2[1 + c] = bf.vel[ns[c]];
         } else if (e[c] >= kFix) {
           const int s = f.fix[pt.f0 + (int)(e[c] - kFix)];
-          c2[1 + c] = s < 0 ? make_double2(0.0, 0.0) : d.vel[s];
+          c2[1 + c] = s < 0 ? make_double2(0.0, 0.0) : bf.vel[s];
         }
       }
     }
@@ -620,12 +629,12 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
                         m2[1].y, m2[2].x, m2[2].y, sg, o);
     const int g = g2;
     if (me < nO) {
-      f.sg_out[g] = make_double2(o.sp[0], o.sp[1]);
-      f.sg_out[f.ls + g] = make_double2(o.sp[2], o.sp[3]);
-      f.sg_out[2 * f.ls + g] = make_double2(o.sm[0], o.sm[1]);
-      f.sg_out[3 * f.ls + g] = make_double2(o.sm[2], o.sm[3]);
-      f.sg_out[4 * f.ls + g] = make_double2(o.s12[0], o.s12[1]);
-      f.sg_out[5 * f.ls + g] = make_double2(o.s12[2], o.s12[3]);
+      bf.sg_out[g] = make_double2(o.sp[0], o.sp[1]);
+      bf.sg_out[f.ls + g] = make_double2(o.sp[2], o.sp[3]);
+      bf.sg_out[2 * f.ls + g] = make_double2(o.sm[0], o.sm[1]);
+      bf.sg_out[3 * f.ls + g] = make_double2(o.sm[2], o.sm[3]);
+      bf.sg_out[4 * f.ls + g] = make_double2(o.s12[0], o.s12[1]);
+      bf.sg_out[5 * f.ls + g] = make_double2(o.s12[2], o.s12[3]);
       if (last) {
         d.s15[g] = make_double2(o.str[0], o.str[4]);
         d.s27[g] = make_double2(o.str[1], o.str[6]);
@@ -647,7 +656,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
   auto level2u = [&](int k, int q, const UIn &u) {
     evp1d::StepuOut o;
     stepu(u, vt[k], o);
-    f.vel_out[q] = make_double2(o.uvel, o.vvel);
+    bf.vel_out[q] = make_double2(o.uvel, o.vvel);
     if (TRAJ && f.tr_end) f.tr_end[q - d.u_begin] = make_double2(o.uvel, o.vvel);
     if (last) {
       d.strintx[q] = o.strintx;
@@ -659,6 +668,83 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
   for (int k = me + B; k < nUo; k += B) {
     const int q = uslot(k);
     level2u(k, q, uload(k, q));
+  }
+}
+
+template <int B, bool X, bool TRAJ = false, bool FIX = false>
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_pair(DevView d, PairView f, int last) {
+  pair_tile<B, TRAJ, FIX>(d, f, PairBufs{d.vel, f.vel_out, f.sg_in, f.sg_out}, last, tile_of(blockIdx.x, gridDim.x, X));
+}
+
+// The pair launches j0..L of a run (numbered as issue_fused numbers them) in
+// one cooperative launch.  The workgroups form P groups (P = 8, one per XCD
+// when the grid is a multiple of 8 and the tile order XCD-aware: workgroup b
+// runs on XCD b mod 8; else P = 1), and group x runs the tiles that the
+// one-launch form gives XCD x (tile_of).  Per launch, the group's workgroups
+// take its tiles from a counter, so tiles of uneven cost balance out as they
+// do when the hardware dispatches a launch; then a grid barrier, and the
+// ghost copy with another barrier when a copy list is set.  The velocities
+// alternate between d.vel and a.vy, the packed stresses between a.sg0 and
+// a.sg1, exactly as the separate launches alternate them, so results are
+// bit-identical.  a.cnt: 2 x 8 counters, zero at the launch; launch j uses
+// row j mod 2 and clears the other row for launch j + 1.
+struct PersistArgs {
+  double2 *vy, *sg0, *sg1;
+  int *cnt;
+  int j0, L, ntiles;
+};
+
+template <int B, bool X, bool FIX>
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_pair_persistent(DevView d, PairView f,
+                                                                                             PersistArgs a) {
+  cg::grid_group grid = cg::this_grid();
+  // The arguments are read in place in the kernel-argument segment (d at
+  // offset 0, f and a after it, each at its alignment), through a pointer
+  // that an empty asm statement makes opaque once per tile: their fields are
+  // then loaded where the tile body uses them, as in k_pair, instead of
+  // being hoisted out of the loops and held in registers for the whole
+  // launch (which spilled).
+  typedef const __attribute__((address_space(4))) char *ArgP;
+  constexpr size_t kOffF = (sizeof(DevView) + alignof(PairView) - 1) / alignof(PairView) * alignof(PairView);
+  constexpr size_t kOffA =
+      (kOffF + sizeof(PairView) + alignof(PersistArgs) - 1) / alignof(PersistArgs) * alignof(PersistArgs);
+  ArgP ka = (ArgP)__builtin_amdgcn_kernarg_segment_ptr();
+  const int j0 = a.j0, L = a.L, ntiles = a.ntiles;
+  // this workgroup's group x and the group's tiles [t0, t0 + nt)
+  const int P = (X && gridDim.x >= kXcds && gridDim.x % kXcds == 0) ? kXcds : 1;
+  const int x = blockIdx.x % P;
+  const int q = ntiles / P, r = ntiles % P;
+  const int t0 = x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q, nt = q + (x < r ? 1 : 0);
+  __shared__ int grab;
+  for (int j = j0; j <= L; ++j) {
+    const bool to_x = ((L - j) % 2) == 0;   // the run ends in the handle's own buffer
+    const bool odd = ((j - j0) % 2) == 1;   // which packed stress buffer is the input
+    int *cnt = a.cnt + (j % 2) * kXcds;
+    if (blockIdx.x == 0 && threadIdx.x < kXcds)   // with the atomics' scope, published by the barrier
+      __hip_atomic_store(a.cnt + ((j + 1) % 2) * kXcds + threadIdx.x, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (;;) {
+      if (threadIdx.x == 0) grab = atomicAdd(cnt + x, 1);
+      __syncthreads();
+      const int k = grab;   // read by every thread before the tile's first barrier
+      if (k >= nt) break;
+      asm volatile("" : "+s"(ka));
+      const DevView &dv = *(const DevView *)(const __attribute__((address_space(4))) DevView *)ka;
+      const PairView &fv = *(const PairView *)(const __attribute__((address_space(4))) PairView *)(ka + kOffF);
+      const PersistArgs &av =
+          *(const PersistArgs *)(const __attribute__((address_space(4))) PersistArgs *)(ka + kOffA);
+      const PairBufs bf{to_x ? av.vy : dv.vel, to_x ? dv.vel : av.vy, odd ? av.sg1 : av.sg0, odd ? av.sg0 : av.sg1};
+      pair_tile<B, false, FIX>(dv, fv, bf, j == L ? 1 : 0, t0 + k);
+      __syncthreads();   // the next tile rewrites the LDS tables and grab
+    }
+    grid.sync();
+    if (d.nhalo > 0) {   // the copy after the launch's second subcycle
+      double2 *out = to_x ? d.vel : a.vy;
+      for (int k = blockIdx.x * B + threadIdx.x; k < d.nhalo; k += gridDim.x * B) {
+        const int dst = d.halo_dst[k], src = d.halo_src[k];
+        out[dst] = src >= 0 ? out[src] : make_double2(0.0, 0.0);
+      }
+      grid.sync();
+    }
   }
 }
 
@@ -1077,6 +1163,7 @@ struct evp1d_handle {
   std::vector<int32_t> m_na, m_navel, a_off, g_off;
   std::vector<uint8_t> idx_mset[6];   // per member: list set
   int coop_blocks = 0;
+  int *pcnt_dev = nullptr;         // persistent pair form: tile counters (2 x 8)
   // fused form: tiling and its device tables (owned here, replaced on relayout)
   int tile_w = 64, tile_h = 16, tile_block = 512, tile_xcd = 1;
   bool tiling_auto = true;         // choose the tile from the domain size (until evp1d_set_tiling)
@@ -1207,6 +1294,7 @@ static void release(evp1d_handle *h) {
   if (h->pconst) (void)hipFree(h->pconst);
   if (h->sig_pair) (void)hipFree(h->sig_pair);
   if (h->nbr4) (void)hipFree(h->nbr4);
+  if (h->pcnt_dev) (void)hipFree(h->pcnt_dev);
   if (h->cnz_dev) (void)hipFree(h->cnz_dev);
   if (h->traj) (void)hipFree(h->traj);
   if (h->adj_ws) (void)hipFree(h->adj_ws);
@@ -1581,6 +1669,16 @@ static const PairFn kPairFixFns[kNumPairBlocks][2] = {
 // the gradient sweep's trajectory-storing form (XCD-aware tile order only)
 static const PairFn kPairTrajFns[kNumPairBlocks] = {k_pair<256, true, true>, k_pair<512, true, true>,
                                                     k_pair<1024, true, true>};
+// one cooperative launch for the pair launches of a run (evp1d_run in
+// EVP1D_MODE_PERSISTENT with the pair form selected)
+typedef void (*PairPersistFn)(DevView, PairView, PersistArgs);
+static const PairPersistFn kPairPersistFns[kNumPairBlocks][2][2] = {
+    {{k_pair_persistent<256, false, false>, k_pair_persistent<256, false, true>},
+     {k_pair_persistent<256, true, false>, k_pair_persistent<256, true, true>}},
+    {{k_pair_persistent<512, false, false>, k_pair_persistent<512, false, true>},
+     {k_pair_persistent<512, true, false>, k_pair_persistent<512, true, true>}},
+    {{k_pair_persistent<1024, false, false>, k_pair_persistent<1024, false, true>},
+     {k_pair_persistent<1024, true, false>, k_pair_persistent<1024, true, true>}}};
 static int pair_block_index(int b) {
   for (int i = 0; i < kNumPairBlocks; ++i)
     if (kPairBlocks[i] == b) return i;
@@ -1677,9 +1775,34 @@ static void fused_step(evp1d_handle *h, FusedView &f, int k) {
 // traj (pair form only; the gradient's forward sweep): also store the stepu
 // slots' velocities after subcycles 1 .. nsub - 1 at traj + k * nu, k the
 // subcycle, nu = u_end - u_begin (the state after the last stays in X only).
-static int issue_fused(evp1d_handle *h, int nsub, double2 *traj = nullptr) {
+// The pair launches j0..L of a run as one cooperative launch (k_pair_persistent):
+// as many workgroups as are co-resident (a multiple of the XCD count), at
+// most one per tile.
+static int launch_pair_persistent(evp1d_handle *h, const DevView &x, PairView pv, double2 *sg0, double2 *sg1,
+                                  int j0, int L, size_t lds) {
+  const int b = pair_block_index(h->pair_block);
+  const PairPersistFn fn = kPairPersistFns[b][h->tile_xcd ? 1 : 0][h->v.nhalo > 0 ? 1 : 0];
+  int per_cu = 0, cus = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, h->pair_block, lds));
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
+  if (per_cu <= 0 || cus <= 0) return fail(EVP1D_EHIP, "no co-resident workgroups for k_pair_persistent");
+  int grid = std::min(per_cu * cus, h->fused_ntiles);
+  if (grid >= kXcds) grid -= grid % kXcds;
+  if (!h->pcnt_dev) HIP_TRY(hipMalloc((void **)&h->pcnt_dev, 2 * kXcds * sizeof(int)));
+  HIP_TRY(hipMemsetAsync(h->pcnt_dev, 0, 2 * kXcds * sizeof(int), h->stream));
+  DevView d = x;
+  PersistArgs a{h->vel_alt, sg0, sg1, h->pcnt_dev, j0, L, h->fused_ntiles};
+  void *args[] = {&d, &pv, &a};
+  HIP_TRY(hipLaunchCooperativeKernel((const void *)fn, dim3(grid), dim3(h->pair_block), args, lds, h->stream));
+  return EVP1D_OK;
+}
+
+// persistent: the pair launches as one cooperative launch (eager only: a
+// cooperative launch is not captured into the run's graph)
+static int issue_fused(evp1d_handle *h, int nsub, double2 *traj = nullptr, bool persistent = false) {
   const bool pair = pair_form(h);
   if (traj && !pair) return fail(EVP1D_ESTATE, "issue_fused: a trajectory needs the pair form");
+  if (persistent && (!pair || traj)) return fail(EVP1D_ESTATE, "issue_fused: the persistent form needs the pair form");
   const bool odd_first = pair && nsub % 2 == 1;
   const long nu = (long)h->v.u_end - h->v.u_begin;
   const bool ip = h->fused_inplace && !pair;
@@ -1727,6 +1850,11 @@ static int issue_fused(evp1d_handle *h, int nsub, double2 *traj = nullptr) {
           HIP_TRY(hipGetLastError());
         }
         packed = true;
+      }
+      if (persistent) {   // this and every later launch, in one
+        if (h->fused_ntiles > 0) RC_TRY(launch_pair_persistent(h, X, pv, P[0], P[1], j, L, lds2));
+        pin = (pin + (L - j + 1)) % 2;
+        break;
       }
       pv.vel_out = out.vel;
       pv.sg_in = P[pin];
@@ -1799,6 +1927,10 @@ static int run_graph(evp1d_handle *h, int nsub, bool fused) {
 }
 
 static int run_persistent(evp1d_handle *h, int nsub) {
+  if (h->fused_ok && pair_form(h)) {   // the pair form's tiles, one grid barrier per two subcycles
+    RC_TRY(ensure_fused_buffers(h));
+    return issue_fused(h, nsub, nullptr, true);
+  }
   if (h->coop_blocks <= 0) {
     int per_cu = 0, cus = 0;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persistent, kBlock, 0));

@@ -104,7 +104,10 @@ enum evp1d_skip { EVP1D_SKIPT = 0, EVP1D_SKIPU = 1 };
 enum evp1d_mode {
   EVP1D_MODE_LAUNCH = 0,      /* one stress and one stepu launch per subcycle */
   EVP1D_MODE_GRAPH = 1,       /* the nsub x (stress, stepu, halo) sequence captured once as a hipGraph */
-  EVP1D_MODE_PERSISTENT = 2,  /* one cooperative kernel, grid barrier between phases */
+  EVP1D_MODE_PERSISTENT = 2,  /* one cooperative kernel for the whole loop: with the pair form selected
+                                 (evp1d_set_fused_depth 2, planned) the pair form's tiles with one grid
+                                 barrier per two subcycles, else stress and stepu over all points with a
+                                 grid barrier between phases (hipLaunchCooperativeKernel) */
   EVP1D_MODE_AUTO = 3,        /* library default: FUSED when the lists tile and nsub >= 4, else GRAPH */
   EVP1D_MODE_FUSED = 4        /* one launch per subcycle: stress and stepu per tile of points, str1..8
                                  kept on chip (hipGraph); needs lists that describe a grid */

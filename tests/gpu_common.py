@@ -52,15 +52,16 @@ def load_golden_v0(case: str):
 
 
 def set_mode(eng: evp1d.Engine, mode: str) -> None:
-    """mode: a loop form, "fused-inplace" (fused, stresses updated in place)
-    or "fused-pair" (fused, two subcycles per launch on 32x16 tiles)."""
+    """mode: a loop form, "fused-inplace" (fused, stresses updated in place),
+    "fused-pair" (fused, two subcycles per launch on 32x16 tiles) or
+    "persistent-pair" (the pair form's launches as one cooperative launch)."""
     if mode == "fused-inplace":
         eng.set_fused_inplace(True)
         mode = "fused"
-    elif mode == "fused-pair":
+    elif mode in ("fused-pair", "persistent-pair"):
         eng.set_fused_depth(2)
         eng.set_tiling(32, 16, 512, 1)
-        mode = "fused"
+        mode = "fused" if mode == "fused-pair" else "persistent"
     eng.set_mode(mode)
 
 

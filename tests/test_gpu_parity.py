@@ -18,7 +18,7 @@ from gpu_common import (GOLD, assert_state_bitexact, fetch_state, golden_cases, 
                         make_engine, oracle_params, set_mode, synthetic_state)
 
 pytestmark = pytest.mark.gpu
-MODES = ("launch", "graph", "persistent", "fused", "fused-inplace", "fused-pair")
+MODES = ("launch", "graph", "persistent", "fused", "fused-inplace", "fused-pair", "persistent-pair")
 
 
 def _log(msg):
@@ -95,7 +95,7 @@ def test_halo_copy_list(gpu):
     for mode in MODES:
         with make_engine(st, 9, mode) as eng:
             eng.set_halo(dst, src)
-            if mode == "fused-pair":   # the copy list runs between the two subcycles of a launch
+            if mode in ("fused-pair", "persistent-pair"):   # the copy list runs between the two subcycles of a launch
                 assert eng.get_fused_depth()["active"]
             eng.run(9)
             got = fetch_state(eng, st)

@@ -4,11 +4,13 @@
   the flang-built reference f0's (tests/golden/golden.json, made by
   tests/golden/make_golden.py evp_c1) in the library default form, the
   two-kernel graph, the persistent kernel, the pair form on 31x15 tiles in
-  Hilbert order (the form the benchmark's tuner picks) and whatever
-  evp1d_autotune_fused picks on the box; the contraction-on build within
-  1e-12 norm-wise of the oracle in the graph and pair forms.
+  Hilbert order (the form the benchmark's tuner picks), the same pair
+  launches as one cooperative launch, and whatever evp1d_autotune_fused
+  picks on the box; the contraction-on build within 1e-12 norm-wise of the
+  oracle in the graph and pair forms.
 * 2560x3040, 1000 subcycles (na = 5,447,680): md5 equal to the reference f0's
-  in the pair form, the one-subcycle fused form and the persistent kernel.
+  in the pair form, the one-subcycle fused form, the persistent kernel and
+  the persistent pair form.
 * 64 ensemble members in one batched handle (25.2 M points, 1000 subcycles):
   every member's output md5 equals its own oracle run
   (tests/golden/make_golden_ensemble.py).
@@ -84,8 +86,8 @@ def set_form(eng: evp1d.Engine, form: str) -> None:
     elif form == "fused-64x16":
         eng.set_mode("fused")
         eng.set_tiling(64, 16, 512, 1)
-    elif form == "pair-31x15-hilbert":
-        eng.set_mode("fused")
+    elif form in ("pair-31x15-hilbert", "persistent-pair-31x15-hilbert"):
+        eng.set_mode("fused" if form.startswith("pair") else "persistent")
         eng.set_tiling(31, 15, 512, 1)
         eng.set_fused_depth(2)
         eng.set_tile_order(1)
@@ -100,7 +102,7 @@ def run_form(st: O.State1D, ndte: int, form: str, fma: bool = False) -> O.State1
     with evp1d.Engine(st.na, st.navel, p, fma=fma) as eng:
         eng.load_state(st.f, st.idx, st.skipT, st.skipU)
         set_form(eng, form)
-        if form == "pair-31x15-hilbert":
+        if form.endswith("pair-31x15-hilbert"):
             d = eng.get_fused_depth()
             assert d["active"], d
         t = time.time()
@@ -117,7 +119,8 @@ def check_md5(got: O.State1D, rec: dict) -> None:
 
 
 @pytest.mark.timeout(400)
-@pytest.mark.parametrize("form", ["auto", "graph", "persistent", "pair-31x15-hilbert", "tuned"])
+@pytest.mark.parametrize("form", ["auto", "graph", "persistent", "pair-31x15-hilbert", "persistent-pair-31x15-hilbert",
+                                  "tuned"])
 def test_c1_md5(c1, form):
     """1280x1520, 1000 subcycles: md5 of both output files equal reference f0's."""
     _log(f"evp_c1 form={form}")
@@ -152,7 +155,7 @@ def test_c1_fma_within_1e12(c1):
 
 
 @pytest.mark.timeout(400)
-@pytest.mark.parametrize("form", ["pair-31x15-hilbert", "fused-64x16", "persistent"])
+@pytest.mark.parametrize("form", ["pair-31x15-hilbert", "fused-64x16", "persistent", "persistent-pair-31x15-hilbert"])
 def test_c2_md5(c2, form):
     """2560x3040, 1000 subcycles: md5 of both output files equal reference f0's."""
     _log(f"evp_c2 form={form}")
