@@ -199,10 +199,7 @@ def calibrate_copy(mib: int) -> int:
     return 0
 
 
-def stream_probe(mib: int, reps: int = 20) -> int:
-    """HBM ceiling at several read/write mixes: lib/libevp1d_probe.so streams
-    nr double2 arrays of mib MiB each in and nw out (independent coalesced
-    loads), timed with HIP events; prints one JSON line."""
+def _probe_lib():
     import ctypes
     path = REPO / "cice-evp1d_amd" / "lib" / "libevp1d_probe.so"
     if not path.exists():
@@ -211,6 +208,27 @@ def stream_probe(mib: int, reps: int = 20) -> int:
     L.evp1d_probe_stream.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64,
                                      ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
     L.evp1d_probe_stream.restype = ctypes.c_int
+    return L
+
+
+def probe_rate(nr: int, nw: int, mib: int = 512, reps: int = 20) -> float:
+    """GB/s (read + written) of the streaming probe at nr read and nw written
+    streams of mib MiB, one point per thread."""
+    import ctypes
+    n = mib * (1 << 20) // 16 // 1024 * 1024
+    us = ctypes.c_double()
+    rc = _probe_lib().evp1d_probe_stream(nr, nw, 1, n, reps, ctypes.byref(us))
+    if rc:
+        raise RuntimeError(f"evp1d_probe_stream({nr}, {nw}, 1) returned {rc}")
+    return (nr + nw) * n * 16 / us.value / 1e3
+
+
+def stream_probe(mib: int, reps: int = 20) -> int:
+    """HBM ceiling at several read/write mixes: lib/libevp1d_probe.so streams
+    nr double2 arrays of mib MiB each in and nw out (independent coalesced
+    loads), timed with HIP events; prints one JSON line."""
+    import ctypes
+    L = _probe_lib()
     n = mib * (1 << 20) // 16 // 1024 * 1024
     rows = []
     for per in (1, 4):
@@ -749,6 +767,19 @@ def main() -> int:
         phys = traffic / (dom["avg_us"] * 1e-6) / 1e9
         roofline.update({"achieved_physical": round(phys, 1), "frac_physical": round(phys / HBM_PEAK_GBS, 4),
                          "traffic_over_algorithmic": round(traffic / dom["bytes"], 3)})
+        # the box's practical ceiling at the kernel's read/write mix (counters:
+        # k_pair 83 % reads, k_fused 76 %, k_stress 56 %, k_stepu 94 %):
+        # the streaming probe at the nearest whole-stream mix, measured now
+        mix = {"k_pair": (5, 1), "k_fused": (3, 1), "k_stress": (1, 1), "k_stepu": (6, 1)}.get(dom["name"])
+        if mix:
+            try:
+                ceil = probe_rate(*mix)
+                roofline["probe"] = {"read_streams": mix[0], "write_streams": mix[1], "GBps": round(ceil, 1),
+                                     "frac_physical_of_probe": round(phys / ceil, 4)}
+                log(f"[bench] streaming probe {mix[0]}:{mix[1]}: {ceil:.1f} GB/s; {dom['name']} at "
+                    f"{phys / ceil:.3f} of it")
+            except Exception as e:   # reported, never fatal
+                roofline["probe"] = {"error": repr(e)[:200]}
     if dom["name"] == "k_pair":
         roofline["note"] = ("frac counts the reference's bytes of two subcycles; the pair form keeps "
                             "str1..8 and the first subcycle's state on chip, so it moves fewer bytes than "

@@ -1188,7 +1188,7 @@ struct evp1d_handle {
   int fused_inplace = 0;           // fused runs update stresses in place
   // pair form (two subcycles per fused launch): its tables, planned on the
   // current tiling when fused_depth is 2
-  int fused_depth = 1;
+  int fused_depth = 2;
   int tile_order = 0;              // 0 rows of tiles, 1 Hilbert curve over the tiles
   // subcycles per checkpointing unit of a gradient with fewer stored states
   // than subcycles (0: chosen per call)
@@ -1508,11 +1508,19 @@ static int relayout_impl(evp1d_handle *h) {
     h->tile_cache_ok = false;
   }
   if (h->tiling_auto && !h->tile_cache_ok) {
-    // enough tiles to fill the GPU: the largest tile of the ladder that
-    // still gives min_tiles tiles (256 CUs)
+    // one subcycle per launch: enough tiles to fill the GPU, the largest
+    // tile of the ladder that still gives min_tiles tiles (256 CUs); the
+    // pair form: 31x19 cells, the tuner's choice from 84k to 5.4M points
+    // (at most 640 level-2 stress points per tile: one 512- or 1024-thread
+    // workgroup; one subcycle per launch on the same tiles where the pair
+    // tables do not fit)
     struct Rung { int w, h, block, min_tiles; };
     static const Rung ladder[3] = {{64, 16, 512, 1024}, {32, 8, 256, 512}, {16, 8, 128, 0}};
-    for (const Rung &r : ladder) {
+    static const Rung pair_rung = {31, 19, 512, 0};
+    const Rung *rungs = h->fused_depth == 2 ? &pair_rung : ladder;
+    const int nrungs = h->fused_depth == 2 ? 1 : 3;
+    for (int k = 0; k < nrungs; ++k) {
+      const Rung &r = rungs[k];
       h->tile_cache = evp1d::tile_ids(h->coords, r.w, r.h, h->tile_order);
       h->tile_w = r.w;
       h->tile_h = r.h;
@@ -2556,6 +2564,7 @@ int evp1d_set_fused_depth(evp1d_handle *h, int32_t depth) {
     h->fused_depth = depth;
     h->layout_dirty = true;                     // plan (or drop) the pair tables
     h->graph_dirty = true;
+    if (h->tiling_auto) h->tile_cache_ok = false;   // the library's tile depends on the depth
   }
   return EVP1D_OK;
 }

@@ -18,9 +18,11 @@
 ! do not line up, src/vars.F90:1801-1838).
 !
 ! Usage:  evp1d_v2_driver [graph|persistent|launch|fused|fused-inplace|fused-pair|tuned]
-! (fused-inplace: fused form with stresses updated in place; fused-pair: two
-! subcycles per fused launch on 31x15 tiles; tuned: fused form with the tile,
-! stress storage and depth chosen by evp1d_autotune_fused)
+! (no argument: the library's default, the pair form where the lists tile;
+! fused: one subcycle per fused launch; fused-inplace: the same with
+! stresses updated in place; fused-pair: two subcycles per fused launch on
+! 31x15 tiles; tuned: fused form with the tile, stress storage and depth
+! chosen by evp1d_autotune_fused)
 !===============================================================================
 program evp1d_v2_driver
   use, intrinsic :: iso_c_binding
@@ -64,15 +66,15 @@ program evp1d_v2_driver
   if (na < 0 .or. navel < na) error stop 'ERROR: kernel.nml needs 0 <= na <= navel'
   write(*,'(a,i0,a,i0,a,i0)') ' na = ', na, '  navel = ', navel, '  ndte = ', ndte
 
-  mode = MODE_AUTO; inplace = 0; depth = 1; tune = .false.
+  mode = MODE_AUTO; inplace = 0; depth = 0; tune = .false.
   if (command_argument_count() >= 1) then
     call get_command_argument(1, arg)
     select case (trim(arg))
     case ('graph');      mode = MODE_GRAPH
     case ('persistent'); mode = MODE_PERSISTENT
     case ('launch');     mode = MODE_LAUNCH
-    case ('fused');      mode = MODE_FUSED
-    case ('fused-inplace'); mode = MODE_FUSED; inplace = 1
+    case ('fused');      mode = MODE_FUSED; depth = 1
+    case ('fused-inplace'); mode = MODE_FUSED; inplace = 1; depth = 1
     case ('fused-pair'); mode = MODE_FUSED; depth = 2
     case ('tuned');      mode = MODE_FUSED; tune = .true.
     case default;        error stop 'usage: evp1d_v2_driver [graph|persistent|launch|fused|fused-inplace|fused-pair|tuned] [grad]'
@@ -133,10 +135,10 @@ program evp1d_v2_driver
     call evp1d_check(evp1d_set_field(h, hid(k), ht(:,k), int(na,c_int32_t)), 'set_field')
   enddo
   call evp1d_check(evp1d_set_fused_inplace(h, inplace), 'set_fused_inplace')
-  if (depth == 2) then
+  if (depth > 0) &   ! 0: the library's default (the pair form where the lists tile)
     call evp1d_check(evp1d_set_fused_depth(h, depth), 'set_fused_depth')
+  if (depth == 2) &
     call evp1d_check(evp1d_set_tiling(h, 31_c_int32_t, 15_c_int32_t, 512_c_int32_t, 1_c_int32_t), 'set_tiling')
-  endif
   if (tune) then
     call evp1d_check(evp1d_autotune_fused(h, int(min(ndte, 100),c_int32_t), tune_us), 'autotune_fused')
     write(*,'(a,f10.2,a)') ' autotune_fused: ', tune_us, ' us per subcycle'

@@ -223,9 +223,11 @@ int evp1d_get_tuning(const evp1d_handle *h, int32_t *stress_block, int32_t *stre
  * the library recovers from the neighbour lists.  evp1d_set_tiling chooses
  * the tile ((tile_w+1)(tile_h+1) <= 2560), the workgroup size (64 to 512)
  * and the XCD-aware tile order (xcd = 0/1); a new tile re-lays the points out
- * at the next run.  Until it is called the library chooses by domain size:
- * 64x16 cells on 512 threads if that gives at least 1024 tiles, else 32x8 on
- * 256 if at least 512, else 16x8 on 128 (XCD-aware order).
+ * at the next run.  Until it is called the library chooses: 31x19 cells on
+ * 512 threads for the pair form (fused depth 2, the default), and for one
+ * subcycle per launch by domain size, 64x16 cells on 512 threads if that
+ * gives at least 1024 tiles, else 32x8 on 256 if at least 512, else 16x8 on
+ * 128 (XCD-aware order).
  * evp1d_get_tiling reports the tiling and the number of tiles (0 when the
  * lists do not describe a grid: the fused form is then unavailable).
  * Results never depend on the tiling.                                       */
@@ -241,7 +243,7 @@ int evp1d_get_tiling(evp1d_handle *h, int32_t *tile_w, int32_t *tile_h, int32_t 
  * results are identical.                                                    */
 int evp1d_set_fused_inplace(evp1d_handle *h, int32_t inplace);
 int evp1d_get_fused_inplace(evp1d_handle *h, int32_t *inplace);
-/* Subcycles per fused launch.  depth = 1 (default): one.  depth = 2 (the
+/* Subcycles per fused launch.  depth = 1: one.  depth = 2 (default; the
  * pair form): each workgroup runs two subcycles of its tile, the first over
  * a two-cell border ring that the neighbouring tiles also evaluate, so state
  * and constant fields cross HBM once per two subcycles; an odd nsub starts

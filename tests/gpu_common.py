@@ -52,11 +52,14 @@ def load_golden_v0(case: str):
 
 
 def set_mode(eng: evp1d.Engine, mode: str) -> None:
-    """mode: a loop form, "fused-inplace" (fused, stresses updated in place),
-    "fused-pair" (fused, two subcycles per launch on 32x16 tiles) or
-    "persistent-pair" (the pair form's launches as one cooperative launch)."""
-    if mode == "fused-inplace":
-        eng.set_fused_inplace(True)
+    """mode: a loop form ("auto" is the library's default: the pair form where
+    the lists tile), "fused" and "fused-inplace" (one subcycle per launch,
+    stresses in two buffers or updated in place), "fused-pair" (two
+    subcycles per launch on 32x16 tiles) or "persistent-pair" (the pair
+    form's launches as one cooperative launch)."""
+    if mode in ("fused", "fused-inplace"):
+        eng.set_fused_depth(1)
+        eng.set_fused_inplace(mode == "fused-inplace")
         mode = "fused"
     elif mode in ("fused-pair", "persistent-pair"):
         eng.set_fused_depth(2)

@@ -18,7 +18,7 @@ from gpu_common import (GOLD, assert_state_bitexact, fetch_state, golden_cases, 
                         make_engine, oracle_params, set_mode, synthetic_state)
 
 pytestmark = pytest.mark.gpu
-MODES = ("launch", "graph", "persistent", "fused", "fused-inplace", "fused-pair", "persistent-pair")
+MODES = ("auto", "launch", "graph", "persistent", "fused", "fused-inplace", "fused-pair", "persistent-pair")
 
 
 def _log(msg):

@@ -85,6 +85,7 @@ def set_form(eng: evp1d.Engine, form: str) -> None:
         eng.set_mode(form)
     elif form == "fused-64x16":
         eng.set_mode("fused")
+        eng.set_fused_depth(1)
         eng.set_tiling(64, 16, 512, 1)
     elif form in ("pair-31x15-hilbert", "persistent-pair-31x15-hilbert"):
         eng.set_mode("fused" if form.startswith("pair") else "persistent")
