@@ -12,6 +12,7 @@ owned points that other tiles evaluate as halo ("shared") lead their tile's
 owned ranges and carry consecutive ids in slot order, since during a run
 their stresses live in a table indexed by those ids.
 """
+import os
 from pathlib import Path
 
 import numpy as np
@@ -280,3 +281,41 @@ def test_hilbert_tile_order(tmp_path):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "all good" in r.stdout and "order 1:" in r.stdout
+
+
+def test_host_planning_clean_under_sanitizers(tmp_path):
+    """The planner checks (pair tables with the schedule model and the
+    corruptions, tile plan corruptions, Hilbert order) and a fuzz of the 1D
+    index builder, built host-only with AddressSanitizer and UBSan: every
+    program must pass and no sanitizer may report."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    repo = Path(__file__).resolve().parent.parent
+    csrc = repo / "cice-evp1d_amd" / "csrc"
+    san = ["-O1", "-g", "-std=c++17", "-pthread", "-fsanitize=address,undefined", "-fno-omit-frame-pointer",
+           "-fno-sanitize-recover=undefined", f"-I{csrc}", f"-I{repo / 'include'}"]
+    g = synth.make_grid(300, 200, 72)
+    ix = evp1d.build_index(g.tmask, g.nghost)
+    from evp1d import layout
+    s = layout.pack(g, ix)
+    skip = (s["skipT"] != 0).astype(np.uint8) | ((s["skipU"] != 0).astype(np.uint8) << 1)
+    with open(tmp_path / "in.bin", "wb") as fh:
+        fh.write(np.array([s["na"], s["navel"]], np.int32).tobytes())
+        for n in evp1d.NBR:
+            fh.write((s["idx"][n].astype(np.int32) - 1).tobytes())
+        fh.write(skip.tobytes())
+    progs = {"pair_plan_check": csrc / "evp1d_tiles.cpp", "plan_check_corruptions": csrc / "evp1d_tiles.cpp",
+             "tile_order_check": csrc / "evp1d_tiles.cpp", "index_fuzz": csrc / "evp1d_index.cpp"}
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", UBSAN_OPTIONS="print_stacktrace=1")
+    for name, lib in progs.items():
+        exe = tmp_path / name
+        subprocess.run(["g++", *san, str(repo / "tests" / "cpp" / f"{name}.cpp"), str(lib), "-o", str(exe)],
+                       check=True, capture_output=True)
+        r = subprocess.run([str(exe), str(tmp_path / "in.bin")], capture_output=True, text=True, timeout=300,
+                           env=env)
+        out = r.stdout + r.stderr
+        assert r.returncode == 0, (name, out[-2000:])
+        assert "AddressSanitizer" not in out and "runtime error" not in out, (name, out[-2000:])
+
