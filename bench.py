@@ -729,8 +729,9 @@ def main() -> int:
     # copies and packing, about 1 % of it, are included), so launches x
     # avg_us equals ms_per_step.
     run_us = secs / args.steps * 1e6
-    if args.mode == "persistent":
-        dom = {"name": "k_persistent", "bytes": B["subcycle"] * ndte, "avg_us": run_us, "launches_per_step": 1}
+    if args.mode == "persistent":   # one cooperative launch per run: the pair form's tiles when it is active
+        dom = {"name": "k_pair_persistent" if tiling["depth"] == 2 else "k_persistent",
+               "bytes": B["subcycle"] * ndte, "avg_us": run_us, "launches_per_step": 1}
     elif fused and tiling["depth"] == 2:
         nl = ndte / 2.0     # one k_pair launch = two subcycles (an odd ndte adds one one-subcycle launch)
         dom = {"name": "k_pair", "bytes": 2 * B["subcycle"], "avg_us": run_us / nl,
@@ -828,7 +829,8 @@ def main() -> int:
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{args.workload} {w['nx']}x{w['ny']} ndte={ndte}", "nx": w["nx"],
                    "ny": w["ny"], "seed": w["seed"], "na": na, "navel": s["navel"], "nT": nT, "nU": nU,
-                   "ndte": ndte, "mode": args.mode, "form": "fused" if fused else "unfused", "tiling": tiling,
+                   "ndte": ndte, "mode": args.mode, "form": "fused" if fused else dom["name"] if args.mode == "persistent" else "unfused",
+                   "tiling": tiling,
                    "tiling_chosen_by": ("evp1d_autotune_fused (setup, untimed)" if tuned_fused else
                                         "--tiling/--inplace/--depth/--order" if (args.tiling or args.inplace is not None
                                                                          or args.depth is not None
