@@ -464,6 +464,13 @@ def ensemble_measure(nmem: int, steps: int, autotune: int, tune_fused: int = 0, 
     if fused_batch:
         log(f"[bench] ensemble: batch autotune_fused: {fused_batch}")
     t_batch = time_steps(batch, ndte, steps, 1) / steps
+    # the whole ensemble's run as one cooperative launch (the persistent form
+    # on the same tiles), results identical
+    t_batch_pers = None
+    if batch.get_fused_depth()["active"]:
+        batch.set_mode("persistent")
+        t_batch_pers = time_steps(batch, ndte, steps, 1) / steps
+        batch.set_mode("auto")
     batch.close()
     singles = [new_engine(s, ndte) for s in members]
     depths = []
@@ -522,10 +529,14 @@ def ensemble_measure(nmem: int, steps: int, autotune: int, tune_fused: int = 0, 
            "batched_pts_sub_per_s": tot * ndte / t_batch, "sequential_pts_sub_per_s": tot * ndte / t_seq,
            "lockstep_pts_sub_per_s": tot * ndte / t_lock,
            "speedup": round(t_seq / t_batch, 3), "speedup_vs_lockstep": round(t_lock / t_batch, 3),
+           "batched_persistent_s": round(t_batch_pers, 4) if t_batch_pers else None,
+           "persistent_speedup_vs_lockstep": round(t_lock / t_batch_pers, 3) if t_batch_pers else None,
            "tuning": tuning, "fused_batch": fused_batch,
            "members_tuned": "each member (evp1d_autotune_fused)" if (tune_fused > 0 and tune_members) else
                             "the batch's choice" if fused_batch else "library default",
            "members_on_pair_form": depths.count(2) if depths else None}
+    if t_batch_pers:
+        log(f"[bench] ensemble: batched as one cooperative launch {t_batch_pers:.3f}s")
     log(f"[bench] ensemble: batched {t_batch:.3f}s, member after member {t_seq:.3f}s (x{t_seq / t_batch:.2f}), "
         f"lockstep {t_lock:.3f}s (x{t_lock / t_batch:.2f}), member after member unfused {t_seq_graph:.3f}s "
         f"(x{t_seq_graph / t_batch:.2f})")
