@@ -436,6 +436,15 @@ __global__ void k_pack_const(DevView d, double2 *pc, long lp) {
   pc[9 * lp + i] = make_double2(d.uvel_init[i], d.vvel_init[i]);
 }
 
+// the buffers that alternate between launches: input velocities and
+// stresses, output velocities and stresses
+struct PairBufs {
+  const double2 *vel;
+  double2 *vel_out;
+  const double2 *sg_in;
+  double2 *sg_out;
+};
+
 // At most 128 VGPRs (4 waves per SIMD): workgroups of 256, 512 or 1024
 // threads then fill 16 waves per CU.  TRAJ: the gradient's forward sweep,
 // which also stores both subcycles' stepu-slot velocities (f.tr_mid,
@@ -446,15 +455,6 @@ __global__ void k_pack_const(DevView d, double2 *pc, long lp) {
 // of the next phase (level-1 stepu, level-2 stress, level-2 stepu) before
 // the barrier that ends the current phase, and that phase starts on values
 // already in registers instead of one memory round trip after the barrier.
-// the buffers that alternate between launches: input velocities and
-// stresses, output velocities and stresses
-struct PairBufs {
-  const double2 *vel;
-  double2 *vel_out;
-  const double2 *sg_in;
-  double2 *sg_out;
-};
-
 template <int B, bool TRAJ, bool FIX>
 __device__ __forceinline__ void pair_tile(const DevView &d, const PairView &f, PairBufs bf, int last, int tix) {
   extern __shared__ double2 lds[];
@@ -1187,7 +1187,7 @@ struct evp1d_handle {
   double *sig_alt = nullptr;       // second stress buffer (12 x ldsig), two-buffer fused runs
   int fused_inplace = 0;           // fused runs update stresses in place
   // pair form (two subcycles per fused launch): its tables, planned on the
-  // current tiling when fused_depth is 2
+  // current tiling when fused_depth is 2 (the default)
   int fused_depth = 2;
   int tile_order = 0;              // 0 rows of tiles, 1 Hilbert curve over the tiles
   // subcycles per checkpointing unit of a gradient with fewer stored states
