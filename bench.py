@@ -664,8 +664,8 @@ def main() -> int:
     if args.calibrate_copy:
         return calibrate_copy(args.calibrate_copy)
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design "
-                                   "note (DESIGN.md), not a measured path", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "the engine runs on one GPU; sharding over several GPUs is designed, "
+                                   "not implemented", "n_gpus": args.gpus}))
         return 2
     if args.path == "v0":
         form = "plain" if args.mode == "graph" else "fused"
