@@ -259,6 +259,49 @@ def test_batch_members_match_independent_oracle_runs(gpu):
                 assert_state_bitexact(got, wants[k])
 
 
+@pytest.mark.parametrize("mode", ["auto", "fused-pair", "persistent-pair", "graph"])
+def test_batch_with_copy_lists_matches_member_oracle_runs(gpu, mode):
+    """A batch whose ghost copy list is the union of member-local lists (each
+    member's ghosts copy that member's active points or zero), written in
+    the handle's global numbering through member_info: every member
+    bit-identical to its own oracle run with its own list, over an odd
+    number of subcycles (the pair forms start with one one-subcycle launch
+    and fold the copy after each launch's first subcycle into their plan)."""
+    sizes = [(40, 30, 61), (64, 40, 62), (9, 7, 63), (50, 44, 64)]
+    states = [synthetic_state(nx, ny, seed, force_perturb=0.02) for nx, ny, seed in sizes]
+    ndte = 11
+    lib, p = oracle_params(states[0], ndte)
+    params = evp1d.set_evp_parameters(ndte)
+    rng = np.random.default_rng(17)
+    local = []
+    for st in states:
+        ghosts = np.arange(st.na + 1, st.navel + 1, dtype=np.int32)
+        dst = rng.choice(ghosts, size=(2 * ghosts.size) // 3, replace=False).astype(np.int32)
+        src = rng.integers(0, st.na + 1, size=dst.size).astype(np.int32)   # 0 = fill with zero
+        local.append((dst, src))
+    _log(f"batch with copy lists mode={mode}")
+    with evp1d.Engine.batch([(st.na, st.navel) for st in states], params) as eng:
+        set_mode(eng, mode)
+        for k, st in enumerate(states):
+            eng.load_member(k, st.f, st.idx, st.skipT, st.skipU)
+        gd, gs = [], []
+        for k, (st, (dst, src)) in enumerate(zip(states, local)):
+            info = eng.member_info(k)
+            gd.append(info["ghost_offset"] + (dst - st.na))
+            gs.append(np.where(src > 0, info["active_offset"] + src, 0))
+        eng.set_halo(np.concatenate(gd).astype(np.int32), np.concatenate(gs).astype(np.int32))
+        if mode in ("fused-pair", "persistent-pair"):
+            assert eng.get_fused_depth()["active"]
+        eng.run(ndte)
+        for k, st in enumerate(states):
+            want = st.copy()
+            lib.run(p, want, ndte, halo=local[k])
+            got = st.copy()
+            for n in got.f:
+                got.f[n][:] = eng.get_field_member(k, n)
+            assert_state_bitexact(got, want)
+
+
 @pytest.mark.parametrize("mode", ["fused", "fused-pair"])
 def test_batch_hilbert_order_matches_independent_oracle_runs(gpu, mode):
     """A batch in the Hilbert tile order (members are separate components of
