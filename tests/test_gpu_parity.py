@@ -218,6 +218,37 @@ def test_skip_flags_change_between_runs(gpu, mode):
     assert_state_bitexact(got, want)
 
 
+@pytest.mark.parametrize("mode", ["auto", "persistent-pair", "fused"])
+def test_skip_flags_change_with_a_copy_list(gpu, mode):
+    """New skip flags between runs with a ghost copy list set: the re-layout
+    re-plans the pair form's tables, copy-list sources included; results stay
+    bit-identical to the oracle's halo run (odd subcycle counts)."""
+    st = synthetic_state(70, 52, seed=57)
+    rng = np.random.default_rng(19)
+    ghosts = np.arange(st.na + 1, st.navel + 1, dtype=np.int32)
+    dst = rng.choice(ghosts, size=ghosts.size // 2, replace=False).astype(np.int32)
+    src = rng.integers(0, st.na + 1, size=dst.size).astype(np.int32)
+    lib, p = oracle_params(st, 9)
+    want = st.copy()
+    lib.run(p, want, 9, halo=(dst, src))
+    newT = (st.skipT | (rng.random(st.na) < 0.3)).astype(np.int32)
+    newU = (st.skipU | (rng.random(st.na) < 0.2)).astype(np.int32)
+    want.skipT[:] = newT
+    want.skipU[:] = newU
+    lib.run(p, want, 7, halo=(dst, src))
+    _log(f"skip change with copy list mode={mode}")
+    with make_engine(st, 9, mode) as eng:
+        eng.set_halo(dst, src)
+        eng.run(9)
+        eng.set_skip("skipT", newT)
+        eng.set_skip("skipU", newU)
+        eng.run(7)
+        if mode != "fused":
+            assert eng.get_fused_depth()["active"]
+        got = fetch_state(eng, want)
+    assert_state_bitexact(got, want)
+
+
 def test_fields_roundtrip_through_layout(gpu):
     """set_field/get_field return the caller's order under any skip pattern."""
     st = synthetic_state(40, 30, seed=61)
