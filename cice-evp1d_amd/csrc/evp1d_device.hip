@@ -1210,6 +1210,9 @@ struct evp1d_handle {
   size_t traj_bytes = 0;
   void *adj_ws = nullptr;          // the gradient's per-call buffers, kept between calls
   size_t adj_ws_bytes = 0;
+  void *ckpt = nullptr;            // a checkpointed gradient's stored states, kept between calls
+  size_t ckpt_bytes = 0;
+  bool pconst_fresh = false;       // pconst holds the current fields (set for a gradient call's duration)
   // 2D grid of the caller's arrays (device-side pack/unpack)
   int grid_nx = 0, grid_ny = 0;
   std::vector<int32_t> grid_ij;    // caller position < na -> 0-based linear 2D index
@@ -1298,6 +1301,7 @@ static void release(evp1d_handle *h) {
   if (h->cnz_dev) (void)hipFree(h->cnz_dev);
   if (h->traj) (void)hipFree(h->traj);
   if (h->adj_ws) (void)hipFree(h->adj_ws);
+  if (h->ckpt) (void)hipFree(h->ckpt);
   if (h->pos2d_dev) (void)hipFree(h->pos2d_dev);
   for (double *g : h->g2d)
     if (g) (void)hipFree(g);
@@ -1830,7 +1834,7 @@ static int issue_fused(evp1d_handle *h, int nsub, double2 *traj = nullptr, bool 
   PairView pv{h->ptiles_dev, h->hlist_dev, h->pxlist_dev, h->pelist_dev, h->puloc_dev, h->pvloc_dev,
               h->pcstr_dev,  h->cnz_dev,   h->pk_total,   nullptr,       h->pair_cap,  h->pconst,
               (long)h->na,   nullptr,      nullptr,       h->ldsig,      h->ptnbr_dev,  h->pfix_dev};
-  if (pair && h->na > 0) {
+  if (pair && h->na > 0 && !h->pconst_fresh) {
     hipLaunchKernelGGL(k_pack_const, dim3(blocks_for(h->na)), dim3(kBlock), 0, h->stream, X, h->pconst, (long)h->na);
     HIP_TRY(hipGetLastError());
   }
@@ -2724,7 +2728,6 @@ struct AdjointRun {
   StateBuf final_state;
   double2 *str_save = nullptr;   // [4][navel]
   double *ut = nullptr, *vt = nullptr, *terms = nullptr;
-  std::vector<void *> mem;
   Schedule sched;
   int seg = 1;                   // subcycles per checkpointing unit (the last unit may be shorter)
   int nunits = 0;                // ceil(nsub / seg)
@@ -2732,16 +2735,52 @@ struct AdjointRun {
   int working = -1;              // unit index i of the state held by the working arrays
   double loss = 0.0;
 
-  int alloc(void **p, size_t bytes, bool zero = true) {
-    const hipError_t e = hipMalloc(p, bytes ? bytes : 256);
-    if (e != hipSuccess) return fail(EVP1D_ENOMEM, "evp1d_adjoint_run: hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
-    mem.push_back(*p);
-    if (zero) HIP_TRY(hipMemsetAsync(*p, 0, bytes ? bytes : 256, h->stream));
-    return EVP1D_OK;
-  }
   ~AdjointRun() {
-    if (h && h->stream) (void)hipStreamSynchronize(h->stream);
-    for (void *p : mem) (void)hipFree(p);
+    if (!h) return;
+    h->pconst_fresh = false;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+  }
+  // The stored states, the final state and its str1..8 and diagnostics:
+  // carved from one allocation that the handle keeps (large allocations and
+  // frees are slow).  Every part is written before it is read: a stored
+  // state by take, the final state and its saves by save_final or
+  // final_step, before restore_final.
+  int carve_checkpoints(int nslots) {
+    const size_t nA = (size_t)h->na * sizeof(double), nV = (size_t)h->navel * sizeof(double2);
+    const size_t nS = (size_t)h->na * sizeof(double2);
+    auto up = [](size_t b) { return (std::max<size_t>(b, 1) + 255) / 256 * 256; };
+    const size_t total = (seg > 1 ? up(3 * nA) : 0) + up(4 * nV) + up(sig_bytes()) + up(nV) +
+                         (size_t)nslots * (up(sig_bytes()) + up(nS));
+    if (h->ckpt_bytes < total) {
+      HIP_TRY(hipStreamSynchronize(h->stream));
+      if (h->ckpt) HIP_TRY(hipFree(h->ckpt));
+      h->ckpt = nullptr;
+      h->ckpt_bytes = 0;
+      const hipError_t e = hipMalloc(&h->ckpt, total);
+      if (e != hipSuccess) {
+        h->ckpt = nullptr;
+        return fail(EVP1D_ENOMEM, "evp1d_adjoint_run: hipMalloc(%zu) for %d stored states: %s", total, nslots,
+                    hipGetErrorString(e));
+      }
+      h->ckpt_bytes = total;
+    }
+    char *p = static_cast<char *>(h->ckpt);
+    auto take_bytes = [&](size_t b) {
+      char *q = p;
+      p += up(b);
+      return q;
+    };
+    diag_save = seg > 1 ? reinterpret_cast<double *>(take_bytes(3 * nA)) : nullptr;
+    str_save = reinterpret_cast<double2 *>(take_bytes(4 * nV));
+    final_state.sig = reinterpret_cast<double *>(take_bytes(sig_bytes()));
+    final_state.vel = reinterpret_cast<double2 *>(take_bytes(nV));
+    slots.resize((size_t)nslots);
+    used.assign((size_t)nslots, 0);
+    for (StateBuf &s : slots) {
+      s.sig = reinterpret_cast<double *>(take_bytes(sig_bytes()));
+      s.vel = reinterpret_cast<double2 *>(take_bytes(nS));
+    }
+    return EVP1D_OK;
   }
   size_t sig_bytes() const { return (size_t)12 * h->ldsig * sizeof(double); }
   int copy_state(const StateBuf &dst, const StateBuf &src, bool ghosts) {
@@ -3106,11 +3145,23 @@ int evp1d_adjoint_run(evp1d_handle *h, int32_t nsub, int32_t nsnap, const double
                        (long)h->v.t_end);
     HIP_TRY(hipGetLastError());
   }
+  h->pconst_fresh = true;   // the forward sweeps of this call reuse the packed fields
   R.a.pc = h->pconst;
   R.a.lp = h->na;
   R.a.nbr = h->nbr4;
   RC_TRY(upload_caller_order(h, R.ut, u_target));
   RC_TRY(upload_caller_order(h, R.vt, v_target));
+  // gradient storage the handle holds from earlier calls: reused, or given
+  // back when a call needs the room
+  auto release = [&](void *&p, size_t &bytes) -> int {
+    if (!p) return EVP1D_OK;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipFree(p));
+    p = nullptr;
+    bytes = 0;
+    return EVP1D_OK;
+  };
+  const size_t held = h->traj_bytes + h->ckpt_bytes;
   if (nsnap <= 0) {
     // automatic: the whole trajectory if it fits in free device memory, else
     // binomial checkpointing with as many stored states as fit
@@ -3119,25 +3170,22 @@ int evp1d_adjoint_run(evp1d_handle *h, int32_t nsub, int32_t nsnap, const double
     const size_t reserve = (size_t)1 << 30;
     const size_t traj = trajectory_bytes(h, nsub);
     const size_t have = h->traj_bytes >= traj ? traj : 0;   // storage the handle already holds
-    if (have > 0 || traj + reserve <= free_b) {
+    if (have > 0 || traj + reserve <= free_b + held) {
       nsnap = nsub;
     } else {
       const size_t slot = R.sig_bytes() + (size_t)h->na * sizeof(double2);
-      const size_t room = free_b + h->traj_bytes > reserve ? (free_b + h->traj_bytes - reserve) / slot : 0;
+      const size_t room = free_b + held > reserve ? (free_b + held - reserve) / slot : 0;
       nsnap = (int)std::max<size_t>(1, std::min<size_t>(room / 2, (size_t)nsub - 1));
     }
   }
   if (nsnap >= nsub) {
+    // a whole-trajectory run that has to allocate its storage gives the
+    // stored states of earlier checkpointed runs back first
+    if (h->traj_bytes < trajectory_bytes(h, nsub)) RC_TRY(release(h->ckpt, h->ckpt_bytes));
     R.seg = nsub;
     R.nunits = 1;
     RC_TRY(R.run_segment(nsub, true));
   } else {
-    if (h->traj) {   // a checkpointed run gives the stored trajectory's memory back
-      HIP_TRY(hipStreamSynchronize(h->stream));
-      HIP_TRY(hipFree(h->traj));
-      h->traj = nullptr;
-      h->traj_bytes = 0;
-    }
     // Checkpointing units of seg subcycles: binomial checkpointing over the
     // units, each unit reversed through a stored velocity trajectory of seg
     // states (seg = 1: one subcycle from recomputed str1..8).  Automatic:
@@ -3150,21 +3198,15 @@ int evp1d_adjoint_run(evp1d_handle *h, int32_t nsub, int32_t nsnap, const double
       size_t free_b = 0, total_b = 0;
       HIP_TRY(hipMemGetInfo(&free_b, &total_b));
       const size_t slot = R.sig_bytes() + (size_t)h->na * sizeof(double2);
-      if (trajectory_bytes(h, seg) + (size_t)S0 * slot + ((size_t)1 << 30) > free_b) seg = 1;
+      if (trajectory_bytes(h, seg) + (size_t)S0 * slot + ((size_t)1 << 30) > free_b + held) seg = 1;
     }
+    // a checkpointed run keeps no more trajectory storage than its units need
+    // (a whole trajectory of an earlier call is given back)
+    if (h->traj_bytes > (seg > 1 ? trajectory_bytes(h, seg) : 0)) RC_TRY(release(h->traj, h->traj_bytes));
     R.seg = seg;
     R.nunits = (nsub + seg - 1) / seg;
     const int S = std::min(S0, R.nunits);   // more slots than units are never used
-    if (seg > 1) RC_TRY(R.alloc((void **)&R.diag_save, 3 * nA));
-    RC_TRY(R.alloc((void **)&R.str_save, 4 * nV));
-    RC_TRY(R.alloc((void **)&R.final_state.sig, R.sig_bytes()));
-    RC_TRY(R.alloc((void **)&R.final_state.vel, nV));
-    R.slots.resize((size_t)S);
-    R.used.assign((size_t)S, 0);
-    for (int k = 0; k < S; ++k) {
-      RC_TRY(R.alloc((void **)&R.slots[(size_t)k].sig, R.sig_bytes()));
-      RC_TRY(R.alloc((void **)&R.slots[(size_t)k].vel, (size_t)h->na * sizeof(double2)));
-    }
+    RC_TRY(R.carve_checkpoints(S));
     R.sched.build(R.nunits, S - 1);
     R.working = 0;
     const int s0 = R.take(0);

@@ -307,11 +307,12 @@ int evp1d_tile_plan(int32_t na, int32_t navel, const int32_t *ee, const int32_t 
  * one scalar drag coefficient is the sum), forcex and forcey (caller order,
  * na each; any output may be NULL).  The reverse sweep uses hand-written
  * adjoint stress and stepu kernels and binomial checkpointing with nsnap
- * stored subcycle states (nsnap >= nsub stores them all: no recomputation;
- * the handle keeps that storage, 16 B per stepu point and subcycle, for later
- * calls until it is destroyed or a call with 0 < nsnap < nsub releases it;
- * nsnap <= 0 chooses: all of them if they fit in free device memory, else as
- * many stored states as fit).
+ * stored subcycle states (nsnap >= nsub stores them all: no recomputation,
+ * 16 B per stepu point and subcycle; nsnap <= 0 chooses: all of them if they
+ * fit in free device memory, else as many stored states as fit).  The handle
+ * keeps the storage of either kind of call for later calls until it is
+ * destroyed; a call with 0 < nsnap < nsub gives a whole trajectory back, and a
+ * whole-trajectory call that has to allocate gives stored states back.
  * Afterwards the handle holds the forward state after nsub subcycles, exactly
  * as evp1d_run(h, nsub) leaves it.  evp1d_loss evaluates L at the current
  * state.                                                                    */

@@ -171,7 +171,7 @@ def test_gradient_is_zero_off_ice(problem):
 
 def test_repeated_gradient_calls_on_one_handle(problem):
     """The stored trajectory's memory is kept and reused by later calls and
-    released by a checkpointed one; every call starts from the state it finds."""
+    given back by a checkpointed one; every call starts from the state it finds."""
     st, ut, vt = problem
     with make_engine(st, NDTE) as eng:
         ref = eng.adjoint_run(NDTE, ut, vt, nsnap=NDTE)
@@ -182,6 +182,45 @@ def test_repeated_gradient_calls_on_one_handle(problem):
             for k in ("strength", "cdn_ocn", "forcex", "forcey"):
                 assert np.array_equal(g[k], ref[k]), (nsnap, k)
             assert g["loss"] == ref["loss"]
+
+
+@pytest.mark.parametrize("mode", ["auto", "fused-pair"])
+def test_gradient_storage_reused_across_calls(problem, mode):
+    """One handle through checkpointed calls of several stored-state counts
+    and units with whole-trajectory calls between them: the stored states and
+    the trajectory that the handle keeps from earlier calls (never cleared)
+    give every call a fresh handle's gradient bit for bit.  Then a changed
+    field: the next forward run and gradient use it (the forward sweeps of a
+    gradient reuse the fields it packed; later runs pack them again)."""
+    st, ut, vt = problem
+    keys = ("strength", "cdn_ocn", "forcex", "forcey")
+    with make_engine(st, NDTE, mode) as eng:
+        ref = eng.adjoint_run(NDTE, ut, vt, nsnap=NDTE)
+    st2 = st.copy()
+    st2.f["strength"] = st2.f["strength"] * 1.1
+    with make_engine(st2, NDTE, mode) as eng:
+        ref2 = eng.adjoint_run(NDTE, ut, vt, nsnap=NDTE)
+    lib, p = oracle_params(st2, NDTE)
+    want2 = st2.copy()
+    lib.run(p, want2, NDTE)
+    with make_engine(st, NDTE, mode) as eng:
+        for nsnap, seg in [(4, 0), (NDTE, 0), (3, 0), (6, 1), (2, 3), (NDTE, 0), (5, 0), (8, 0)]:
+            eng.load_state(st.f, st.idx, st.skipT, st.skipU)
+            eng.set_adjoint_segment(seg)
+            g = eng.adjoint_run(NDTE, ut, vt, nsnap=nsnap)
+            for k in keys:
+                assert np.array_equal(g[k], ref[k]), (nsnap, seg, k)
+            assert g["loss"] == ref["loss"], (nsnap, seg)
+        eng.set_adjoint_segment(0)
+        eng.load_state(st2.f, st2.idx, st2.skipT, st2.skipU)
+        eng.run(NDTE)
+        assert_state_bitexact(fetch_state(eng, st2), want2)
+        for nsnap in (5, NDTE):
+            eng.load_state(st2.f, st2.idx, st2.skipT, st2.skipU)
+            g = eng.adjoint_run(NDTE, ut, vt, nsnap=nsnap)
+            for k in keys:
+                assert np.array_equal(g[k], ref2[k]), (nsnap, k)
+            assert_state_bitexact(fetch_state(eng, st2), want2)
 
 
 @pytest.mark.parametrize("nsnap", [NDTE, 5])
