@@ -686,8 +686,10 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
 // ghost copy with another barrier when a copy list is set.  The velocities
 // alternate between d.vel and a.vy, the packed stresses between a.sg0 and
 // a.sg1, exactly as the separate launches alternate them, so results are
-// bit-identical.  a.cnt: 2 x 8 counters, zero at the launch; launch j uses
-// row j mod 2 and clears the other row for launch j + 1.
+// bit-identical.  a.cnt: 2 x 8 counters, zero at the launch, each on a
+// 256-byte line of its own (the groups' atomics do not queue on one line);
+// launch j uses row j mod 2 and clears the other row for launch j + 1.
+constexpr int kCntPad = 64;   // ints from one tile counter to the next
 struct PersistArgs {
   double2 *vy, *sg0, *sg1;
   int *cnt;
@@ -719,11 +721,12 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
   for (int j = j0; j <= L; ++j) {
     const bool to_x = ((L - j) % 2) == 0;   // the run ends in the handle's own buffer
     const bool odd = ((j - j0) % 2) == 1;   // which packed stress buffer is the input
-    int *cnt = a.cnt + (j % 2) * kXcds;
+    int *cnt = a.cnt + (j % 2) * kXcds * kCntPad;
     if (blockIdx.x == 0 && threadIdx.x < kXcds)   // with the atomics' scope, published by the barrier
-      __hip_atomic_store(a.cnt + ((j + 1) % 2) * kXcds + threadIdx.x, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(a.cnt + (((j + 1) % 2) * kXcds + threadIdx.x) * kCntPad, 0, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
     for (;;) {
-      if (threadIdx.x == 0) grab = atomicAdd(cnt + x, 1);
+      if (threadIdx.x == 0) grab = atomicAdd(cnt + x * kCntPad, 1);
       __syncthreads();
       const int k = grab;   // read by every thread before the tile's first barrier
       if (k >= nt) break;
@@ -1163,7 +1166,7 @@ struct evp1d_handle {
   std::vector<int32_t> m_na, m_navel, a_off, g_off;
   std::vector<uint8_t> idx_mset[6];   // per member: list set
   int coop_blocks = 0;
-  int *pcnt_dev = nullptr;         // persistent pair form: tile counters (2 x 8)
+  int *pcnt_dev = nullptr;         // persistent pair form: tile counters (2 x 8, kCntPad apart)
   // fused form: tiling and its device tables (owned here, replaced on relayout)
   int tile_w = 64, tile_h = 16, tile_block = 512, tile_xcd = 1;
   bool tiling_auto = true;         // choose the tile from the domain size (until evp1d_set_tiling)
@@ -1800,8 +1803,9 @@ static int launch_pair_persistent(evp1d_handle *h, const DevView &x, PairView pv
   if (per_cu <= 0 || cus <= 0) return fail(EVP1D_EHIP, "no co-resident workgroups for k_pair_persistent");
   int grid = std::min(per_cu * cus, h->fused_ntiles);
   if (grid >= kXcds) grid -= grid % kXcds;
-  if (!h->pcnt_dev) HIP_TRY(hipMalloc((void **)&h->pcnt_dev, 2 * kXcds * sizeof(int)));
-  HIP_TRY(hipMemsetAsync(h->pcnt_dev, 0, 2 * kXcds * sizeof(int), h->stream));
+  const size_t cnt_bytes = (size_t)2 * kXcds * kCntPad * sizeof(int);
+  if (!h->pcnt_dev) HIP_TRY(hipMalloc((void **)&h->pcnt_dev, cnt_bytes));
+  HIP_TRY(hipMemsetAsync(h->pcnt_dev, 0, cnt_bytes, h->stream));
   DevView d = x;
   PersistArgs a{h->vel_alt, sg0, sg1, h->pcnt_dev, j0, L, h->fused_ntiles};
   void *args[] = {&d, &pv, &a};
