@@ -2590,6 +2590,26 @@ int evp1d_get_adjoint_segment(evp1d_handle *h, int32_t *seg) {
   return EVP1D_OK;
 }
 
+int evp1d_get_gradient_storage(evp1d_handle *h, int64_t *bytes) {
+  if (!h) return fail(EVP1D_EINVAL, "evp1d_get_gradient_storage: NULL handle");
+  if (bytes) *bytes = (int64_t)(h->traj_bytes + h->ckpt_bytes + h->adj_ws_bytes);
+  return EVP1D_OK;
+}
+
+int evp1d_release_gradient_storage(evp1d_handle *h) {
+  if (!h) return fail(EVP1D_EINVAL, "evp1d_release_gradient_storage: NULL handle");
+  if (!h->traj && !h->ckpt && !h->adj_ws) return EVP1D_OK;
+  HIP_TRY(hipStreamSynchronize(h->stream));   // no queued work may still use them
+  void **bufs[3] = {&h->traj, &h->ckpt, &h->adj_ws};
+  size_t *sizes[3] = {&h->traj_bytes, &h->ckpt_bytes, &h->adj_ws_bytes};
+  for (int k = 0; k < 3; ++k) {
+    if (*bufs[k]) HIP_TRY(hipFree(*bufs[k]));
+    *bufs[k] = nullptr;
+    *sizes[k] = 0;
+  }
+  return EVP1D_OK;
+}
+
 int evp1d_get_fused_depth(evp1d_handle *h, int32_t *depth, int32_t *active, int32_t *block) {
   if (!h) return fail(EVP1D_EINVAL, "evp1d_get_fused_depth: NULL handle");
   if (ready(h)) RC_TRY(ensure(h));

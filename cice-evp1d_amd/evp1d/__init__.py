@@ -55,7 +55,7 @@ EXPORTS = (
     "evp1d_get_field_member", "evp1d_set_index_member", "evp1d_set_skip_member",
     "evp1d_autotune", "evp1d_set_tuning", "evp1d_get_tuning",
     "evp1d_adjoint_run", "evp1d_loss", "evp1d_loss_noalloc", "evp1d_tangent_run", "evp1d_set_adjoint_segment",
-    "evp1d_get_adjoint_segment",
+    "evp1d_get_adjoint_segment", "evp1d_get_gradient_storage", "evp1d_release_gradient_storage",
     "evp1d_set_tiling", "evp1d_get_tiling", "evp1d_tile_plan",
     "evp1d_set_fused_inplace", "evp1d_get_fused_inplace", "evp1d_autotune_fused",
     "evp1d_set_fused_depth", "evp1d_get_fused_depth", "evp1d_set_tile_order", "evp1d_get_tile_order",
@@ -150,6 +150,8 @@ def load(fma: bool = False, path: str | os.PathLike | None = None) -> ctypes.CDL
         "evp1d_set_tile_order": [_H, i32],
         "evp1d_get_tile_order": [_H, _IP],
         "evp1d_get_adjoint_segment": [_H, _IP],
+        "evp1d_get_gradient_storage": [_H, ctypes.POINTER(ctypes.c_int64)],
+        "evp1d_release_gradient_storage": [_H],
         "evp1d_tangent_run": [_H, i32, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP],
         "evp1d_set_tiling": [_H, i32, i32, i32, i32],
         "evp1d_set_grid": [_H, i32, i32, _IP, i32],
@@ -449,6 +451,16 @@ class Engine:
         s = ctypes.c_int32()
         _check(self.L, self.L.evp1d_get_adjoint_segment(self.h, ctypes.byref(s)))
         return s.value
+
+    def gradient_storage(self) -> int:
+        """Bytes of device memory that gradient calls keep with the handle."""
+        b = ctypes.c_int64()
+        _check(self.L, self.L.evp1d_get_gradient_storage(self.h, ctypes.byref(b)))
+        return b.value
+
+    def release_gradient_storage(self) -> None:
+        """Give that memory back (evp1d_release_gradient_storage)."""
+        _check(self.L, self.L.evp1d_release_gradient_storage(self.h))
 
     def get_fused_depth(self) -> dict:
         """{"depth": setting, "active": runs use the pair form, "block": its workgroup size}."""

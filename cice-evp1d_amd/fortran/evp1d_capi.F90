@@ -68,6 +68,7 @@ module evp1d_capi
             evp1d_set_fused_depth, evp1d_get_fused_depth,                                &
             evp1d_adjoint_run, evp1d_loss, evp1d_tangent_run, evp1d_set_adjoint_segment, &
             evp1d_get_adjoint_segment, evp1d_set_tile_order, evp1d_get_tile_order,       &
+            evp1d_get_gradient_storage, evp1d_release_gradient_storage,                  &
             evp2d_create, evp2d_destroy, evp2d_set_params, evp2d_set_cells, evp2d_set_halo, &
             evp2d_set_field, evp2d_get_field, evp2d_run, evp2d_stress, evp2d_stepu,      &
             evp2d_halo, evp2d_run_individual, evp2d_synchronize, evp2d_timer_start,    &
@@ -449,6 +450,15 @@ module evp1d_capi
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: h
       integer(c_int32_t), intent(out) :: seg
+    end function
+    integer(c_int) function evp1d_get_gradient_storage(h, bytes) bind(C, name='evp1d_get_gradient_storage')
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int64_t), intent(out) :: bytes
+    end function
+    integer(c_int) function evp1d_release_gradient_storage(h) bind(C, name='evp1d_release_gradient_storage')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
     end function
     integer(c_int) function evp1d_set_tile_order(h, order) bind(C, name='evp1d_set_tile_order')
       import :: c_int, c_int32_t, c_ptr

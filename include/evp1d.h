@@ -344,6 +344,14 @@ int evp1d_loss_noalloc(evp1d_handle *h, const double *u_target, const double *v_
  * the same bit for bit for every seg and nsnap.                            */
 int evp1d_set_adjoint_segment(evp1d_handle *h, int32_t seg);
 int evp1d_get_adjoint_segment(evp1d_handle *h, int32_t *seg);
+/* Device memory that gradient calls keep with the handle (no counterpart in
+ * the reference): the stored trajectory, the stored states of checkpointed
+ * calls and the adjoint work arrays.  evp1d_get_gradient_storage reports the
+ * bytes held; evp1d_release_gradient_storage gives them back (the next
+ * gradient call allocates again), for callers that need the memory for other
+ * work without destroying the handle.                                       */
+int evp1d_get_gradient_storage(evp1d_handle *h, int64_t *bytes);
+int evp1d_release_gradient_storage(evp1d_handle *h);
 /* Tangent-linear run: nsub subcycles from the current state propagating the
  * perturbation (d_strength, d_cdn_ocn, d_forcex, d_forcey) (caller order, na
  * each, NULL = zero) by forward-mode differentiation of the same point

@@ -223,6 +223,33 @@ def test_gradient_storage_reused_across_calls(problem, mode):
             assert_state_bitexact(fetch_state(eng, st2), want2)
 
 
+def test_gradient_storage_released_and_regained(problem):
+    """The handle reports the memory gradient calls keep with it; releasing
+    it leaves nothing held (twice is harmless), and the next call allocates
+    the same amount again and gives the same gradient bit for bit."""
+    st, ut, vt = problem
+    keys = ("strength", "cdn_ocn", "forcex", "forcey")
+    with make_engine(st, NDTE) as eng:
+        assert eng.gradient_storage() == 0
+        out = []
+        for nsnap in (5, NDTE):
+            eng.load_state(st.f, st.idx, st.skipT, st.skipU)
+            g1 = eng.adjoint_run(NDTE, ut, vt, nsnap=nsnap)
+            held = eng.gradient_storage()
+            assert held > 0, nsnap
+            eng.release_gradient_storage()
+            assert eng.gradient_storage() == 0
+            eng.release_gradient_storage()
+            eng.load_state(st.f, st.idx, st.skipT, st.skipU)
+            g2 = eng.adjoint_run(NDTE, ut, vt, nsnap=nsnap)
+            assert eng.gradient_storage() == held, nsnap
+            for k in keys:
+                assert np.array_equal(g1[k], g2[k]), (nsnap, k)
+            out.append(g2)
+        for k in keys:
+            assert np.array_equal(out[0][k], out[1][k]), k
+
+
 @pytest.mark.parametrize("nsnap", [NDTE, 5])
 def test_batch_gradient_equals_member_gradients(gpu, nsnap):
     """A batched handle's gradient: each member's part is bit-identical to the

@@ -31,6 +31,18 @@ def test_library_exports_every_declared_symbol(fma):
     assert set(declared) == set(evp1d.EXPORTS)
 
 
+def test_gradient_storage_calls_reject_a_null_handle():
+    """The storage query and release check their handle before any device
+    call (EVP1D_EINVAL = 1)."""
+    L = ctypes.CDLL(str(evp1d.lib_path(False)))
+    L.evp1d_get_gradient_storage.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]
+    L.evp1d_release_gradient_storage.argtypes = [ctypes.c_void_p]
+    b = ctypes.c_int64(-1)
+    assert L.evp1d_get_gradient_storage(None, ctypes.byref(b)) == 1
+    assert b.value == -1
+    assert L.evp1d_release_gradient_storage(None) == 1
+
+
 def test_stream_probe_library_rejects_unsupported_mixes():
     """The HBM streaming probe (bench.py --stream-probe) is built beside the
     library and refuses a read/write mix it has no kernel for before any
