@@ -651,6 +651,8 @@ def main() -> int:
                     help="tile order of the fused forms: 0 rows of tiles, 1 Hilbert curve (evp1d_set_tile_order)")
     ap.add_argument("--depth", type=int, default=None, choices=(1, 2),
                     help="subcycles per fused launch: 2 the pair form (implies --tune-fused 0)")
+    ap.add_argument("--hints", type=int, default=None, choices=(-1, 0, 1),
+                    help="pair form's cache hints: -1 chosen by size, 0 off, 1 on (implies --tune-fused 0)")
     ap.add_argument("--tuning", default=None, metavar="SB,SX,UB,UX",
                     help="fixed launch tuning: stress block, stress xcd order, stepu block, stepu xcd order")
     ap.add_argument("--stream-probe", type=int, default=0, metavar="MIB",
@@ -697,6 +699,8 @@ def main() -> int:
         eng.set_fused_depth(args.depth)
     if args.order is not None:
         eng.set_tile_order(args.order)
+    if args.hints is not None:
+        eng.set_cache_hints(args.hints)
     if args.autotune:
         log(f"[bench] autotune: {eng.autotune(args.autotune)}")
     tiling = eng.get_tiling()
@@ -704,7 +708,7 @@ def main() -> int:
     fused = args.mode == "fused" or (args.mode == "auto" and tiling["ntiles"] > 0 and ndte >= 4)
     tuned_fused = None
     if (fused and args.tune_fused > 0 and not args.tiling and args.inplace is None and args.depth is None
-            and args.order is None):
+            and args.order is None and args.hints is None):
         t0 = time.perf_counter()
         tuned_fused = eng.autotune_fused(args.tune_fused)
         log(f"[bench] autotune_fused ({time.perf_counter() - t0:.1f}s, setup): {tuned_fused}")
@@ -714,6 +718,7 @@ def main() -> int:
     tiling["depth"] = 2 if dinfo["active"] else 1
     if dinfo["active"]:
         tiling.update(inplace=0, pair_block=dinfo["block"])   # the pair form keeps two stress buffers
+        tiling["cache_hints"] = int(eng.get_cache_hints()["active"])
     log(f"[bench] mode {args.mode} ({'fused' if fused else 'unfused'}), tiling {tiling}, tuning {tuning}")
     secs = time_steps(eng, ndte, args.steps, args.warmup)
     value = na * ndte * args.steps / secs
@@ -763,8 +768,9 @@ def main() -> int:
         key = dom["name"]
         if key == "k_fused":
             key += "@inplace" if tiling.get("inplace") else "@twobuf"
-        elif key == "k_pair":   # measured per tile shape and order (the tuner's choice varies between boxes)
+        elif key == "k_pair":   # measured per tile shape, order and cache hints (the tuner's choice varies)
             key += f"@{tiling['tile_w']}x{tiling['tile_h']}" + ("@hilbert" if tiling.get("order") == 1 else "")
+            key += "@hints" if tiling.get("cache_hints") else ""
         if args.fma:   # counters were taken on the exact build
             key += "@fma"
         traffic = tab.get(key)

@@ -455,7 +455,41 @@ struct PairBufs {
 // of the next phase (level-1 stepu, level-2 stress, level-2 stepu) before
 // the barrier that ends the current phase, and that phase starts on values
 // already in registers instead of one memory round trip after the barrier.
-template <int B, bool TRAJ, bool FIX>
+// Cache hints of the pair form (NT): the streams a launch reads or writes
+// once (stresses in and out, the T points' constant fields, velocities and
+// trajectory written) as non-temporal loads and stores, so that the
+// last-level cache keeps what every launch reads again (the U points'
+// constant fields, velocities, tables).  Hints only: the same values.
+typedef double NtD2 __attribute__((ext_vector_type(2)));
+template <bool NT>
+__device__ __forceinline__ void out2(double2 *p, double2 v) {
+  if (NT) {
+    const NtD2 x = {v.x, v.y};
+    __builtin_nontemporal_store(x, reinterpret_cast<NtD2 *>(p));
+  } else {
+    *p = v;
+  }
+}
+template <bool NT>
+__device__ __forceinline__ void out1(double *p, double v) {
+  if (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+template <bool NT>
+__device__ __forceinline__ double2 in2(const double2 *p) {
+  if (NT) {
+    const NtD2 x = __builtin_nontemporal_load(reinterpret_cast<const NtD2 *>(p));
+    return make_double2(x.x, x.y);
+  }
+  return *p;
+}
+template <bool NT>
+__device__ __forceinline__ double in1(const double *p) {
+  if (NT) return __builtin_nontemporal_load(p);
+  return *p;
+}
+
+template <int B, bool TRAJ, bool FIX, bool NT>
 __device__ __forceinline__ void pair_tile(const DevView &d, const PairView &f, PairBufs bf, int last, int tix) {
   extern __shared__ double2 lds[];
   const int cap = f.cap;
@@ -513,12 +547,12 @@ __device__ __forceinline__ void pair_tile(const DevView &d, const PairView &f, P
       double sig[12];
 #pragma unroll
       for (int c = 0; c < 6; ++c) {
-        const double2 v = bf.sg_in[c * f.ls + g];
+        const double2 v = in2<NT>(bf.sg_in + (c * f.ls + g));
         sig[2 * c] = v.x;
         sig[2 * c + 1] = v.y;
       }
-      const double2 a0 = pc[g], a1 = pc[lp + g], a2 = pc[2 * lp + g];
-      const double st = d.strength[g];
+      const double2 a0 = in2<NT>(pc + g), a1 = in2<NT>(pc + (lp + g)), a2 = in2<NT>(pc + (2 * lp + g));
+      const double st = in1<NT>(d.strength + g);
       evp1d::StressOut o;
       const double2 cc = bf.vel[g], ve = bf.vel[n.y], vn = bf.vel[n.z], vs = bf.vel[n.w];
       evp1d::stress_point(d.s, cc.x, cc.y, ve.x, ve.y, vs.x, vs.y, vn.x, vn.y, a0.x, a0.y, st, a1.x, a1.y, a2.x,
@@ -572,7 +606,7 @@ __device__ __forceinline__ void pair_tile(const DevView &d, const PairView &f, P
     evp1d::StepuOut o;
     stepu(u1, v1, o);
     vt[me] = make_double2(o.uvel, o.vvel);
-    if (TRAJ && me < nUo) f.tr_mid[q1 - d.u_begin] = vt[me];
+    if (TRAJ && me < nUo) out2<NT>(f.tr_mid + (q1 - d.u_begin), vt[me]);
   }
   for (int k = me + B; k < nU; k += B) {
     const int q = uslot(k);
@@ -580,7 +614,7 @@ __device__ __forceinline__ void pair_tile(const DevView &d, const PairView &f, P
     evp1d::StepuOut o;
     stepu(u, bf.vel[q], o);
     vt[k] = make_double2(o.uvel, o.vvel);
-    if (TRAJ && k < nUo) f.tr_mid[q - d.u_begin] = vt[k];
+    if (TRAJ && k < nUo) out2<NT>(f.tr_mid + (q - d.u_begin), vt[k]);
   }
   // level 2, stress of entry me: memory inputs loaded before the barrier
   // (velocities that are no U1 entry never change, kNoEntry: from memory)
@@ -629,17 +663,17 @@ __device__ __forceinline__ void pair_tile(const DevView &d, const PairView &f, P
                         m2[1].y, m2[2].x, m2[2].y, sg, o);
     const int g = g2;
     if (me < nO) {
-      bf.sg_out[g] = make_double2(o.sp[0], o.sp[1]);
-      bf.sg_out[f.ls + g] = make_double2(o.sp[2], o.sp[3]);
-      bf.sg_out[2 * f.ls + g] = make_double2(o.sm[0], o.sm[1]);
-      bf.sg_out[3 * f.ls + g] = make_double2(o.sm[2], o.sm[3]);
-      bf.sg_out[4 * f.ls + g] = make_double2(o.s12[0], o.s12[1]);
-      bf.sg_out[5 * f.ls + g] = make_double2(o.s12[2], o.s12[3]);
+      out2<NT>(bf.sg_out + g, make_double2(o.sp[0], o.sp[1]));
+      out2<NT>(bf.sg_out + (f.ls + g), make_double2(o.sp[2], o.sp[3]));
+      out2<NT>(bf.sg_out + (2 * f.ls + g), make_double2(o.sm[0], o.sm[1]));
+      out2<NT>(bf.sg_out + (3 * f.ls + g), make_double2(o.sm[2], o.sm[3]));
+      out2<NT>(bf.sg_out + (4 * f.ls + g), make_double2(o.s12[0], o.s12[1]));
+      out2<NT>(bf.sg_out + (5 * f.ls + g), make_double2(o.s12[2], o.s12[3]));
       if (last) {
-        d.s15[g] = make_double2(o.str[0], o.str[4]);
-        d.s27[g] = make_double2(o.str[1], o.str[6]);
-        d.s36[g] = make_double2(o.str[2], o.str[5]);
-        d.s48[g] = make_double2(o.str[3], o.str[7]);
+        out2<NT>(d.s15 + g, make_double2(o.str[0], o.str[4]));
+        out2<NT>(d.s27 + g, make_double2(o.str[1], o.str[6]));
+        out2<NT>(d.s36 + g, make_double2(o.str[2], o.str[5]));
+        out2<NT>(d.s48 + g, make_double2(o.str[3], o.str[7]));
       }
     }
     put(me, o);
@@ -656,12 +690,12 @@ __device__ __forceinline__ void pair_tile(const DevView &d, const PairView &f, P
   auto level2u = [&](int k, int q, const UIn &u) {
     evp1d::StepuOut o;
     stepu(u, vt[k], o);
-    bf.vel_out[q] = make_double2(o.uvel, o.vvel);
-    if (TRAJ && f.tr_end) f.tr_end[q - d.u_begin] = make_double2(o.uvel, o.vvel);
+    out2<NT>(bf.vel_out + q, make_double2(o.uvel, o.vvel));
+    if (TRAJ && f.tr_end) out2<NT>(f.tr_end + (q - d.u_begin), make_double2(o.uvel, o.vvel));
     if (last) {
-      d.strintx[q] = o.strintx;
-      d.strinty[q] = o.strinty;
-      d.cb[q] = o.Cb;
+      out1<NT>(d.strintx + q, o.strintx);
+      out1<NT>(d.strinty + q, o.strinty);
+      out1<NT>(d.cb + q, o.Cb);
     }
   };
   if (me < nUo) level2u(me, q2, u2);
@@ -671,9 +705,10 @@ __device__ __forceinline__ void pair_tile(const DevView &d, const PairView &f, P
   }
 }
 
-template <int B, bool X, bool TRAJ = false, bool FIX = false>
+template <int B, bool X, bool TRAJ = false, bool FIX = false, bool NT = false>
 __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_pair(DevView d, PairView f, int last) {
-  pair_tile<B, TRAJ, FIX>(d, f, PairBufs{d.vel, f.vel_out, f.sg_in, f.sg_out}, last, tile_of(blockIdx.x, gridDim.x, X));
+  pair_tile<B, TRAJ, FIX, NT>(d, f, PairBufs{d.vel, f.vel_out, f.sg_in, f.sg_out}, last,
+                              tile_of(blockIdx.x, gridDim.x, X));
 }
 
 // The pair launches j0..L of a run (numbered as issue_fused numbers them) in
@@ -736,7 +771,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4))) void k_p
       const PersistArgs &av =
           *(const PersistArgs *)(const __attribute__((address_space(4))) PersistArgs *)(ka + kOffA);
       const PairBufs bf{to_x ? av.vy : dv.vel, to_x ? dv.vel : av.vy, odd ? av.sg1 : av.sg0, odd ? av.sg0 : av.sg1};
-      pair_tile<B, false, FIX>(dv, fv, bf, j == L ? 1 : 0, t0 + k);
+      pair_tile<B, false, FIX, false>(dv, fv, bf, j == L ? 1 : 0, t0 + k);
       __syncthreads();   // the next tile rewrites the LDS tables and grab
     }
     grid.sync();
@@ -1192,6 +1227,7 @@ struct evp1d_handle {
   // pair form (two subcycles per fused launch): its tables, planned on the
   // current tiling when fused_depth is 2 (the default)
   int fused_depth = 2;
+  int cache_hints = -1;            // pair form's cache hints: -1 chosen by size (pair_hints), 0 off, 1 on
   int tile_order = 0;              // 0 rows of tiles, 1 Hilbert curve over the tiles
   // subcycles per checkpointing unit of a gradient with fewer stored states
   // than subcycles (0: chosen per call)
@@ -1673,17 +1709,22 @@ static const FusedFn kFusedFns[4][2] = {{k_fused<64, false>, k_fused<64, true>},
 static const FusedFn kFusedSplitFns[4] = {k_fused<64, true, true>, k_fused<128, true, true>,
                                          k_fused<256, true, true>, k_fused<512, true, true>};
 typedef void (*PairFn)(DevView, PairView, int);
-static const PairFn kPairFns[kNumPairBlocks][2] = {{k_pair<256, false>, k_pair<256, true>},
-                                                   {k_pair<512, false>, k_pair<512, true>},
-                                                   {k_pair<1024, false>, k_pair<1024, true>}};
+// [workgroup size][XCD-aware order][cache hints]
+#define PAIR_ROW(B, FIX)                                                                      \
+  {{k_pair<B, false, false, FIX, false>, k_pair<B, false, false, FIX, true>},                 \
+   {k_pair<B, true, false, FIX, false>, k_pair<B, true, false, FIX, true>}}
+static const PairFn kPairFns[kNumPairBlocks][2][2] = {PAIR_ROW(256, false), PAIR_ROW(512, false),
+                                                      PAIR_ROW(1024, false)};
 // with a ghost copy list (level-2 reads of listed ghosts through PairPlan::fix)
-static const PairFn kPairFixFns[kNumPairBlocks][2] = {
-    {k_pair<256, false, false, true>, k_pair<256, true, false, true>},
-    {k_pair<512, false, false, true>, k_pair<512, true, false, true>},
-    {k_pair<1024, false, false, true>, k_pair<1024, true, false, true>}};
-// the gradient sweep's trajectory-storing form (XCD-aware tile order only)
-static const PairFn kPairTrajFns[kNumPairBlocks] = {k_pair<256, true, true>, k_pair<512, true, true>,
-                                                    k_pair<1024, true, true>};
+static const PairFn kPairFixFns[kNumPairBlocks][2][2] = {PAIR_ROW(256, true), PAIR_ROW(512, true),
+                                                         PAIR_ROW(1024, true)};
+#undef PAIR_ROW
+// the gradient sweep's trajectory-storing form (XCD-aware tile order only),
+// [workgroup size][cache hints]
+static const PairFn kPairTrajFns[kNumPairBlocks][2] = {
+    {k_pair<256, true, true, false, false>, k_pair<256, true, true, false, true>},
+    {k_pair<512, true, true, false, false>, k_pair<512, true, true, false, true>},
+    {k_pair<1024, true, true, false, false>, k_pair<1024, true, true, false, true>}};
 // one cooperative launch for the pair launches of a run (evp1d_run in
 // EVP1D_MODE_PERSISTENT with the pair form selected)
 typedef void (*PairPersistFn)(DevView, PairView, PersistArgs);
@@ -1702,6 +1743,23 @@ static int pair_block_index(int b) {
 // the pair form runs when selected and planned
 static bool pair_form(const evp1d_handle *h) { return h->fused_depth == 2 && h->pair_ok; }
 
+// Whether the pair form's launches use the cache hints (NT).  By default:
+// when all the bytes a launch touches exceed the last-level cache (256 MiB
+// on MI355X) but the part every launch reads again (the U points' constant
+// fields and tables, about 152 B per U point, and the velocities) is at most
+// 1.2 times it.  Measured on the pair form at 31x19: 8 % less time per
+// subcycle at 1.36M and 1.68M points (reread part 221 and 273 MB); 9-13 %
+// more at 0.34M and 0.63M (everything fits: 113 and 211 MB), 4-7 % more at
+// 2.5M and 5.4M (reread part 409 MB and more).
+constexpr double kLastLevelCache = 256.0 * 1024 * 1024;
+static bool pair_hints(const evp1d_handle *h) {
+  if (h->cache_hints >= 0) return h->cache_hints != 0;
+  const double nT = h->v.t_end, nU = h->v.u_end - h->v.u_begin;
+  const double reread = 152.0 * nU + 16.0 * h->navel;
+  const double all = reread + 248.0 * nT;   // + stresses in and out, the T points' constant fields
+  return all > kLastLevelCache && reread <= 1.2 * kLastLevelCache;
+}
+
 static int fused_block_index(int b) {
   for (int i = 0; i < 4; ++i)
     if (kFusedBlocks[i] == b) return i;
@@ -1719,14 +1777,14 @@ static int ensure_fused_buffers(evp1d_handle *h) {
     for (FusedFn fn : kFusedSplitFns)
       HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   kFusedMaxEntries * 4 * (int)sizeof(double2)));
-    for (auto &row : kPairFns)
+    for (const auto *tab : {&kPairFns, &kPairFixFns})
+      for (const auto &row : *tab)
+        for (const auto &pair : row)
+          for (PairFn fn : pair)
+            HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, kPairMaxLds));
+    for (const auto &row : kPairTrajFns)
       for (PairFn fn : row)
         HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, kPairMaxLds));
-    for (auto &row : kPairFixFns)
-      for (PairFn fn : row)
-        HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, kPairMaxLds));
-    for (PairFn fn : kPairTrajFns)
-      HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, kPairMaxLds));
   }
   if (!h->vel_alt) HIP_TRY(hipMalloc((void **)&h->vel_alt, (size_t)h->navel * sizeof(double2)));
   if ((!h->fused_inplace || pair_form(h)) && !h->sig_alt)
@@ -1876,12 +1934,13 @@ static int issue_fused(evp1d_handle *h, int nsub, double2 *traj = nullptr, bool 
       pv.sg_in = P[pin];
       pv.sg_out = P[1 - pin];
       pin = 1 - pin;
-      PairFn fn = (h->v.nhalo > 0 ? kPairFixFns : kPairFns)[pair_block_index(h->pair_block)][h->tile_xcd ? 1 : 0];
+      const int nt = pair_hints(h) ? 1 : 0;
+      PairFn fn = (h->v.nhalo > 0 ? kPairFixFns : kPairFns)[pair_block_index(h->pair_block)][h->tile_xcd ? 1 : 0][nt];
       if (traj) {
         const int before = odd_first ? 2 * j - 3 : 2 * j - 2;   // subcycles done before this launch
         pv.tr_mid = traj + (size_t)(before + 1) * nu;
         pv.tr_end = before + 2 < nsub ? traj + (size_t)(before + 2) * nu : nullptr;
-        fn = kPairTrajFns[pair_block_index(h->pair_block)];
+        fn = kPairTrajFns[pair_block_index(h->pair_block)][nt];
       }
       if (h->fused_ntiles > 0) {
         hipLaunchKernelGGL(fn, dim3(h->fused_ntiles), dim3(h->pair_block), lds2, h->stream, in, pv, j == L ? 1 : 0);
@@ -2619,6 +2678,24 @@ int evp1d_get_fused_depth(evp1d_handle *h, int32_t *depth, int32_t *active, int3
   return EVP1D_OK;
 }
 
+int evp1d_set_cache_hints(evp1d_handle *h, int32_t hints) {
+  if (!h) return fail(EVP1D_EINVAL, "evp1d_set_cache_hints: NULL handle");
+  if (hints < -1 || hints > 1) return fail(EVP1D_EINVAL, "evp1d_set_cache_hints: hints=%d not -1, 0 or 1", hints);
+  if (hints != h->cache_hints) {
+    h->cache_hints = hints;
+    h->graph_dirty = true;   // the captured launches name the kernel
+  }
+  return EVP1D_OK;
+}
+
+int evp1d_get_cache_hints(evp1d_handle *h, int32_t *hints, int32_t *active) {
+  if (!h) return fail(EVP1D_EINVAL, "evp1d_get_cache_hints: NULL handle");
+  if (ready(h)) RC_TRY(ensure(h));
+  if (hints) *hints = h->cache_hints;
+  if (active) *active = pair_form(h) && pair_hints(h) ? 1 : 0;
+  return EVP1D_OK;
+}
+
 int evp1d_autotune_fused(evp1d_handle *h, int32_t nsub, float *us) {
   if (!h) return fail(EVP1D_EINVAL, "evp1d_autotune_fused: NULL handle");
   if (!ready(h)) return fail(EVP1D_ESTATE, "evp1d_autotune_fused: neighbour lists not set");
@@ -2643,9 +2720,10 @@ int evp1d_autotune_fused(evp1d_handle *h, int32_t nsub, float *us) {
   constexpr int ncand = (int)(sizeof(cands) / sizeof(cands[0]));
   const bool keep_auto = h->tiling_auto;
   const int keep_w = h->tile_w, keep_h = h->tile_h, keep_b = h->tile_block, keep_x = h->tile_xcd,
-            keep_ip = h->fused_inplace, keep_d = h->fused_depth, keep_o = h->tile_order;
+            keep_ip = h->fused_inplace, keep_d = h->fused_depth, keep_o = h->tile_order,
+            keep_hints = h->cache_hints;
   float best = 1e30f;
-  int win = -1, win_ip = 0, win_o = 0;
+  int win = -1, win_ip = 0, win_o = 0, win_h = -1;
   int rc = EVP1D_OK;
   for (int c = 0; c < ncand && !rc; ++c) {
     const int order = cands[c].order;
@@ -2657,18 +2735,21 @@ int evp1d_autotune_fused(evp1d_handle *h, int32_t nsub, float *us) {
     if (cands[c].depth == 2 && !pair_form(h)) continue;     // pair tables do not fit
     SavedState st;
     rc = st.save(h, "evp1d_autotune_fused");
-    for (int ip = 0; ip < (cands[c].depth == 1 ? 2 : 1) && !rc; ++ip) {
-      h->fused_inplace = ip;
+    // depth 1: both stress storages; depth 2: without and with cache hints
+    for (int v = 0; v < 2 && !rc; ++v) {
+      h->fused_inplace = cands[c].depth == 1 ? v : 0;
+      h->cache_hints = cands[c].depth == 2 ? v : keep_hints;
       rc = ensure_fused_buffers(h);
       float t = 0.f;
       if (!rc) rc = time_launches(h, 1, [&] { return issue_fused(h, nsub); }, &t);
       t /= (float)nsub;
-      if (!rc && t < best) { best = t; win = c; win_ip = ip; win_o = order; }
+      if (!rc && t < best) { best = t; win = c; win_ip = h->fused_inplace; win_o = order; win_h = h->cache_hints; }
     }
     rc = st.restore(h, rc, "evp1d_autotune_fused");
   }
   h->graph_dirty = true;
   if (rc || win < 0) {
+    h->cache_hints = keep_hints;
     h->fused_inplace = keep_ip;
     (void)evp1d_set_fused_depth(h, keep_d);
     (void)evp1d_set_tile_order(h, keep_o);
@@ -2682,6 +2763,7 @@ int evp1d_autotune_fused(evp1d_handle *h, int32_t nsub, float *us) {
     return rc ? rc : fail(EVP1D_ESTATE, "evp1d_autotune_fused: no tiling fits these lists");
   }
   h->fused_inplace = win_ip;
+  h->cache_hints = cands[win].depth == 2 ? win_h : keep_hints;
   RC_TRY(evp1d_set_fused_depth(h, cands[win].depth));
   RC_TRY(evp1d_set_tile_order(h, win_o));
   RC_TRY(evp1d_set_tiling(h, cands[win].w, cands[win].h, cands[win].block, 1));

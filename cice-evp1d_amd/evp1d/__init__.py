@@ -59,6 +59,7 @@ EXPORTS = (
     "evp1d_set_tiling", "evp1d_get_tiling", "evp1d_tile_plan",
     "evp1d_set_fused_inplace", "evp1d_get_fused_inplace", "evp1d_autotune_fused",
     "evp1d_set_fused_depth", "evp1d_get_fused_depth", "evp1d_set_tile_order", "evp1d_get_tile_order",
+    "evp1d_set_cache_hints", "evp1d_get_cache_hints",
     "evp1d_set_grid", "evp1d_put_2d", "evp1d_get_2d", "evp1d_host_alloc", "evp1d_host_free",
     "evp2d_create", "evp2d_destroy", "evp2d_set_params", "evp2d_set_cells", "evp2d_set_halo",
     "evp2d_set_field", "evp2d_get_field", "evp2d_run", "evp2d_stress", "evp2d_stepu", "evp2d_halo",
@@ -165,6 +166,8 @@ def load(fma: bool = False, path: str | os.PathLike | None = None) -> ctypes.CDL
         "evp1d_autotune_fused": [_H, i32, ctypes.POINTER(ctypes.c_float)],
         "evp1d_set_fused_depth": [_H, i32],
         "evp1d_get_fused_depth": [_H, _IP, _IP, _IP],
+        "evp1d_set_cache_hints": [_H, i32],
+        "evp1d_get_cache_hints": [_H, _IP, _IP],
         "evp2d_create": [i32, i32, i32, ctypes.POINTER(Params), ctypes.POINTER(_H)],
         "evp2d_destroy": [_H],
         "evp2d_set_params": [_H, ctypes.POINTER(Params)],
@@ -468,13 +471,24 @@ class Engine:
         _check(self.L, self.L.evp1d_get_fused_depth(self.h, ctypes.byref(d), ctypes.byref(a), ctypes.byref(b)))
         return {"depth": d.value, "active": bool(a.value), "block": b.value}
 
+    def set_cache_hints(self, hints: int) -> None:
+        """Pair form's cache hints: -1 chosen by size (default), 0 off, 1 on."""
+        _check(self.L, self.L.evp1d_set_cache_hints(self.h, int(hints)))
+
+    def get_cache_hints(self) -> dict:
+        """{"hints": setting, "active": runs use them}."""
+        s, a = ctypes.c_int32(), ctypes.c_int32()
+        _check(self.L, self.L.evp1d_get_cache_hints(self.h, ctypes.byref(s), ctypes.byref(a)))
+        return {"hints": s.value, "active": bool(a.value)}
+
     def autotune_fused(self, nsub: int = 100) -> dict:
-        """Time the fused tile ladder x stress storage (evp1d_autotune_fused),
-        keep the fastest; returns it with its time per subcycle."""
+        """Time the fused tile ladder x stress storage and the pair form's tiles
+        x cache hints (evp1d_autotune_fused), keep the fastest; returns it with
+        its time per subcycle."""
         us = ctypes.c_float()
         _check(self.L, self.L.evp1d_autotune_fused(self.h, nsub, ctypes.byref(us)))
         return dict(self.get_tiling(), inplace=self.get_fused_inplace(), depth=self.get_fused_depth()["depth"],
-                    order=self.get_tile_order(), us=us.value)
+                    order=self.get_tile_order(), hints=self.get_cache_hints()["hints"], us=us.value)
 
     # lifetime ------------------------------------------------------------
     def close(self) -> None:

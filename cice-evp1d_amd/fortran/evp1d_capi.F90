@@ -66,6 +66,7 @@ module evp1d_capi
             evp1d_set_grid, evp1d_put_2d, evp1d_get_2d, evp1d_host_alloc, evp1d_host_free, &
             evp1d_set_fused_inplace, evp1d_get_fused_inplace, evp1d_autotune_fused,    &
             evp1d_set_fused_depth, evp1d_get_fused_depth,                                &
+            evp1d_set_cache_hints, evp1d_get_cache_hints,                                &
             evp1d_adjoint_run, evp1d_loss, evp1d_tangent_run, evp1d_set_adjoint_segment, &
             evp1d_get_adjoint_segment, evp1d_set_tile_order, evp1d_get_tile_order,       &
             evp1d_get_gradient_storage, evp1d_release_gradient_storage,                  &
@@ -320,6 +321,16 @@ module evp1d_capi
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: h
       integer(c_int32_t), intent(out) :: depth, active, block
+    end function
+    integer(c_int) function evp1d_set_cache_hints(h, hints) bind(C, name='evp1d_set_cache_hints')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: hints
+    end function
+    integer(c_int) function evp1d_get_cache_hints(h, hints, active) bind(C, name='evp1d_get_cache_hints')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(out) :: hints, active
     end function
     ! ---- the v0 2D block path -------------------------------------------
     integer(c_int) function evp2d_create(nxb, nyb, nblocks, p, h) bind(C, name='evp2d_create')

@@ -265,15 +265,30 @@ int evp1d_set_tile_order(evp1d_handle *h, int32_t order);
 int evp1d_get_tile_order(evp1d_handle *h, int32_t *order);
 int evp1d_set_fused_depth(evp1d_handle *h, int32_t depth);
 int evp1d_get_fused_depth(evp1d_handle *h, int32_t *depth, int32_t *active, int32_t *block);
+/* Cache hints of the pair form (no counterpart in the reference).  Its
+ * launches can mark what each launch reads or writes once (stresses in and
+ * out, the T points' constant fields, velocities and trajectory written) as
+ * non-temporal loads and stores, so that the 256 MiB last-level cache keeps
+ * what every launch reads again (the U points' constant fields and tables,
+ * the velocities).  Same results bit for bit.  That pays when a launch's
+ * bytes exceed that cache and the reread part fits it: 8 % less time per
+ * subcycle at 1280x1520 (1.36M points) and at 1.68M; 9-13 % more time
+ * below 0.7M points, 4-7 % more from 2.5M.  hints = -1 (default): chosen by
+ * those sizes; 0: off; 1: on.  evp1d_get_cache_hints reports the setting and
+ * whether the pair form's fused launches use them (pair form selected and
+ * hints on; the persistent launch and the unfused forms never do).         */
+int evp1d_set_cache_hints(evp1d_handle *h, int32_t hints);
+int evp1d_get_cache_hints(evp1d_handle *h, int32_t *hints, int32_t *active);
 /* Times one fused run of nsub subcycles (after an untimed one) for each tile
  * of 64x16/512, 32x16/512, 32x8/256, 16x8/128 (XCD-aware order) that fits
  * the lists, each with both stress storages, and the pair form on tiles of
  * 31x19, 27x23, 39x15, 33x17, 31x15, 62x19, 62x15, 15x19, 23x11, 15x15 and
- * 62x7 along the Hilbert curve and 31x19, 31x15, 15x15 in rows of tiles, on
- * the handle's own data (the state is saved and restored), keeps the
- * fastest (as evp1d_set_tiling,
- * evp1d_set_fused_inplace, evp1d_set_fused_depth and evp1d_set_tile_order
- * would) and reports its time per subcycle in microseconds.                 */
+ * 62x7 along the Hilbert curve and 31x19, 31x15, 15x15 in rows of tiles,
+ * each without and with cache hints, on the handle's own data (the state is
+ * saved and restored), keeps the fastest (as evp1d_set_tiling,
+ * evp1d_set_fused_inplace, evp1d_set_fused_depth, evp1d_set_tile_order and
+ * evp1d_set_cache_hints would) and reports its time per subcycle in
+ * microseconds.                                                             */
 int evp1d_autotune_fused(evp1d_handle *h, int32_t nsub, float *us);
 
 /* Host-only tiling of a 1D list (no device use; for inspection and tests).

@@ -55,16 +55,18 @@ def set_mode(eng: evp1d.Engine, mode: str) -> None:
     """mode: a loop form ("auto" is the library's default: the pair form where
     the lists tile), "fused" and "fused-inplace" (one subcycle per launch,
     stresses in two buffers or updated in place), "fused-pair" (two
-    subcycles per launch on 32x16 tiles) or "persistent-pair" (the pair
-    form's launches as one cooperative launch)."""
+    subcycles per launch on 32x16 tiles), "pair-hints" (the same with the
+    cache hints on) or "persistent-pair" (the pair form's launches as one
+    cooperative launch)."""
     if mode in ("fused", "fused-inplace"):
         eng.set_fused_depth(1)
         eng.set_fused_inplace(mode == "fused-inplace")
         mode = "fused"
-    elif mode in ("fused-pair", "persistent-pair"):
+    elif mode in ("fused-pair", "pair-hints", "persistent-pair"):
         eng.set_fused_depth(2)
         eng.set_tiling(32, 16, 512, 1)
-        mode = "fused" if mode == "fused-pair" else "persistent"
+        eng.set_cache_hints(1 if mode == "pair-hints" else 0)
+        mode = "persistent" if mode == "persistent-pair" else "fused"
     eng.set_mode(mode)
 
 

@@ -18,7 +18,8 @@ from gpu_common import (GOLD, assert_state_bitexact, fetch_state, golden_cases, 
                         make_engine, oracle_params, set_mode, synthetic_state)
 
 pytestmark = pytest.mark.gpu
-MODES = ("auto", "launch", "graph", "persistent", "fused", "fused-inplace", "fused-pair", "persistent-pair")
+MODES = ("auto", "launch", "graph", "persistent", "fused", "fused-inplace", "fused-pair", "pair-hints",
+         "persistent-pair")
 
 
 def _log(msg):
@@ -95,8 +96,9 @@ def test_halo_copy_list(gpu):
     for mode in MODES:
         with make_engine(st, 9, mode) as eng:
             eng.set_halo(dst, src)
-            if mode in ("fused-pair", "persistent-pair"):   # the copy list runs between the two subcycles of a launch
+            if mode in ("fused-pair", "pair-hints", "persistent-pair"):   # the copy list runs between the two subcycles of a launch
                 assert eng.get_fused_depth()["active"]
+            assert eng.get_cache_hints()["active"] == (mode == "pair-hints")
             eng.run(9)
             got = fetch_state(eng, st)
         assert_state_bitexact(got, want)
@@ -290,7 +292,7 @@ def test_batch_members_match_independent_oracle_runs(gpu):
                 assert_state_bitexact(got, wants[k])
 
 
-@pytest.mark.parametrize("mode", ["auto", "fused-pair", "persistent-pair", "graph"])
+@pytest.mark.parametrize("mode", ["auto", "fused-pair", "pair-hints", "persistent-pair", "graph"])
 def test_batch_with_copy_lists_matches_member_oracle_runs(gpu, mode):
     """A batch whose ghost copy list is the union of member-local lists (each
     member's ghosts copy that member's active points or zero), written in
@@ -321,7 +323,7 @@ def test_batch_with_copy_lists_matches_member_oracle_runs(gpu, mode):
             gd.append(info["ghost_offset"] + (dst - st.na))
             gs.append(np.where(src > 0, info["active_offset"] + src, 0))
         eng.set_halo(np.concatenate(gd).astype(np.int32), np.concatenate(gs).astype(np.int32))
-        if mode in ("fused-pair", "persistent-pair"):
+        if mode in ("fused-pair", "pair-hints", "persistent-pair"):
             assert eng.get_fused_depth()["active"]
         eng.run(ndte)
         for k, st in enumerate(states):
@@ -438,6 +440,10 @@ def test_setting_errors_are_reported(gpu):
         assert eng.get_adjoint_segment() == 0
         eng.set_adjoint_segment(7)
         assert eng.get_adjoint_segment() == 7
+        for bad in (-2, 2):
+            with pytest.raises(evp1d.Evp1dError):
+                eng.set_cache_hints(bad)
+        assert eng.get_cache_hints()["hints"] == -1
 
 
 @pytest.mark.parametrize("mode", ["fused", "fused-inplace", "fused-pair"])

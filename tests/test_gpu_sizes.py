@@ -4,13 +4,15 @@
   the flang-built reference f0's (tests/golden/golden.json, made by
   tests/golden/make_golden.py evp_c1) in the library default form, the
   two-kernel graph, the persistent kernel, the pair form on 31x15 tiles in
-  Hilbert order (the form the benchmark's tuner picks), the same pair
-  launches as one cooperative launch, and whatever evp1d_autotune_fused
-  picks on the box; the contraction-on build within 1e-12 norm-wise of the
-  oracle in the graph and pair forms.
+  Hilbert order (the form the benchmark's tuner picks; cache hints on, as
+  the size rule picks here, and off), the same pair launches as one
+  cooperative launch, and whatever evp1d_autotune_fused picks on the box;
+  the contraction-on build within 1e-12 norm-wise of the oracle in the
+  graph and pair forms.
 * 2560x3040, 1000 subcycles (na = 5,447,680): md5 equal to the reference f0's
-  in the pair form, the one-subcycle fused form, the persistent kernel and
-  the persistent pair form.
+  in the pair form (cache hints off, as the size rule picks here, and on),
+  the one-subcycle fused form, the persistent kernel and the persistent
+  pair form.
 * 64 ensemble members in one batched handle (25.2 M points, 1000 subcycles):
   every member's output md5 equals its own oracle run
   (tests/golden/make_golden_ensemble.py).
@@ -87,11 +89,16 @@ def set_form(eng: evp1d.Engine, form: str) -> None:
         eng.set_mode("fused")
         eng.set_fused_depth(1)
         eng.set_tiling(64, 16, 512, 1)
-    elif form in ("pair-31x15-hilbert", "persistent-pair-31x15-hilbert"):
+    elif form.startswith(("pair-31x15-hilbert", "persistent-pair-31x15-hilbert")):
+        # cache hints chosen by size, or forced on (-hints) or off (-nohints)
         eng.set_mode("fused" if form.startswith("pair") else "persistent")
         eng.set_tiling(31, 15, 512, 1)
         eng.set_fused_depth(2)
         eng.set_tile_order(1)
+        if form.endswith("-hints"):
+            eng.set_cache_hints(1)
+        elif form.endswith("-nohints"):
+            eng.set_cache_hints(0)
     elif form == "tuned":
         _log(f"autotune_fused: {eng.autotune_fused(100)}")
     else:
@@ -103,9 +110,11 @@ def run_form(st: O.State1D, ndte: int, form: str, fma: bool = False) -> O.State1
     with evp1d.Engine(st.na, st.navel, p, fma=fma) as eng:
         eng.load_state(st.f, st.idx, st.skipT, st.skipU)
         set_form(eng, form)
-        if form.endswith("pair-31x15-hilbert"):
+        if "pair-31x15-hilbert" in form:
             d = eng.get_fused_depth()
             assert d["active"], d
+            if form.startswith("pair") and form.endswith("hints"):
+                assert eng.get_cache_hints()["active"] == form.endswith("-hints")
         t = time.time()
         eng.run(ndte)
         eng.synchronize()
@@ -120,8 +129,8 @@ def check_md5(got: O.State1D, rec: dict) -> None:
 
 
 @pytest.mark.timeout(400)
-@pytest.mark.parametrize("form", ["auto", "graph", "persistent", "pair-31x15-hilbert", "persistent-pair-31x15-hilbert",
-                                  "tuned"])
+@pytest.mark.parametrize("form", ["auto", "graph", "persistent", "pair-31x15-hilbert", "pair-31x15-hilbert-nohints",
+                                  "persistent-pair-31x15-hilbert", "tuned"])
 def test_c1_md5(c1, form):
     """1280x1520, 1000 subcycles: md5 of both output files equal reference f0's."""
     _log(f"evp_c1 form={form}")
@@ -156,7 +165,8 @@ def test_c1_fma_within_1e12(c1):
 
 
 @pytest.mark.timeout(400)
-@pytest.mark.parametrize("form", ["pair-31x15-hilbert", "fused-64x16", "persistent", "persistent-pair-31x15-hilbert"])
+@pytest.mark.parametrize("form", ["pair-31x15-hilbert", "pair-31x15-hilbert-hints", "fused-64x16", "persistent",
+                                  "persistent-pair-31x15-hilbert"])
 def test_c2_md5(c2, form):
     """2560x3040, 1000 subcycles: md5 of both output files equal reference f0's."""
     _log(f"evp_c2 form={form}")
