@@ -43,6 +43,7 @@
 #include "evp1d_adjoint.hpp"
 #include "evp1d_tiles.hpp"
 #include "evp1d_point.hpp"
+#include "evp1d_cache.hpp"
 #include "evp1d_xcd.hpp"
 
 #ifndef EVP1D_FMA
@@ -455,39 +456,11 @@ struct PairBufs {
 // of the next phase (level-1 stepu, level-2 stress, level-2 stepu) before
 // the barrier that ends the current phase, and that phase starts on values
 // already in registers instead of one memory round trip after the barrier.
-// Cache hints of the pair form (NT): the streams a launch reads or writes
-// once (stresses in and out, the T points' constant fields, velocities and
-// trajectory written) as non-temporal loads and stores, so that the
-// last-level cache keeps what every launch reads again (the U points'
-// constant fields, velocities, tables).  Hints only: the same values.
-typedef double NtD2 __attribute__((ext_vector_type(2)));
-template <bool NT>
-__device__ __forceinline__ void out2(double2 *p, double2 v) {
-  if (NT) {
-    const NtD2 x = {v.x, v.y};
-    __builtin_nontemporal_store(x, reinterpret_cast<NtD2 *>(p));
-  } else {
-    *p = v;
-  }
-}
-template <bool NT>
-__device__ __forceinline__ void out1(double *p, double v) {
-  if (NT) __builtin_nontemporal_store(v, p);
-  else *p = v;
-}
-template <bool NT>
-__device__ __forceinline__ double2 in2(const double2 *p) {
-  if (NT) {
-    const NtD2 x = __builtin_nontemporal_load(reinterpret_cast<const NtD2 *>(p));
-    return make_double2(x.x, x.y);
-  }
-  return *p;
-}
-template <bool NT>
-__device__ __forceinline__ double in1(const double *p) {
-  if (NT) return __builtin_nontemporal_load(p);
-  return *p;
-}
+// Cache hints of the pair form (NT, evp1d_cache.hpp): the streams a launch
+// reads or writes once (stresses in and out, the T points' constant fields,
+// velocities and trajectory written) as non-temporal loads and stores, so
+// that the last-level cache keeps what every launch reads again (the U
+// points' constant fields, velocities, tables).
 
 template <int B, bool TRAJ, bool FIX, bool NT>
 __device__ __forceinline__ void pair_tile(const DevView &d, const PairView &f, PairBufs bf, int last, int tix) {
@@ -1747,17 +1720,15 @@ static bool pair_form(const evp1d_handle *h) { return h->fused_depth == 2 && h->
 // when all the bytes a launch touches exceed the last-level cache (256 MiB
 // on MI355X) but the part every launch reads again (the U points' constant
 // fields and tables, about 152 B per U point, and the velocities) is at most
-// 1.2 times it.  Measured on the pair form at 31x19: 8 % less time per
+// 1.3 times it (hints_pay).  Measured on the pair form at 31x19: 8 % less time per
 // subcycle at 1.36M and 1.68M points (reread part 221 and 273 MB); 9-13 %
 // more at 0.34M and 0.63M (everything fits: 113 and 211 MB), 4-7 % more at
 // 2.5M and 5.4M (reread part 409 MB and more).
-constexpr double kLastLevelCache = 256.0 * 1024 * 1024;
 static bool pair_hints(const evp1d_handle *h) {
   if (h->cache_hints >= 0) return h->cache_hints != 0;
   const double nT = h->v.t_end, nU = h->v.u_end - h->v.u_begin;
   const double reread = 152.0 * nU + 16.0 * h->navel;
-  const double all = reread + 248.0 * nT;   // + stresses in and out, the T points' constant fields
-  return all > kLastLevelCache && reread <= 1.2 * kLastLevelCache;
+  return hints_pay(reread, reread + 248.0 * nT);   // + stresses in and out, the T points' constant fields
 }
 
 static int fused_block_index(int b) {

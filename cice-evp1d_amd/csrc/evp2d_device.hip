@@ -37,6 +37,7 @@
 #include <vector>
 
 #include "../../include/evp1d.h"
+#include "evp1d_cache.hpp"
 #include "evp1d_point.hpp"
 #include "evp1d_tiles.hpp"
 #include "evp1d_xcd.hpp"
@@ -233,6 +234,10 @@ __global__ void k2_unpack_sig(double *sig, long n, const int32_t *ocell, const d
   }
 }
 
+// NT: the packed stresses (read and written once per launch) and str1..8 as
+// non-temporal loads and stores (evp1d_cache.hpp), so that the last-level
+// cache keeps the packed constants, tables and velocities between launches.
+template <bool NT>
 __global__ __launch_bounds__(kB2F) void k2_fused(View2 v, Fused2 f, int last) {
   __shared__ double2 lds[4 * kCap2];
   const Tile2 t = f.tiles[tile_of(blockIdx.x, gridDim.x, true)];   // each XCD a contiguous band of tiles
@@ -251,7 +256,7 @@ __global__ __launch_bounds__(kB2F) void k2_fused(View2 v, Fused2 f, int last) {
     double sig[12];
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
-      const double2 s = f.sg_in[q * f.no + p];
+      const double2 s = in2<NT>(f.sg_in + (q * f.no + p));
       sig[2 * q] = s.x;
       sig[2 * q + 1] = s.y;
     }
@@ -263,17 +268,17 @@ __global__ __launch_bounds__(kB2F) void k2_fused(View2 v, Fused2 f, int last) {
     const double2 p15 = make_double2(o.str[0], o.str[4]), p27 = make_double2(o.str[1], o.str[6]);
     const double2 p36 = make_double2(o.str[2], o.str[5]), p48 = make_double2(o.str[3], o.str[7]);
     if (k < t.nto) {
-      f.sg_out[p] = make_double2(o.sp[0], o.sp[1]);
-      f.sg_out[f.no + p] = make_double2(o.sp[2], o.sp[3]);
-      f.sg_out[2 * f.no + p] = make_double2(o.sm[0], o.sm[1]);
-      f.sg_out[3 * f.no + p] = make_double2(o.sm[2], o.sm[3]);
-      f.sg_out[4 * f.no + p] = make_double2(o.s12[0], o.s12[1]);
-      f.sg_out[5 * f.no + p] = make_double2(o.s12[2], o.s12[3]);
+      out2<NT>(f.sg_out + p, make_double2(o.sp[0], o.sp[1]));
+      out2<NT>(f.sg_out + (f.no + p), make_double2(o.sp[2], o.sp[3]));
+      out2<NT>(f.sg_out + (2 * f.no + p), make_double2(o.sm[0], o.sm[1]));
+      out2<NT>(f.sg_out + (3 * f.no + p), make_double2(o.sm[2], o.sm[3]));
+      out2<NT>(f.sg_out + (4 * f.no + p), make_double2(o.s12[0], o.s12[1]));
+      out2<NT>(f.sg_out + (5 * f.no + p), make_double2(o.s12[2], o.s12[3]));
       if (last) {
-        v.s15[c] = p15;
-        v.s27[c] = p27;
-        v.s36[c] = p36;
-        v.s48[c] = p48;
+        out2<NT>(v.s15 + c, p15);
+        out2<NT>(v.s27 + c, p27);
+        out2<NT>(v.s36 + c, p36);
+        out2<NT>(v.s48 + c, p48);
       }
     }
     lds[el] = p15;
@@ -354,6 +359,7 @@ struct evp2d_handle {
   int32_t *fill_dev = nullptr, *dst_dev = nullptr, *src_dev = nullptr;
   // fused form (default): tile tables and the second velocity / stress buffers
   int mode = EVP2D_MODE_FUSED;
+  int cache_hints = -1;   // k2_fused's cache hints: -1 chosen by size (hints2), 0 off, 1 on
   Tile2 *tiles_dev = nullptr;
   int32_t *ftcell_dev = nullptr, *fucell_dev = nullptr;
   uint16_t *ftlds_dev = nullptr, *fulds_dev = nullptr;
@@ -611,6 +617,16 @@ int launch_halo2_on(evp2d_handle *h, double2 *vel) {
   return EVP1D_OK;
 }
 
+// Whether k2_fused uses the cache hints.  By default (hints_pay): what every
+// launch reads again is the packed constants (5 pairs per T entry, 7 per U
+// entry), the tables (20 B per T entry, 12 per U entry) and the two velocity
+// arrays (16 B per cell each); the packed stresses add 192 B per owned T cell.
+bool hints2(const evp2d_handle *h) {
+  if (h->cache_hints >= 0) return h->cache_hints != 0;
+  const double reread = 100.0 * h->et + 124.0 * h->eu + 32.0 * h->n;
+  return hints_pay(reread, reread + 192.0 * h->no);
+}
+
 // nsub fused subcycles.  The velocities start in the handle's buffer (X),
 // are copied to the second one (Y), and the subcycles alternate so that the
 // last one writes X: subcycle k writes X when nsub - k is even.  The
@@ -628,6 +644,7 @@ int issue_fused2(evp2d_handle *h, int nsub) {
     hipLaunchKernelGGL(k2_pack_sig, dim3(blocks2(h->no)), dim3(kB2), 0, h->stream, (const double *)X.sig, h->n,
                        (const int32_t *)h->ocell_dev, h->sg_dev[0], h->no);
   HIP2(hipGetLastError());
+  const auto fused = hints2(h) ? k2_fused<true> : k2_fused<false>;
   int cur = 0;
   for (int k = 1; k <= nsub; ++k) {
     const bool to_x = ((nsub - k) % 2) == 0;
@@ -637,7 +654,7 @@ int issue_fused2(evp2d_handle *h, int nsub) {
              h->tc_dev,    h->uc_dev,     h->et,         h->eu,        h->no,         h->sg_dev[cur],
              h->sg_dev[1 - cur], out.vel};
     if (h->ntiles > 0)
-      hipLaunchKernelGGL(k2_fused, dim3(h->ntiles), dim3(kB2F), 0, h->stream, in, f, k == nsub ? 1 : 0);
+      hipLaunchKernelGGL(fused, dim3(h->ntiles), dim3(kB2F), 0, h->stream, in, f, k == nsub ? 1 : 0);
     HIP2(hipGetLastError());
     RC2(launch_halo2_on(h, out.vel));
     cur = 1 - cur;
@@ -934,6 +951,24 @@ int evp2d_set_mode(evp2d_handle *h, int32_t mode) {
 int evp2d_get_mode(evp2d_handle *h, int32_t *mode) {
   if (!h || !mode) return fail2(EVP1D_EINVAL, "evp2d_get_mode: NULL argument");
   *mode = h->mode;
+  return EVP1D_OK;
+}
+
+int evp2d_set_cache_hints(evp2d_handle *h, int32_t hints) {
+  if (!h) return fail2(EVP1D_EINVAL, "evp2d_set_cache_hints: NULL handle");
+  if (hints < -1 || hints > 1) return fail2(EVP1D_EINVAL, "evp2d_set_cache_hints: hints=%d not -1, 0 or 1", hints);
+  if (hints != h->cache_hints) {
+    h->cache_hints = hints;
+    h->graph_dirty = true;   // the captured launches name the kernel
+  }
+  return EVP1D_OK;
+}
+
+int evp2d_get_cache_hints(evp2d_handle *h, int32_t *hints, int32_t *active) {
+  if (!h) return fail2(EVP1D_EINVAL, "evp2d_get_cache_hints: NULL handle");
+  RC2(ensure2(h));
+  if (hints) *hints = h->cache_hints;
+  if (active) *active = h->mode == EVP2D_MODE_FUSED && h->ntiles > 0 && hints2(h) ? 1 : 0;
   return EVP1D_OK;
 }
 

@@ -64,7 +64,7 @@ EXPORTS = (
     "evp2d_create", "evp2d_destroy", "evp2d_set_params", "evp2d_set_cells", "evp2d_set_halo",
     "evp2d_set_field", "evp2d_get_field", "evp2d_run", "evp2d_stress", "evp2d_stepu", "evp2d_halo",
     "evp2d_run_individual", "evp2d_synchronize", "evp2d_timer_start", "evp2d_timer_stop",
-    "evp2d_get_cells", "evp2d_set_mode", "evp2d_get_mode",
+    "evp2d_get_cells", "evp2d_set_mode", "evp2d_get_mode", "evp2d_set_cache_hints", "evp2d_get_cache_hints",
 )
 
 
@@ -183,6 +183,8 @@ def load(fma: bool = False, path: str | os.PathLike | None = None) -> ctypes.CDL
         "evp2d_get_cells": [_H, _IP, _IP],
         "evp2d_set_mode": [_H, i32],
         "evp2d_get_mode": [_H, _IP],
+        "evp2d_set_cache_hints": [_H, i32],
+        "evp2d_get_cache_hints": [_H, _IP, _IP],
         "evp1d_tile_plan": [i32, i32] + [_IP] * 8 + [i32, i32, i32] + [_IP] * 5 + [i32, _IP, i32, _IP, _IP, i32,
                                                                                 _IP, _IP],
     }
@@ -667,6 +669,16 @@ class Engine2D:
         m = ctypes.c_int32()
         _check(self.L, self.L.evp2d_get_mode(self.h, ctypes.byref(m)))
         return ("plain", "fused")[m.value]
+
+    def set_cache_hints(self, hints: int) -> None:
+        """Fused form's cache hints: -1 chosen by size (default), 0 off, 1 on."""
+        _check(self.L, self.L.evp2d_set_cache_hints(self.h, int(hints)))
+
+    def get_cache_hints(self) -> dict:
+        """{"hints": setting, "active": fused runs use them}."""
+        s, a = ctypes.c_int32(), ctypes.c_int32()
+        _check(self.L, self.L.evp2d_get_cache_hints(self.h, ctypes.byref(s), ctypes.byref(a)))
+        return {"hints": s.value, "active": bool(a.value)}
 
     def stress(self) -> None:
         _check(self.L, self.L.evp2d_stress(self.h))

@@ -73,7 +73,8 @@ module evp1d_capi
             evp2d_create, evp2d_destroy, evp2d_set_params, evp2d_set_cells, evp2d_set_halo, &
             evp2d_set_field, evp2d_get_field, evp2d_run, evp2d_stress, evp2d_stepu,      &
             evp2d_halo, evp2d_run_individual, evp2d_synchronize, evp2d_timer_start,    &
-            evp2d_timer_stop, evp2d_get_cells, evp2d_set_mode, evp2d_get_mode
+            evp2d_timer_stop, evp2d_get_cells, evp2d_set_mode, evp2d_get_mode,           &
+            evp2d_set_cache_hints, evp2d_get_cache_hints
 
   interface
     integer(c_int) function evp1d_set_evp_parameters(ndte, dt, p) bind(C, name='evp1d_set_evp_parameters')
@@ -420,6 +421,16 @@ module evp1d_capi
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: h
       integer(c_int32_t), intent(out) :: mode
+    end function
+    integer(c_int) function evp2d_set_cache_hints(h, hints) bind(C, name='evp2d_set_cache_hints')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: hints
+    end function
+    integer(c_int) function evp2d_get_cache_hints(h, hints, active) bind(C, name='evp2d_get_cache_hints')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(out) :: hints, active
     end function
     integer(c_int) function evp2d_get_cells(h, nt, nu) bind(C, name='evp2d_get_cells')
       import :: c_int, c_int32_t, c_ptr

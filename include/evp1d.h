@@ -464,6 +464,16 @@ int evp2d_run(evp2d_handle *h, int32_t nsub);
 enum evp2d_mode { EVP2D_MODE_PLAIN = 0, EVP2D_MODE_FUSED = 1 };
 int evp2d_set_mode(evp2d_handle *h, int32_t mode);
 int evp2d_get_mode(evp2d_handle *h, int32_t *mode);
+/* Cache hints of the fused form (no counterpart in the reference), as
+ * evp1d_set_cache_hints: the tile kernel reads and writes its packed
+ * stresses (and the last subcycle's str1..8) as non-temporal loads and
+ * stores, so that the last-level cache keeps the packed constants, tables
+ * and velocities between subcycles.  Same results bit for bit.  hints = -1
+ * (default): chosen by the domain size; 0: off; 1: on.
+ * evp2d_get_cache_hints reports the setting and whether fused runs use
+ * them.                                                                     */
+int evp2d_set_cache_hints(evp2d_handle *h, int32_t hints);
+int evp2d_get_cache_hints(evp2d_handle *h, int32_t *hints, int32_t *active);
 int evp2d_stress(evp2d_handle *h);
 int evp2d_stepu(evp2d_handle *h);
 int evp2d_halo(evp2d_handle *h);

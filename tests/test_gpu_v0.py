@@ -30,12 +30,20 @@ def _params(st, ndte):
     return p
 
 
-MODES2D = ("fused", "plain")
+MODES2D = ("fused", "fused-hints", "plain")
+
+
+def _set_mode(eng, mode):
+    """"fused" (cache hints chosen by size: off at these sizes), "fused-hints"
+    (forced on) or "plain"."""
+    eng.set_mode("fused" if mode.startswith("fused") else mode)
+    if mode == "fused-hints":
+        eng.set_cache_hints(1)
 
 
 def _engine(st, ndte, mode="fused"):
     eng = evp1d.Engine2D(st.nxb, st.nyb, st.nblocks, _params(st, ndte))
-    eng.set_mode(mode)
+    _set_mode(eng, mode)
     eng.load_state(st)
     return eng
 
@@ -205,6 +213,10 @@ def test_v0_bad_inputs_rejected(gpu):
             eng.set_halo(h["ilo"], h["jlo"], h["ihi"], h["jhi"], src, dst)
         with pytest.raises(evp1d.Evp1dError):
             eng.set_halo(h["ilo"], h["jlo"], np.full_like(h["ihi"], nxb), h["jhi"], h["src"], h["dst"])
+        for bad in (-2, 2):
+            with pytest.raises(evp1d.Evp1dError):
+                eng.set_cache_hints(bad)
+        assert eng.get_cache_hints() == {"hints": -1, "active": False}
         # the handle still works after the rejections
         want = st.copy()
         V = O.OracleV0()
@@ -243,7 +255,7 @@ def test_v0_fma_build_within_1e12(gpu, mode):
     V.run(V.params(25), want, 25)
     eng = evp1d.Engine2D(st.nxb, st.nyb, st.nblocks, _params(st, 25), fma=True)
     with eng:
-        eng.set_mode(mode)
+        _set_mode(eng, mode)
         eng.load_state(st)
         eng.run(25)
         worst = 0.0
