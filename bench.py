@@ -543,7 +543,8 @@ def ensemble_measure(nmem: int, steps: int, autotune: int, tune_fused: int = 0, 
     return out
 
 
-def v0_measure(workload: str, steps: int, warmup: int, kernel_reps: int, form: str = "fused") -> dict:
+def v0_measure(workload: str, steps: int, warmup: int, kernel_reps: int, form: str = "fused",
+               hints: int | None = None) -> dict:
     """The v0 2D block path (evp2d_*) on the workload's grid as one block:
     CICE's (nx_block, ny_block) arrays, per-block cell lists.  Algorithmic
     bytes per subcycle in the reference's data model: stress 360 B per T cell
@@ -552,7 +553,8 @@ def v0_measure(workload: str, steps: int, warmup: int, kernel_reps: int, form: s
     (indxui/indxuj 8, uvel/vvel 16, 14 forcing/metric arrays 112, str 64,
     uvel/vvel 16, strintx/y 16, taubx/y 16); the reference's clearing of the
     whole str scratch in every stress call is not counted.  form: "fused"
-    (one tile kernel per subcycle, the default) or "plain" (--mode graph)."""
+    (one tile kernel per subcycle, the default) or "plain" (--mode graph);
+    hints: the fused form's cache hints (None: the library's choice by size)."""
     import evp1d
     from evp1d import layout, synth
     w = dict(synth.WORKLOADS[workload])
@@ -562,8 +564,11 @@ def v0_measure(workload: str, steps: int, warmup: int, kernel_reps: int, form: s
     nT, nU = int(blk.icellt[0]), int(blk.icellu[0])
     eng = evp1d.Engine2D(blk.nxb, blk.nyb, 1, evp1d.set_evp_parameters(ndte))
     eng.set_mode(form)
+    if hints is not None:
+        eng.set_cache_hints(hints)
     eng.load_state(blk)
-    log(f"[bench] v0 path: {blk.nxb}x{blk.nyb} one block, icellt={nT} icellu={nU}")
+    hints_on = eng.get_cache_hints()["active"]
+    log(f"[bench] v0 path: {blk.nxb}x{blk.nyb} one block, icellt={nT} icellu={nU}, cache hints {int(hints_on)}")
     for k in range(warmup):
         eng.run(ndte)
         eng.synchronize()
@@ -593,7 +598,7 @@ def v0_measure(workload: str, steps: int, warmup: int, kernel_reps: int, form: s
     traffic = None
     tf = REPO / "profiles" / f"traffic_{workload}.json"
     if tf.exists():
-        traffic = json.loads(tf.read_text()).get("k2_" + name)
+        traffic = json.loads(tf.read_text()).get("k2_" + name + ("@hints" if name == "fused" and hints_on else ""))
     step_us = secs / (steps * ndte) * 1e6
     sub_bytes = B["stress"] + B["stepu"]
     return {
@@ -602,7 +607,7 @@ def v0_measure(workload: str, steps: int, warmup: int, kernel_reps: int, form: s
         "warmup": warmup, "ms_per_step": secs / steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{workload} {w['nx']}x{w['ny']} ndte={ndte}, v0 2D block path (one block)",
-                   "form": mode,
+                   "form": mode, "cache_hints": int(hints_on),
                    "nx_block": blk.nxb, "ny_block": blk.nyb, "icellt": nT, "icellu": nU, "ndte": ndte,
                    "us_per_subcycle": round(step_us, 3),
                    "loop_GBps_algorithmic": round(sub_bytes / (step_us * 1e-6) / 1e9, 1)},
@@ -652,7 +657,8 @@ def main() -> int:
     ap.add_argument("--depth", type=int, default=None, choices=(1, 2),
                     help="subcycles per fused launch: 2 the pair form (implies --tune-fused 0)")
     ap.add_argument("--hints", type=int, default=None, choices=(-1, 0, 1),
-                    help="pair form's cache hints: -1 chosen by size, 0 off, 1 on (implies --tune-fused 0)")
+                    help="cache hints of the pair form (or, with --path v0, of the 2D fused form): -1 chosen "
+                         "by size, 0 off, 1 on (implies --tune-fused 0)")
     ap.add_argument("--tuning", default=None, metavar="SB,SX,UB,UX",
                     help="fixed launch tuning: stress block, stress xcd order, stepu block, stepu xcd order")
     ap.add_argument("--stream-probe", type=int, default=0, metavar="MIB",
@@ -671,7 +677,8 @@ def main() -> int:
         return 2
     if args.path == "v0":
         form = "plain" if args.mode == "graph" else "fused"
-        print(json.dumps(v0_measure(args.workload, args.steps, args.warmup, args.kernel_reps, form)), flush=True)
+        print(json.dumps(v0_measure(args.workload, args.steps, args.warmup, args.kernel_reps, form, args.hints)),
+              flush=True)
         return 0
     if args.api:
         out = api_measure(args.workload, 3)
