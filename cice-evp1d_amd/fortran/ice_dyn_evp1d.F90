@@ -26,9 +26,10 @@
 ! fields (convert_2d_1d_dyn / convert_1d_2d_dyn, ice_dyn_evp1d.F90:818-947)
 ! also run on the device: whole 2D arrays cross PCIe and the library gathers
 ! and scatters them; .false. keeps the host loops.  With lautotune (default
-! .false.) the first run first times the fused form's tiles and stress
-! storages on the first step's own fields (evp1d_autotune_fused, state
-! restored) and keeps the fastest; results do not depend on it.
+! .false.) the first run first times the fused form's tiles, stress
+! storages, depth and cache hints on the first step's own fields
+! (evp1d_autotune_fused, state restored) and keeps the fastest, written to
+! evp1d.log with linfo; results do not depend on it.
 !===============================================================================
 module ice_dyn_evp1d
   use, intrinsic :: iso_c_binding
@@ -247,6 +248,7 @@ contains
     logical(kind=log_kind), allocatable :: GT(:,:), GU(:,:)
     type(evp1d_params_t) :: p
     real(c_float) :: tune_us
+    integer(c_int32_t) :: tw, th, tb, tx, ntl, dep, dact, dblk, hset, hact
     integer(kind=int_kind) :: iw
 
     if (.not. c_associated(handle)) error stop 'dyn_evp1d_run: dyn_evp1d_init was not called'
@@ -328,7 +330,14 @@ contains
     call evp1d_check(evp1d_set_params(handle, p), 'evp1d_set_params')
     if (lautotune .and. .not. tuned) then
       call evp1d_check(evp1d_autotune_fused(handle, int(min(ndte, 100),c_int32_t), tune_us), 'autotune_fused')
-      if (linfo .and. log_open) write(iu06, '(a,f10.2,a)') ' evp1d: autotune_fused ', tune_us, ' us/subcycle'
+      if (linfo .and. log_open) then
+        call evp1d_check(evp1d_get_tiling(handle, tw, th, tb, tx, ntl), 'evp1d_get_tiling')
+        call evp1d_check(evp1d_get_fused_depth(handle, dep, dact, dblk), 'evp1d_get_fused_depth')
+        call evp1d_check(evp1d_get_cache_hints(handle, hset, hact), 'evp1d_get_cache_hints')
+        write(iu06, '(a,f10.2,a,i0,a,i0,a,i0,a,i0,a,i0)') ' evp1d: autotune_fused ', tune_us, &
+          ' us/subcycle, tile ', tw, 'x', th, ', workgroup ', tb, ', depth ', merge(2, 1, dact /= 0), &
+          ', cache hints ', hact
+      endif
       tuned = .true.
     endif
     call evp1d_check(evp1d_run(handle, int(ndte,c_int32_t)), 'evp1d_run')

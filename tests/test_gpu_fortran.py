@@ -19,6 +19,8 @@ import sys
 import tempfile
 from pathlib import Path
 
+import re
+
 import numpy as np
 import pytest
 
@@ -169,6 +171,8 @@ def test_api_driver_matches_reference_dataflow(gpu, nx, ny, seed, ndte, nrun, de
         log = (Path(d) / "evp1d.log").read_text()
     assert "nActive" in log
     assert ("autotune_fused" in log) == autotune
+    if autotune:   # the choice: tile, workgroup, depth and cache hints
+        assert re.search(r"tile \d+x\d+, workgroup \d+, depth [12], cache hints [01]", log), log
     assert "Time to die for evp1d" in log      # dyn_evp1d_finalize closes the log
     shape = g.tmask.shape
     got = raw.reshape((29,) + shape[::-1]).transpose(0, 2, 1)
