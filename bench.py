@@ -208,18 +208,22 @@ def _probe_lib():
     L.evp1d_probe_stream.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64,
                                      ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
     L.evp1d_probe_stream.restype = ctypes.c_int
+    L.evp1d_probe_stream_hints.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+    L.evp1d_probe_stream_hints.restype = ctypes.c_int
     return L
 
 
-def probe_rate(nr: int, nw: int, mib: int = 512, reps: int = 20) -> float:
+def probe_rate(nr: int, nw: int, mib: int = 512, reps: int = 20, hints: int = 0) -> float:
     """GB/s (read + written) of the streaming probe at nr read and nw written
-    streams of mib MiB, one point per thread."""
+    streams of mib MiB, one point per thread; hints = 1: non-temporal loads
+    and stores."""
     import ctypes
     n = mib * (1 << 20) // 16 // 1024 * 1024
     us = ctypes.c_double()
-    rc = _probe_lib().evp1d_probe_stream(nr, nw, 1, n, reps, ctypes.byref(us))
+    rc = _probe_lib().evp1d_probe_stream_hints(nr, nw, 1, hints, n, reps, ctypes.byref(us))
     if rc:
-        raise RuntimeError(f"evp1d_probe_stream({nr}, {nw}, 1) returned {rc}")
+        raise RuntimeError(f"evp1d_probe_stream_hints({nr}, {nw}, 1, {hints}) returned {rc}")
     return (nr + nw) * n * 16 / us.value / 1e3
 
 
@@ -231,17 +235,19 @@ def stream_probe(mib: int, reps: int = 20) -> int:
     L = _probe_lib()
     n = mib * (1 << 20) // 16 // 1024 * 1024
     rows = []
-    for per in (1, 4):
-        for nr, nw in ((1, 0), (1, 1), (2, 2), (2, 1), (3, 1), (4, 1), (5, 1), (6, 1)):
-            us = ctypes.c_double()
-            rc = L.evp1d_probe_stream(nr, nw, per, n, reps, ctypes.byref(us))
-            if rc:
-                raise RuntimeError(f"evp1d_probe_stream({nr}, {nw}, {per}) returned {rc}")
-            b = (nr + nw) * n * 16
-            rows.append({"read_streams": nr, "write_streams": nw, "points_per_thread": per,
-                         "read_fraction": round(nr / (nr + nw), 3), "bytes": b, "us": round(us.value, 2),
-                         "GB_s": round(b / us.value / 1e3, 1)})
-            log(f"[probe] {nr} in {nw} out, {per} per thread: {us.value:.1f} us, {b / us.value / 1e3:.1f} GB/s")
+    for hints in (0, 1):
+        for per in (1, 4):
+            for nr, nw in ((1, 0), (1, 1), (2, 2), (2, 1), (3, 1), (4, 1), (5, 1), (6, 1)):
+                us = ctypes.c_double()
+                rc = L.evp1d_probe_stream_hints(nr, nw, per, hints, n, reps, ctypes.byref(us))
+                if rc:
+                    raise RuntimeError(f"evp1d_probe_stream_hints({nr}, {nw}, {per}, {hints}) returned {rc}")
+                b = (nr + nw) * n * 16
+                rows.append({"read_streams": nr, "write_streams": nw, "points_per_thread": per, "hints": hints,
+                             "read_fraction": round(nr / (nr + nw), 3), "bytes": b, "us": round(us.value, 2),
+                             "GB_s": round(b / us.value / 1e3, 1)})
+                log(f"[probe] {nr} in {nw} out, {per} per thread, hints {hints}: {us.value:.1f} us, "
+                    f"{b / us.value / 1e3:.1f} GB/s")
     print(json.dumps({"stream_probe_mib_per_stream": mib, "launches": reps, "mixes": rows}))
     return 0
 
@@ -798,9 +804,15 @@ def main() -> int:
         mix = {"k_pair": (5, 1), "k_fused": (3, 1), "k_stress": (1, 1), "k_stepu": (6, 1)}.get(dom["name"])
         if mix:
             try:
+                # a kernel that runs with the cache hints is held to the higher of
+                # the plain and the hinted probe
                 ceil = probe_rate(*mix)
-                roofline["probe"] = {"read_streams": mix[0], "write_streams": mix[1], "GBps": round(ceil, 1),
-                                     "frac_physical_of_probe": round(phys / ceil, 4)}
+                roofline["probe"] = {"read_streams": mix[0], "write_streams": mix[1], "GBps": round(ceil, 1)}
+                if tiling.get("cache_hints"):
+                    ceil_nt = probe_rate(*mix, hints=1)
+                    roofline["probe"]["GBps_hints"] = round(ceil_nt, 1)
+                    ceil = max(ceil, ceil_nt)
+                roofline["probe"]["frac_physical_of_probe"] = round(phys / ceil, 4)
                 log(f"[bench] streaming probe {mix[0]}:{mix[1]}: {ceil:.1f} GB/s; {dom['name']} at "
                     f"{phys / ceil:.3f} of it")
             except Exception as e:   # reported, never fatal

@@ -9,24 +9,30 @@
 // loads stay but nothing is written.  A thread handles `per` points, one
 // grid's width apart (every access stays coalesced), with all their loads
 // issued before the first sum: per = 4 keeps four times as many bytes in
-// flight per thread.
+// flight per thread.  With hints = 1 every load and store is non-temporal
+// (the cache hints of evp1d_cache.hpp), the ceiling for kernels that use them.
 //
 //   int evp1d_probe_stream(int32_t nr, int32_t nw, int32_t per, int64_t n,
 //                          int32_t reps, double *us_per_launch);
+//   int evp1d_probe_stream_hints(int32_t nr, int32_t nw, int32_t per, int32_t hints,
+//                                int64_t n, int32_t reps, double *us_per_launch);
 //
 // allocates (nr + nw) * n * 16 bytes, fills the inputs, runs two untimed
 // launches and then reps timed ones (HIP events on its own stream), frees
 // everything and returns 0, or -1 for an unsupported mix, -2 for a HIP
 // error.  Supported: nr in 1..6 with nw 0 or 1, and nr = nw = 2; per 1 or 4
-// with n a multiple of 256 * per.
+// with n a multiple of 256 * per; hints 0 or 1.  evp1d_probe_stream is
+// hints = 0.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 
+#include "evp1d_cache.hpp"
+
 namespace {
 
 // grid: n / (256 * PER) workgroups of 256 threads (n a multiple of 256 * PER)
-template <int NR, int NW, int PER>
+template <int NR, int NW, int PER, bool NT>
 __global__ __launch_bounds__(256) void k_stream(const double2 *__restrict__ src, double2 *__restrict__ dst,
                                                 long n) {
   const long w = (long)gridDim.x * 256;
@@ -35,7 +41,7 @@ __global__ __launch_bounds__(256) void k_stream(const double2 *__restrict__ src,
 #pragma unroll
   for (int p = 0; p < PER; ++p)
 #pragma unroll
-    for (int j = 0; j < NR; ++j) v[p][j] = src[(long)j * n + i0 + p * w];
+    for (int j = 0; j < NR; ++j) v[p][j] = in2<NT>(src + ((long)j * n + i0 + p * w));
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
     double2 s = make_double2(0.0, 0.0);
@@ -46,10 +52,10 @@ __global__ __launch_bounds__(256) void k_stream(const double2 *__restrict__ src,
     }
     const long i = i0 + p * w;
     if (NW == 0) {
-      if (s.x == -1.0) dst[i] = s;   // inputs are >= 0: never taken
+      if (s.x == -1.0) out2<NT>(dst + i, s);   // inputs are >= 0: never taken
     } else {
 #pragma unroll
-      for (int j = 0; j < NW; ++j) dst[(long)j * n + i] = make_double2(s.x + j, s.y);
+      for (int j = 0; j < NW; ++j) out2<NT>(dst + ((long)j * n + i), make_double2(s.x + j, s.y));
     }
   }
 }
@@ -61,28 +67,33 @@ __global__ void k_fill(double2 *p, long n) {
 
 using Fn = void (*)(const double2 *, double2 *, long);
 
-template <int PER>
+template <int PER, bool NT>
 Fn pick_per(int nr, int nw) {
   if (nw == 0) {
-    static const Fn f[6] = {k_stream<1, 0, PER>, k_stream<2, 0, PER>, k_stream<3, 0, PER>,
-                            k_stream<4, 0, PER>, k_stream<5, 0, PER>, k_stream<6, 0, PER>};
+    static const Fn f[6] = {k_stream<1, 0, PER, NT>, k_stream<2, 0, PER, NT>, k_stream<3, 0, PER, NT>,
+                            k_stream<4, 0, PER, NT>, k_stream<5, 0, PER, NT>, k_stream<6, 0, PER, NT>};
     return nr >= 1 && nr <= 6 ? f[nr - 1] : nullptr;
   }
   if (nw == 1) {
-    static const Fn f[6] = {k_stream<1, 1, PER>, k_stream<2, 1, PER>, k_stream<3, 1, PER>,
-                            k_stream<4, 1, PER>, k_stream<5, 1, PER>, k_stream<6, 1, PER>};
+    static const Fn f[6] = {k_stream<1, 1, PER, NT>, k_stream<2, 1, PER, NT>, k_stream<3, 1, PER, NT>,
+                            k_stream<4, 1, PER, NT>, k_stream<5, 1, PER, NT>, k_stream<6, 1, PER, NT>};
     return nr >= 1 && nr <= 6 ? f[nr - 1] : nullptr;
   }
-  return nr == 2 && nw == 2 ? k_stream<2, 2, PER> : nullptr;
+  return nr == 2 && nw == 2 ? k_stream<2, 2, PER, NT> : nullptr;
 }
 
-Fn pick(int nr, int nw, int per) { return per == 1 ? pick_per<1>(nr, nw) : per == 4 ? pick_per<4>(nr, nw) : nullptr; }
+Fn pick(int nr, int nw, int per, int hints) {
+  if (hints != 0 && hints != 1) return nullptr;
+  if (per == 1) return hints ? pick_per<1, true>(nr, nw) : pick_per<1, false>(nr, nw);
+  if (per == 4) return hints ? pick_per<4, true>(nr, nw) : pick_per<4, false>(nr, nw);
+  return nullptr;
+}
 
 }  // namespace
 
-extern "C" int evp1d_probe_stream(int32_t nr, int32_t nw, int32_t per, int64_t n, int32_t reps,
-                                  double *us_per_launch) {
-  const Fn fn = pick(nr, nw, per);
+extern "C" int evp1d_probe_stream_hints(int32_t nr, int32_t nw, int32_t per, int32_t hints, int64_t n,
+                                        int32_t reps, double *us_per_launch) {
+  const Fn fn = pick(nr, nw, per, hints);
   if (!fn || n < 1 || n % (256L * per) != 0 || reps < 1 || !us_per_launch) return -1;
   double2 *src = nullptr, *dst = nullptr;
   hipStream_t st = nullptr;
@@ -112,4 +123,9 @@ done:
   if (src) (void)hipFree(src);
   if (dst) (void)hipFree(dst);
   return rc;
+}
+
+extern "C" int evp1d_probe_stream(int32_t nr, int32_t nw, int32_t per, int64_t n, int32_t reps,
+                                  double *us_per_launch) {
+  return evp1d_probe_stream_hints(nr, nw, per, 0, n, reps, us_per_launch);
 }

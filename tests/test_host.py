@@ -60,6 +60,13 @@ def test_stream_probe_library_rejects_unsupported_mixes():
     assert f(1, 1, 4, (1 << 20) + 256, 1, ctypes.byref(us)) == -1   # not a multiple of 256 x 4
     assert f(1, 1, 1, 0, 1, ctypes.byref(us)) == -1
     assert f(1, 1, 1, 1 << 20, 0, ctypes.byref(us)) == -1
+    g = L.evp1d_probe_stream_hints
+    g.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
+                  ctypes.POINTER(ctypes.c_double)]
+    g.restype = ctypes.c_int
+    for hints in (-1, 2):
+        assert g(1, 1, 1, hints, 1 << 20, 1, ctypes.byref(us)) == -1, hints
+    assert g(7, 1, 1, 1, 1 << 20, 1, ctypes.byref(us)) == -1
 
 
 def _c_prototypes():
