@@ -13,7 +13,8 @@ Checks:
   * the gradient does not depend on the number of stored states, from one
     stored state (recompute everything) to all of them (no recomputation),
     nor on the checkpointing unit (single subcycles, or segments reversed
-    through a stored velocity trajectory), in the graph and pair forms;
+    through a stored velocity trajectory), in the graph and pair forms (the
+    latter also with the cache hints forced on);
   * the pair-form forward sweep (two stored subcycles per launch, odd and
     tiny nsub) gives the same gradient as one subcycle per launch.
 """
@@ -113,7 +114,7 @@ def test_central_finite_differences(problem):
     assert worst <= 1e-6
 
 
-@pytest.mark.parametrize("mode", ["graph", "fused-pair"])
+@pytest.mark.parametrize("mode", ["graph", "fused-pair", "pair-hints"])
 def test_gradient_independent_of_snapshot_count(problem, mode):
     """From one stored state to all of them, and for every checkpointing
     unit: one subcycle (binomial checkpointing with recomputed str1..8),
