@@ -3,9 +3,11 @@
 // MI355X keeps a 256 MiB last-level cache (Infinity Cache) in front of HBM.
 // A kernel that streams through more data than that per launch evicts what
 // the next launch reads again.  Marking the streams that a launch reads or
-// writes once as non-temporal (global_load ... nt, global_store ... nt) lets
-// the cache keep the rest.  Hints only: every load and store returns and
-// writes the same values either way.
+// writes once as non-temporal (global_load ... nt, global_store ... nt) is
+// meant to let the cache keep the rest; the gain appears only in the size
+// window that this explanation predicts (hints_pay), and no counter on the
+// chip separates cache hits from DRAM reads.  Hints only: every load and
+// store returns and writes the same values either way.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -456,12 +456,12 @@ struct PairBufs {
 // of the next phase (level-1 stepu, level-2 stress, level-2 stepu) before
 // the barrier that ends the current phase, and that phase starts on values
 // already in registers instead of one memory round trip after the barrier.
-// Cache hints of the pair form (NT, evp1d_cache.hpp): the streams a launch
-// reads or writes once (stresses in and out, the T points' constant fields,
-// velocities and trajectory written) as non-temporal loads and stores, so
-// that the last-level cache keeps what every launch reads again (the U
-// points' constant fields, velocities, tables).
-
+//
+// NT: the cache hints (evp1d_cache.hpp).  The streams a launch reads or
+// writes once (stresses in and out, the T points' constant fields,
+// velocities and trajectory written) become non-temporal loads and stores,
+// so that the cache can keep what every launch reads again (the U points'
+// constant fields, velocities, tables).
 template <int B, bool TRAJ, bool FIX, bool NT>
 __device__ __forceinline__ void pair_tile(const DevView &d, const PairView &f, PairBufs bf, int last, int tix) {
   extern __shared__ double2 lds[];

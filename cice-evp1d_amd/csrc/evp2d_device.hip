@@ -236,7 +236,7 @@ __global__ void k2_unpack_sig(double *sig, long n, const int32_t *ocell, const d
 
 // NT: the packed stresses (read and written once per launch) and str1..8 as
 // non-temporal loads and stores (evp1d_cache.hpp), so that the last-level
-// cache keeps the packed constants, tables and velocities between launches.
+// cache can keep the packed constants, tables and velocities between launches.
 template <bool NT>
 __global__ __launch_bounds__(kB2F) void k2_fused(View2 v, Fused2 f, int last) {
   __shared__ double2 lds[4 * kCap2];
