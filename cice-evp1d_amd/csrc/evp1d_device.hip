@@ -1225,6 +1225,7 @@ struct evp1d_handle {
   void *ckpt = nullptr;            // a checkpointed gradient's stored states, kept between calls
   size_t ckpt_bytes = 0;
   bool pconst_fresh = false;       // pconst holds the current fields (set for a gradient call's duration)
+  bool fills_fresh = false;        // str1..8 of the constant entries and *cnz filled by this gradient call
   // 2D grid of the caller's arrays (device-side pack/unpack)
   int grid_nx = 0, grid_ny = 0;
   std::vector<int32_t> grid_ij;    // caller position < na -> 0-based linear 2D index
@@ -1771,10 +1772,12 @@ static int ensure_fused_buffers(evp1d_handle *h) {
 // Before the first fused subcycle of a run: the constant entries' str1..8
 // and, for an in-place run, the shared points' stresses (into copy 0).
 // Fills the run's view; fused_step points it at subcycle k.
-static int fused_prologue(evp1d_handle *h, const DevView &d, FusedView &f, bool ip) {
-  hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, h->stream, h->cnz_dev);
-  HIP_TRY(hipGetLastError());
-  if (h->nk_total > 0) {
+static int fused_prologue(evp1d_handle *h, const DevView &d, FusedView &f, bool ip, bool fills = true) {
+  if (fills) {
+    hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, h->stream, h->cnz_dev);
+    HIP_TRY(hipGetLastError());
+  }
+  if (fills && h->nk_total > 0) {
     hipLaunchKernelGGL(k_fill_const, dim3((unsigned)((h->nk_total + 255) / 256)), dim3(256), 0, h->stream, d,
                        (const int32_t *)h->klist_dev, h->cstr_dev, h->nk_total, h->cnz_dev);
     HIP_TRY(hipGetLastError());
@@ -1844,11 +1847,21 @@ static int launch_pair_persistent(evp1d_handle *h, const DevView &x, PairView pv
 
 // persistent: the pair launches as one cooperative launch (eager only: a
 // cooperative launch is not captured into the run's graph)
-static int issue_fused(evp1d_handle *h, int nsub, double2 *traj = nullptr, bool persistent = false) {
+// sig_src (pair form, even nsub): the starting stresses are read from there
+// (a stored state of a checkpointed gradient) instead of the handle's own;
+// unpack = false leaves the handle's stresses stale after the run (the
+// gradient's segment sweeps, whose end state nothing reads).  Within a
+// gradient call (pconst_fresh) the constant entries' str1..8 and *cnz are
+// filled by the first run only: they do not change during the call.
+static int issue_fused(evp1d_handle *h, int nsub, double2 *traj = nullptr, bool persistent = false,
+                       const double *sig_src = nullptr, bool unpack = true) {
   const bool pair = pair_form(h);
   if (traj && !pair) return fail(EVP1D_ESTATE, "issue_fused: a trajectory needs the pair form");
   if (persistent && (!pair || traj)) return fail(EVP1D_ESTATE, "issue_fused: the persistent form needs the pair form");
   const bool odd_first = pair && nsub % 2 == 1;
+  if ((sig_src || !unpack) && (!pair || odd_first))
+    return fail(EVP1D_ESTATE, "issue_fused: stresses in a stored state need the pair form and an even run");
+  const bool fills = !(h->pconst_fresh && h->fills_fresh);
   const long nu = (long)h->v.u_end - h->v.u_begin;
   const bool ip = h->fused_inplace && !pair;
   const DevView X = h->v;
@@ -1862,7 +1875,7 @@ static int issue_fused(evp1d_handle *h, int nsub, double2 *traj = nullptr, bool 
                            h->stream));
   }
   FusedView f;
-  RC_TRY(fused_prologue(h, X, f, ip));
+  RC_TRY(fused_prologue(h, X, f, ip, fills));
   const long t_end = h->v.t_end;
   PairView pv{h->ptiles_dev, h->hlist_dev, h->pxlist_dev, h->pelist_dev, h->puloc_dev, h->pvloc_dev,
               h->pcstr_dev,  h->cnz_dev,   h->pk_total,   nullptr,       h->pair_cap,  h->pconst,
@@ -1874,8 +1887,8 @@ static int issue_fused(evp1d_handle *h, int nsub, double2 *traj = nullptr, bool 
   double2 *P[2] = {reinterpret_cast<double2 *>(h->sig_pair), reinterpret_cast<double2 *>(h->sig_alt)};
   int pin = 0;                 // P[pin]: the pair launches' current stresses
   bool packed = false;
-  const double *soa = X.sig;   // the current stresses before packing
-  if (pair && h->pk_total > 0) {
+  const double *soa = sig_src ? sig_src : X.sig;   // the current stresses before packing
+  if (pair && h->pk_total > 0 && fills) {
     hipLaunchKernelGGL(k_fill_const, dim3((unsigned)((h->pk_total + 255) / 256)), dim3(256), 0, h->stream, X,
                        (const int32_t *)h->pklist_dev, h->pcstr_dev, h->pk_total, h->cnz_dev);
     HIP_TRY(hipGetLastError());
@@ -1940,11 +1953,12 @@ static int issue_fused(evp1d_handle *h, int nsub, double2 *traj = nullptr, bool 
       HIP_TRY(hipMemcpyAsync(traj + nu, out.vel + h->v.u_begin, (size_t)nu * sizeof(double2),
                              hipMemcpyDeviceToDevice, h->stream));
   }
-  if (packed && t_end > 0) {
+  if (packed && t_end > 0 && unpack) {
     hipLaunchKernelGGL(k_unpack_sig, dim3(blocks_for((int)t_end)), dim3(kBlock), 0, h->stream, X.sig, h->ldsig,
                        (const double2 *)P[pin], h->ldsig, t_end);
     HIP_TRY(hipGetLastError());
   }
+  if (h->pconst_fresh) h->fills_fresh = true;
   return fused_epilogue(h, X, nsub, ip);
 }
 
@@ -2810,11 +2824,18 @@ struct AdjointRun {
   int nunits = 0;                // ceil(nsub / seg)
   double *diag_save = nullptr;   // [3][na]: strintx, strinty, Cb of x_nsub (checkpointed, seg > 1)
   int working = -1;              // unit index i of the state held by the working arrays
+  // lazy: pair-form units of even length.  A restore then copies only the
+  // velocities and leaves sig_at pointing at the stored state's stresses,
+  // which the next sweep packs from directly; a segment sweep that is not
+  // the last leaves the handle's stresses stale (working = -1).
+  bool lazy = false;
+  const double *sig_at = nullptr;
   double loss = 0.0;
 
   ~AdjointRun() {
     if (!h) return;
     h->pconst_fresh = false;
+    h->fills_fresh = false;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
   }
   // The stored states, the final state and its str1..8 and diagnostics:
@@ -2866,10 +2887,19 @@ struct AdjointRun {
     HIP_TRY(hipMemcpyAsync(dst.vel, src.vel, n, hipMemcpyDeviceToDevice, h->stream));
     return EVP1D_OK;
   }
-  StateBuf work() const { return StateBuf{h->v.sig, h->v.vel}; }
+  // the handle's arrays, and the working state's (its stresses may still be
+  // in a stored state: sig_at)
+  StateBuf own() const { return StateBuf{h->v.sig, h->v.vel}; }
+  StateBuf work() const { return StateBuf{sig_at ? const_cast<double *>(sig_at) : h->v.sig, h->v.vel}; }
   int restore(int slot, int index) {
     if (working == index) return EVP1D_OK;
-    RC_TRY(copy_state(work(), slots[(size_t)slot], false));
+    if (lazy) {
+      HIP_TRY(hipMemcpyAsync(h->v.vel, slots[(size_t)slot].vel, (size_t)h->na * sizeof(double2),
+                             hipMemcpyDeviceToDevice, h->stream));
+      sig_at = slots[(size_t)slot].sig;
+    } else {
+      RC_TRY(copy_state(own(), slots[(size_t)slot], false));
+    }
     working = index;
     return EVP1D_OK;
   }
@@ -2881,7 +2911,8 @@ struct AdjointRun {
     const int m = unit_start(working + n) - unit_start(working);
     if (seg > 1 && m > 0 && h->fused_ok) {
       RC_TRY(ensure_fused_buffers(h));
-      RC_TRY(issue_fused(h, m));
+      RC_TRY(issue_fused(h, m, nullptr, false, sig_at));
+      sig_at = nullptr;
     } else {
       for (int k = 0; k < m; ++k) {
         RC_TRY(launch_stress(h));
@@ -2906,7 +2937,7 @@ struct AdjointRun {
     return EVP1D_OK;
   }
   int restore_final() {
-    RC_TRY(copy_state(work(), final_state, false));
+    RC_TRY(copy_state(own(), final_state, false));
     const size_t nV = (size_t)h->navel * sizeof(double2), nA = (size_t)h->na * sizeof(double);
     double2 *pairs[4] = {h->v.s15, h->v.s27, h->v.s36, h->v.s48};
     for (int k = 0; k < 4; ++k)
@@ -2967,7 +2998,9 @@ struct AdjointRun {
     if (seg > 1) {
       const int b = unit_start(k - 1), e = unit_start(k);
       RC_TRY(run_segment(e - b, k == nunits));
-      working = k;   // the segment's forward sweep left its end state
+      // the segment's forward sweep left its end state (with lazy units, a
+      // segment other than the last leaves the stresses stale)
+      working = lazy && k != nunits ? -1 : k;
       return EVP1D_OK;
     }
     if (k == nsub) RC_TRY(final_step());
@@ -3039,7 +3072,8 @@ return EVP1D_OK;
     if (pair_form(h)) {
       // two subcycles per launch, both stored (k_pair<.., TRAJ>)
       RC_TRY(ensure_fused_buffers(h));
-      RC_TRY(issue_fused(h, N, U));
+      RC_TRY(issue_fused(h, N, U, false, sig_at, !lazy || final));
+      sig_at = nullptr;
     } else if (h->fused_ok) {
       // fused forward sweep; two-buffer stresses ping-pong between the
       // handle's buffers and end in its own (subcycle k writes it when N - k
@@ -3282,6 +3316,7 @@ int evp1d_adjoint_run(evp1d_handle *h, int32_t nsub, int32_t nsnap, const double
     if (h->traj_bytes > (seg > 1 ? trajectory_bytes(h, seg) : 0)) RC_TRY(release(h->traj, h->traj_bytes));
     R.seg = seg;
     R.nunits = (nsub + seg - 1) / seg;
+    R.lazy = pair_form(h) && h->fused_ok && seg > 1 && seg % 2 == 0 && (nsub - (R.nunits - 1) * seg) % 2 == 0;
     const int S = std::min(S0, R.nunits);   // more slots than units are never used
     RC_TRY(R.carve_checkpoints(S));
     R.sched.build(R.nunits, S - 1);
