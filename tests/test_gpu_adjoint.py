@@ -127,7 +127,10 @@ def test_gradient_independent_of_snapshot_count(problem, mode):
     want = st.copy()
     lib.run(p, want, NDTE)
     out = []
-    cases = [(1, 1), (3, 1), (7, 1), (1, 0), (3, 0), (7, 0), (3, 5), (2, 3), (NDTE + 5, 0), (0, 0)]
+    # (2, 2) and (3, 4): units of even length with few stored states, so the
+    # pair form's sweeps start from stored stresses also when the schedule
+    # recomputes from one stored state
+    cases = [(1, 1), (3, 1), (7, 1), (1, 0), (3, 0), (7, 0), (3, 5), (2, 3), (2, 2), (3, 4), (NDTE + 5, 0), (0, 0)]
     for nsnap, seg in cases:     # nsnap 0: chosen by the library (all, here)
         with make_engine(st, NDTE, mode) as eng:
             eng.set_adjoint_segment(seg)
