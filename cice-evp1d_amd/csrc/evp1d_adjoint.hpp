@@ -12,6 +12,7 @@
 // are taken as products with its reciprocal), so no bit-exactness claim.
 #pragma once
 
+#include "evp1d_cache.hpp"
 #include "evp1d_point.hpp"
 
 namespace evp1d {
@@ -141,6 +142,9 @@ __host__ __device__ inline void combos_adjoint(double dxT, double dyT, double dx
 // reversed one after the other, each from its own strain rates, and feed the
 // velocity accumulators as they go: few values stay live (no per-corner
 // arrays), so the kernel keeps several waves per SIMD without spills.
+// NT: the adjoint stresses read and written with the cache hints
+// (evp1d_cache.hpp).
+template <bool NT = false>
 __host__ __device__ inline double stress_adjoint(const Scalars &s,
     double uvel_cc, double vvel_cc, double uvel_ee, double vvel_ee,
     double uvel_se, double vvel_se, double uvel_ne, double vvel_ne,
@@ -158,7 +162,7 @@ __host__ __device__ inline double stress_adjoint(const Scalars &s,
   // adjoint of the updated stresses including the str1..8 path
   double sb[12];
 #pragma unroll
-  for (int k = 0; k < 12; ++k) sb[k] = sig_b_new[k * ss];
+  for (int k = 0; k < 12; ++k) sb[k] = in1<NT>(sig_b_new + k * ss);
   combos_adjoint(dxT, dyT, dxhy, dyhx, str_b, sb);
 
   const double keep = c1 - s.arlx1i * s.revp;
@@ -212,9 +216,9 @@ __host__ __device__ inline double stress_adjoint(const Scalars &s,
     // sp = (sp0*keep + arlx1i*(zeta*div - rep))*denom1, rep = (1-K)*T*D
     // sm = (sm0*keep + arlx1i*eta*ten)*denom1, s12 = (s120*keep + arlx1i*p5*eta*shr)*denom1
     const double P = sb[c] * s.denom1, M = sb[4 + c] * s.denom1, Q = sb[8 + c] * s.denom1;
-    sig_b_out[c * ss] = P * keep;
-    sig_b_out[(4 + c) * ss] = M * keep;
-    sig_b_out[(8 + c) * ss] = Q * keep;
+    out1<NT>(sig_b_out + c * ss, P * keep);
+    out1<NT>(sig_b_out + (4 + c) * ss, M * keep);
+    out1<NT>(sig_b_out + (8 + c) * ss, Q * keep);
     const double zeta_b0 = P * s.arlx1i * dv;
     const double rep_b = -P * s.arlx1i;
     const double eta_b = M * s.arlx1i * tn + Q * s.arlx1i * p5 * sh;

@@ -17,7 +17,7 @@ constexpr double kLastLevelCache = 256.0 * 1024 * 1024;
 typedef double NtD2 __attribute__((ext_vector_type(2)));
 
 template <bool NT>
-__device__ __forceinline__ void out2(double2 *p, double2 v) {
+__host__ __device__ __forceinline__ void out2(double2 *p, double2 v) {
   if (NT) {
     const NtD2 x = {v.x, v.y};
     __builtin_nontemporal_store(x, reinterpret_cast<NtD2 *>(p));
@@ -26,12 +26,12 @@ __device__ __forceinline__ void out2(double2 *p, double2 v) {
   }
 }
 template <bool NT>
-__device__ __forceinline__ void out1(double *p, double v) {
+__host__ __device__ __forceinline__ void out1(double *p, double v) {
   if (NT) __builtin_nontemporal_store(v, p);
   else *p = v;
 }
 template <bool NT>
-__device__ __forceinline__ double2 in2(const double2 *p) {
+__host__ __device__ __forceinline__ double2 in2(const double2 *p) {
   if (NT) {
     const NtD2 x = __builtin_nontemporal_load(reinterpret_cast<const NtD2 *>(p));
     return make_double2(x.x, x.y);
@@ -39,7 +39,7 @@ __device__ __forceinline__ double2 in2(const double2 *p) {
   return *p;
 }
 template <bool NT>
-__device__ __forceinline__ double in1(const double *p) {
+__host__ __device__ __forceinline__ double in1(const double *p) {
   if (NT) return __builtin_nontemporal_load(p);
   return *p;
 }

@@ -899,22 +899,38 @@ __global__ __launch_bounds__(256) void k_str_recompute(DevView d) {
 // trajectory (SPLIT), or recomputed from the str1..8 that k_str_recompute
 // left in the working arrays: stepu_point is the forward kernels' own
 // function, so both give the same bits.
-template <bool SPLIT>
+//
+// NT (with SPLIT): the cache hints (evp1d_cache.hpp) on the stored
+// trajectory's velocities, the adjoint stresses, the velocity contributions
+// and the gradient accumulators; the constant fields, ub and g stay cached.
+
+// a velocity of subcycle k's start state: from the stored trajectory (split
+// view, stepu slots) or the handle's arrays
+template <bool SPLIT, bool NT>
+__device__ __forceinline__ double2 adj_vel(const DevView &d, int j) {
+  if (SPLIT && j >= d.u_begin && j < d.u_end) return in2<NT>(d.vel + (j - d.u_begin));
+  return Ld<SPLIT>::vel(d, j);
+}
+template <bool NT>
+__device__ __forceinline__ void adj_acc(double *p, double v) {
+  out1<NT>(p, in1<NT>(p) + v);
+}
+
+template <bool SPLIT, bool NT = false>
 __global__ __launch_bounds__(256) void k_stepu_adj(DevView d, AdjView a) {
-  using L = Ld<SPLIT>;
   const int iw = d.u_begin + blockIdx.x * 256 + threadIdx.x;
   if (iw >= d.u_end) return;
   const int jw = d.nw[iw], js = d.sse[iw], jsw = d.sw[iw];
   const long na = d.na;
   double2 ub = a.ub[iw];
-  if (iw < d.t_end) { const double2 t = a.vc[iw]; ub.x += t.x; ub.y += t.y; }
-  if (jw < d.t_end) { const double2 t = a.vc[na + jw]; ub.x += t.x; ub.y += t.y; }
-  if (jsw < d.t_end) { const double2 t = a.vc[3 * na + jsw]; ub.x += t.x; ub.y += t.y; }
-  if (js < d.t_end) { const double2 t = a.vc[2 * na + js]; ub.x += t.x; ub.y += t.y; }
-  const double2 v = L::vel(d, iw);
+  if (iw < d.t_end) { const double2 t = in2<NT>(a.vc + iw); ub.x += t.x; ub.y += t.y; }
+  if (jw < d.t_end) { const double2 t = in2<NT>(a.vc + (na + jw)); ub.x += t.x; ub.y += t.y; }
+  if (jsw < d.t_end) { const double2 t = in2<NT>(a.vc + (3 * na + jsw)); ub.x += t.x; ub.y += t.y; }
+  if (js < d.t_end) { const double2 t = in2<NT>(a.vc + (2 * na + js)); ub.x += t.x; ub.y += t.y; }
+  const double2 v = adj_vel<SPLIT, NT>(d, iw);
   double2 vn;
   if (SPLIT) {
-    vn = d.vel_out[iw - d.u_begin];
+    vn = in2<NT>(d.vel_out + (iw - d.u_begin));
   } else {
     // str1..8 of slots stress never writes are +0.0 unless *cnz: no loads then
     const bool cz = *a.cnz == 0;
@@ -934,36 +950,36 @@ __global__ __launch_bounds__(256) void k_stepu_adj(DevView d, AdjView a) {
                        ub.x, ub.y, o);
   a.ub[iw] = make_double2(o.uold_b, o.vold_b);
   a.g[iw] = make_double2(o.gx, o.gy);
-  a.gfx[iw] += o.fx_b;
-  a.gfy[iw] += o.fy_b;
-  a.gcw[iw] += o.cw_b;
+  adj_acc<NT>(a.gfx + iw, o.fx_b);
+  adj_acc<NT>(a.gfy + iw, o.fy_b);
+  adj_acc<NT>(a.gcw + iw, o.cw_b);
 }
 
 // Reverse of stress for subcycle k.  str_b of point q: its own terms and the
 // terms stepu took from q as the nw, sse and sw neighbour of the points
 // ee(q), se(q) and ne(q).
-template <bool SPLIT>
+template <bool SPLIT, bool NT = false>
 __global__ __launch_bounds__(256) void k_stress_adj(DevView d, AdjView a) {
-  using L = Ld<SPLIT>;
   const int iw = blockIdx.x * 256 + threadIdx.x;
   if (iw >= d.t_end) return;
   const int4 nb = a.nbr[iw];
   const int ie = nb.x, in = nb.y, is = nb.z;
-  const double2 cc = L::vel(d, iw), ve = L::vel(d, ie), vn = L::vel(d, in), vs = L::vel(d, is);
+  const double2 cc = adj_vel<SPLIT, NT>(d, iw), ve = adj_vel<SPLIT, NT>(d, ie), vn = adj_vel<SPLIT, NT>(d, in),
+                vs = adj_vel<SPLIT, NT>(d, is);
   const double2 g0 = a.g[iw], ge = a.g[ie], gs = a.g[is], gn = a.g[in];
   const double str_b[8] = {g0.x, ge.x, gs.x, gn.x, g0.y, gs.y, ge.y, gn.y};
   const long lp = a.lp;
   const double2 m0 = a.pc[iw], m1 = a.pc[lp + iw], m2 = a.pc[2 * lp + iw];
   double vb[8];
-  const double gs_b = evp1d::stress_adjoint(d.s, cc.x, cc.y, ve.x, ve.y, vs.x, vs.y, vn.x, vn.y, m0.x, m0.y,
+  const double gs_b = evp1d::stress_adjoint<NT>(d.s, cc.x, cc.y, ve.x, ve.y, vs.x, vs.y, vn.x, vn.y, m0.x, m0.y,
                                             d.strength[iw], m1.x, m1.y, m2.x, m2.y, a.sigb + iw, str_b,
                                             a.sigb + iw, d.ldsig, vb);
   const long na = d.na;
-  a.vc[iw] = make_double2(vb[0], vb[1]);
-  a.vc[na + iw] = make_double2(vb[2], vb[3]);
-  a.vc[2 * na + iw] = make_double2(vb[4], vb[5]);
-  a.vc[3 * na + iw] = make_double2(vb[6], vb[7]);
-  a.gstr[iw] += gs_b;
+  out2<NT>(a.vc + iw, make_double2(vb[0], vb[1]));
+  out2<NT>(a.vc + (na + iw), make_double2(vb[2], vb[3]));
+  out2<NT>(a.vc + (2 * na + iw), make_double2(vb[4], vb[5]));
+  out2<NT>(a.vc + (3 * na + iw), make_double2(vb[6], vb[7]));
+  adj_acc<NT>(a.gstr + iw, gs_b);
 }
 
 // ---------------------------------------------------------------------------
@@ -3126,10 +3142,14 @@ return EVP1D_OK;
       loss = 0.0;
       for (double x : t) loss += x;
     }
+    // the reverse kernels take the handle's cache hints (pair_hints)
+    const bool hints = pair_hints(h);
+    const auto stepu_adj = hints ? k_stepu_adj<true, true> : k_stepu_adj<true, false>;
+    const auto stress_adj = hints ? k_stress_adj<true, true> : k_stress_adj<true, false>;
     for (int k = N; k >= 1; --k) {
       const DevView dk = view(k);
-      if (nu > 0) hipLaunchKernelGGL(k_stepu_adj<true>, dim3(blocks(nu)), dim3(256), 0, h->stream, dk, a);
-      if (nt > 0) hipLaunchKernelGGL(k_stress_adj<true>, dim3(blocks(nt)), dim3(256), 0, h->stream, dk, a);
+      if (nu > 0) hipLaunchKernelGGL(stepu_adj, dim3(blocks(nu)), dim3(256), 0, h->stream, dk, a);
+      if (nt > 0) hipLaunchKernelGGL(stress_adj, dim3(blocks(nt)), dim3(256), 0, h->stream, dk, a);
       HIP_TRY(hipGetLastError());
     }
     return EVP1D_OK;
