@@ -164,6 +164,21 @@ def test_c1_fma_within_1e12(c1):
         assert errs[worst] <= 1e-12, (form, worst, errs[worst])
 
 
+@pytest.mark.timeout(200)
+def test_cache_hints_chosen_by_size(c1, c2):
+    """The library's default (hints = -1) turns the pair form's cache hints on
+    at 1280x1520 (the reread part, 221 MB, fits the 256 MiB last-level cache
+    while the whole launch does not) and leaves them off at 2560x3040 (926 MB
+    reread); the default tile and depth are the library's own."""
+    for st, want in ((c1, True), (c2, False)):
+        p = evp1d.set_evp_parameters(1000, **st.scalars)
+        with evp1d.Engine(st.na, st.navel, p) as eng:
+            eng.load_state(st.f, st.idx, st.skipT, st.skipU)
+            assert eng.get_fused_depth()["active"]
+            h = eng.get_cache_hints()
+            assert h == {"hints": -1, "active": want}, (st.na, h)
+
+
 @pytest.mark.timeout(400)
 @pytest.mark.parametrize("form", ["pair-31x15-hilbert", "pair-31x15-hilbert-hints", "fused-64x16", "persistent",
                                   "persistent-pair-31x15-hilbert"])
