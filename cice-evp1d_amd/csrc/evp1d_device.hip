@@ -3596,14 +3596,16 @@ int evp1d_set_grid(evp1d_handle *h, int32_t nx, int32_t ny, const int32_t *indxt
   return EVP1D_OK;
 }
 
-int evp1d_put_2d(evp1d_handle *h, int32_t nfield, const int32_t *fields, const double *arrays) {
-  if (!h) return fail(EVP1D_EINVAL, "evp1d_put_2d: NULL handle");
-  if (h->grid_nx == 0) return fail(EVP1D_ESTATE, "evp1d_put_2d: evp1d_set_grid was not called");
-  if (nfield < 0 || (nfield > 0 && (!fields || !arrays))) return fail(EVP1D_EINVAL, "evp1d_put_2d: bad arguments");
+// wait = false: evp1d_put_2d_async (the copies stay queued)
+static int put_2d_impl(evp1d_handle *h, int32_t nfield, const int32_t *fields, const double *arrays, bool wait,
+                       const char *name) {
+  if (!h) return fail(EVP1D_EINVAL, "%s: NULL handle", name);
+  if (h->grid_nx == 0) return fail(EVP1D_ESTATE, "%s: evp1d_set_grid was not called", name);
+  if (nfield < 0 || (nfield > 0 && (!fields || !arrays))) return fail(EVP1D_EINVAL, "%s: bad arguments", name);
   const size_t n2 = (size_t)h->grid_nx * h->grid_ny;
   FieldRef r;
   for (int k = 0; k < nfield; ++k)
-    if (!field_ref(h, fields[k], r)) return fail(EVP1D_EINVAL, "evp1d_put_2d: unknown field %d", fields[k]);
+    if (!field_ref(h, fields[k], r)) return fail(EVP1D_EINVAL, "%s: unknown field %d", name, fields[k]);
   RC_TRY(ensure_pos2d(h));
   const int na = h->na, navel = h->navel;
   for (int k = 0; k < nfield; ++k) {
@@ -3623,8 +3625,16 @@ int evp1d_put_2d(evp1d_handle *h, int32_t nfield, const int32_t *fields, const d
     }
     HIP_TRY(hipGetLastError());
   }
-  HIP_TRY(hipStreamSynchronize(h->stream));   // the caller may reuse its arrays
+  if (wait) HIP_TRY(hipStreamSynchronize(h->stream));   // the caller may reuse its arrays
   return EVP1D_OK;
+}
+
+int evp1d_put_2d(evp1d_handle *h, int32_t nfield, const int32_t *fields, const double *arrays) {
+  return put_2d_impl(h, nfield, fields, arrays, true, "evp1d_put_2d");
+}
+
+int evp1d_put_2d_async(evp1d_handle *h, int32_t nfield, const int32_t *fields, const double *arrays) {
+  return put_2d_impl(h, nfield, fields, arrays, false, "evp1d_put_2d_async");
 }
 
 int evp1d_get_2d(evp1d_handle *h, int32_t nfield, const int32_t *fields, double *arrays) {

@@ -60,7 +60,7 @@ EXPORTS = (
     "evp1d_set_fused_inplace", "evp1d_get_fused_inplace", "evp1d_autotune_fused",
     "evp1d_set_fused_depth", "evp1d_get_fused_depth", "evp1d_set_tile_order", "evp1d_get_tile_order",
     "evp1d_set_cache_hints", "evp1d_get_cache_hints",
-    "evp1d_set_grid", "evp1d_put_2d", "evp1d_get_2d", "evp1d_host_alloc", "evp1d_host_free",
+    "evp1d_set_grid", "evp1d_put_2d", "evp1d_put_2d_async", "evp1d_get_2d", "evp1d_host_alloc", "evp1d_host_free",
     "evp2d_create", "evp2d_destroy", "evp2d_set_params", "evp2d_set_cells", "evp2d_set_halo",
     "evp2d_set_field", "evp2d_get_field", "evp2d_run", "evp2d_stress", "evp2d_stepu", "evp2d_halo",
     "evp2d_run_individual", "evp2d_synchronize", "evp2d_timer_start", "evp2d_timer_stop",
@@ -157,6 +157,7 @@ def load(fma: bool = False, path: str | os.PathLike | None = None) -> ctypes.CDL
         "evp1d_set_tiling": [_H, i32, i32, i32, i32],
         "evp1d_set_grid": [_H, i32, i32, _IP, i32],
         "evp1d_put_2d": [_H, i32, _IP, _DP],
+        "evp1d_put_2d_async": [_H, i32, _IP, _DP],
         "evp1d_get_2d": [_H, i32, _IP, _DP],
         "evp1d_host_alloc": [ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)],
         "evp1d_host_free": [ctypes.c_void_p],

@@ -63,7 +63,8 @@ module evp1d_capi
             evp1d_member_info, evp1d_set_field_member, evp1d_get_field_member,          &
             evp1d_set_index_member, evp1d_set_skip_member, evp1d_autotune,              &
             evp1d_set_tuning, evp1d_get_tuning, evp1d_set_tiling, evp1d_get_tiling,       &
-            evp1d_set_grid, evp1d_put_2d, evp1d_get_2d, evp1d_host_alloc, evp1d_host_free, &
+            evp1d_set_grid, evp1d_put_2d, evp1d_put_2d_async, evp1d_get_2d, evp1d_host_alloc, &
+            evp1d_host_free, &
             evp1d_set_fused_inplace, evp1d_get_fused_inplace, evp1d_autotune_fused,    &
             evp1d_set_fused_depth, evp1d_get_fused_depth,                                &
             evp1d_set_cache_hints, evp1d_get_cache_hints,                                &
@@ -204,6 +205,15 @@ module evp1d_capi
       integer(c_int32_t), intent(in) :: indxtij(*)
     end function
     integer(c_int) function evp1d_put_2d(h, nfield, fields, arrays) bind(C, name='evp1d_put_2d')
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nfield
+      integer(c_int32_t), intent(in) :: fields(*)
+      real(c_double), intent(in) :: arrays(*)
+    end function
+    ! arrays must stay unchanged until a call that waits (see include/evp1d.h):
+    ! pass page-locked memory that outlives the queued copies
+    integer(c_int) function evp1d_put_2d_async(h, nfield, fields, arrays) bind(C, name='evp1d_put_2d_async')
       import :: c_int, c_int32_t, c_double, c_ptr
       type(c_ptr), value :: h
       integer(c_int32_t), value :: nfield

@@ -269,39 +269,28 @@ contains
                               count(GT), count(GU)
 
     if (ldevpack) then
-      ! 2D -> 1D on the device; fin's order is the argument order
-      call gather_global_ext(gall(:,:,1), L_stressp_1, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,2), L_stressp_2, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,3), L_stressp_3, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,4), L_stressp_4, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,5), L_stressm_1, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,6), L_stressm_2, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,7), L_stressm_3, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,8), L_stressm_4, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,9), L_stress12_1, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,10), L_stress12_2, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,11), L_stress12_3, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,12), L_stress12_4, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,13), L_strength, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,14), L_cdn_ocn, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,15), L_aiu, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,16), L_uocn, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,17), L_vocn, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,18), L_waterxU, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,19), L_wateryU, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,20), L_forcexU, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,21), L_forceyU, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,22), L_umassdti, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,23), L_fmU, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,24), L_strintxU, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,25), L_strintyU, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,26), L_Tbu, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,27), L_Cb, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,28), L_uvel, master_task, distrb_info)
-      call gather_global_ext(gall(:,:,29), L_vvel, master_task, distrb_info)
-      gall(:,:,30) = gall(:,:,28)
-      gall(:,:,31) = gall(:,:,29)
-      call evp1d_check(evp1d_put_2d(handle, int(size(fin),c_int32_t), fin, gall), 'evp1d_put_2d')
+      ! 2D -> 1D on the device; fin's order is the argument order.  Each field
+      ! is queued for the device as soon as it is gathered, so its copy runs
+      ! while the next one is gathered (gall is page-locked and stays
+      ! untouched until evp1d_get_2d below has waited for the stream); the
+      ! initial velocities are uvel and vvel's own staging.
+      call gather_put_async(1, L_stressp_1);    call gather_put_async(2, L_stressp_2)
+      call gather_put_async(3, L_stressp_3);    call gather_put_async(4, L_stressp_4)
+      call gather_put_async(5, L_stressm_1);    call gather_put_async(6, L_stressm_2)
+      call gather_put_async(7, L_stressm_3);    call gather_put_async(8, L_stressm_4)
+      call gather_put_async(9, L_stress12_1);   call gather_put_async(10, L_stress12_2)
+      call gather_put_async(11, L_stress12_3);  call gather_put_async(12, L_stress12_4)
+      call gather_put_async(13, L_strength);    call gather_put_async(14, L_cdn_ocn)
+      call gather_put_async(15, L_aiu);         call gather_put_async(16, L_uocn)
+      call gather_put_async(17, L_vocn);        call gather_put_async(18, L_waterxU)
+      call gather_put_async(19, L_wateryU);     call gather_put_async(20, L_forcexU)
+      call gather_put_async(21, L_forceyU);     call gather_put_async(22, L_umassdti)
+      call gather_put_async(23, L_fmU);         call gather_put_async(24, L_strintxU)
+      call gather_put_async(25, L_strintyU);    call gather_put_async(26, L_Tbu)
+      call gather_put_async(27, L_Cb);          call gather_put_async(28, L_uvel)
+      call gather_put_async(29, L_vvel)
+      call evp1d_check(evp1d_put_2d_async(handle, 1_c_int32_t, fin(30:30), gall(:,:,28)), 'evp1d_put_2d_async')
+      call evp1d_check(evp1d_put_2d_async(handle, 1_c_int32_t, fin(31:31), gall(:,:,29)), 'evp1d_put_2d_async')
     else
       call gather_put(L_stressp_1, F_STRESSP_1);   call gather_put(L_stressp_2, F_STRESSP_2)
       call gather_put(L_stressp_3, F_STRESSP_3);   call gather_put(L_stressp_4, F_STRESSP_4)
@@ -385,6 +374,15 @@ contains
       call gather_global_ext(G, L, master_task, distrb_info)
       call put_dyn(G, field, .false.)
     end subroutine gather_put
+
+    ! field k of fin: gathered into its page-locked staging gall(:,:,k), then
+    ! queued for the device
+    subroutine gather_put_async(k, L)
+      integer, intent(in) :: k
+      real(kind=dbl_kind), intent(in) :: L(:,:,:)
+      call gather_global_ext(gall(:,:,k), L, master_task, distrb_info)
+      call evp1d_check(evp1d_put_2d_async(handle, 1_c_int32_t, fin(k:k), gall(:,:,k)), 'evp1d_put_2d_async')
+    end subroutine gather_put_async
 
     subroutine get_scatter(L, field, with_ghosts)
       real(kind=dbl_kind), intent(inout) :: L(:,:,:)

@@ -202,6 +202,14 @@ int evp1d_host_alloc(int64_t bytes, void **ptr);
 int evp1d_host_free(void *ptr);
 int evp1d_put_2d(evp1d_handle *h, int32_t nfield, const int32_t *fields, const double *arrays);
 int evp1d_get_2d(evp1d_handle *h, int32_t nfield, const int32_t *fields, double *arrays);
+/* evp1d_put_2d without its final wait (no counterpart in the reference): the
+ * copies and gathers are queued on the handle's stream and the call returns
+ * at once, so that the caller can fill its next array meanwhile.  arrays
+ * must stay unchanged until a call that waits for the stream
+ * (evp1d_synchronize, evp1d_get_2d, evp1d_get_field, evp1d_put_2d, ...),
+ * and should be page-locked (evp1d_host_alloc) for the copies to overlap
+ * the caller's work.                                                        */
+int evp1d_put_2d_async(evp1d_handle *h, int32_t nfield, const int32_t *fields, const double *arrays);
 
 /* ---- launch tuning (no counterpart in the reference) ---------------------
  * Block size (64, 128, 256, 512 or 1024 threads) and XCD-aware tile order of
